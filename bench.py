@@ -1,0 +1,204 @@
+"""Benchmark of the electrodiffusion step on one MI355X.
+
+Workload (default): the single large cluster -- jittered hex lattice (10 um
+spacing, +-10% jitter, numpy seed 0) clipped to an 800 um disc (23,222 cells,
+139,332 membranes), 256x256 environment grid, ions Na/K/Cl/Ca/P/M, dt 1e-4 s.
+A step is one iteration of the reference loop over that cluster; the timed
+region is ``--steps`` steps with all state resident in HBM.
+
+Printed: one JSON line with cell-steps/s, the roofline of the dominant kernel
+(algorithmic bytes per launch / HIP-event launch time), and the CPU baseline
+(the FP64 NumPy oracle, timed on a bounded prefix of the same workload on this
+host), plus |dVmem| of the GPU path against that oracle on the same prefix.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str) -> dict:
+    """Compulsory bytes per launch of each step kernel (each array element the
+    kernel must touch, counted once at its storage width)."""
+    C, M = cells.n_cells, cells.n_mems
+    G2 = cells.X.size
+    n_in = len(cells.envInds_inClust)
+    ni = n_ions
+    b = {}
+    b["mem_cell"] = (M * (4 * 4 + 8 + 8 + 8 * ni)          # mem_cell, nn_cell, nn_mem, mem_sq, mem_sa, gjopen, Dm
+                     + M * 8 * ni                           # fsa write
+                     + C * (8 * ni * 2 + 8 * ni + 8 * ni     # c_cell r, c_lag r, c_cell w, c_lag w
+                            + 16 + 16 + 4)                   # vm r/w, cell_vol, diviterm, mem_ptr
+                     + n_in * 8 * (ni + (1 if ca else 0)))  # env gathers (pre- and post-diffusion Ca)
+    b["env_diffuse"] = G2 * (8 * ni + 8 * ni + 16 + 8 * ni)  # c_env r, D_env r, E r, c_mid w
+    b["env_exchange"] = G2 * (8 * ni + 8 * ni + 8 + 1 + 4) + M * (4 + 8 * ni)
+    b["gauss_rows"] = G2 * 16
+    b["gauss_cols_grad"] = G2 * (8 + 24)
+    b["step"] = sum(b.values())
+    return b
+
+
+def make_problem(name: str):
+    from betse_amd.cells import Cells
+    from betse_amd.parameters import Parameters
+    p = Parameters.preset(name)
+    t0 = time.time()
+    cells = Cells.make_world(p)
+    return p, cells, time.time() - t0
+
+
+def gpu_session(p, cells, flux_mode):
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    sim = Simulator(p.copy(), device="cuda:0", flux_mode=flux_mode, use_graph=True)
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    sim.prepare(ph)
+    return sim
+
+
+def time_gpu(p, cells, flux_mode, steps, warmup):
+    import torch
+    sim = gpu_session(p, cells, flux_mode)
+    sim.step_n(warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(sim.stream)
+    sim.step_n(steps, sync=False)
+    e1.record(sim.stream)
+    sim.pull()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    kernels = {k: sim.time_kernel(k, 100) for k in ("mem_cell", "env_diffuse", "env_exchange",
+                                                        "gauss_rows", "gauss_cols_grad")}
+    return sim, wall, ev_ms, kernels
+
+
+def cpu_baseline(p, cells, target_s: float):
+    """FP64 oracle (sparse operators) on this host: a bounded prefix of the run."""
+    from oracle.betse_oracle import OracleSim
+    o = OracleSim(cells, p.copy(), mode="sparse")
+    o.init_core()
+    o.init_dynamics("init")
+    o.p.set_phase_time("init")
+    o.update_V(observables=False)
+    t0 = time.perf_counter()
+    o.step("init", observables=False)
+    one = time.perf_counter() - t0
+    n = max(2, int(target_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    for _ in range(n - 1):
+        o.step("init", observables=False)
+    el = one + (time.perf_counter() - t0)
+    return o, n, el
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--config", default="large")
+    ap.add_argument("--flux", default="both", choices=["fp64", "fp32", "both"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(each GPU box has one GPU; the sharding design is in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    p, cells, t_world = make_problem(a.config)
+    n_ions = len(p.ion_names())
+    has_ca = "Ca" in p.ion_names()
+    C = cells.n_cells
+    print(f"# world: {C} cells, {cells.n_mems} membranes, grid {cells.X.shape}, ions {p.ion_names()}, "
+          f"built in {t_world:.1f}s", file=sys.stderr, flush=True)
+
+    modes = ["fp64", "fp32"] if a.flux == "both" else [a.flux]
+    res = {}
+    for mode in modes:
+        sim, wall, ev_ms, kern = time_gpu(p, cells, mode, a.steps, a.warmup)
+        res[mode] = dict(sim=sim, wall=wall, ev_ms=ev_ms, kernels=kern)
+        print(f"# {mode}: {a.steps} steps in {wall:.3f}s wall ({ev_ms/a.steps*1e3:.1f} us/step by events), "
+              f"kernels(us): " + ", ".join(f"{k}={v*1e3:.2f}" for k, v in kern.items()),
+              file=sys.stderr, flush=True)
+
+    # CPU baseline + parity on the same prefix
+    cpu = None
+    parity = {}
+    if not a.no_cpu:
+        o, n_cpu, el = cpu_baseline(p, cells, a.cpu_seconds)
+        cpu = {"value": C * n_cpu / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 NumPy oracle (sparse operators), first {n_cpu} steps of the same "
+                         f"workload, {el:.1f}s on this host, single thread"}
+        for mode in modes:
+            sim = gpu_session(p, cells, mode)
+            sim.step_n(n_cpu)
+            sim.pull()
+            parity[mode] = {
+                "steps": n_cpu,
+                "max_abs_dVm_V": float(np.abs(sim.vm - o.vm).max()),
+                "max_rel_dc_cell": float((np.abs(sim.cc_cells - o.cc_cells) / np.abs(o.cc_cells)).max()),
+                "max_rel_dc_env": float((np.abs(sim.cc_env - o.cc_env) / np.abs(o.cc_env)).max()),
+            }
+        print(f"# cpu oracle: {n_cpu} steps in {el:.2f}s; parity {parity}", file=sys.stderr, flush=True)
+
+    # headline: fastest path that holds |dVm| < 1 uV against the oracle prefix
+    ok = [m for m in modes if (not parity) or parity[m]["max_abs_dVm_V"] < 1e-6]
+    best = min(ok or modes, key=lambda m: res[m]["ev_ms"])
+    r = res[best]
+    ms_step = r["ev_ms"] / a.steps
+    value = C * a.steps / (r["ev_ms"] / 1e3)
+    ab = algorithmic_bytes(cells, n_ions, has_ca, best)
+    kern = r["kernels"]
+    dom = max(kern, key=kern.get)
+    achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
+    line = {
+        "metric": "cell-steps/s at |dVmem|<1uV vs FP64 oracle",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "f32-flux/f64-state",
+        "data": "synthetic",
+        "config": {"workload": f"single large cluster ({a.config}): {C} cells, {cells.n_mems} membranes, "
+                               f"{cells.X.shape[0]}x{cells.X.shape[1]} grid, ions {'/'.join(p.ion_names())}, "
+                               f"dt {p.dt:g} s; Nav/Kv channels not yet included",
+                   "cells": C, "membranes": cells.n_mems, "grid": list(cells.X.shape), "n_ions": n_ions,
+                   "flux_mode": best},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launch_us": kern[dom] * 1e3, "algorithmic_bytes": ab[dom],
+                     "step_algorithmic_bytes": ab["step"],
+                     "step_achieved_GBs": ab["step"] / (ms_step * 1e-3) / 1e9},
+        "cpu_baseline": cpu,
+        "parity_prefix": parity,
+        "paths": {m: {"cell_steps_per_s": C * a.steps / (res[m]["ev_ms"] / 1e3),
+                      "us_per_step": res[m]["ev_ms"] / a.steps * 1e3,
+                      "kernels_us": {k: v * 1e3 for k, v in res[m]["kernels"].items()}} for m in modes},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

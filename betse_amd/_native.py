@@ -1,0 +1,96 @@
+"""ctypes binding of ``libbetse_amd.so`` (declarations in ``include/betse_amd.h``).
+
+The library is built in-tree by ``betse_amd.build.build_library`` (``hipcc
+--offload-arch=gfx950``). Loading fails loudly when it is missing: there is no
+CPU fallback for the step.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+LIB_NAME = "libbetse_amd.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+BT_OK = 0
+BT_UNSTABLE = 1
+BT_EINVAL = -1
+BT_EHIP = -2
+BT_MAX_IONS = 8
+BT_FLUX_FP64 = 0
+BT_FLUX_FP32 = 1
+
+KERNEL_IDS = {"mem_cell": 0, "env_diffuse": 1, "env_exchange": 2, "gauss_rows": 3, "gauss_cols_grad": 4}
+
+_d8 = ctypes.c_double * BT_MAX_IONS
+_d5 = ctypes.c_double * 5
+_vp = ctypes.c_void_p
+
+
+class BtParams(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int32), ("n_mems", ctypes.c_int32), ("grid_ny", ctypes.c_int32),
+        ("grid_nx", ctypes.c_int32), ("n_ions", ctypes.c_int32),
+        ("i_na", ctypes.c_int32), ("i_k", ctypes.c_int32), ("i_ca", ctypes.c_int32),
+        ("flux_mode", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("z", _d8), ("d_gj", _d8), ("c_bound", _d8),
+        ("dt", ctypes.c_double), ("T", ctypes.c_double), ("T_gj", ctypes.c_double),
+        ("F", ctypes.c_double), ("R", ctypes.c_double), ("q", ctypes.c_double), ("kb", ctypes.c_double),
+        ("cm", ctypes.c_double), ("tm", ctypes.c_double),
+        ("gj_len", ctypes.c_double), ("gj_surface", ctypes.c_double),
+        ("alpha_NaK", ctypes.c_double), ("NaK_block", ctypes.c_double), ("deltaGATP", ctypes.c_double),
+        ("cATP", ctypes.c_double), ("cADP", ctypes.c_double), ("cPi", ctypes.c_double),
+        ("KmNK_Na", ctypes.c_double), ("KmNK_K", ctypes.c_double), ("KmNK_ATP", ctypes.c_double),
+        ("alpha_Ca", ctypes.c_double), ("KmCa_Ca", ctypes.c_double), ("KmCa_ATP", ctypes.c_double),
+        ("delta", ctypes.c_double), ("cell_height", ctypes.c_double),
+        ("mean_sa", ctypes.c_double),
+        ("ko_env", ctypes.c_double), ("eo", ctypes.c_double), ("er", ctypes.c_double),
+        ("cell_radius", ctypes.c_double), ("true_cell_size", ctypes.c_double),
+        ("gauss_w", _d5),
+    ]
+
+
+class BtBuffers(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in (
+        "mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "tile_ptr",
+        "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
+        "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env")] + [
+        ("n_tiles", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+# symbol -> (restype, argtypes); every function include/betse_amd.h declares
+SIGNATURES = {
+    "bt_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(BtParams), ctypes.POINTER(BtBuffers), _vp]),
+    "bt_destroy": (ctypes.c_int, [_vp]),
+    "bt_update_v": (ctypes.c_int, [_vp]),
+    "bt_step": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
+    "bt_sync": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "bt_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "bt_steps_done": (ctypes.c_int64, [_vp]),
+    "bt_last_error": (ctypes.c_char_p, [_vp]),
+    "bt_version": (ctypes.c_int, []),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load(path: Optional[str] = None) -> ctypes.CDLL:
+    """Load the HIP library (once). Raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: build the gfx950 library first "
+            "(python -c 'import __graft_entry__ as g; g.build()'). The GPU step has no CPU fallback.")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib

@@ -1,0 +1,257 @@
+"""Simulation parameters for the electrodiffusion step.
+
+This is the subset of BETSE's ``Parameters`` (``betse/science/parameters.py``)
+that the per-timestep electrodiffusion loop reads, with the same attribute
+names so host code and tests read like the reference:
+
+* fundamental constants and pump constants: ``parameters.py:1061-1122``;
+* time-step bookkeeping (``dt``, ``t_resample``): ``parameters.py:525-526`` and
+  ``phase/phasecls.py:315-330``;
+* ion profile and charge balance with the anion M: ``parameters.py:1296-1496``
+  and ``_balance_charge`` ``parameters.py:1569-1605``;
+* defaults of the YAML schema: ``betse/data/yaml/sim_config.yaml`` (line numbers
+  cited per attribute).
+
+YAML loading is not reproduced: presets are built in Python (``Parameters.preset``)
+for the workloads this package is measured on.
+"""
+
+from __future__ import annotations
+
+import copy
+import math
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+# Insertion order of BETSE's ions_dict (parameters.py:1296,1333,1373,1476):
+# Protein P comes *before* the charge-balancing anion M.
+ION_ORDER = ("Na", "K", "Cl", "Ca", "H", "P", "M")
+
+ION_CHARGE = {"Na": 1, "K": 1, "Cl": -1, "Ca": 2, "H": 1, "P": -1, "M": -1}   # parameters.py:1082-1088
+
+# Free diffusion constants, sim_config.yaml:1021-1026.
+FREE_DIFF = {"Na": 1.33e-9, "K": 1.96e-9, "Cl": 2.03e-9, "Ca": 1.0e-10, "H": 2.5e-9, "P": 0.0, "M": 1.0e-9}
+
+# Default tissue membrane diffusion constants, sim_config.yaml:226-231.
+DEFAULT_DM = {"Na": 2.0e-18, "K": 1.0e-18, "Cl": 1.0e-18, "Ca": 1.0e-18, "H": 1.0e-18, "P": 0.0, "M": 1.0e-18}
+
+ION_LONG_NAME = {"Na": "sodium", "K": "potassium", "Cl": "chloride", "Ca": "calcium",
+                 "H": "protons", "P": "proteins", "M": "anion"}
+
+
+def balance_charge(concentrations, zs):
+    """Concentration and valence of anion M that zeroes the net charge.
+
+    Follows ``_balance_charge`` (``parameters.py:1569-1605``): sum ``c*z`` in
+    order, return ``(|-q|, sign(-q))``.
+    """
+    q = 0
+    for conc, z in zip(concentrations, zs):
+        q = q + conc * z
+    to_zero = -q
+    bal_conc = abs(to_zero)
+    valence = np.sign(to_zero)
+    if bal_conc < 0:
+        raise ValueError("negative balancing concentration")
+    return bal_conc, valence
+
+
+@dataclass
+class ChannelSpec:
+    """Hodgkin-Huxley style voltage-gated channel on every membrane.
+
+    ``kind`` is the reference class name (``Nav1p2`` in ``channels/vg_na.py``,
+    ``Kv1p1`` in ``channels/vg_k.py``); ``max_dm`` the maximum membrane
+    diffusion constant of the channel (``chan.maxDm`` in
+    ``chemistry/networks.py:3150``).
+    """
+    kind: str
+    max_dm: float
+
+
+@dataclass
+class Parameters:
+    """Constants and settings consumed by the step.
+
+    Attribute names are the reference's (``p.F``, ``p.cm``, ``p.alpha_NaK`` ...).
+    """
+
+    # ---- fundamental constants (parameters.py:1106-1116) ----
+    F: float = 96485.0
+    R: float = 8.314
+    eo: float = 8.854e-12
+    kb: float = 1.3806e-23
+    q: float = 1.602e-19
+    mu: float = 1.275e-6
+    NAv: float = 6.022e23
+    er: float = 80.0
+    deltaGATP: float = -37000.0
+
+    # ---- metabolite levels and pump constants (parameters.py:1065-1080) ----
+    cATP: float = 1.5
+    cADP: float = 0.1
+    cPi: float = 0.1
+    KmNK_Na: float = 12.0
+    KmNK_K: float = 0.2
+    KmNK_ATP: float = 0.5
+    KmCa_Ca: float = 1.0e-3
+    KmCa_ATP: float = 0.5
+    alpha_NaK: float = 1.0e-7        # sim_config.yaml:1030
+    alpha_Ca: float = 5.0e-8         # sim_config.yaml:1032
+
+    # ---- membrane and geometry (parameters.py:1119-1133; sim_config.yaml:160-164) ----
+    scale_cell: float = 0.99
+    cm: float = 0.05                 # sim_config.yaml:1038
+    tm: float = 7.5e-9
+    cell_height: float = 10.0e-6
+    cell_space: float = 26.0e-9
+    cell_radius: float = 5.0e-6      # half the lattice spacing (d_cell = 2*cell_radius)
+    true_cell_size: float = 1.0e-5   # sim_config.yaml:1050
+    gj_surface: float = 5.0e-8       # sim_config.yaml:700
+    T: float = 310.0                 # sim_config.yaml:670
+
+    # ---- switches fixed by the electrodiffusion configurations ----
+    is_ecm: bool = True
+    cluster_open: bool = True        # parameters.py:924 (hard-coded)
+    cell_polarizability: float = 0.0 # sim_config.yaml:1040
+    sharpness: float = 1.0           # sim_config.yaml:1046 (no integrator smoothing)
+    smooth_cells: float = 2.0        # 1/'sharpness cell' (sim_config.yaml:1048)
+    fast_update_ecm: bool = False    # sim_config.yaml:1044
+    v_sensitive_gj: bool = False
+    channel_noise_level: float = 0.0 # sim_config.yaml:688
+    dynamic_noise: bool = False      # sim_config.yaml:691
+    D_adh: float = 1.0
+    D_tj: float = 1.0
+    Dtj_rel: Dict[str, float] = field(default_factory=lambda: {k: 1.0 for k in ION_ORDER})
+    gj_block: float = 1.0            # global_options['gj_block'] == 0 -> scalar 1 (sim.py:845-848)
+    NaKATP_block: float = 1.0        # sim.py:851-854
+
+    # ---- ion profile ----
+    ions_dict: Dict[str, int] = field(default_factory=dict)
+    cell_concs: Dict[str, float] = field(default_factory=dict)
+    env_concs: Dict[str, float] = field(default_factory=dict)
+    mem_perms: Dict[str, float] = field(default_factory=dict)
+    free_diff: Dict[str, float] = field(default_factory=lambda: dict(FREE_DIFF))
+    ion_charge: Dict[str, int] = field(default_factory=lambda: dict(ION_CHARGE))
+    cbnd: Optional[Dict[str, float]] = None
+
+    # ---- time settings (phase/phasecls.py:315-330) ----
+    init_time_step: float = 1.0e-3
+    init_time_total: float = 1.0
+    init_time_sampling: float = 0.1
+    sim_time_step: float = 1.0e-3
+    sim_time_total: float = 1.0
+    sim_time_sampling: float = 0.1
+    dt: float = 1.0e-3
+    resample: float = 0.1
+    t_resample: float = 100.0
+
+    # ---- cluster / grid description (consumed by cells.make_cluster) ----
+    cluster_radius: float = 100.0e-6
+    lattice_spacing: float = 10.0e-6
+    lattice_jitter: float = 0.1
+    seed: int = 0
+    grid_size: int = 50
+    world_scale: float = 1.25
+
+    # ---- optional voltage-gated channels (configs with Nav/Kv) ----
+    channels: tuple = ()
+
+    # ------------------------------------------------------------------
+    @property
+    def init_tsteps(self) -> int:
+        return int(self.init_time_total / self.init_time_step)    # parameters.py:525
+
+    @property
+    def sim_tsteps(self) -> int:
+        return int(self.sim_time_total / self.sim_time_step)      # parameters.py:526
+
+    @property
+    def d_cell(self) -> float:
+        return 2.0 * self.cell_radius
+
+    def set_phase_time(self, kind: str) -> None:
+        """Set ``dt``, ``resample`` and ``t_resample`` for a phase
+        (``SimPhase._init_kludge_time``, ``phase/phasecls.py:315-330``)."""
+        if kind == "init":
+            self.dt = self.init_time_step
+            self.resample = self.init_time_sampling
+        elif kind == "sim":
+            self.dt = self.sim_time_step
+            self.resample = self.sim_time_sampling
+        else:
+            raise ValueError(f"phase kind {kind!r} unsupported")
+        self.t_resample = self.resample / self.dt
+
+    def ion_names(self):
+        """Enabled ions in ``ions_dict`` insertion order (the ``sim.i<Ion>`` indices)."""
+        return [n for n in ION_ORDER if self.ions_dict.get(n, 0) == 1]
+
+    def copy(self) -> "Parameters":
+        return copy.deepcopy(self)
+
+    # ------------------------------------------------------------------
+    def set_ion_profile(self, cell: Dict[str, float], env: Dict[str, float],
+                        mem_perms: Optional[Dict[str, float]] = None) -> None:
+        """Customised ion profile with M set by charge balance.
+
+        Same construction as the CUSTOM profile (``parameters.py:1433-1480``):
+        concentrations of every listed ion other than M are given, and M is
+        ``_balance_charge`` of the rest, in ``ION_ORDER``.
+        """
+        names = [n for n in ION_ORDER if n != "M" and (n in cell)]
+        zs = [ION_CHARGE[n] for n in names]
+        cM_env, zM_env = balance_charge([env[n] for n in names], zs)
+        cM_cell, zM_cell = balance_charge([cell[n] for n in names], zs)
+        if zM_env != -1 or zM_cell != -1:
+            raise ValueError("ion profile needs a negative balancing anion M")
+        self.ions_dict = {n: (1 if (n in cell or n == "M") else 0) for n in ION_ORDER}
+        self.cell_concs = {n: float(cell[n]) for n in names}
+        self.cell_concs["M"] = float(cM_cell)
+        self.env_concs = {n: float(env[n]) for n in names}
+        self.env_concs["M"] = float(cM_env)
+        perms = dict(DEFAULT_DM)
+        if mem_perms:
+            perms.update(mem_perms)
+        self.mem_perms = {n: float(perms[n]) for n in self.ion_names()}
+
+    # ------------------------------------------------------------------
+    @classmethod
+    def preset(cls, name: str) -> "Parameters":
+        """Named workloads.
+
+        ``tiny``   few dozen cells on a 16x16 grid (fast unit tests);
+        ``small``  ~350 cells, 100 um disc, 50x50 grid, Na/K/Cl/P/M, dt 1e-3;
+        ``large``  ~23k cells, 800 um disc, 256x256 grid, +Ca, dt 1e-4.
+        """
+        p = cls()
+        cell = {"Na": 8.0, "K": 125.0, "Cl": 10.0, "P": 100.0}
+        env = {"Na": 145.0, "K": 5.0, "Cl": 105.0, "P": 10.0}
+        perms = {"Na": 1.0e-18, "K": 15.0e-18, "Cl": 2.0e-18}
+        if name == "tiny":
+            p.cluster_radius = 30.0e-6
+            p.grid_size = 16
+            p.init_time_step = p.sim_time_step = 1.0e-3
+            p.init_time_total = p.sim_time_total = 0.05
+            p.init_time_sampling = p.sim_time_sampling = 0.01
+        elif name == "small":
+            p.cluster_radius = 100.0e-6
+            p.grid_size = 50
+            p.init_time_step = p.sim_time_step = 1.0e-3
+            p.init_time_total = p.sim_time_total = 1.0
+            p.init_time_sampling = p.sim_time_sampling = 0.1
+        elif name in ("large", "large_plain"):
+            p.cluster_radius = 800.0e-6
+            p.grid_size = 256
+            cell = dict(cell, Ca=1.0e-4)
+            env = dict(env, Ca=1.0)
+            p.init_time_step = p.sim_time_step = 1.0e-4
+            p.init_time_total = p.sim_time_total = 1.0
+            p.init_time_sampling = p.sim_time_sampling = 0.01
+        else:
+            raise ValueError(f"unknown preset {name!r}")
+        p.set_ion_profile(cell, env, perms)
+        p.set_phase_time("init")
+        return p

@@ -1,0 +1,537 @@
+"""GPU ``Simulator``: the drop-in for ``Simulator._run_sim_core_loop``.
+
+Mirrors the reference's ``betse/science/sim.py`` interface for the full-solver
+electrodiffusion path:
+
+* ``init_core(phase)``      -- ``sim.py:452-754``: state arrays from the ion profile;
+* ``init_dynamics(phase)``  -- ``sim.py:758-1012``: per-phase set-up (tissue Dm,
+  D_env, ko_env, noise, blocks);
+* ``run_sim_core(phase)``   -- ``sim.py:1016-1128``: init_dynamics, storage reset,
+  ``update_V``, time vector, the loop, instability handling;
+* ``_run_sim_core_loop(phase, time_steps, time_steps_sampled, anim_cells)`` --
+  ``sim.py:1132-1390``: every step runs on the GPU through the C ABI of
+  ``libbetse_amd.so``; the host only wakes at sampled steps to run
+  ``write2storage`` (``sim.py:1789-1884``) and the progress callbacks.
+
+State lives in device tensors (PyTorch holds the memory; the kernels are the
+library's own HIP code). Host NumPy mirrors (``cc_cells``, ``cc_env``, ``vm`` ...)
+are refreshed at sampled steps and at the end of a phase.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _native
+from .cells import Cells
+from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableException
+from .parameters import Parameters
+
+TILE_M = 256          # membranes per k_mem_cell block (matches the HIP source)
+
+
+class SimPhaseKind(enum.Enum):
+    """``betse/science/enum/enumphase.py:17``."""
+    SEED = 1
+    INIT = 2
+    SIM = 3
+
+
+class SolverType(enum.Enum):
+    """``betse/science/enum/enumconf.py:120-125``; only FULL has a GPU loop."""
+    FULL = 1
+    FAST = 2
+
+
+class SimCallbacks:
+    """Progress callbacks (``phase/phasecallbacks.py:20-110``)."""
+
+    def __init__(self):
+        self.progress_max = 0
+        self.progress = 0
+
+    def progress_ranged(self, progress_max: int, progress_min: int = 0) -> None:
+        self.progress_max = progress_max
+        self.progress = progress_min
+
+    def progressed_next(self) -> None:
+        self.progress += 1
+
+
+class SimPhase:
+    """A seed/init/sim phase (``phase/phasecls.py:46-204``).
+
+    Constructing an INIT or SIM phase sets ``p.dt``, ``p.resample`` and
+    ``p.t_resample`` (``_init_kludge_time``, ``phasecls.py:315-330``).
+    """
+
+    def __init__(self, kind: SimPhaseKind, p: Parameters, cells: Optional[Cells] = None,
+                 sim: Optional["Simulator"] = None, callbacks: Optional[SimCallbacks] = None,
+                 save_path: Optional[str] = None):
+        self.kind = kind
+        self.p = p
+        self.cells = cells
+        self.sim = sim
+        self.callbacks = callbacks or SimCallbacks()
+        self.save_path = save_path
+        if kind is SimPhaseKind.INIT:
+            p.set_phase_time("init")
+        elif kind is SimPhaseKind.SIM:
+            p.set_phase_time("sim")
+
+    def die_unless_kind_init_or_sim(self) -> None:
+        if self.kind not in (SimPhaseKind.INIT, SimPhaseKind.SIM):
+            raise BetseSimException(f"phase {self.kind} is neither init nor sim")
+
+
+def gaussian_taps(sigma: float = 1.0, truncate: float = 4.0) -> np.ndarray:
+    """Taps of ``scipy.ndimage.gaussian_filter1d`` (centre first), built with
+    SciPy's published formula ``exp(-x^2/(2 sigma^2))`` normalised to sum 1."""
+    radius = int(truncate * float(sigma) + 0.5)
+    sigma2 = sigma * sigma
+    x = np.arange(-radius, radius + 1)
+    phi_x = np.exp(-0.5 / sigma2 * x ** 2)
+    phi_x = phi_x / phi_x.sum()
+    return phi_x[radius:]
+
+
+def phase_time_steps(n_steps: int, dt: float, t_resample: float):
+    """``time_steps`` and ``time_steps_sampled`` (``sim.py:2455-2462``, Q16)."""
+    time_steps = np.linspace(0, n_steps * dt, n_steps)
+    sampled = set()
+    i = 0
+    while i < len(time_steps) - t_resample:
+        i += t_resample
+        i = int(i)
+        sampled.add(time_steps[i])
+    return time_steps, sampled
+
+
+def make_tiles(mem_ptr: np.ndarray, tile_m: int = TILE_M) -> np.ndarray:
+    """Greedy packing of consecutive whole cells into tiles of <= tile_m membranes."""
+    n = len(mem_ptr) - 1
+    counts = np.diff(mem_ptr)
+    if counts.max(initial=0) > tile_m:
+        raise BetseSimException(f"a cell has more than {tile_m} membranes")
+    tiles = [0]
+    acc = 0
+    for c in range(n):
+        if acc + counts[c] > tile_m:
+            tiles.append(c)
+            acc = 0
+        acc += counts[c]
+    tiles.append(n)
+    return np.asarray(tiles, dtype=np.int32)
+
+
+class Simulator:
+    """GPU-backed BETSE ``Simulator`` for the full electrodiffusion solver.
+
+    ``flux_mode`` is ``"fp64"`` (every operation in FP64, the parity path) or
+    ``"fp32"`` (flux laws in FP32; concentrations, charge sums and the
+    charge-to-voltage step in FP64).
+    """
+
+    def __init__(self, p: Optional[Parameters] = None, *, device: Optional[str] = None,
+                 flux_mode: str = "fp64", use_graph: bool = True):
+        import torch
+        if not torch.cuda.is_available():
+            raise BetseSimException("no GPU visible: the GPU simulator has no CPU fallback")
+        if flux_mode not in ("fp64", "fp32"):
+            raise ValueError(flux_mode)
+        self.p = p
+        self.flux_mode = flux_mode
+        self.use_graph = use_graph
+        self.torch = torch
+        self.device = torch.device(device or "cuda")
+        self.stream = torch.cuda.Stream(device=self.device)     # default priority
+        self._lib = _native.load()
+        self._ctx = ctypes.c_void_p()
+        self._geom_key = None
+        self._dev: Dict[str, "torch.Tensor"] = {}
+        self.exception_instability = None
+
+    # ------------------------------------------------------------------
+    # set-up mirrored from the reference
+    # ------------------------------------------------------------------
+    def init_core(self, phase: SimPhase) -> None:
+        p, cells = phase.p, phase.cells
+        self.p = p
+        self.cells = cells
+        self.mdl = cells.n_mems
+        self.cdl = cells.n_cells
+        self.edl = cells.X.size
+        self.T = p.T
+        self.names = p.ion_names()
+        self.ionlabel = {}
+        cc_cells, cc_env, zs, Dm, D_free, D_gj, D_env, cbound, Dtj = [], [], [], [], [], [], [], [], []
+        for i, name in enumerate(self.names):
+            setattr(self, "i" + name, i)
+            self.ionlabel[i] = name
+            cc_cells.append(np.full(self.cdl, p.cell_concs[name]))
+            cc_env.append(np.full(self.edl, p.env_concs[name]))
+            zs.append(p.ion_charge[name])
+            Dm.append(np.full(self.mdl, p.mem_perms[name]))
+            D_gj.append(np.full(self.mdl, p.free_diff[name]))
+            D_env.append(np.full(self.edl, p.free_diff[name]))
+            D_free.append(p.free_diff[name])
+            cbound.append(p.env_concs[name])
+            Dtj.append(p.Dtj_rel[name])
+        self.movingIons = list(range(len(self.names)))
+        self.cc_cells = np.asarray(cc_cells)
+        self.cc_env = np.asarray(cc_env)
+        self.zs = np.asarray(zs)
+        self.Dm_cells = np.asarray(Dm)
+        self.D_free = np.asarray(D_free)
+        self.D_gj = np.asarray(D_gj)
+        self.D_env = np.asarray(D_env)
+        self.Dtj_rel = Dtj
+        self.c_env_bound = cbound
+        self.bound_V = {"T": 0, "B": 0, "L": 0, "R": 0}
+        self.initDenv(cells, p)
+        self.gj_block = 1
+        self.gjopen = np.ones(self.mdl) * cells.gj_default_weights
+        # cc_at_mem = cc_cells[mem_to_cells] (sim.py:744-745) is constant over a
+        # cell's membranes on this path, so the device keeps it per cell.
+        self.c_lag = self.cc_cells.copy()
+        self.vm_cell = np.zeros(self.cdl)
+        self.E_env_x = np.zeros(cells.X.shape)
+        self.E_env_y = np.zeros(cells.X.shape)
+        self.v_env = np.zeros(self.edl)
+        self.Phi_b = np.zeros(self.edl)
+
+    def initDenv(self, cells: Cells, p: Parameters) -> None:
+        """``sim.py:2347-2400`` (ECM branch)."""
+        for i in range(len(self.D_env)):
+            Denv_o = np.ones(self.edl) * self.D_free[i]
+            Denv_o[cells.envInds_inClust] = self.D_free[i] * p.D_adh
+            Denv_o[cells.all_bound_mem_inds] = self.D_free[i] * p.D_tj * self.Dtj_rel[i]
+            Denv_o[cells.interior_bound_mem_inds] = self.D_free[i] * p.D_tj * self.Dtj_rel[i]
+            Denv_o[cells.ecm_inds_bound_cell] = self.D_free[i] * p.D_tj * self.Dtj_rel[i]
+            self.D_env[i] = Denv_o * 1.0
+        self.TJ_modulator = np.ones(self.D_env.shape)
+
+    def init_dynamics(self, phase: SimPhase) -> None:
+        """``sim.py:758-1012`` for the electrodiffusion path."""
+        p, cells = phase.p, phase.cells
+        if p.v_sensitive_gj:
+            raise BetseSimException("voltage-sensitive gap junctions are not supported by the GPU step")
+        if p.sharpness < 1.0:
+            raise BetseSimException("env smoothing (sharpness < 1) is not supported by the GPU step")
+        if p.cell_polarizability != 0.0 or not p.is_ecm or p.fast_update_ecm:
+            raise BetseSimException("the GPU step covers the ECM, non-polarizable, slow-update path")
+        nfrac = p.smooth_cells
+        nm = cells.num_mems[cells.mem_to_cells]
+        self.smooth_weight_mem = (nfrac * nm - 1) / (nfrac * nm)
+        self.smooth_weight_o = 1 / (nfrac * nm)
+        self.initDenv(cells, p)
+        if p.cbnd is not None:
+            for i, name in enumerate(self.names):
+                self.c_env_bound[i] = p.cbnd[name]
+        # tissue profiles (tishandler.py:363-443): default tissue, then extras
+        for i, name in enumerate(self.names):
+            self.Dm_cells[i][:] = p.mem_perms[name]
+        for cell_inds, dms in getattr(p, "tissue_profiles", ()):
+            mems = np.concatenate([cells.cell_to_mems[c] for c in cell_inds]).astype(np.int64)
+            for name, val in dms.items():
+                if name in self.names:
+                    self.Dm_cells[self.names.index(name)][mems] = val
+        self.D_env_base = np.copy(self.D_env)
+        self.Dm_base = np.copy(self.Dm_cells)
+        self.Dm_scheduled = np.zeros_like(self.Dm_cells)
+        self.channel_noise_factor = np.random.random(self.mdl)
+        if "K" in self.names:
+            iK = self.iK
+            self.Dm_cells[iK] = (p.channel_noise_level * self.channel_noise_factor + 1) * self.Dm_cells[iK]
+        self.gj_block = 1
+        self.NaKATP_block = p.NaKATP_block
+        self.CaATP_block = np.ones(self.mdl)
+        self.rho_pump = 1
+        self.rho_channel = 1
+        self.ko_env = (np.sqrt(np.dot((p.NAv * (p.q ** 2) * self.zs ** 2) / (p.er * p.eo * p.kb * p.T),
+                                      self.cc_env))).mean()
+        self.ko_cell = (np.sqrt(np.dot((p.NAv * (p.q ** 2) * self.zs ** 2) / (p.er * p.eo * p.kb * p.T),
+                                       self.cc_cells))).mean()
+
+    def clear_storage(self, cells=None, p=None) -> None:
+        """Time-series lists (``sim.py:1680-1786``, the fields this path fills)."""
+        self.time: List[float] = []
+        self.cc_time: List[np.ndarray] = []
+        self.cc_env_time: List[np.ndarray] = []
+        self.dd_time: List[np.ndarray] = []
+        self.vm_time: List[np.ndarray] = []
+        self.vm_ave_time: List[np.ndarray] = []
+        self.rho_cells_time: List[np.ndarray] = []
+        self.venv_time: List[np.ndarray] = []
+        self.efield_ecm_x_time: List[np.ndarray] = []
+        self.efield_ecm_y_time: List[np.ndarray] = []
+        self.gjopen_time: List[np.ndarray] = []
+
+    # ------------------------------------------------------------------
+    # host views of the device state
+    # ------------------------------------------------------------------
+    @property
+    def vm(self) -> np.ndarray:
+        """Vmem per membrane: ``(1/cm)*rho_surf[mem_to_cells] - Phi_b[map_mem2ecm]``."""
+        return self.vm_cell[self.cells.mem_to_cells] - self.Phi_b[self.cells.map_mem2ecm]
+
+    @property
+    def cc_at_mem(self) -> np.ndarray:
+        return self.c_lag[:, self.cells.mem_to_cells]
+
+    @property
+    def vm_ave(self) -> np.ndarray:
+        cells = self.cells
+        return np.add.reduceat(self.vm, cells.mem_ptr[:-1]) / cells.num_mems
+
+    @property
+    def rho_cells(self) -> np.ndarray:
+        return np.dot(self.zs * self.p.F, self.cc_cells)
+
+    # ------------------------------------------------------------------
+    # device set-up
+    # ------------------------------------------------------------------
+    def _t(self, a, dtype):
+        torch = self.torch
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).to(self.device, non_blocking=False)
+
+    def _upload_geometry(self, cells: Cells) -> None:
+        if self._geom_key is cells:
+            return
+        torch = self.torch
+        i32 = torch.int32
+        f64 = torch.float64
+        G2 = cells.X.size
+        counts = np.diff(cells.sq_ptr)
+        d = self._dev
+        d["mem_ptr"] = self._t(cells.mem_ptr, i32)
+        d["mem_cell"] = self._t(cells.mem_to_cells, i32)
+        d["nn_mem"] = self._t(cells.nn_i, i32)
+        d["nn_cell"] = self._t(cells.mem_to_cells[cells.nn_i], i32)
+        d["mem_sq"] = self._t(cells.map_mem2ecm, i32)
+        d["sq_ptr"] = self._t(cells.sq_ptr, i32)
+        d["sq_mems"] = self._t(cells.sq_mems, i32)
+        tiles = make_tiles(cells.mem_ptr)
+        d["tile_ptr"] = self._t(tiles, i32)
+        self.n_tiles = len(tiles) - 1
+        d["mem_sa"] = self._t(cells.mem_sa, f64)
+        d["cell_vol"] = self._t(cells.cell_vol, f64)
+        d["diviterm"] = self._t(cells.diviterm, f64)
+        d["sq_has_mem"] = self._t((counts > 0).astype(np.uint8), torch.uint8)
+        if G2 != len(counts):
+            raise BetseSimException("grid-side CSR does not match the grid")
+        self._geom_key = cells
+
+    def _device_setup(self, phase: SimPhase) -> None:
+        p, cells = phase.p, phase.cells
+        torch = self.torch
+        f64 = torch.float64
+        self._destroy_ctx()
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            self._upload_geometry(cells)
+            d = self._dev
+            Dm = self.Dm_cells if phase.kind is SimPhaseKind.INIT else (self.Dm_scheduled + self.Dm_base)
+            d["Dm"] = self._t(Dm, f64)
+            d["D_env"] = self._t(self.D_env * self.TJ_modulator, f64)
+            d["gjopen"] = self._t(self.gj_block * np.ones(self.mdl) * cells.gj_default_weights, f64)
+            d["c_cell"] = self._t(self.cc_cells, f64)
+            d["c_lag"] = self._t(self.c_lag, f64)
+            d["vm_cell"] = self._t(self.vm_cell, f64)
+            d["c_env"] = self._t(self.cc_env, f64)
+            d["E_x"] = self._t(self.E_env_x.ravel(), f64)
+            d["E_y"] = self._t(self.E_env_y.ravel(), f64)
+            d["v_env"] = self._t(self.v_env, f64)
+            self.stream.synchronize()
+
+        prm = _native.BtParams()
+        prm.n_cells, prm.n_mems = self.cdl, self.mdl
+        prm.grid_ny, prm.grid_nx = cells.X.shape
+        prm.n_ions = len(self.names)
+        prm.i_na = self.names.index("Na") if ("Na" in self.names and "K" in self.names and p.alpha_NaK > 0.0) else -1
+        prm.i_k = self.names.index("K") if prm.i_na >= 0 else -1
+        prm.i_ca = self.names.index("Ca") if ("Ca" in self.names and p.alpha_Ca > 0.0) else -1
+        prm.flux_mode = _native.BT_FLUX_FP32 if self.flux_mode == "fp32" else _native.BT_FLUX_FP64
+        for i in range(prm.n_ions):
+            prm.z[i] = float(self.zs[i])
+            if not np.all(self.D_gj[i] == self.D_gj[i][0]):
+                raise BetseSimException("the GPU step needs a uniform D_gj per ion")
+            prm.d_gj[i] = float(self.D_gj[i][0])
+            prm.c_bound[i] = float(self.c_env_bound[i])
+        prm.dt, prm.T, prm.T_gj = p.dt, self.T, p.T
+        prm.F, prm.R, prm.q, prm.kb, prm.cm, prm.tm = p.F, p.R, p.q, p.kb, p.cm, p.tm
+        prm.gj_len, prm.gj_surface = cells.gj_len, p.gj_surface
+        prm.alpha_NaK, prm.NaK_block, prm.deltaGATP = p.alpha_NaK, float(self.NaKATP_block), p.deltaGATP
+        prm.cATP, prm.cADP, prm.cPi = p.cATP, p.cADP, p.cPi
+        prm.KmNK_Na, prm.KmNK_K, prm.KmNK_ATP = p.KmNK_Na, p.KmNK_K, p.KmNK_ATP
+        prm.alpha_Ca, prm.KmCa_Ca, prm.KmCa_ATP = p.alpha_Ca, p.KmCa_Ca, p.KmCa_ATP
+        prm.delta, prm.cell_height = cells.delta, p.cell_height
+        prm.mean_sa = float(cells.memSa_per_envSquare[cells.map_mem2ecm].mean())
+        prm.ko_env, prm.eo, prm.er = float(self.ko_env), p.eo, p.er
+        prm.cell_radius, prm.true_cell_size = p.cell_radius, p.true_cell_size
+        for j, w in enumerate(gaussian_taps(1.0)):
+            prm.gauss_w[j] = float(w)
+
+        buf = _native.BtBuffers()
+        for name in ("mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "tile_ptr",
+                     "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
+                     "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env"):
+            setattr(buf, name, d[name].data_ptr())
+        buf.n_tiles = self.n_tiles
+        self._prm, self._buf = prm, buf
+        with torch.cuda.device(self.device):
+            rc = self._lib.bt_create(ctypes.byref(self._ctx), ctypes.byref(prm), ctypes.byref(buf),
+                                     ctypes.c_void_p(self.stream.cuda_stream))
+        if rc != _native.BT_OK:
+            raise BetseNativeError(f"bt_create failed ({rc}): {self._lib.bt_last_error(None).decode()}")
+
+    def _destroy_ctx(self) -> None:
+        if self._ctx:
+            self._lib.bt_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self._destroy_ctx()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc != _native.BT_OK:
+            raise BetseNativeError(f"libbetse_amd error {rc}: {self._lib.bt_last_error(self._ctx).decode()}")
+
+    def _pull_state(self) -> None:
+        d = self._dev
+        self.stream.synchronize()
+        self.cc_cells = d["c_cell"].cpu().numpy().copy()
+        self.c_lag = d["c_lag"].cpu().numpy().copy()
+        self.vm_cell = d["vm_cell"].cpu().numpy().copy()
+        self.cc_env = d["c_env"].cpu().numpy().copy()
+        shp = self.cells.X.shape
+        self.E_env_x = d["E_x"].cpu().numpy().reshape(shp).copy()
+        self.E_env_y = d["E_y"].cpu().numpy().reshape(shp).copy()
+        self.v_env = d["v_env"].cpu().numpy().copy()
+
+    def _sync(self, phase_step0: int = 0) -> None:
+        bad = ctypes.c_int64(-1)
+        rc = self._lib.bt_sync(self._ctx, ctypes.byref(bad))
+        if rc == _native.BT_UNSTABLE:
+            self._pull_state()
+            step = None if bad.value < 0 else int(bad.value) - phase_step0
+            raise BetseSimUnstableException(
+                "Your simulation has become unstable. Please try a smaller time step,"
+                "reduce gap junction radius, and/or reduce pump rate coefficients.", step=step)
+        self._check(rc)
+
+    # ------------------------------------------------------------------
+    # update_V and the loop
+    # ------------------------------------------------------------------
+    def update_V(self, cells=None, p=None) -> None:
+        """Voltage part of ``update_V`` (``sim.py:2007-2038``) on the device."""
+        self._check(self._lib.bt_update_v(self._ctx))
+
+    def write2storage(self, t, cells=None, p=None) -> None:
+        """``sim.py:1789-1884``: the state fields of this path."""
+        self.time.append(t)
+        self.cc_time.append(np.copy(self.cc_cells))
+        self.cc_env_time.append(np.copy(self.cc_env))
+        self.dd_time.append(np.copy(self.Dm_cells))
+        self.vm_time.append(self.vm * 1)
+        self.vm_ave_time.append(self.vm_ave)
+        self.rho_cells_time.append(self.rho_cells * 1)
+        self.venv_time.append(self.v_env * 1)
+        self.efield_ecm_x_time.append(self.E_env_x * 1)
+        self.efield_ecm_y_time.append(self.E_env_y * 1)
+        self.gjopen_time.append(self.gjopen * 1)
+
+    def run_sim_core(self, phase: SimPhase) -> None:
+        """``sim.py:1016-1128``."""
+        phase.die_unless_kind_init_or_sim()
+        self.cells = phase.cells
+        self.init_dynamics(phase)
+        self.clear_storage(phase.cells, phase.p)
+        self._device_setup(phase)
+        self.update_V(phase.cells, phase.p)
+        n = phase.p.init_tsteps if phase.kind is SimPhaseKind.INIT else phase.p.sim_tsteps
+        time_steps, time_steps_sampled = phase_time_steps(n, phase.p.dt, phase.p.t_resample)
+        phase.callbacks.progress_ranged(progress_max=len(time_steps_sampled))
+        self.exception_instability = None
+        if getattr(phase.p, "solver_type", SolverType.FULL) is not SolverType.FULL:
+            raise BetseSimException("only the full solver runs on the GPU")
+        try:
+            self._run_sim_core_loop(phase=phase, time_steps=time_steps,
+                                    time_steps_sampled=time_steps_sampled, anim_cells=None)
+        except BetseSimUnstableException as exception:
+            self.exception_instability = exception
+        if phase.save_path:
+            self.save(phase.save_path)
+        if self.exception_instability is not None:
+            raise self.exception_instability
+
+    def _run_sim_core_loop(self, phase: SimPhase, time_steps: np.ndarray, time_steps_sampled: set,
+                           anim_cells=None) -> None:
+        """Run ``len(time_steps)`` steps on the GPU; wake at sampled steps."""
+        if anim_cells is not None:
+            raise BetseSimException("in-loop animation is not supported by the GPU step")
+        step0 = int(self._lib.bt_steps_done(self._ctx))
+        sampled_idx = [i for i, t in enumerate(time_steps) if t in time_steps_sampled]
+        pos = 0
+        for s in sampled_idx:
+            self._check(self._lib.bt_step(self._ctx, s + 1 - pos, 1 if self.use_graph else 0))
+            pos = s + 1
+            self._sync(step0)
+            self._pull_state()
+            phase.callbacks.progressed_next()
+            self.write2storage(time_steps[s], phase.cells, phase.p)
+        if pos < len(time_steps):
+            self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, 1 if self.use_graph else 0))
+        self._sync(step0)
+        self._pull_state()
+
+    # ------------------------------------------------------------------
+    # raw stepping (bench / tests): no sampling, state stays on the device
+    # ------------------------------------------------------------------
+    def prepare(self, phase: SimPhase) -> None:
+        """init_dynamics + device set-up + initial update_V, without running steps."""
+        self.cells = phase.cells
+        self.init_dynamics(phase)
+        self.clear_storage()
+        self._device_setup(phase)
+        self.update_V()
+
+    def step_n(self, n: int, sync: bool = True) -> None:
+        self._check(self._lib.bt_step(self._ctx, int(n), 1 if self.use_graph else 0))
+        if sync:
+            self._sync()
+
+    def pull(self) -> None:
+        self._sync()
+        self._pull_state()
+
+    def time_kernel(self, name: str, n_launches: int = 50) -> float:
+        """Average device ms of one launch of a step kernel (HIP events)."""
+        ms = ctypes.c_double(0.0)
+        self._check(self._lib.bt_time_kernel(self._ctx, _native.KERNEL_IDS[name], int(n_launches),
+                                             ctypes.byref(ms)))
+        return ms.value
+
+    # ------------------------------------------------------------------
+    def save(self, path: str) -> None:
+        """Phase-end state (stand-in for ``_pickle_phase``, ``sim.py:1888-1950``)."""
+        np.savez_compressed(path, cc_cells=self.cc_cells, c_lag=self.c_lag, vm_cell=self.vm_cell,
+                            cc_env=self.cc_env, E_env_x=self.E_env_x, E_env_y=self.E_env_y,
+                            v_env=self.v_env, names=np.asarray(self.names))
+
+    def load(self, path: str) -> None:
+        with np.load(path, allow_pickle=False) as z:
+            self.cc_cells = z["cc_cells"]
+            self.c_lag = z["c_lag"]
+            self.vm_cell = z["vm_cell"]
+            self.cc_env = z["cc_env"]
+            self.E_env_x = z["E_env_x"]
+            self.E_env_y = z["E_env_y"]
+            self.v_env = z["v_env"]

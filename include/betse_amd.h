@@ -1,0 +1,141 @@
+/*
+ * betse_amd.h -- C ABI of the MI355X (gfx950) electrodiffusion step.
+ *
+ * The reference has no FFI: its seam is the solver dispatch in
+ * Simulator.run_sim_core (betse/science/sim.py:1064-1102), which calls
+ * Simulator._run_sim_core_loop(phase, time_steps, time_steps_sampled,
+ * anim_cells) (sim.py:1132-1138). Each entry point below replaces a part of
+ * that loop body; the Python host (betse_amd/sim.py) binds them with ctypes.
+ *
+ * All pointers are device pointers owned by the caller (PyTorch tensors in the
+ * Python host), except the workspace the library allocates in bt_create and
+ * frees in bt_destroy. Every call is asynchronous on the stream given to
+ * bt_create; bt_sync waits for it. Return codes: BT_OK (0), BT_UNSTABLE (>0,
+ * a NaN was produced: BetseSimUnstableException in the reference,
+ * sim_toolbox.py:332-344 and 475-503), BT_EINVAL / BT_EHIP (<0, see
+ * bt_last_error).
+ */
+#ifndef BETSE_AMD_H
+#define BETSE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BT_OK        0
+#define BT_UNSTABLE  1
+#define BT_EINVAL   -1
+#define BT_EHIP     -2
+
+#define BT_MAX_IONS  8
+
+/* Flux arithmetic of the membrane and environment flux laws.
+ * Cell/environment state, charge sums and the charge->voltage step are FP64
+ * in both modes. */
+#define BT_FLUX_FP64 0
+#define BT_FLUX_FP32 1
+
+typedef struct bt_ctx bt_ctx;
+
+/* Sizes, constants and switches; the names follow betse/science/parameters.py. */
+typedef struct {
+    int32_t n_cells, n_mems, grid_ny, grid_nx, n_ions;
+    int32_t i_na, i_k, i_ca;            /* ion slots, -1 if the ion is absent */
+    int32_t flux_mode;                  /* BT_FLUX_FP64 / BT_FLUX_FP32 */
+    int32_t reserved;
+    double z[BT_MAX_IONS];              /* valence (parameters.py:1082-1088) */
+    double d_gj[BT_MAX_IONS];           /* D_gj per ion (sim.py:570-574) */
+    double c_bound[BT_MAX_IONS];        /* c_env_bound (sim.py:608, 800-806) */
+    double dt, T, T_gj;                 /* sim.T for membrane/pump/env, p.T for GJ (Q7) */
+    double F, R, q, kb, cm, tm;
+    double gj_len, gj_surface;
+    double alpha_NaK, NaK_block, deltaGATP, cATP, cADP, cPi;
+    double KmNK_Na, KmNK_K, KmNK_ATP;
+    double alpha_Ca, KmCa_Ca, KmCa_ATP;
+    double delta, cell_height;
+    double mean_sa;                     /* memSa_per_envSquare[map_mem2ecm].mean() (ion_current.py:93-95) */
+    double ko_env, eo, er;              /* ko_env is frozen per phase (sim.py:974, Q13) */
+    double cell_radius, true_cell_size;
+    double gauss_w[5];                  /* scipy gaussian_filter1d taps, centre first */
+} bt_params;
+
+/* Device buffers. Layouts: per-ion fields are [n_ions][n], grid fields are
+ * row-major [grid_ny][grid_nx] (rows = y = axis 0), indices are int32. */
+typedef struct {
+    /* geometry (read only) */
+    const int32_t* mem_ptr;     /* [n_cells+1] CSR of each cell's membranes       */
+    const int32_t* mem_cell;    /* [n_mems] mem_to_cells                          */
+    const int32_t* nn_mem;      /* [n_mems] nn_i (self for boundary membranes)    */
+    const int32_t* nn_cell;     /* [n_mems] mem_to_cells[nn_i]                    */
+    const int32_t* mem_sq;      /* [n_mems] map_mem2ecm                           */
+    const int32_t* sq_ptr;      /* [ny*nx+1] grid-side CSR of M_divmap_mem2ecm    */
+    const int32_t* sq_mems;     /* [n_mems] membranes sorted by square            */
+    const int32_t* tile_ptr;    /* [n_tiles+1] first cell of each 256-membrane tile */
+    const double* mem_sa;       /* [n_mems]                                       */
+    const double* gjopen;       /* [n_mems] gj_block*gj_default_weights           */
+    const double* cell_vol;     /* [n_cells]                                      */
+    const double* diviterm;     /* [n_cells]                                      */
+    const double* Dm;           /* [n_ions][n_mems] Dm_cells                      */
+    const double* D_env;        /* [n_ions][ny*nx] D_env*TJ_modulator             */
+    const uint8_t* sq_has_mem;  /* [ny*nx] 1 where some membrane maps            */
+    /* state (read/write, in place between calls) */
+    double* c_cell;             /* [n_ions][n_cells] cc_cells                     */
+    double* c_lag;              /* [n_ions][n_cells] cc_at_mem per cell (Q1)      */
+    double* vm_cell;            /* [n_cells] Vmem of each cell (Phi_b == 0)       */
+    double* c_env;              /* [n_ions][ny*nx] cc_env                         */
+    double* E_x;                /* [ny*nx] E_env_x                                */
+    double* E_y;                /* [ny*nx] E_env_y                                */
+    double* v_env;              /* [ny*nx] v_env                                  */
+    int32_t n_tiles;
+    int32_t reserved;
+} bt_buffers;
+
+/* Create a context: validates sizes, copies the parameter block, allocates the
+ * workspace (second cell-state buffer, post-diffusion env buffer, per-membrane
+ * exchange fluxes, Gaussian scratch, status word) on the current device.
+ * `stream` is a hipStream_t (may be NULL for the default stream; graphs need a
+ * non-default stream). Replaces the state set-up of init_core/init_dynamics
+ * (sim.py:452-1012) on the device side. */
+int bt_create(bt_ctx** out, const bt_params* params, const bt_buffers* buffers, void* stream);
+
+/* Release the workspace, graphs and the context. */
+int bt_destroy(bt_ctx* ctx);
+
+/* update_V's voltage part for the current state (sim.py:2007-2083 with
+ * ion_current.py:13-114): rho_cells -> vm_cell, rho_env -> Gaussian -> v_env
+ * -> E_env. Called once before a phase's loop (sim.py:1041). */
+int bt_update_v(bt_ctx* ctx);
+
+/* Enqueue n_steps iterations of the _run_sim_core_loop body (sim.py:1169-1366):
+ * NaK pump, per-ion GHK + gap-junction fluxes, env Nernst-Planck update,
+ * Ca pump, cell/env concentration update, update_V, NaN checks. Steps are
+ * replayed from a captured hipGraph when use_graph != 0. Asynchronous. */
+int bt_step(bt_ctx* ctx, int32_t n_steps, int32_t use_graph);
+
+/* Wait for the stream; return BT_OK or BT_UNSTABLE (a NaN was produced; the
+ * state is frozen at the step that produced it and *bad_step receives its
+ * 0-based index counted from context creation, or -1). */
+int bt_sync(bt_ctx* ctx, int64_t* bad_step);
+
+/* Average device time in ms of one launch of a step kernel (0 membrane/cell,
+ * 1 env diffusion, 2 env exchange, 3 Gaussian rows, 4 Gaussian cols+gradient),
+ * timed with HIP events over n_launches back-to-back launches on the context
+ * stream. Leaves the state undefined: call after the timed work. */
+int bt_time_kernel(bt_ctx* ctx, int32_t kernel, int32_t n_launches, double* ms_out);
+
+/* Steps enqueued since bt_create. */
+int64_t bt_steps_done(const bt_ctx* ctx);
+
+/* Last error message of this context (or of the last failed bt_create). */
+const char* bt_last_error(const bt_ctx* ctx);
+
+/* ABI version (major*100+minor). */
+int bt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BETSE_AMD_H */

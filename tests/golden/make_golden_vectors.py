@@ -1,0 +1,186 @@
+"""Write input/output vectors of the reference's own step functions.
+
+Run in a container that has the BETSE sources at /root/reference (never on the
+GPU box, never by the tests): it loads three reference modules straight from
+their files -- ``betse/exceptions.py``, ``betse/science/math/finitediff.py`` and
+``betse/science/sim_toolbox.py`` -- without executing ``betse/__init__.py``
+(which needs packages this image does not have), calls the functions on seeded
+inputs and writes ``reference_vectors.npz`` next to this script. The tests
+compare the oracle against that file.
+
+Functions recorded (reference file:line):
+  finitediff.gradient   math/finitediff.py:1236-1266
+  finitediff.diff       math/finitediff.py:1268-1303
+  finitediff.divergence math/finitediff.py:1305-1311
+  finitediff.integrator math/finitediff.py:1479-1512
+  sim_toolbox.electroflux        sim_toolbox.py:18-69 (also the mutated vBA)
+  sim_toolbox.pumpNaKATP         sim_toolbox.py:71-122
+  sim_toolbox.pumpCaATP          sim_toolbox.py:124-182 (also the clamped inputs)
+  sim_toolbox.nernst_planck_flux sim_toolbox.py:385-413
+  sim_toolbox.no_negs            sim_toolbox.py:475-503
+  sim_toolbox.update_Co          sim_toolbox.py:1155-1207 (+ div_env :1209-1234)
+  sim_toolbox.HH_Decomp          sim_toolbox.py:1236-1290
+"""
+
+import importlib.util
+import os
+import sys
+import types
+from types import SimpleNamespace
+
+import numpy as np
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _load(modname, relpath):
+    spec = importlib.util.spec_from_file_location(modname, os.path.join(REF, relpath))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[modname] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_reference():
+    # package namespaces only, so that betse/__init__.py is not executed
+    for pkg in ("betse", "betse.science", "betse.science.math"):
+        if pkg not in sys.modules:
+            m = types.ModuleType(pkg)
+            m.__path__ = []
+            sys.modules[pkg] = m
+    _load("betse.exceptions", "betse/exceptions.py")
+    fd = _load("betse.science.math.finitediff", "betse/science/math/finitediff.py")
+    sys.modules["betse.science.math"].finitediff = fd
+    stb = _load("betse.science.sim_toolbox", "betse/science/sim_toolbox.py")
+    return fd, stb
+
+
+def params():
+    return SimpleNamespace(
+        F=96485, R=8.314, q=1.602e-19, kb=1.3806e-23, deltaGATP=-37000, cATP=1.5, cADP=0.1, cPi=0.1,
+        KmNK_Na=12.0, KmNK_K=0.2, KmNK_ATP=0.5, KmCa_Ca=1.0e-3, KmCa_ATP=0.5, alpha_NaK=1.0e-7,
+        alpha_Ca=5.0e-8, dt=1.0e-3, is_ecm=True, fast_update_ecm=False, cell_height=10.0e-6)
+
+
+def main():
+    fd, stb = load_reference()
+    rng = np.random.RandomState(20240601)
+    out = {}
+    # ---- finite differences --------------------------------------------------
+    for n, shape in enumerate([(3, 3), (4, 7), (6, 5), (9, 8)]):
+        F = rng.randn(*shape) * 10.0 ** rng.uniform(-2, 2)
+        G = rng.randn(*shape)
+        d = float(10.0 ** rng.uniform(-6, -5))
+        out[f"fd{n}_F"], out[f"fd{n}_G"], out[f"fd{n}_delta"] = F, G, np.float64(d)
+        gx, gy = fd.gradient(F, d)
+        out[f"fd{n}_grad_x"], out[f"fd{n}_grad_y"] = gx, gy
+        out[f"fd{n}_diff0"] = fd.diff(F, d, axis=0)
+        out[f"fd{n}_diff1"] = fd.diff(F, d, axis=1)
+        out[f"fd{n}_div"] = fd.divergence(F, G, d, d)
+        out[f"fd{n}_integ"] = fd.integrator(F, sharp=0.5)
+    # ---- GHK electroflux -----------------------------------------------------
+    p = params()
+    n = 64
+    cA = 10.0 ** rng.uniform(-4, 2.3, n)
+    cB = 10.0 ** rng.uniform(-4, 2.3, n)
+    Dc = 10.0 ** rng.uniform(-19, -8, n)
+    d = np.where(rng.rand(n) < 0.5, 7.5e-9, 26e-9)
+    z = rng.choice([-1, 1, 2], n).astype(np.float64)
+    v = rng.uniform(-0.12, 0.12, n)
+    v[:4] = 0.0
+    v[4:8] = rng.uniform(-1e-20, 1e-20, 4)
+    v[8:12] = 1e-12 * rng.randn(4)
+    for k, a in dict(cA=cA, cB=cB, Dc=Dc, d=d, z=z, v=v).items():
+        out["ghk_" + k] = a.copy()
+    T = 310.0
+    vv = v.copy()
+    out["ghk_flux"] = stb.electroflux(cA, cB, Dc, d, z.copy(), vv, T, p)
+    out["ghk_v_after"] = vv
+    # ---- pumps ---------------------------------------------------------------
+    cNai = 10.0 ** rng.uniform(-1, 2.3, n)
+    cNao = 10.0 ** rng.uniform(-1, 2.3, n)
+    cKi = 10.0 ** rng.uniform(-1, 2.3, n)
+    cKo = 10.0 ** rng.uniform(-1, 2.3, n)
+    cNai[0] = 0.0
+    cKo[1] = 0.0
+    Vm = rng.uniform(-0.1, 0.05, n)
+    for k, a in dict(cNai=cNai, cNao=cNao, cKi=cKi, cKo=cKo, Vm=Vm).items():
+        out["nak_" + k] = a.copy()
+    fNa, fK, rate = stb.pumpNaKATP(cNai, cNao, cKi, cKo, Vm, T, p, 1)
+    out["nak_fNa"], out["nak_fK"] = fNa, fK
+    cCai = 10.0 ** rng.uniform(-6, 0, n)
+    cCao = 10.0 ** rng.uniform(-3, 0.5, n)
+    cCai[0] = 0.0
+    cCai[1] = -1e-5
+    cCao[2] = -1e-4
+    out["ca_cCai"], out["ca_cCao"], out["ca_Vm"] = cCai.copy(), cCao.copy(), Vm.copy()
+    ci, co = cCai.copy(), cCao.copy()
+    out["ca_fCa"] = stb.pumpCaATP(ci, co, Vm, T, p, np.ones(n))
+    out["ca_cCai_after"], out["ca_cCao_after"] = ci, co
+    # ---- Nernst-Planck flux ----------------------------------------------------
+    shape = (5, 6)
+    c = 10.0 ** rng.uniform(0, 2, shape)
+    gcx, gcy = rng.randn(*shape) * 1e5, rng.randn(*shape) * 1e5
+    gvx, gvy = rng.randn(*shape) * 1e2, rng.randn(*shape) * 1e2
+    D = 10.0 ** rng.uniform(-10, -8, shape)
+    zz = -1
+    for k, a in dict(c=c, gcx=gcx, gcy=gcy, gvx=gvx, gvy=gvy, D=D).items():
+        out["np_" + k] = a
+    out["np_z"] = np.float64(zz)
+    fx, fy = stb.nernst_planck_flux(c, gcx, gcy, gvx, gvy, np.zeros(shape), np.zeros(shape), D, zz, T, p)
+    out["np_fx"], out["np_fy"] = fx, fy
+    # ---- no_negs -------------------------------------------------------------------
+    x = rng.randn(32)
+    out["nn_in"] = x.copy()
+    out["nn_out"] = stb.no_negs(x).copy()
+    # ---- update_Co / div_env on a small dense geometry ------------------------------
+    C, M, G = 5, 17, 4
+    m2c = np.sort(rng.randint(0, C, M))
+    m2c[:C] = np.arange(C)
+    m2c = np.sort(m2c)
+    Msum = np.zeros((C, M))
+    Msum[m2c, np.arange(M)] = 1.0
+    mem_sa = 10.0 ** rng.uniform(-11, -10, M)
+    cell_vol = 10.0 ** rng.uniform(-16, -15, C)
+    mapk = rng.randint(0, G * G, M)
+    Mdiv = np.zeros((G * G, M))
+    for m, k in enumerate(mapk):
+        Mdiv[k, m] += mem_sa[m]
+    cells = SimpleNamespace(M_sum_mems=Msum, mem_sa=mem_sa, cell_vol=cell_vol, mem_to_cells=m2c,
+                            M_divmap_mem2ecm=Mdiv, delta=6e-6, xypts=np.zeros((G * G, 2)))
+    cX_cell = 10.0 ** rng.uniform(0, 2, C)
+    cX_env = 10.0 ** rng.uniform(0, 2, G * G)
+    flux = rng.randn(M) * 1e-8
+    out["uc_m2c"], out["uc_mem_sa"], out["uc_cell_vol"], out["uc_map"] = m2c, mem_sa, cell_vol, mapk
+    out["uc_cell"], out["uc_env"], out["uc_flux"] = cX_cell, cX_env, flux
+    out["uc_delta"], out["uc_G"] = np.float64(6e-6), np.int64(G)
+    a, b, e = stb.update_Co(None, cX_cell.copy(), cX_cell[m2c].copy(), cX_env.copy(), flux, cells, p)
+    out["uc_cell_out"], out["uc_mem_out"], out["uc_env_out"] = a, b, e
+    # ---- HH_Decomp ---------------------------------------------------------------------
+    ny, nx, dl = 7, 6, 5e-6
+    X = np.zeros((ny, nx))
+    from scipy.sparse import csr_matrix  # noqa: F401  (finitediff imports scipy.spatial only)
+    n2 = ny * nx
+    A = np.zeros((n2, n2))
+    for k in range(n2):
+        i, j = divmod(k, nx)
+        if 0 < i < ny - 1 and 0 < j < nx - 1:
+            A[k, k] = -4
+            for off in (nx, -nx, 1, -1):
+                A[k, k + off] = 1
+        else:
+            A[k, k] = 1
+    A = A / dl ** 2
+    cells_h = SimpleNamespace(X=X, delta=dl, lapENVinv=np.linalg.pinv(A))
+    JJx, JJy = rng.randn(ny, nx), rng.randn(ny, nx)
+    out["hh_JJx"], out["hh_JJy"], out["hh_delta"] = JJx, JJy, np.float64(dl)
+    res = stb.HH_Decomp(JJx, JJy, cells_h, bounds={"L": 0.0, "R": 0.0, "T": 0.0, "B": 0.0})
+    for k, r in zip(("AA", "Fx", "Fy", "BB", "Gx", "Gy"), res):
+        out["hh_" + k] = np.asarray(r)
+    np.savez_compressed(os.path.join(HERE, "reference_vectors.npz"), **out)
+    print("wrote", len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,65 @@
+"""The C ABI: the library loads, exports every declared entry point, and the
+ctypes structures have the C layout (no compute calls: runs without a GPU)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from betse_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "betse_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(bt_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_bound_functions():
+    decl = declared_functions()
+    assert decl == sorted(_native.SIGNATURES), (decl, sorted(_native.SIGNATURES))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_native.LIB_PATH):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    lib = _native.load()
+    assert lib.bt_version() >= 100
+
+
+def test_load_fails_loudly_when_missing(tmp_path):
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _native.load(str(tmp_path / "nope.so"))
+
+
+def test_struct_layout_matches_c(tmp_path):
+    fields_p = [f[0] for f in _native.BtParams._fields_]
+    fields_b = [f[0] for f in _native.BtBuffers._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "betse_amd.h"', 'int main(void){',
+           'printf("%zu %zu\\n", sizeof(bt_params), sizeof(bt_buffers));']
+    for f in fields_p:
+        src.append(f'printf("p {f} %zu\\n", offsetof(bt_params, {f}));')
+    for f in fields_b:
+        src.append(f'printf("b {f} %zu\\n", offsetof(bt_buffers, {f}));')
+    src.append("return 0;}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    sp, sb = map(int, out[0].split())
+    assert sp == ctypes.sizeof(_native.BtParams)
+    assert sb == ctypes.sizeof(_native.BtBuffers)
+    for line in out[1:]:
+        if not line:
+            continue
+        kind, name, off = line.split()
+        cls = _native.BtParams if kind == "p" else _native.BtBuffers
+        assert getattr(cls, name).offset == int(off), name
