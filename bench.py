@@ -116,7 +116,19 @@ def main(argv=None):
     ap.add_argument("--flux", default="both", choices=["fp64", "fp32", "both"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args(argv)
+    if a.copy_calib_mib:
+        import torch
+        n = a.copy_calib_mib * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8, "launches": 5}))
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
               "(each GPU box has one GPU; the sharding design is in DESIGN.md)", file=sys.stderr)

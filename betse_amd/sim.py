@@ -118,6 +118,8 @@ def make_tiles(mem_ptr: np.ndarray, tile_m: int = TILE_M) -> np.ndarray:
     counts = np.diff(mem_ptr)
     if counts.max(initial=0) > tile_m:
         raise BetseSimException(f"a cell has more than {tile_m} membranes")
+    if counts.min(initial=1) < 1:
+        raise BetseSimException("a cell has no membranes")
     tiles = [0]
     acc = 0
     for c in range(n):
