@@ -29,7 +29,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str) -> dict:
+def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0) -> dict:
     """Compulsory bytes per launch of each step kernel (each array element the
     kernel must touch, counted once at its storage width)."""
     C, M = cells.n_cells, cells.n_mems
@@ -37,15 +37,14 @@ def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str) -> dict:
     n_in = len(cells.envInds_inClust)
     ni = n_ions
     b = {}
-    b["mem_cell"] = (M * (4 * 4 + 8 + 8 + 8 * ni)          # mem_cell, nn_cell, nn_mem, mem_sq, mem_sa, gjopen, Dm
+    b["mem_cell"] = (M * (5 * 4 + 8 + 8 + 8 * ni)          # mem_cell, nn_cell, nn_mem, mem_sq, mem_pos, mem_sa, gjopen, Dm
                      + M * 8 * ni                           # fsa write
+                     + M * n_ch * (16 + 16 + 8)              # gates r/w, channel flux write
                      + C * (8 * ni * 2 + 8 * ni + 8 * ni     # c_cell r, c_lag r, c_cell w, c_lag w
-                            + 16 + 16 + 4)                   # vm r/w, cell_vol, diviterm, mem_ptr
-                     + n_in * 8 * (ni + (1 if ca else 0)))  # env gathers (pre- and post-diffusion Ca)
-    b["env_diffuse"] = G2 * (8 * ni + 8 * ni + 16 + 8 * ni)  # c_env r, D_env r, E r, c_mid w
-    b["env_exchange"] = G2 * (8 * ni + 8 * ni + 8 + 1 + 4) + M * (4 + 8 * ni)
-    b["gauss_rows"] = G2 * 16
-    b["gauss_cols_grad"] = G2 * (8 + 24)
+                            + 16 + 16 + 4 + 8 * n_ch)        # vm r/w, cell_vol, diviterm, mem_ptr, modulators
+                     + n_in * 8 * (ni + (1 if ca else 0) + n_ch))  # env gathers (pre- and post-diffusion)
+    b["env_field"] = G2 * (8 + 8 * ni + 8 * ni + 8 * ni)    # u r, c_env r, D_env r, c_mid w
+    b["env_exchange"] = G2 * (8 * ni + 8 * ni + 8 + 1 + 4) + M * 8 * (ni + n_ch)
     b["step"] = sum(b.values())
     return b
 
@@ -83,8 +82,7 @@ def time_gpu(p, cells, flux_mode, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev_ms = e0.elapsed_time(e1)
-    kernels = {k: sim.time_kernel(k, 100) for k in ("mem_cell", "env_diffuse", "env_exchange",
-                                                        "gauss_rows", "gauss_cols_grad")}
+    kernels = {k: sim.time_kernel(k, 100) for k in ("mem_cell", "env_field", "env_exchange")}
     return sim, wall, ev_ms, kernels
 
 
@@ -176,7 +174,7 @@ def main(argv=None):
     r = res[best]
     ms_step = r["ev_ms"] / a.steps
     value = C * a.steps / (r["ev_ms"] / 1e3)
-    ab = algorithmic_bytes(cells, n_ions, has_ca, best)
+    ab = algorithmic_bytes(cells, n_ions, has_ca, best, len(p.channels))
     kern = r["kernels"]
     dom = max(kern, key=kern.get)
     achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
@@ -195,7 +193,7 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"single large cluster ({a.config}): {C} cells, {cells.n_mems} membranes, "
                                f"{cells.X.shape[0]}x{cells.X.shape[1]} grid, ions {'/'.join(p.ion_names())}, "
-                               f"dt {p.dt:g} s; Nav/Kv channels not yet included",
+                               f"dt {p.dt:g} s; channels {'/'.join(c.kind for c in p.channels) or 'none'}",
                    "cells": C, "membranes": cells.n_mems, "grid": list(cells.X.shape), "n_ions": n_ions,
                    "flux_mode": best},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -19,13 +19,19 @@ BT_UNSTABLE = 1
 BT_EINVAL = -1
 BT_EHIP = -2
 BT_MAX_IONS = 8
+BT_MAX_CHAN = 4
+BT_CHAN_KINDS = {"Nav1p2": 0, "Kv1p1": 1}
 BT_FLUX_FP64 = 0
 BT_FLUX_FP32 = 1
 
-KERNEL_IDS = {"mem_cell": 0, "env_diffuse": 1, "env_exchange": 2, "gauss_rows": 3, "gauss_cols_grad": 4}
+BT_TILE_MEMBRANES = 64
+
+KERNEL_IDS = {"mem_cell": 0, "env_field": 1, "env_exchange": 2, "env_field_only": 3}
 
 _d8 = ctypes.c_double * BT_MAX_IONS
 _d5 = ctypes.c_double * 5
+_i4 = ctypes.c_int32 * BT_MAX_CHAN
+_d4 = ctypes.c_double * BT_MAX_CHAN
 _vp = ctypes.c_void_p
 
 
@@ -34,7 +40,8 @@ class BtParams(ctypes.Structure):
         ("n_cells", ctypes.c_int32), ("n_mems", ctypes.c_int32), ("grid_ny", ctypes.c_int32),
         ("grid_nx", ctypes.c_int32), ("n_ions", ctypes.c_int32),
         ("i_na", ctypes.c_int32), ("i_k", ctypes.c_int32), ("i_ca", ctypes.c_int32),
-        ("flux_mode", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("flux_mode", ctypes.c_int32), ("n_chan", ctypes.c_int32),
+        ("chan_kind", _i4), ("chan_ion", _i4), ("chan_max_dm", _d4), ("chan_dt", ctypes.c_double),
         ("z", _d8), ("d_gj", _d8), ("c_bound", _d8),
         ("dt", ctypes.c_double), ("T", ctypes.c_double), ("T_gj", ctypes.c_double),
         ("F", ctypes.c_double), ("R", ctypes.c_double), ("q", ctypes.c_double), ("kb", ctypes.c_double),
@@ -49,15 +56,26 @@ class BtParams(ctypes.Structure):
         ("ko_env", ctypes.c_double), ("eo", ctypes.c_double), ("er", ctypes.c_double),
         ("cell_radius", ctypes.c_double), ("true_cell_size", ctypes.c_double),
         ("gauss_w", _d5),
+        ("smooth_cells", ctypes.c_double), ("mu", ctypes.c_double),
     ]
 
 
 class BtBuffers(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
-        "mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "tile_ptr",
+        "mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "mem_pos", "tile_ptr",
         "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
-        "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env")] + [
-        ("n_tiles", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+        "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "mem_nx", "mem_ny", "cell_sa",
+        "gates", "chan_mod")] + [
+        ("n_tiles", ctypes.c_int32), ("tile_m", ctypes.c_int32)]
+
+
+OBS_FIELDS = ("fluxes_mem", "fluxes_gj", "fluxes_env_x", "fluxes_env_y", "chan_flux", "rate_NaKATP",
+              "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave",
+              "J_env_x", "J_env_y", "Jtx", "Jty", "B_field")
+
+
+class BtObs(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in OBS_FIELDS]
 
 
 # symbol -> (restype, argtypes); every function include/betse_amd.h declares
@@ -67,6 +85,9 @@ SIGNATURES = {
     "bt_update_v": (ctypes.c_int, [_vp]),
     "bt_step": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "bt_sync": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "bt_step_observed": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(BtObs)]),
+    "bt_observables": (ctypes.c_int, [_vp, ctypes.POINTER(BtObs)]),
+    "bt_gemm_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "bt_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "bt_steps_done": (ctypes.c_int64, [_vp]),
     "bt_last_error": (ctypes.c_char_p, [_vp]),

@@ -2,26 +2,29 @@
 // and the C ABI declared in include/betse_amd.h.
 //
 // One step of Simulator._run_sim_core_loop (betse/science/sim.py:1169-1366) is
-// five kernels on one stream:
+// three kernels on one stream:
 //
-//   k_env_diffuse   per grid square, all ions: boundary reset (Q3), gradient,
-//                   Nernst-Planck flux, sign-flipped boundary divergence (Q4),
-//                   explicit Euler (sim.py:2209-2254)          c_env -> c_mid
-//   k_mem_cell      one block per tile of whole cells (<= 256 membranes):
-//                   per membrane NaK pump, GHK per ion, GHK gap junction per
-//                   ion, Ca pump; per cell the CSR segment sums from LDS, the
-//                   update_Co/GJ update, no_negs clamp, rho_cells -> Vmem
-//                   (sim.py:1196-1283, 2086-2111, 2007-2038)   cell A -> cell B
-//   k_env_exchange  per square: grid-side CSR gather of the membrane fluxes
-//                   (M_divmap_mem2ecm), env update, rho_env -> surface charge
+//   k_env_field     one block per 16x16 tile of the environment grid, staged
+//                   through LDS: the surface-charge field u of the previous
+//                   update_V -> SciPy Gaussian (sigma 1, constant 0) -> v_env ->
+//                   E = -grad(screen*v_env) (ion_current.py:93-109), then for
+//                   every ion the boundary reset (Q3), gradient, Nernst-Planck
+//                   flux and sign-flipped boundary divergence (Q4) of update_ecm
+//                   (sim.py:2209-2254)                          c_env -> c_mid
+//   k_mem_cell      one block per tile of whole cells (<= 64 membranes), one
+//                   wavefront per ion: NaK pump, GHK, GHK gap junction, Ca pump,
+//                   voltage-gated channel per (membrane, ion); per cell the CSR
+//                   segment sums from LDS, update_Co + GJ update, no_negs clamp,
+//                   rho_cells -> Vmem (sim.py:1196-1283, 2086-2111, 2007-2038,
+//                   networks.py:3115-3215)                      cell A -> cell B
+//   k_env_exchange  per square: the square's membrane fluxes (written by
+//                   k_mem_cell in square-sorted order, the sparse form of
+//                   M_divmap_mem2ecm) -> env update -> rho_env -> u
 //                   (sim_toolbox.py:1191-1229, ion_current.py:75-97) c_mid -> c_env
-//   k_gauss_rows    Gaussian sigma=1 along axis 0, constant-0 edges (ion_current.py:104)
-//   k_gauss_cols_grad  Gaussian along axis 1, then E = -grad(screen*v_env)
-//                   (ion_current.py:104-109)
 //
-// Floating-point expressions keep the reference's operation order (the code
-// is compiled with -ffp-contract=off) so the FP64 path tracks the NumPy
-// reference to a few ulps per step.
+// Floating-point expressions keep the reference's operation order (the code is
+// compiled with -ffp-contract=off) so the FP64 path tracks the NumPy reference
+// to a few ulps per step.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -35,13 +38,19 @@
 
 namespace {
 
-constexpr int TILE_M = 256;              // membranes per k_mem_cell block
-constexpr int BLOCK = 256;               // threads per grid kernel block
+constexpr int TMB = BT_TILE_MEMBRANES;   // membranes per k_mem_cell tile (one wavefront per ion)
+constexpr int BLOCK = 256;               // threads per per-square kernel block
+constexpr int TY = 16, TX = 16;          // k_env_field tile
 constexpr double FLOAT_NONCE = 1.0e-25;  // sim_toolbox.py:50
 
 // Constants derived on the host, each in the reference's evaluation order.
 struct KParams {
-    int n_cells, n_mems, ny, nx, n_sq, n_ions, i_na, i_k, i_ca, pad;
+    int n_cells, n_mems, ny, nx, n_sq, n_ions, i_na, i_k, i_ca, n_chan;
+    int chan_kind[BT_MAX_CHAN];
+    int chan_ion[BT_MAX_CHAN];
+    int ion_chan[BT_MAX_IONS];      // channel acting on each ion, -1 if none
+    double chan_max_dm[BT_MAX_CHAN];
+    double chan_dt;
     double zplus[BT_MAX_IONS];      // z + 1e-25          (electroflux zc += nonce)
     double zF[BT_MAX_IONS];         // zs * p.F           (get_current)
     double zq[BT_MAX_IONS];         // z (for alpha = (D*z*q)/(kb*T))
@@ -54,7 +63,7 @@ struct KParams {
     double dG_RT;                   // deltaGATP/(R*T)
     double qn_scal;                 // (cADP*1e-3)*(cPi*1e-3)
     double qd_scal;                 // cATP*1e-3
-    double inv_km_na, km_na, km_k, atp_km, one_p_atp_km;
+    double km_na, km_k, atp_km, one_p_atp_km;
     // Ca-ATPase (sim_toolbox.py:124-182)
     double ca_coef;                 // -alpha_Ca
     double qca_num;                 // cADP*cPi
@@ -119,197 +128,360 @@ __device__ __forceinline__ FT pump_ca(FT cCai, FT cCao, FT Vm, const KParams& P)
     return (FT)P.ca_coef * frwd * ((FT)1 - (Q / Keq));
 }
 
+// Nav1p2._calculate_state (channels/vg_na.py:169-186); V in mV.
+template <typename FT>
+__device__ __forceinline__ void nav1p2_rates(FT V, FT& mInf, FT& mTau, FT& hInf, FT& hTau) {
+    const FT a = V - (FT)10.0;
+    const FT mAlpha = ((FT)0.182 * (a - (FT)-35.0)) / ((FT)1 - exp(-(a - (FT)-35.0) / (FT)9));
+    const FT mBeta = ((FT)0.124 * (-a - (FT)35.0)) / ((FT)1 - exp(-(-a - (FT)35.0) / (FT)9));
+    mInf = mAlpha / (mAlpha + mBeta);
+    mTau = (FT)1 / (mAlpha + mBeta);
+    hInf = (FT)1.0 / ((FT)1 + exp((V - (FT)-65.0 - (FT)10.0) / (FT)6.2));
+    const FT t1 = ((FT)0.024 * (a - (FT)-50.0)) / ((FT)1 - exp(-(a - (FT)-50.0) / (FT)5));
+    const FT t2 = ((FT)0.0091 * (-a - (FT)75.000123)) / ((FT)1 - exp(-(-a - (FT)75.000123) / (FT)5));
+    hTau = (FT)1 / (t1 + t2);
+}
+
+// Kv1p1._calculate_state (channels/vg_k.py:167-178); V in mV.
+template <typename FT>
+__device__ __forceinline__ void kv1p1_rates(FT V, FT& mInf, FT& mTau, FT& hInf, FT& hTau) {
+    mInf = (FT)1.0 / ((FT)1 + exp((V - (FT)-30.5) / (FT)-11.3943));
+    mTau = (FT)30.0 / ((FT)1 + exp((V - (FT)-76.56) / (FT)26.1479));
+    hInf = (FT)1.0 / ((FT)1 + exp((V - (FT)-30.0) / (FT)27.3943));
+    hTau = (FT)15000.0 / ((FT)1 + exp((V - (FT)-160.56) / (FT)-100.0));
+}
+
+// Vmem carrying n sequential +1e-25 nonces (electroflux mutates sim.vm in place, Q5)
+__device__ __forceinline__ double with_nonces(double v, int n) {
+    for (int q = 0; q < n; ++q) v = v + FLOAT_NONCE;
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // k_mem_cell: membrane fluxes + per-cell update + Vmem
 // ---------------------------------------------------------------------------
 struct MemCellArgs {
-    const int32_t* mem_ptr; const int32_t* mem_cell; const int32_t* nn_mem; const int32_t* nn_cell; const int32_t* mem_sq;
-    const int32_t* tile_ptr; const double* mem_sa; const double* gjopen; const double* cell_vol;
-    const double* diviterm; const double* Dm;
+    const int32_t* mem_ptr; const int32_t* mem_cell; const int32_t* nn_mem; const int32_t* nn_cell;
+    const int32_t* mem_sq; const int32_t* mem_pos; const int32_t* tile_ptr;
+    const double* mem_sa; const double* gjopen; const double* cell_vol; const double* diviterm; const double* Dm;
     const double* c_env;      // pre-diffusion env (membrane + NaK gathers, Q2)
-    const double* c_mid;      // post-diffusion env (Ca pump gather)
+    const double* c_mid;      // post-diffusion env (Ca pump and channel gathers)
     const double* c_cell_in; const double* c_lag_in; const double* vm_in;
     double* c_cell_out; double* c_lag_out; double* vm_out;
-    double* fsa;              // [n_ions][n_mems] flux_mem * mem_sa
+    double* fsa;              // [n_ions][n_mems] flux_mem * mem_sa, square-sorted
+    double* gates;            // [n_chan][2][n_mems]
+    const double* chan_mod;   // [n_chan][n_cells]
+    double* fch;              // [n_chan][n_mems] channel flux * mem_sa, square-sorted
     unsigned* flags; unsigned long long* bad_step; const long long* step_ctr;
+    // observed step only (NULL otherwise): raw fluxes in membrane order
+    double* obs_fm; double* obs_fgj; double* obs_fch; double* obs_rate;
 };
 
 template <int NI, typename FT>
-__global__ void __launch_bounds__(TILE_M)
+__global__ void __launch_bounds__(NI * TMB)
 k_mem_cell(MemCellArgs A, KParams P) {
     if (halted(A.flags)) return;
-    __shared__ double s_mem[NI][TILE_M];
-    __shared__ double s_gj[NI][TILE_M];
+    __shared__ double s_mem[NI][TMB];
+    __shared__ double s_gj[NI][TMB];
+    __shared__ double s_ch[BT_MAX_CHAN][TMB];
+    __shared__ double s_c2[NI][TMB];
+    __shared__ int s_bad;
     const int tile = blockIdx.x;
     const int c0 = A.tile_ptr[tile];
     const int c1 = A.tile_ptr[tile + 1];
     const int m0 = A.mem_ptr[c0];
     const int m1 = A.mem_ptr[c1];
-    const int t = threadIdx.x;
-    const int m = m0 + t;
+    const int i = threadIdx.x / TMB;      // ion of this wavefront
+    const int j = threadIdx.x % TMB;
+    const int m = m0 + j;
     const int M = P.n_mems, C = P.n_cells, G2 = P.n_sq;
+    if (threadIdx.x == 0) s_bad = 0;
 
     if (m < m1) {
         const int c = A.mem_cell[m];
         const int cn = A.nn_cell[m];
         const bool bnd = (A.nn_mem[m] == m);
         const int k = A.mem_sq[m];
+        const int pos = A.mem_pos[m];
         const double sa = A.mem_sa[m];
         const double v = A.vm_in[c];
         const double vn = A.vm_in[cn];
-        // NaK-ATPase on the lagged membrane concentrations (sim.py:1196-1234, Q1)
-        FT fNa = (FT)0, fK = (FT)0;
-        const bool nak = (P.i_na >= 0);
-        if (nak) {
-            fNa = pump_nak<FT>((FT)A.c_lag_in[P.i_na * C + c], (FT)A.c_env[P.i_na * G2 + k],
-                               (FT)A.c_lag_in[P.i_k * C + c], (FT)A.c_env[P.i_k * G2 + k], (FT)v, P);
-            fK = (FT)P.k_per_na * fNa;
+        double f = 0.0;                                  // fluxes_mem.fill(0)
+        // NaK-ATPase on the lagged membrane concentrations (sim.py:1196-1234, Q1);
+        // the Na and the K wavefronts evaluate the same pump
+        if (P.i_na >= 0 && (i == P.i_na || i == P.i_k)) {
+            const FT fNa = pump_nak<FT>((FT)A.c_lag_in[P.i_na * C + c], (FT)A.c_env[P.i_na * G2 + k],
+                                        (FT)A.c_lag_in[P.i_k * C + c], (FT)A.c_env[P.i_k * G2 + k], (FT)v, P);
+            f = f + (double)((i == P.i_na) ? fNa : (FT)P.k_per_na * fNa);
+            if (A.obs_rate && i == P.i_na) A.obs_rate[m] = -(double)fNa;
         }
-        double fm[NI];
-        double vmut = v, vnmut = vn;
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            double f = 0.0;                          // fluxes_mem.fill(0)
-            if (nak && i == P.i_na) f = f + (double)fNa;
-            if (nak && i == P.i_k) f = f + (double)fK;
-            // electroflux mutates sim.vm in place: +1e-25 per ion (Q5)
-            vmut = vmut + FLOAT_NONCE;
-            vnmut = vnmut + FLOAT_NONCE;
-            const double clag = A.c_lag_in[i * C + c];
-            const FT fED = ghk<FT>((FT)A.c_env[i * G2 + k], (FT)clag, (FT)A.Dm[i * M + m], (FT)P.tm,
-                                   (FT)P.zplus[i], (FT)vmut, (FT)P.F, (FT)P.RT);
-            fm[i] = f + (double)fED;
-            double fgj = 0.0;                        // fgj[bflags_mems] = 0 (sim.py:2203)
-            if (!bnd) {
-                const double vgj = vnmut - vmut;
-                const double vBA = vgj + FLOAT_NONCE;
-                fgj = (double)ghk<FT>((FT)clag, (FT)A.c_lag_in[i * C + cn],
-                                      (FT)(P.dgj_surf[i] * A.gjopen[m]), (FT)P.gj_len,
-                                      (FT)P.zplus[i], (FT)vBA, (FT)P.F, (FT)P.RT_gj);
-            }
-            s_gj[i][t] = (-fgj) * sa;
+        // GHK across the membrane and across the gap junction (sim.py:1253-1269, 2162-2206)
+        const double vmut = with_nonces(v, i + 1);
+        const double vnmut = with_nonces(vn, i + 1);
+        const double clag = A.c_lag_in[i * C + c];
+        const FT fED = ghk<FT>((FT)A.c_env[i * G2 + k], (FT)clag, (FT)A.Dm[i * M + m], (FT)P.tm,
+                               (FT)P.zplus[i], (FT)vmut, (FT)P.F, (FT)P.RT);
+        f = f + (double)fED;
+        double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
+        if (!bnd) {
+            const double vBA = (vnmut - vmut) + FLOAT_NONCE;
+            fgj = (double)ghk<FT>((FT)clag, (FT)A.c_lag_in[i * C + cn], (FT)(P.dgj_surf[i] * A.gjopen[m]),
+                                  (FT)P.gj_len, (FT)P.zplus[i], (FT)vBA, (FT)P.F, (FT)P.RT_gj);
         }
         // Ca-ATPase after the ion loop: refreshed cell Ca, post-diffusion env Ca,
         // Vmem carrying all n_ions nonces (sim.py:2126-2159)
-        double fCa = 0.0;
-        if (P.i_ca >= 0) {
-            double cai = A.c_cell_in[P.i_ca * C + c];
-            double cao = A.c_mid[P.i_ca * G2 + k];
+        if (i == P.i_ca) {
+            double cai = A.c_cell_in[i * C + c];
+            double cao = A.c_mid[i * G2 + k];
             if (cai < 0.0) cai = 0.0;
             if (cao < 0.0) cao = 0.0;
-            fCa = (double)pump_ca<FT>((FT)cai, (FT)cao, (FT)vmut, P);
+            f = f + (double)pump_ca<FT>((FT)cai, (FT)cao, (FT)with_nonces(v, NI), P);
         }
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            double f = fm[i];
-            if (i == P.i_ca) f = f + fCa;
-            const double x = f * sa;
-            s_mem[i][t] = x;
-            A.fsa[i * M + m] = x;
+        // voltage-gated channel on this ion (networks.py:3115-3215): gates from
+        // Vmem carrying the nonces so far, update_mh, open probability, GHK with
+        // the current cell and post-diffusion env concentrations. Gating stays
+        // FP64 in both flux modes: the Hodgkin-Huxley rates have removable 0/0
+        // points (V = -25 mV, -40 mV for Nav1p2) that FP32 rounding would hit.
+        const int ch = P.ion_chan[i];
+        if (ch >= 0) {
+            double* gm = A.gates + (size_t)(2 * ch) * M;
+            double* gh = gm + M;
+            const double vg = with_nonces(v, NI + ch);
+            const double V = vg * 1000.0 + 0.0;
+            double mInf, mTau, hInf, hTau;
+            const bool nav = (P.chan_kind[ch] == BT_CHAN_NAV1P2);
+            if (nav) nav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
+            else kv1p1_rates<double>(V, mInf, mTau, hInf, hTau);
+            const double dtm = P.chan_dt;
+            double mm = gm[m], hh = gh[m];
+            mm = (mTau * mm + dtm * mInf) / (mTau + dtm);
+            hh = (hTau * hh + dtm * hInf) / (hTau + dtm);
+            gm[m] = mm;
+            gh[m] = hh;
+            const double Pch = nav ? cube(mm) * hh : mm * (hh * hh);
+            const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * A.chan_mod[(size_t)ch * C + c]);
+            const FT fc = ghk<FT>((FT)A.c_mid[i * G2 + k], (FT)A.c_cell_in[i * C + c], Dch, (FT)P.tm,
+                                  (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE), (FT)P.F, (FT)P.RT);
+            const double x = (double)fc * sa;
+            s_ch[ch][j] = x;
+            A.fch[(size_t)ch * M + pos] = x;
+            if (A.obs_fch) A.obs_fch[(size_t)ch * M + m] = (double)fc;
         }
+        if (A.obs_fm) {
+            A.obs_fm[(size_t)i * M + m] = f;
+            A.obs_fgj[(size_t)i * M + m] = fgj;
+        }
+        const double x = f * sa;
+        s_mem[i][j] = x;
+        A.fsa[(size_t)i * M + pos] = x;
+        s_gj[i][j] = (-fgj) * sa;
     }
     __syncthreads();
 
-    // per-cell segment sums in membrane order (np.dot(M_sum_mems, .) / reduceat)
-    for (int c = c0 + t; c < c1; c += TILE_M) {
+    // per-cell segment sums in membrane order (np.dot(M_sum_mems, .)), one ion per wavefront
+    const int c = c0 + j;
+    if (c < c1) {
         const int a = A.mem_ptr[c] - m0;
         const int b = A.mem_ptr[c + 1] - m0;
         const double vol = A.cell_vol[c];
-        double rho = 0.0;
-        bool bad = false;
+        double base = A.c_cell_in[i * C + c];
+        const int ch = P.ion_chan[i];
+        if (ch >= 0) {             // immediate update_Co of the channel flux
+            double Sc = s_ch[ch][a];
+            for (int q = a + 1; q < b; ++q) Sc = Sc + s_ch[ch][q];
+            base = base + (Sc / vol) * P.dt;
+        }
+        double S = s_mem[i][a];
+        double Sg = s_gj[i][a];
+        for (int q = a + 1; q < b; ++q) {
+            S = S + s_mem[i][q];
+            Sg = Sg + s_gj[i][q];
+        }
+        const double c1v = base + (S / vol) * P.dt;
+        A.c_lag_out[i * C + c] = c1v;
+        double c2 = c1v + P.dt * (Sg / vol);
+        if (c2 < 0.0) c2 = 0.0;                    // no_negs clamp (Q9)
+        if (isnan(c2)) s_bad = 1;
+        A.c_cell_out[i * C + c] = c2;
+        s_c2[i][j] = c2;
+    }
+    __syncthreads();
+    if (i == 0 && c < c1) {
+        double rho = P.zF[0] * s_c2[0][j];
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            double S = s_mem[i][a];
-            double Sg = s_gj[i][a];
-            for (int j = a + 1; j < b; ++j) {
-                S = S + s_mem[i][j];
-                Sg = Sg + s_gj[i][j];
-            }
-            const double delta_cells = S / vol;
-            const double c1v = A.c_cell_in[i * C + c] + delta_cells * P.dt;
-            A.c_lag_out[i * C + c] = c1v;
-            const double delta_cgj = Sg / vol;
-            double c2 = c1v + P.dt * delta_cgj;
-            if (c2 < 0.0) c2 = 0.0;          // no_negs clamp (Q9)
-            if (isnan(c2)) bad = true;
-            A.c_cell_out[i * C + c] = c2;
-            rho = (i == 0) ? P.zF[0] * c2 : rho + P.zF[i] * c2;
-        }
+        for (int ii = 1; ii < NI; ++ii) rho = rho + P.zF[ii] * s_c2[ii][j];
         const double vm = P.cm_inv * (rho * A.diviterm[c]);
-        if (isnan(vm)) bad = true;
+        if (isnan(vm)) s_bad = 1;
         A.vm_out[c] = vm;
-        if (bad) {
-            atomicOr(A.flags, 1u);
-            atomicMin(A.bad_step, (unsigned long long)(*A.step_ctr));
-        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_bad) {
+        atomicOr(A.flags, 1u);
+        atomicMin(A.bad_step, (unsigned long long)(*A.step_ctr));
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_env_diffuse: update_ecm for every ion (sim.py:2209-2254)
+// k_env_field: Gaussian + E field from u, then update_ecm for every ion
 // ---------------------------------------------------------------------------
-struct EnvArgs {
-    const double* c_env; const double* D_env; const double* Ex; const double* Ey;
-    double* c_mid; double* fex; double* fey; const unsigned* flags;
+struct FieldArgs {
+    const double* u;          // scaled surface charge of the last update_V
+    const double* c_env; const double* D_env;
+    double* c_mid;            // post-diffusion env (diffuse mode)
+    double* v_env; double* Ex; double* Ey;   // written in field-only mode
+    const unsigned* flags;
+    double* obs_fx; double* obs_fy;          // fluxes_env_x/y of the observed step, or NULL
 };
 
-template <typename FT>
-__global__ void __launch_bounds__(BLOCK)
-k_env_diffuse(EnvArgs A, KParams P) {
+template <bool DIFFUSE>
+__global__ void __launch_bounds__(TY * TX)
+k_env_field(FieldArgs A, KParams P) {
     if (halted(A.flags)) return;
-    const int idx = blockIdx.x * BLOCK + threadIdx.x;
-    if (idx >= P.n_sq) return;
+    constexpr int UH = 6, UR = TY + 2 * UH, UC = TX + 2 * UH;    // u on tile +- 6
+    constexpr int TR = TY + 4;                                  // row-blurred rows tile +- 2
+    constexpr int VR = TY + 4, VC = TX + 4;                     // v on tile +- 2
+    constexpr int ER = TY + 2, EC = TX + 2;                     // E on tile +- 1
+    __shared__ double sU[UR][UC];
+    __shared__ double sT[TR][UC];
+    __shared__ double sV[VR][VC];
+    __shared__ double sEx[ER][EC];
+    __shared__ double sEy[ER][EC];
+    __shared__ double sC[VR][VC];
+    __shared__ double sD[ER][EC];
+    __shared__ double sFx[TY][TX + 2];
+    __shared__ double sFy[TY + 2][TX];
     const int ny = P.ny, nx = P.nx;
-    const int r = idx / nx, cc = idx % nx;
+    const int r0 = blockIdx.y * TY, cc0 = blockIdx.x * TX;
+    const int tid = threadIdx.x;
     const double dl = P.delta, d2 = P.two_delta;
+
+    // u with zero padding outside the grid (gaussian_filter mode='constant')
+    for (int e = tid; e < UR * UC; e += TY * TX) {
+        const int rr = r0 - UH + e / UC, cc = cc0 - UH + e % UC;
+        sU[e / UC][e % UC] = (rr >= 0 && rr < ny && cc >= 0 && cc < nx) ? A.u[rr * nx + cc] : 0.0;
+    }
+    __syncthreads();
+    // axis-0 pass (scipy correlate1d: centre tap, then pairs outermost first)
+    for (int e = tid; e < TR * UC; e += TY * TX) {
+        const int lr = e / UC + 4, lc = e % UC;          // row in sU
+        double o = sU[lr][lc] * P.gw[0];
+#pragma unroll
+        for (int q = 4; q >= 1; --q) o = o + (sU[lr - q][lc] + sU[lr + q][lc]) * P.gw[q];
+        sT[e / UC][lc] = o;
+    }
+    __syncthreads();
+    // axis-1 pass -> v_env (+ Phi_b == 0)
+    for (int e = tid; e < VR * VC; e += TY * TX) {
+        const int lr = e / VC, lc = e % VC + 4;          // column in sT
+        double o = sT[lr][lc] * P.gw[0];
+#pragma unroll
+        for (int q = 4; q >= 1; --q) o = o + (sT[lr][lc - q] + sT[lr][lc + q]) * P.gw[q];
+        sV[lr][e % VC] = o;
+    }
+    __syncthreads();
+    // E = -grad(screen * v_env), one-sided at the world edges (finitediff.py:1236-1266)
+    for (int e = tid; e < ER * EC; e += TY * TX) {
+        const int er = e / EC, ec = e % EC;
+        const int r = r0 - 1 + er, cc = cc0 - 1 + ec;
+        double ex = 0.0, ey = 0.0;
+        if (r >= 0 && r < ny && cc >= 0 && cc < nx) {
+            const int vr = er + 1, vc = ec + 1;         // (r, cc) in sV
+            double gx, gy;
+            if (cc == 0) gx = (P.screen * sV[vr][vc + 1] - P.screen * sV[vr][vc]) / dl;
+            else if (cc == nx - 1) gx = (P.screen * sV[vr][vc] - P.screen * sV[vr][vc - 1]) / dl;
+            else gx = -(P.screen * sV[vr][vc - 1] - P.screen * sV[vr][vc + 1]) / d2;
+            if (r == 0) gy = (P.screen * sV[vr + 1][vc] - P.screen * sV[vr][vc]) / dl;
+            else if (r == ny - 1) gy = (P.screen * sV[vr][vc] - P.screen * sV[vr - 1][vc]) / dl;
+            else gy = -(P.screen * sV[vr - 1][vc] - P.screen * sV[vr + 1][vc]) / d2;
+            ex = -gx;
+            ey = -gy;
+        }
+        sEx[er][ec] = ex;
+        sEy[er][ec] = ey;
+    }
+    __syncthreads();
+    if (!DIFFUSE) {
+        const int r = r0 + tid / TX, cc = cc0 + tid % TX;
+        if (r < ny && cc < nx) {
+            A.v_env[r * nx + cc] = sV[tid / TX + 2][tid % TX + 2];
+            A.Ex[r * nx + cc] = sEx[tid / TX + 1][tid % TX + 1];
+            A.Ey[r * nx + cc] = sEy[tid / TX + 1][tid % TX + 1];
+        }
+        return;
+    }
+
     for (int i = 0; i < P.n_ions; ++i) {
         const double* cenv = A.c_env + (size_t)i * P.n_sq;
         const double* D = A.D_env + (size_t)i * P.n_sq;
         const double cb = P.c_bound[i];
         const double zi = P.zq[i];
-        // F(r,c): state with the world boundary reset to c_env_bound (Q3)
-        auto Fv = [&](int rr, int c2) -> double {
-            if (rr == 0 || rr == ny - 1 || c2 == 0 || c2 == nx - 1) return cb;
-            return cenv[rr * nx + c2];
-        };
-        auto gcx = [&](int rr, int c2) -> double {
-            if (c2 == 0) return (Fv(rr, 1) - Fv(rr, 0)) / dl;
-            if (c2 == nx - 1) return (Fv(rr, nx - 1) - Fv(rr, nx - 2)) / dl;
-            return -(Fv(rr, c2 - 1) - Fv(rr, c2 + 1)) / d2;
-        };
-        auto gcy = [&](int rr, int c2) -> double {
-            if (rr == 0) return (Fv(1, c2) - Fv(0, c2)) / dl;
-            if (rr == ny - 1) return (Fv(ny - 1, c2) - Fv(ny - 2, c2)) / dl;
-            return -(Fv(rr - 1, c2) - Fv(rr + 1, c2)) / d2;
-        };
-        // nernst_planck_flux with gv = -E, u = 0, mu = 0 (sim_toolbox.py:409-411)
-        auto fxat = [&](int rr, int c2) -> double {
-            const int kk = rr * nx + c2;
-            const double Dv = D[kk];
-            const double alpha = (Dv * zi * P.q) / P.kbT;
-            const double gvx = -A.Ex[kk];
-            return (-Dv) * gcx(rr, c2) - (alpha * gvx) * Fv(rr, c2);
-        };
-        auto fyat = [&](int rr, int c2) -> double {
-            const int kk = rr * nx + c2;
-            const double Dv = D[kk];
-            const double alpha = (Dv * zi * P.q) / P.kbT;
-            const double gvy = -A.Ey[kk];
-            return (-Dv) * gcy(rr, c2) - (alpha * gvy) * Fv(rr, c2);
-        };
-        // fd.divergence(-fx, -fy): diff axis 0 / axis 1 with flipped boundary signs (Q4)
-        double gx, gy;
-        if (cc == 0) gx = ((-fxat(r, 0)) - (-fxat(r, 1))) / dl;
-        else if (cc == nx - 1) gx = ((-fxat(r, nx - 2)) - (-fxat(r, nx - 1))) / dl;
-        else gx = -((-fxat(r, cc - 1)) - (-fxat(r, cc + 1))) / d2;
-        if (r == 0) gy = -((-fyat(1, cc)) - (-fyat(0, cc))) / dl;
-        else if (r == ny - 1) gy = -((-fyat(ny - 1, cc)) - (-fyat(ny - 2, cc))) / dl;
-        else gy = -((-fyat(r - 1, cc)) - (-fyat(r + 1, cc))) / d2;
-        const double div = gx + gy;
-        A.c_mid[(size_t)i * P.n_sq + idx] = Fv(r, cc) + div * P.dt;
-        if (A.fex) {
-            A.fex[(size_t)i * P.n_sq + idx] = fxat(r, cc);
-            A.fey[(size_t)i * P.n_sq + idx] = fyat(r, cc);
+        __syncthreads();
+        // concentrations with the world boundary reset to c_env_bound (Q3)
+        for (int e = tid; e < VR * VC; e += TY * TX) {
+            const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
+            double val = 0.0;
+            if (r >= 0 && r < ny && cc >= 0 && cc < nx)
+                val = (r == 0 || r == ny - 1 || cc == 0 || cc == nx - 1) ? cb : cenv[r * nx + cc];
+            sC[e / VC][e % VC] = val;
+        }
+        for (int e = tid; e < ER * EC; e += TY * TX) {
+            const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
+            sD[e / EC][e % EC] = (r >= 0 && r < ny && cc >= 0 && cc < nx) ? D[r * nx + cc] : 0.0;
+        }
+        __syncthreads();
+        // Nernst-Planck fluxes (sim_toolbox.py:409-411, gv = -E, u = 0, mu = 0)
+        for (int e = tid; e < TY * (TX + 2); e += TY * TX) {
+            const int lr = e / (TX + 2), lc = e % (TX + 2);  // fx at (r0+lr, cc0-1+lc)
+            const int r = r0 + lr, cc = cc0 - 1 + lc;
+            double fx = 0.0;
+            if (r < ny && cc >= 0 && cc < nx) {
+                const int sr = lr + 2, sc = lc + 1;            // in sC
+                double g;
+                if (cc == 0) g = (sC[sr][sc + 1] - sC[sr][sc]) / dl;
+                else if (cc == nx - 1) g = (sC[sr][sc] - sC[sr][sc - 1]) / dl;
+                else g = -(sC[sr][sc - 1] - sC[sr][sc + 1]) / d2;
+                const double Dv = sD[lr + 1][lc];
+                const double alpha = (Dv * zi * P.q) / P.kbT;
+                fx = (-Dv) * g - (alpha * (-sEx[lr + 1][lc])) * sC[sr][sc];
+            }
+            sFx[lr][lc] = fx;
+        }
+        for (int e = tid; e < (TY + 2) * TX; e += TY * TX) {
+            const int lr = e / TX, lc = e % TX;              // fy at (r0-1+lr, cc0+lc)
+            const int r = r0 - 1 + lr, cc = cc0 + lc;
+            double fy = 0.0;
+            if (r >= 0 && r < ny && cc < nx) {
+                const int sr = lr + 1, sc = lc + 2;
+                double g;
+                if (r == 0) g = (sC[sr + 1][sc] - sC[sr][sc]) / dl;
+                else if (r == ny - 1) g = (sC[sr][sc] - sC[sr - 1][sc]) / dl;
+                else g = -(sC[sr - 1][sc] - sC[sr + 1][sc]) / d2;
+                const double Dv = sD[lr][lc + 1];
+                const double alpha = (Dv * zi * P.q) / P.kbT;
+                fy = (-Dv) * g - (alpha * (-sEy[lr][lc + 1])) * sC[sr][sc];
+            }
+            sFy[lr][lc] = fy;
+        }
+        __syncthreads();
+        // fd.divergence(-fx, -fy) with the sign-flipped one-sided edges (Q4), Euler step
+        {
+            const int lr = tid / TX, lc = tid % TX;
+            const int r = r0 + lr, cc = cc0 + lc;
+            if (r < ny && cc < nx) {
+                double gx, gy;
+                if (cc == 0) gx = ((-sFx[lr][lc + 1]) - (-sFx[lr][lc + 2])) / dl;
+                else if (cc == nx - 1) gx = ((-sFx[lr][lc]) - (-sFx[lr][lc + 1])) / dl;
+                else gx = -((-sFx[lr][lc]) - (-sFx[lr][lc + 2])) / d2;
+                if (r == 0) gy = -((-sFy[lr + 2][lc]) - (-sFy[lr + 1][lc])) / dl;
+                else if (r == ny - 1) gy = -((-sFy[lr + 1][lc]) - (-sFy[lr][lc])) / dl;
+                else gy = -((-sFy[lr][lc]) - (-sFy[lr + 2][lc])) / d2;
+                const double div = gx + gy;
+                A.c_mid[(size_t)i * P.n_sq + r * nx + cc] = sC[lr + 2][lc + 2] + div * P.dt;
+                if (A.obs_fx) {
+                    A.obs_fx[(size_t)i * P.n_sq + r * nx + cc] = sFx[lr][lc + 1];
+                    A.obs_fy[(size_t)i * P.n_sq + r * nx + cc] = sFy[lr + 1][lc];
+                }
+            }
         }
     }
 }
@@ -318,22 +490,30 @@ k_env_diffuse(EnvArgs A, KParams P) {
 // k_env_exchange: membrane -> env exchange, rho_env -> surface charge
 // ---------------------------------------------------------------------------
 struct ExArgs {
-    const int32_t* sq_ptr; const int32_t* sq_mems; const uint8_t* has_mem;
-    const double* c_mid; const double* fsa; double* c_env; double* u; const unsigned* flags;
+    const int32_t* sq_ptr; const uint8_t* has_mem;
+    const double* c_mid; const double* fsa; const double* fch; double* c_env; double* u;
+    const unsigned* flags; long long* step_ctr; int count_step;
 };
 
 __global__ void __launch_bounds__(BLOCK)
 k_env_exchange(ExArgs A, KParams P) {
     if (halted(A.flags)) return;
     const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (A.count_step && k == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
     if (k >= P.n_sq) return;
     const int a = A.sq_ptr[k], b = A.sq_ptr[k + 1];
     double rho = 0.0;
     for (int i = 0; i < P.n_ions; ++i) {
+        double cn = A.c_mid[(size_t)i * P.n_sq + k];
+        const int ch = P.ion_chan[i];
+        if (ch >= 0) {
+            double Sc = 0.0;
+            for (int j = a; j < b; ++j) Sc = Sc + (-A.fch[(size_t)ch * P.n_mems + j]);
+            cn = cn + (Sc / P.env_vol) * P.dt;
+        }
         double S = 0.0;
-        for (int j = a; j < b; ++j) S = S + (-A.fsa[(size_t)i * P.n_mems + A.sq_mems[j]]);
-        const double delta_env = S / P.env_vol;
-        const double cn = A.c_mid[(size_t)i * P.n_sq + k] + delta_env * P.dt;
+        for (int j = a; j < b; ++j) S = S + (-A.fsa[(size_t)i * P.n_mems + j]);
+        cn = cn + (S / P.env_vol) * P.dt;
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         rho = (i == 0) ? P.zF[0] * cn : rho + P.zF[i] * cn;
     }
@@ -342,7 +522,7 @@ k_env_exchange(ExArgs A, KParams P) {
     A.u[k] = uu;
 }
 
-// rho_env -> u for the current env state (used before a phase's loop)
+// rho_env -> u for the current env state (before a phase's loop)
 __global__ void __launch_bounds__(BLOCK)
 k_env_charge(const double* c_env, const uint8_t* has_mem, double* u, KParams P) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
@@ -369,55 +549,174 @@ k_cell_vm(const double* c_cell, const double* diviterm, double* vm, KParams P) {
 }
 
 // ---------------------------------------------------------------------------
-// Gaussian (scipy correlate1d symmetric order: centre, then pairs outermost
-// first) along axis 0, then axis 1 + gradient.
+// sampled-step observables
 // ---------------------------------------------------------------------------
+struct ObsMemArgs {
+    const int32_t* mem_ptr; const int32_t* nn_mem; const double* mem_sa; const double* mem_nx;
+    const double* mem_ny; const double* cell_sa; const double* vm_cell; const int32_t* mem_cell;
+    const double* fm; const double* fgj; const double* fch;
+    double* Jn; double* I_mem; double* E_gj_x; double* E_gj_y;
+    double* J_cell_x; double* J_cell_y; double* vm_ave;
+    double smooth_cells;
+};
+
+// get_current cell/membrane part (ion_current.py:18-47), E_gj (sim.py:2166-2172),
+// vm_ave (sim.py:2038); one thread per cell, membranes in order
 __global__ void __launch_bounds__(BLOCK)
-k_gauss_rows(const double* u, double* t, const unsigned* flags, KParams P) {
-    if (halted(flags)) return;
+k_obs_mem(ObsMemArgs A, KParams P) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= P.n_cells) return;
+    const int a = A.mem_ptr[c], b = A.mem_ptr[c + 1];
+    const int M = P.n_mems;
+    auto jn_raw = [&](int m) -> double {
+        double extra = 0.0;                       // extra_J_mem: channel currents
+        for (int ch = 0; ch < P.n_chan; ++ch) {
+            const int ion = P.chan_ion[ch];
+            extra = extra + ((-A.fch[(size_t)ch * M + m]) * P.F) * P.zq[ion];
+        }
+        double d = P.zF[0] * A.fm[m];
+        double g = P.zF[0] * A.fgj[m];
+        for (int i = 1; i < P.n_ions; ++i) {
+            d = d + P.zF[i] * A.fm[(size_t)i * M + m];
+            g = g + P.zF[i] * A.fgj[(size_t)i * M + m];
+        }
+        return (-d + extra) + g;
+    };
+    double s = 0.0;
+    for (int m = a; m < b; ++m) s = (m == a) ? jn_raw(m) * A.mem_sa[m] : s + jn_raw(m) * A.mem_sa[m];
+    const double jn_ave = s / A.cell_sa[c];
+    const double nm = A.smooth_cells * (double)(b - a);
+    const double w_mem = (nm - 1) / nm, w_o = 1 / nm;
+    double sx = 0.0, sy = 0.0, svm = 0.0;
+    const double v = A.vm_cell[c];
+    for (int m = a; m < b; ++m) {
+        const double jn = w_mem * jn_raw(m) + jn_ave * w_o;
+        A.Jn[m] = jn;
+        A.I_mem[m] = -jn * A.mem_sa[m];
+        const double jx = (jn * A.mem_nx[m]) * A.mem_sa[m];
+        const double jy = (jn * A.mem_ny[m]) * A.mem_sa[m];
+        sx = (m == a) ? jx : sx + jx;
+        sy = (m == a) ? jy : sy + jy;
+        svm = (m == a) ? v : svm + v;
+        const double vgj = A.vm_cell[A.mem_cell[A.nn_mem[m]]] - v;
+        const double egj = -vgj / P.gj_len;
+        A.E_gj_x[m] = egj * A.mem_nx[m];
+        A.E_gj_y[m] = egj * A.mem_ny[m];
+    }
+    A.J_cell_x[c] = sx / A.cell_sa[c];
+    A.J_cell_y[c] = sy / A.cell_sa[c];
+    A.vm_ave[c] = svm / (double)(b - a);
+}
+
+// env current J_env_o = F*sum(z*fluxes_env) (ion_current.py:53-58) and the two
+// right-hand sides of HH_Decomp (sim_toolbox.py:1253-1282) on the interior,
+// zero-padded to the GEMM tile (boundary values are zero without bound_V)
+__global__ void __launch_bounds__(BLOCK)
+k_obs_env_rhs(const double* fex, const double* fey, double* jxo, double* jyo, KParams P) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if (k >= P.n_sq) return;
-    const int nx = P.nx, ny = P.ny;
-    const int r = k / nx, c = k % nx;
-    double o = u[k] * P.gw[0];
-    for (int j = 4; j >= 1; --j) {
-        const double lo = (r - j >= 0) ? u[(r - j) * nx + c] : 0.0;
-        const double hi = (r + j < ny) ? u[(r + j) * nx + c] : 0.0;
-        o = o + (lo + hi) * P.gw[j];
+    double x = (P.F * P.zq[0]) * fex[k], y = (P.F * P.zq[0]) * fey[k];
+    for (int i = 1; i < P.n_ions; ++i) {
+        x = x + (P.F * P.zq[i]) * fex[(size_t)i * P.n_sq + k];
+        y = y + (P.F * P.zq[i]) * fey[(size_t)i * P.n_sq + k];
     }
-    t[k] = o;
+    jxo[k] = x;
+    jyo[k] = y;
+}
+
+// fd.divergence of (Fx, Fy) at interior square (r, c) (finitediff.py:1268-1311)
+__device__ __forceinline__ double div_at(const double* Fx, const double* Fy, int r, int c, int ny, int nx,
+                                         double dl, double d2, double sx, double sy) {
+    auto X = [&](int rr, int cc) { return sx * Fx[rr * nx + cc]; };
+    auto Y = [&](int rr, int cc) { return sy * Fy[rr * nx + cc]; };
+    double gx, gy;
+    if (c == 0) gx = (X(r, 0) - X(r, 1)) / dl;
+    else if (c == nx - 1) gx = (X(r, nx - 2) - X(r, nx - 1)) / dl;
+    else gx = -(X(r, c - 1) - X(r, c + 1)) / d2;
+    if (r == 0) gy = -(Y(1, c) - Y(0, c)) / dl;
+    else if (r == ny - 1) gy = -(Y(ny - 1, c) - Y(ny - 2, c)) / dl;
+    else gy = -(Y(r - 1, c) - Y(r + 1, c)) / d2;
+    return gx + gy;
+}
+
+// rhs of the stream-function solve (-div(-Jy, Jx)) and of the potential solve
+// (div(Jx, Jy)) on the interior, written row-major into ld-wide padded matrices
+__global__ void __launch_bounds__(BLOCK)
+k_obs_rhs(const double* jxo, const double* jyo, double* b1, double* b2, int ld, KParams P) {
+    const int n1 = P.ny - 2, n2 = P.nx - 2;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n1 * n2) return;
+    const int r = e / n2 + 1, c = e % n2 + 1;
+    // Jxr = -Jy, Jyr = Jx
+    const double dr = div_at(jyo, jxo, r, c, P.ny, P.nx, P.delta, P.two_delta, -1.0, 1.0);
+    const double dd = div_at(jxo, jyo, r, c, P.ny, P.nx, P.delta, P.two_delta, 1.0, 1.0);
+    b1[(r - 1) * ld + (c - 1)] = -dr;
+    b2[(r - 1) * ld + (c - 1)] = dd;
 }
 
 __global__ void __launch_bounds__(BLOCK)
-k_gauss_cols_grad(const double* t, double* v_env, double* Ex, double* Ey,
-                  unsigned* flags, long long* step_ctr, int count_step, KParams P) {
-    if (halted(flags)) return;
+k_scale(double* x, const double* s, int n) {
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e < n) x[e] = x[e] * s[e];
+}
+
+// interior solution -> full grid (boundary rows of A are identity/delta^2 and
+// their right-hand sides are zero here)
+__global__ void __launch_bounds__(BLOCK)
+k_scatter_interior(const double* X, int ld, double* out, KParams P) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (count_step && k == 0) step_ctr[0] = step_ctr[0] + 1;
     if (k >= P.n_sq) return;
-    const int nx = P.nx, ny = P.ny;
-    const int r = k / nx, c = k % nx;
-    auto vat = [&](int rr, int c2) -> double {
-        const double* row = t + rr * nx;
-        double o = row[c2] * P.gw[0];
-        for (int j = 4; j >= 1; --j) {
-            const double lo = (c2 - j >= 0) ? row[c2 - j] : 0.0;
-            const double hi = (c2 + j < nx) ? row[c2 + j] : 0.0;
-            o = o + (lo + hi) * P.gw[j];
-        }
-        return o;
+    const int r = k / P.nx, c = k % P.nx;
+    const bool bnd = (r == 0 || c == 0 || r == P.ny - 1 || c == P.nx - 1);
+    out[k] = bnd ? 0.0 : X[(r - 1) * ld + (c - 1)];
+}
+
+// gradients of the stream function AA and potential BB -> HH_Decomp outputs
+__global__ void __launch_bounds__(BLOCK)
+k_obs_hh_out(const double* AA, const double* BB, double* Jx, double* Jy, double* Jtx, double* Jty,
+             double* Bf, double mu, KParams P) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= P.n_sq) return;
+    const int ny = P.ny, nx = P.nx, r = k / nx, c = k % nx;
+    auto grad = [&](const double* F, double& gx, double& gy) {
+        if (c == 0) gx = (F[r * nx + 1] - F[r * nx]) / P.delta;
+        else if (c == nx - 1) gx = (F[r * nx + nx - 1] - F[r * nx + nx - 2]) / P.delta;
+        else gx = -(F[r * nx + c - 1] - F[r * nx + c + 1]) / P.two_delta;
+        if (r == 0) gy = (F[nx + c] - F[c]) / P.delta;
+        else if (r == ny - 1) gy = (F[(ny - 1) * nx + c] - F[(ny - 2) * nx + c]) / P.delta;
+        else gy = -(F[(r - 1) * nx + c] - F[(r + 1) * nx + c]) / P.two_delta;
     };
-    auto Fs = [&](int rr, int c2) -> double { return P.screen * vat(rr, c2); };
-    double gx, gy;
-    if (c == 0) gx = (Fs(r, 1) - Fs(r, 0)) / P.delta;
-    else if (c == nx - 1) gx = (Fs(r, nx - 1) - Fs(r, nx - 2)) / P.delta;
-    else gx = -(Fs(r, c - 1) - Fs(r, c + 1)) / P.two_delta;
-    if (r == 0) gy = (Fs(1, c) - Fs(0, c)) / P.delta;
-    else if (r == ny - 1) gy = (Fs(ny - 1, c) - Fs(ny - 2, c)) / P.delta;
-    else gy = -(Fs(r - 1, c) - Fs(r + 1, c)) / P.two_delta;
-    v_env[k] = vat(r, c);
-    Ex[k] = -gx;
-    Ey[k] = -gy;
+    double gAx, gAy, Gx, Gy;
+    grad(AA, gAx, gAy);
+    grad(BB, Gx, Gy);
+    const double Fx = -gAy, Fy = gAx;
+    Jx[k] = Fx;
+    Jy[k] = Fy;
+    Jtx[k] = Fx + Gx;
+    Jty[k] = Fy + Gy;
+    Bf[k] = AA[k] * mu;
+}
+
+// C = A*B (row-major, all sizes multiples of 16) with v_mfma_f64_16x16x4f64:
+// one wavefront per 16x16 tile of C. Lane l holds A[l%16][k0+l/16] and
+// B[k0+l/16][l%16]; result element r of lane l is C[4*(l/16)+r][l%16].
+typedef double v4d __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256)
+k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int tiles_n = n / 16;
+    const int tile = blockIdx.x * 4 + wave;
+    if (tile >= (m / 16) * tiles_n) return;
+    const int tm = tile / tiles_n, tn = tile % tiles_n;
+    const int row = tm * 16 + (lane % 16), col = tn * 16 + (lane % 16), kk = lane / 16;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < kd; k0 += 4) {
+        const double a = A[(size_t)row * kd + k0 + kk];
+        const double b = B[(size_t)(k0 + kk) * n + col];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) C[(size_t)(tm * 16 + 4 * kk + r) * n + col] = acc[r];
 }
 
 }  // namespace
@@ -434,10 +733,16 @@ struct bt_ctx {
     int n_tiles = 0;
     // workspace
     double* c_cell_b = nullptr; double* c_lag_b = nullptr; double* vm_b = nullptr;
-    double* c_mid = nullptr; double* fsa = nullptr; double* u = nullptr; double* tmp = nullptr;
+    double* c_mid = nullptr; double* fsa = nullptr; double* fch = nullptr; double* u = nullptr;
     unsigned* flags = nullptr; unsigned long long* bad_step = nullptr; long long* step_ctr = nullptr;
     hipGraph_t graph = nullptr; hipGraphExec_t gexec = nullptr;
     long long steps_done = 0;
+    const bt_obs* capture = nullptr;   // set while enqueueing an observed step
+    // observables workspace (allocated on first use)
+    int n1 = 0, n2 = 0, n1p = 0, n2p = 0;
+    double* S1 = nullptr; double* S2 = nullptr; double* invlam = nullptr;
+    double* jxo = nullptr; double* jyo = nullptr; double* B1 = nullptr; double* B2 = nullptr;
+    double* T1 = nullptr; double* T2 = nullptr; double* AA = nullptr; double* BB = nullptr;
     std::string err;
 };
 
@@ -460,6 +765,15 @@ static void derive_params(const bt_params& p, KParams& k) {
     k.n_cells = p.n_cells; k.n_mems = p.n_mems; k.ny = p.grid_ny; k.nx = p.grid_nx;
     k.n_sq = p.grid_ny * p.grid_nx; k.n_ions = p.n_ions;
     k.i_na = p.i_na; k.i_k = p.i_k; k.i_ca = p.i_ca;
+    k.n_chan = p.n_chan;
+    for (int i = 0; i < BT_MAX_IONS; ++i) k.ion_chan[i] = -1;
+    for (int c = 0; c < p.n_chan; ++c) {
+        k.chan_kind[c] = p.chan_kind[c];
+        k.chan_ion[c] = p.chan_ion[c];
+        k.chan_max_dm[c] = p.chan_max_dm[c];
+        k.ion_chan[p.chan_ion[c]] = c;
+    }
+    k.chan_dt = p.chan_dt;
     for (int i = 0; i < p.n_ions; ++i) {
         k.zplus[i] = p.z[i] + FLOAT_NONCE;
         k.zF[i] = p.z[i] * p.F;
@@ -495,9 +809,9 @@ static void derive_params(const bt_params& p, KParams& k) {
 template <int NI>
 static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a) {
     if (c->prm.flux_mode == BT_FLUX_FP32)
-        hipLaunchKernelGGL((k_mem_cell<NI, float>), dim3(c->n_tiles), dim3(TILE_M), 0, c->stream, a, c->kp);
+        hipLaunchKernelGGL((k_mem_cell<NI, float>), dim3(c->n_tiles), dim3(NI * TMB), 0, c->stream, a, c->kp);
     else
-        hipLaunchKernelGGL((k_mem_cell<NI, double>), dim3(c->n_tiles), dim3(TILE_M), 0, c->stream, a, c->kp);
+        hipLaunchKernelGGL((k_mem_cell<NI, double>), dim3(c->n_tiles), dim3(NI * TMB), 0, c->stream, a, c->kp);
 }
 
 static void launch_mem_cell(bt_ctx* c, const MemCellArgs& a) {
@@ -515,7 +829,19 @@ static void launch_mem_cell(bt_ctx* c, const MemCellArgs& a) {
 
 static inline int grid_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
+static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr) {
+    const bt_buffers& B = c->buf;
+    FieldArgs f{c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
+                ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr};
+    dim3 grid((c->kp.nx + TX - 1) / TX, (c->kp.ny + TY - 1) / TY);
+    if (diffuse)
+        hipLaunchKernelGGL(k_env_field<true>, grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+    else
+        hipLaunchKernelGGL(k_env_field<false>, grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+}
+
 // One step. parity 0: cell state caller -> workspace; parity 1: workspace -> caller.
+// which: -1 all, 0 membrane/cell, 1 env field+diffusion, 2 exchange, 3 field only
 static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     const bt_buffers& B = c->buf;
     const int nsq = c->kp.n_sq;
@@ -526,30 +852,23 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     double* lo = parity ? B.c_lag : c->c_lag_b;
     double* vo = parity ? B.vm_cell : c->vm_b;
 
-    if (which < 0 || which == 1) {
-        EnvArgs e{B.c_env, B.D_env, B.E_x, B.E_y, c->c_mid, nullptr, nullptr, c->flags};
-        if (c->prm.flux_mode == BT_FLUX_FP32)
-            hipLaunchKernelGGL((k_env_diffuse<float>), dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, e, c->kp);
-        else
-            hipLaunchKernelGGL((k_env_diffuse<double>), dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, e, c->kp);
-    }
+    const bt_obs* ob = c->capture;
+    if (which < 0 || which == 1) launch_field(c, true, ob);
     if (which < 0 || which == 0) {
-        MemCellArgs a{B.mem_ptr, B.mem_cell, B.nn_mem, B.nn_cell, B.mem_sq, B.tile_ptr, B.mem_sa, B.gjopen,
-                      B.cell_vol, B.diviterm, B.Dm, B.c_env, c->c_mid, ci, li, vi, co, lo, vo,
-                      c->fsa, c->flags, c->bad_step, c->step_ctr};
+        MemCellArgs a{B.mem_ptr, B.mem_cell, B.nn_mem, B.nn_cell, B.mem_sq, B.mem_pos, B.tile_ptr,
+                      B.mem_sa, B.gjopen, B.cell_vol, B.diviterm, B.Dm, B.c_env, c->c_mid,
+                      ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->fch,
+                      c->flags, c->bad_step, c->step_ctr,
+                      ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
+                      ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr};
         launch_mem_cell(c, a);
     }
     if (which < 0 || which == 2) {
-        ExArgs x{B.sq_ptr, B.sq_mems, B.sq_has_mem, c->c_mid, c->fsa, B.c_env, c->u, c->flags};
+        ExArgs x{B.sq_ptr, B.sq_has_mem, c->c_mid, c->fsa, c->fch, B.c_env, c->u, c->flags,
+                 c->step_ctr, which < 0 ? 1 : 0};
         hipLaunchKernelGGL(k_env_exchange, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, x, c->kp);
     }
-    if (which < 0 || which == 3)
-        hipLaunchKernelGGL(k_gauss_rows, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream,
-                           (const double*)c->u, c->tmp, (const unsigned*)c->flags, c->kp);
-    if (which < 0 || which == 4)
-        hipLaunchKernelGGL(k_gauss_cols_grad, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream,
-                           (const double*)c->tmp, B.v_env, B.E_x, B.E_y, c->flags, c->step_ctr,
-                           which < 0 ? 1 : 0, c->kp);
+    if (which == 3) launch_field(c, false);
 }
 
 static int copy_back(bt_ctx* c) {
@@ -563,7 +882,7 @@ static int copy_back(bt_ctx* c) {
 
 extern "C" {
 
-int bt_version(void) { return 100; }
+int bt_version(void) { return 200; }
 
 const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
@@ -582,7 +901,19 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_buffers* b, void* strea
     if (p->flux_mode != BT_FLUX_FP64 && p->flux_mode != BT_FLUX_FP32)
         return set_err(nullptr, BT_EINVAL, "unknown flux_mode");
     if (b->n_tiles < 1) return set_err(nullptr, BT_EINVAL, "n_tiles < 1");
-    const void* req[] = {b->mem_ptr, b->mem_cell, b->nn_mem, b->nn_cell, b->mem_sq, b->sq_ptr, b->sq_mems,
+    if (b->tile_m != BT_TILE_MEMBRANES) return set_err(nullptr, BT_EINVAL, "tiles must hold <= 64 membranes");
+    if (p->n_chan < 0 || p->n_chan > BT_MAX_CHAN) return set_err(nullptr, BT_EINVAL, "n_chan out of range");
+    for (int c = 0; c < p->n_chan; ++c) {
+        if (p->chan_kind[c] != BT_CHAN_NAV1P2 && p->chan_kind[c] != BT_CHAN_KV1P1)
+            return set_err(nullptr, BT_EINVAL, "unknown channel kind");
+        if (p->chan_ion[c] < 0 || p->chan_ion[c] >= p->n_ions)
+            return set_err(nullptr, BT_EINVAL, "channel ion slot out of range");
+        for (int d = 0; d < c; ++d)
+            if (p->chan_ion[d] == p->chan_ion[c]) return set_err(nullptr, BT_EINVAL, "two channels on one ion");
+    }
+    if (p->n_chan > 0 && (!b->gates || !b->chan_mod))
+        return set_err(nullptr, BT_EINVAL, "channel buffers missing");
+    const void* req[] = {b->mem_ptr, b->mem_cell, b->nn_mem, b->nn_cell, b->mem_sq, b->mem_pos, b->sq_ptr,
                          b->tile_ptr, b->mem_sa, b->gjopen, b->cell_vol, b->diviterm, b->Dm, b->D_env,
                          b->sq_has_mem, b->c_cell, b->c_lag, b->vm_cell, b->c_env, b->E_x, b->E_y, b->v_env};
     for (const void* q : req)
@@ -605,14 +936,15 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_buffers* b, void* strea
     alloc((void**)&c->vm_b, nc * 8);
     alloc((void**)&c->c_mid, ni * ns * 8);
     alloc((void**)&c->fsa, ni * nm * 8);
+    alloc((void**)&c->fch, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * nm * 8);
     alloc((void**)&c->u, ns * 8);
-    alloc((void**)&c->tmp, ns * 8);
     alloc((void**)&c->flags, 256);
     alloc((void**)&c->bad_step, 256);
     alloc((void**)&c->step_ctr, 256);
     if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->bad_step, 0xff, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->step_ctr, 0, 256, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->u, 0, ns * 8, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         std::string msg = std::string("workspace allocation: ") + hipGetErrorString(e);
@@ -628,8 +960,9 @@ int bt_destroy(bt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->u, c->tmp,
-                    c->flags, c->bad_step, c->step_ctr};
+    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u,
+                    c->flags, c->bad_step, c->step_ctr, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
+                    c->B1, c->B2, c->T1, c->T2, c->AA, c->BB};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     delete c;
@@ -643,8 +976,7 @@ int bt_update_v(bt_ctx* c) {
                        (const double*)c->buf.c_cell, c->buf.diviterm, c->buf.vm_cell, c->kp);
     hipLaunchKernelGGL(k_env_charge, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream,
                        (const double*)c->buf.c_env, c->buf.sq_has_mem, c->u, c->kp);
-    enqueue_step(c, 0, 3);
-    enqueue_step(c, 0, 4);
+    launch_field(c, false);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }
@@ -652,6 +984,7 @@ int bt_update_v(bt_ctx* c) {
 int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     if (!c) return BT_EINVAL;
     if (n_steps < 0) return set_err(c, BT_EINVAL, "n_steps < 0");
+    if (n_steps == 0) return BT_OK;
     int pairs = n_steps / 2;
     if (use_graph && pairs > 0) {
         if (!c->gexec) {
@@ -674,6 +1007,8 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
         int rc = copy_back(c);
         if (rc != BT_OK) return rc;
     }
+    // v_env and E_env of the final update_V, for the host
+    enqueue_step(c, 0, 3);
     HIP_TRY(c, hipGetLastError());
     c->steps_done += n_steps;
     return BT_OK;
@@ -692,7 +1027,7 @@ int bt_sync(bt_ctx* c, int64_t* bad_step) {
 }
 
 int bt_time_kernel(bt_ctx* c, int32_t kernel, int32_t n_launches, double* ms_out) {
-    if (!c || !ms_out || n_launches < 1 || kernel < 0 || kernel > 4) return BT_EINVAL;
+    if (!c || !ms_out || n_launches < 1 || kernel < 0 || kernel > 3) return BT_EINVAL;
     hipEvent_t e0, e1;
     HIP_TRY(c, hipEventCreate(&e0));
     HIP_TRY(c, hipEventCreate(&e1));
@@ -706,6 +1041,144 @@ int bt_time_kernel(bt_ctx* c, int32_t kernel, int32_t n_launches, double* ms_out
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *ms_out = (double)ms / n_launches;
+    return BT_OK;
+}
+
+int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs* obs) {
+    if (!c || !obs) return BT_EINVAL;
+    if (n_steps < 1) return set_err(c, BT_EINVAL, "an observed run needs at least one step");
+    if (!obs->fluxes_mem || !obs->fluxes_gj || !obs->fluxes_env_x || !obs->fluxes_env_y ||
+        !obs->rate_NaKATP || (c->prm.n_chan > 0 && !obs->chan_flux))
+        return set_err(c, BT_EINVAL, "observation buffers missing");
+    if (c->prm.i_na < 0)
+        HIP_TRY(c, hipMemsetAsync(obs->rate_NaKATP, 0, (size_t)c->prm.n_mems * 8, c->stream));
+    const int ahead = n_steps - 1;
+    if (ahead > 0) {
+        // keep the graph on the caller buffers: an even number of graph steps, then eager ones
+        int rc = bt_step(c, ahead - (ahead & 1), use_graph);
+        if (rc != BT_OK) return rc;
+    }
+    int rest = (ahead & 1) + 1;           // 1 or 2 eager steps, the last one observed
+    if (rest == 2) {
+        enqueue_step(c, 0);
+        c->capture = obs;
+        enqueue_step(c, 1);
+        c->capture = nullptr;
+    } else {
+        c->capture = obs;
+        enqueue_step(c, 0);
+        c->capture = nullptr;
+        int rc = copy_back(c);
+        if (rc != BT_OK) return rc;
+    }
+    enqueue_step(c, 0, 3);
+    HIP_TRY(c, hipGetLastError());
+    c->steps_done += rest;
+    return BT_OK;
+}
+
+static int launch_gemm(bt_ctx* c, const double* A, const double* B, double* C, int m, int n, int k) {
+    const int tiles = (m / 16) * (n / 16);
+    hipLaunchKernelGGL(k_gemm_f64, dim3((tiles + 3) / 4), dim3(256), 0, c->stream, A, B, C, m, n, k);
+    HIP_TRY(c, hipGetLastError());
+    return BT_OK;
+}
+
+int bt_gemm_f64(bt_ctx* c, const double* A, const double* B, double* C, int32_t m, int32_t n, int32_t k) {
+    if (!c || !A || !B || !C || m < 16 || n < 16 || k < 16 || (m | n | k) % 16)
+        return set_err(c, BT_EINVAL, "gemm sizes must be positive multiples of 16");
+    return launch_gemm(c, A, B, C, m, n, k);
+}
+
+// Dirichlet Poisson solve on the interior by DST-I: X = S1 ((S1 B S2) .* invlam) S2
+static int dst_solve(bt_ctx* c, double* Bm, double* out) {
+    int rc;
+    if ((rc = launch_gemm(c, c->S1, Bm, c->T1, c->n1p, c->n2p, c->n1p))) return rc;
+    if ((rc = launch_gemm(c, c->T1, c->S2, c->T2, c->n1p, c->n2p, c->n2p))) return rc;
+    const int nn = c->n1p * c->n2p;
+    hipLaunchKernelGGL(k_scale, dim3(grid_blocks(nn)), dim3(BLOCK), 0, c->stream, c->T2, c->invlam, nn);
+    if ((rc = launch_gemm(c, c->S1, c->T2, c->T1, c->n1p, c->n2p, c->n1p))) return rc;
+    if ((rc = launch_gemm(c, c->T1, c->S2, c->T2, c->n1p, c->n2p, c->n2p))) return rc;
+    hipLaunchKernelGGL(k_scatter_interior, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
+                       (const double*)c->T2, c->n2p, out, c->kp);
+    HIP_TRY(c, hipGetLastError());
+    return BT_OK;
+}
+
+static int obs_workspace(bt_ctx* c) {
+    if (c->S1) return BT_OK;
+    const int n1 = c->kp.ny - 2, n2 = c->kp.nx - 2;
+    const int n1p = (n1 + 15) / 16 * 16, n2p = (n2 + 15) / 16 * 16;
+    c->n1 = n1; c->n2 = n2; c->n1p = n1p; c->n2p = n2p;
+    const size_t ns = (size_t)c->kp.n_sq;
+    HIP_TRY(c, hipMalloc((void**)&c->S1, (size_t)n1p * n1p * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->S2, (size_t)n2p * n2p * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->invlam, (size_t)n1p * n2p * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->B1, (size_t)n1p * n2p * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->B2, (size_t)n1p * n2p * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->T1, (size_t)n1p * n2p * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->T2, (size_t)n1p * n2p * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->jxo, ns * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->jyo, ns * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->AA, ns * 8));
+    HIP_TRY(c, hipMalloc((void**)&c->BB, ns * 8));
+    // DST-I basis S[j][k] = sin(pi (j+1)(k+1) / (n+1)), zero in the padding;
+    // eigenvalues of the 1-D second difference 2 cos(pi (j+1)/(n+1)) - 2
+    std::string err;
+    double* h1 = new double[(size_t)n1p * n1p]();
+    double* h2 = new double[(size_t)n2p * n2p]();
+    double* hl = new double[(size_t)n1p * n2p]();
+    for (int j = 0; j < n1; ++j)
+        for (int k = 0; k < n1; ++k) h1[(size_t)j * n1p + k] = sin(M_PI * (j + 1) * (k + 1) / (n1 + 1));
+    for (int j = 0; j < n2; ++j)
+        for (int k = 0; k < n2; ++k) h2[(size_t)j * n2p + k] = sin(M_PI * (j + 1) * (k + 1) / (n2 + 1));
+    const double dl2 = c->prm.delta * c->prm.delta;
+    const double scale = 4.0 / ((double)(n1 + 1) * (double)(n2 + 1));
+    for (int j = 0; j < n1; ++j)
+        for (int k = 0; k < n2; ++k) {
+            const double lam = (2.0 * cos(M_PI * (j + 1) / (n1 + 1)) - 2.0) + (2.0 * cos(M_PI * (k + 1) / (n2 + 1)) - 2.0);
+            hl[(size_t)j * n2p + k] = dl2 * scale / lam;
+        }
+    hipError_t e = hipMemcpy(c->S1, h1, (size_t)n1p * n1p * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->S2, h2, (size_t)n2p * n2p * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->invlam, hl, (size_t)n1p * n2p * 8, hipMemcpyHostToDevice);
+    delete[] h1;
+    delete[] h2;
+    delete[] hl;
+    if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("observables set-up: ") + hipGetErrorString(e));
+    return BT_OK;
+}
+
+int bt_observables(bt_ctx* c, const bt_obs* o) {
+    if (!c || !o) return BT_EINVAL;
+    const void* need[] = {o->fluxes_mem, o->fluxes_gj, o->fluxes_env_x, o->fluxes_env_y, o->Jn, o->I_mem,
+                          o->E_gj_x, o->E_gj_y, o->J_cell_x, o->J_cell_y, o->vm_ave, o->J_env_x, o->J_env_y,
+                          o->Jtx, o->Jty, o->B_field};
+    for (const void* q : need)
+        if (!q) return set_err(c, BT_EINVAL, "observable buffer missing");
+    if (c->prm.n_chan > 0 && !o->chan_flux) return set_err(c, BT_EINVAL, "chan_flux missing");
+    if (!c->buf.mem_nx || !c->buf.mem_ny || !c->buf.cell_sa)
+        return set_err(c, BT_EINVAL, "observables need mem_nx, mem_ny and cell_sa");
+    int rc = obs_workspace(c);
+    if (rc != BT_OK) return rc;
+    const bt_buffers& B = c->buf;
+    ObsMemArgs a{B.mem_ptr, B.nn_mem, B.mem_sa, B.mem_nx, B.mem_ny, B.cell_sa, B.vm_cell, B.mem_cell,
+                 o->fluxes_mem, o->fluxes_gj, o->chan_flux, o->Jn, o->I_mem, o->E_gj_x, o->E_gj_y,
+                 o->J_cell_x, o->J_cell_y, o->vm_ave, c->prm.smooth_cells};
+    hipLaunchKernelGGL(k_obs_mem, dim3(grid_blocks(c->kp.n_cells)), dim3(BLOCK), 0, c->stream, a, c->kp);
+    hipLaunchKernelGGL(k_obs_env_rhs, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
+                       (const double*)o->fluxes_env_x, (const double*)o->fluxes_env_y, c->jxo, c->jyo, c->kp);
+    const size_t pb = (size_t)c->n1p * c->n2p * 8;
+    HIP_TRY(c, hipMemsetAsync(c->B1, 0, pb, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->B2, 0, pb, c->stream));
+    hipLaunchKernelGGL(k_obs_rhs, dim3(grid_blocks(c->n1 * c->n2)), dim3(BLOCK), 0, c->stream,
+                       (const double*)c->jxo, (const double*)c->jyo, c->B1, c->B2, c->n2p, c->kp);
+    if ((rc = dst_solve(c, c->B1, c->AA))) return rc;
+    if ((rc = dst_solve(c, c->B2, c->BB))) return rc;
+    hipLaunchKernelGGL(k_obs_hh_out, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
+                       (const double*)c->AA, (const double*)c->BB, o->J_env_x, o->J_env_y, o->Jtx, o->Jty,
+                       o->B_field, c->prm.mu, c->kp);
+    HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }
 

@@ -157,7 +157,13 @@ class Parameters:
     world_scale: float = 1.25
 
     # ---- optional voltage-gated channels (configs with Nav/Kv) ----
-    channels: tuple = ()
+    channels: tuple = ()                 # ChannelSpec per channel, in network order
+    channels_init_active: bool = True    # 'init active' (networks.py:1054)
+    channel_seed: int = 1
+    channel_cell_mult: Optional[list] = None   # per channel: per-cell modulator, or None
+    channel_gates0: Optional[list] = None      # per channel: (m, h) per membrane, or None
+    channel_random_state: bool = False         # draw multipliers/gates from channel_seed
+    tissue_profiles: tuple = ()          # (cell indices, {ion: Dm}) applied after the default tissue
 
     # ------------------------------------------------------------------
     @property
@@ -224,7 +230,9 @@ class Parameters:
 
         ``tiny``   few dozen cells on a 16x16 grid (fast unit tests);
         ``small``  ~350 cells, 100 um disc, 50x50 grid, Na/K/Cl/P/M, dt 1e-3;
-        ``large``  ~23k cells, 800 um disc, 256x256 grid, +Ca, dt 1e-4.
+        ``large``  ~23k cells, 800 um disc, 256x256 grid, +Ca, Nav/Kv, dt 1e-4;
+        ``large_plain`` the same without channels; ``tiny_channels`` the
+        ``large`` physics on the ``tiny`` cluster.
         """
         p = cls()
         cell = {"Na": 8.0, "K": 125.0, "Cl": 10.0, "P": 100.0}
@@ -242,14 +250,23 @@ class Parameters:
             p.init_time_step = p.sim_time_step = 1.0e-3
             p.init_time_total = p.sim_time_total = 1.0
             p.init_time_sampling = p.sim_time_sampling = 0.1
-        elif name in ("large", "large_plain"):
-            p.cluster_radius = 800.0e-6
-            p.grid_size = 256
+        elif name in ("large", "large_plain", "tiny_channels"):
+            if name == "tiny_channels":
+                p.cluster_radius = 30.0e-6
+                p.grid_size = 16
+            else:
+                p.cluster_radius = 800.0e-6
+                p.grid_size = 256
             cell = dict(cell, Ca=1.0e-4)
             env = dict(env, Ca=1.0)
             p.init_time_step = p.sim_time_step = 1.0e-4
             p.init_time_total = p.sim_time_total = 1.0
             p.init_time_sampling = p.sim_time_sampling = 0.01
+            if name != "large_plain":
+                # Hodgkin-Huxley style Nav/Kv on every membrane, gates U[0,1],
+                # per-cell density multipliers log-uniform on [0.5, 2]
+                p.channels = (ChannelSpec("Nav1p2", 5.0e-17), ChannelSpec("Kv1p1", 5.0e-17))
+                p.channel_random_state = True
         else:
             raise ValueError(f"unknown preset {name!r}")
         p.set_ion_profile(cell, env, perms)

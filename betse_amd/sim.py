@@ -29,10 +29,11 @@ import numpy as np
 
 from . import _native
 from .cells import Cells
+from .channels import CHANNEL_KINDS, draw_channel_state, init_gates
 from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableException
 from .parameters import Parameters
 
-TILE_M = 256          # membranes per k_mem_cell block (matches the HIP source)
+TILE_M = _native.BT_TILE_MEMBRANES   # membranes per cell tile of k_mem_cell
 
 
 class SimPhaseKind(enum.Enum):
@@ -206,6 +207,35 @@ class Simulator:
         self.E_env_y = np.zeros(cells.X.shape)
         self.v_env = np.zeros(self.edl)
         self.Phi_b = np.zeros(self.edl)
+        self.init_channels(cells, p)
+
+    def init_channels(self, cells: Cells, p: Parameters) -> None:
+        """``Channel.init_channel`` (networks.py:6550-6628) for every configured
+        channel, on all membranes: gates from ``_init_state`` at the current
+        Vmem, or the workload's seeded draws."""
+        specs = tuple(getattr(p, "channels", ()))
+        if len(specs) > _native.BT_MAX_CHAN:
+            raise BetseSimException("too many channels")
+        if specs and p.channel_random_state and p.channel_gates0 is None:
+            draw_channel_state(p, self.cdl, self.mdl)
+        self.channel_specs = specs
+        self.chan_gates = np.zeros((len(specs), 2, self.mdl))
+        self.chan_mod = np.ones((len(specs), self.cdl))
+        ions = []
+        for k, spec in enumerate(specs):
+            ion = CHANNEL_KINDS[spec.kind][0]
+            if ion not in self.names:
+                raise BetseSimException(f"channel {spec.kind} needs ion {ion}")
+            if ion in ions:
+                raise BetseSimException("the GPU step supports one channel per ion")
+            ions.append(ion)
+            m, h = init_gates(spec.kind, self.vm)
+            if p.channel_gates0 is not None and p.channel_gates0[k] is not None:
+                m, h = p.channel_gates0[k]
+            self.chan_gates[k, 0] = m
+            self.chan_gates[k, 1] = h
+            if p.channel_cell_mult is not None and p.channel_cell_mult[k] is not None:
+                self.chan_mod[k] = p.channel_cell_mult[k]
 
     def initDenv(self, cells: Cells, p: Parameters) -> None:
         """``sim.py:2347-2400`` (ECM branch)."""
@@ -273,6 +303,15 @@ class Simulator:
         self.efield_ecm_x_time: List[np.ndarray] = []
         self.efield_ecm_y_time: List[np.ndarray] = []
         self.gjopen_time: List[np.ndarray] = []
+        self.efield_gj_x_time: List[np.ndarray] = []
+        self.efield_gj_y_time: List[np.ndarray] = []
+        self.I_cell_x_time: List[np.ndarray] = []
+        self.I_cell_y_time: List[np.ndarray] = []
+        self.I_mem_time: List[np.ndarray] = []
+        self.rate_NaKATP_time: List[np.ndarray] = []
+        self.I_tot_x_time: List[np.ndarray] = []
+        self.I_tot_y_time: List[np.ndarray] = []
+        self.Jn = None
 
     # ------------------------------------------------------------------
     # host views of the device state
@@ -318,6 +357,9 @@ class Simulator:
         d["mem_sq"] = self._t(cells.map_mem2ecm, i32)
         d["sq_ptr"] = self._t(cells.sq_ptr, i32)
         d["sq_mems"] = self._t(cells.sq_mems, i32)
+        pos = np.empty(cells.n_mems, dtype=np.int64)
+        pos[cells.sq_mems] = np.arange(cells.n_mems)
+        d["mem_pos"] = self._t(pos, i32)
         tiles = make_tiles(cells.mem_ptr)
         d["tile_ptr"] = self._t(tiles, i32)
         self.n_tiles = len(tiles) - 1
@@ -325,6 +367,9 @@ class Simulator:
         d["cell_vol"] = self._t(cells.cell_vol, f64)
         d["diviterm"] = self._t(cells.diviterm, f64)
         d["sq_has_mem"] = self._t((counts > 0).astype(np.uint8), torch.uint8)
+        d["mem_nx"] = self._t(cells.mem_vects_flat[:, 2], f64)
+        d["mem_ny"] = self._t(cells.mem_vects_flat[:, 3], f64)
+        d["cell_sa"] = self._t(cells.cell_sa, f64)
         if G2 != len(counts):
             raise BetseSimException("grid-side CSR does not match the grid")
         self._geom_key = cells
@@ -348,6 +393,9 @@ class Simulator:
             d["E_x"] = self._t(self.E_env_x.ravel(), f64)
             d["E_y"] = self._t(self.E_env_y.ravel(), f64)
             d["v_env"] = self._t(self.v_env, f64)
+            nch = len(self.channel_specs)
+            d["gates"] = self._t(self.chan_gates if nch else np.zeros((1, 2, 1)), f64)
+            d["chan_mod"] = self._t(self.chan_mod if nch else np.ones((1, 1)), f64)
             self.stream.synchronize()
 
         prm = _native.BtParams()
@@ -358,6 +406,13 @@ class Simulator:
         prm.i_k = self.names.index("K") if prm.i_na >= 0 else -1
         prm.i_ca = self.names.index("Ca") if ("Ca" in self.names and p.alpha_Ca > 0.0) else -1
         prm.flux_mode = _native.BT_FLUX_FP32 if self.flux_mode == "fp32" else _native.BT_FLUX_FP64
+        active = phase.kind is SimPhaseKind.SIM or getattr(p, "channels_init_active", True)
+        prm.n_chan = len(self.channel_specs) if active else 0
+        for k, spec in enumerate(self.channel_specs):
+            prm.chan_kind[k] = _native.BT_CHAN_KINDS[spec.kind]
+            prm.chan_ion[k] = self.names.index(CHANNEL_KINDS[spec.kind][0])
+            prm.chan_max_dm[k] = float(spec.max_dm)
+        prm.chan_dt = p.dt * 1.0e3        # time_unit of the supported channel models (ms)
         for i in range(prm.n_ions):
             prm.z[i] = float(self.zs[i])
             if not np.all(self.D_gj[i] == self.D_gj[i][0]):
@@ -377,13 +432,17 @@ class Simulator:
         prm.cell_radius, prm.true_cell_size = p.cell_radius, p.true_cell_size
         for j, w in enumerate(gaussian_taps(1.0)):
             prm.gauss_w[j] = float(w)
+        prm.smooth_cells = p.smooth_cells
+        prm.mu = p.mu
 
         buf = _native.BtBuffers()
-        for name in ("mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "tile_ptr",
+        for name in ("mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "mem_pos", "tile_ptr",
                      "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
-                     "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env"):
+                     "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "mem_nx", "mem_ny",
+                     "cell_sa", "gates", "chan_mod"):
             setattr(buf, name, d[name].data_ptr())
         buf.n_tiles = self.n_tiles
+        buf.tile_m = _native.BT_TILE_MEMBRANES
         self._prm, self._buf = prm, buf
         with torch.cuda.device(self.device):
             rc = self._lib.bt_create(ctypes.byref(self._ctx), ctypes.byref(prm), ctypes.byref(buf),
@@ -417,6 +476,8 @@ class Simulator:
         self.E_env_x = d["E_x"].cpu().numpy().reshape(shp).copy()
         self.E_env_y = d["E_y"].cpu().numpy().reshape(shp).copy()
         self.v_env = d["v_env"].cpu().numpy().copy()
+        if len(self.channel_specs):
+            self.chan_gates = d["gates"].cpu().numpy().copy()
 
     def _sync(self, phase_step0: int = 0) -> None:
         bad = ctypes.c_int64(-1)
@@ -430,6 +491,59 @@ class Simulator:
         self._check(rc)
 
     # ------------------------------------------------------------------
+    # sampled-step observables
+    # ------------------------------------------------------------------
+    def _obs_buffers(self):
+        if getattr(self, "_obs", None) is not None and self._obs_key == (self.cdl, self.mdl, self.edl):
+            return self._obs
+        torch = self.torch
+        f64 = torch.float64
+        ni, M, C, G2 = len(self.names), self.mdl, self.cdl, self.edl
+        nch = max(len(self.channel_specs), 1)
+        shapes = {"fluxes_mem": (ni, M), "fluxes_gj": (ni, M), "fluxes_env_x": (ni, G2), "fluxes_env_y": (ni, G2),
+                  "chan_flux": (nch, M), "rate_NaKATP": (M,), "Jn": (M,), "I_mem": (M,), "E_gj_x": (M,),
+                  "E_gj_y": (M,), "J_cell_x": (C,), "J_cell_y": (C,), "vm_ave": (C,), "J_env_x": (G2,),
+                  "J_env_y": (G2,), "Jtx": (G2,), "Jty": (G2,), "B_field": (G2,)}
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            t = {k: torch.zeros(v, dtype=f64, device=self.device) for k, v in shapes.items()}
+        obs = _native.BtObs()
+        for k in _native.OBS_FIELDS:
+            setattr(obs, k, t[k].data_ptr())
+        self._obs = (obs, t)
+        self._obs_key = (self.cdl, self.mdl, self.edl)
+        return self._obs
+
+    def _step_observed(self, n: int) -> None:
+        obs, _ = self._obs_buffers()
+        self._check(self._lib.bt_step_observed(self._ctx, int(n), 1 if self.use_graph else 0, ctypes.byref(obs)))
+
+    def _pull_observables(self) -> None:
+        """Observables of the last observed step (get_current, ion_current.py:13-109)."""
+        obs, t = self._obs_buffers()
+        self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
+        self.stream.synchronize()
+        shp = self.cells.X.shape
+        h = {k: v.cpu().numpy().copy() for k, v in t.items()}
+        self.fluxes_mem, self.fluxes_gj = h["fluxes_mem"], h["fluxes_gj"]
+        self.fluxes_env_x, self.fluxes_env_y = h["fluxes_env_x"], h["fluxes_env_y"]
+        self.rate_NaKATP = h["rate_NaKATP"]
+        self.chan_flux = h["chan_flux"][:len(self.channel_specs)]
+        self.Jn, self.I_mem = h["Jn"], h["I_mem"]
+        self.E_gj_x, self.E_gj_y = h["E_gj_x"], h["E_gj_y"]
+        self.J_cell_x, self.J_cell_y = h["J_cell_x"], h["J_cell_y"]
+        self.vm_ave_obs = h["vm_ave"]
+        self.J_env_x, self.J_env_y = h["J_env_x"].reshape(shp), h["J_env_y"].reshape(shp)
+        self.Jtx, self.Jty = h["Jtx"].reshape(shp), h["Jty"].reshape(shp)
+        self.B_field = h["B_field"].reshape(shp)
+
+    def step_observed(self, n: int) -> None:
+        """Run n steps; the last one's observables are pulled to the host."""
+        self._step_observed(n)
+        self._sync()
+        self._pull_state()
+        self._pull_observables()
+
+    # ------------------------------------------------------------------
     # update_V and the loop
     # ------------------------------------------------------------------
     def update_V(self, cells=None, p=None) -> None:
@@ -437,8 +551,17 @@ class Simulator:
         self._check(self._lib.bt_update_v(self._ctx))
 
     def write2storage(self, t, cells=None, p=None) -> None:
-        """``sim.py:1789-1884``: the state fields of this path."""
+        """``sim.py:1789-1884`` for the fields of this path."""
         self.time.append(t)
+        if getattr(self, "Jn", None) is not None:
+            self.efield_gj_x_time.append(self.E_gj_x * 1)
+            self.efield_gj_y_time.append(self.E_gj_y * 1)
+            self.I_cell_x_time.append(self.J_cell_x * 1)
+            self.I_cell_y_time.append(self.J_cell_y * 1)
+            self.I_mem_time.append(self.I_mem * 1)
+            self.rate_NaKATP_time.append(self.rate_NaKATP * 1)
+            self.I_tot_x_time.append(self.J_env_x * 1)
+            self.I_tot_y_time.append(self.J_env_y * 1)
         self.cc_time.append(np.copy(self.cc_cells))
         self.cc_env_time.append(np.copy(self.cc_env))
         self.dd_time.append(np.copy(self.Dm_cells))
@@ -483,10 +606,11 @@ class Simulator:
         sampled_idx = [i for i, t in enumerate(time_steps) if t in time_steps_sampled]
         pos = 0
         for s in sampled_idx:
-            self._check(self._lib.bt_step(self._ctx, s + 1 - pos, 1 if self.use_graph else 0))
+            self._step_observed(s + 1 - pos)
             pos = s + 1
             self._sync(step0)
             self._pull_state()
+            self._pull_observables()
             phase.callbacks.progressed_next()
             self.write2storage(time_steps[s], phase.cells, phase.p)
         if pos < len(time_steps):

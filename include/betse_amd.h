@@ -30,6 +30,12 @@ extern "C" {
 #define BT_EHIP     -2
 
 #define BT_MAX_IONS  8
+#define BT_MAX_CHAN  4
+#define BT_TILE_MEMBRANES 64   /* membranes per cell tile (whole cells, one wavefront per ion) */
+
+/* Voltage-gated channel models (betse/science/channels/vg_na.py, vg_k.py). */
+#define BT_CHAN_NAV1P2 0
+#define BT_CHAN_KV1P1  1
 
 /* Flux arithmetic of the membrane and environment flux laws.
  * Cell/environment state, charge sums and the charge->voltage step are FP64
@@ -44,7 +50,11 @@ typedef struct {
     int32_t n_cells, n_mems, grid_ny, grid_nx, n_ions;
     int32_t i_na, i_k, i_ca;            /* ion slots, -1 if the ion is absent */
     int32_t flux_mode;                  /* BT_FLUX_FP64 / BT_FLUX_FP32 */
-    int32_t reserved;
+    int32_t n_chan;                     /* voltage-gated channels on every membrane (0..BT_MAX_CHAN) */
+    int32_t chan_kind[BT_MAX_CHAN];     /* BT_CHAN_* */
+    int32_t chan_ion[BT_MAX_CHAN];      /* ion slot the channel conducts; one channel per ion */
+    double chan_max_dm[BT_MAX_CHAN];    /* chan.maxDm (networks.py:6557) */
+    double chan_dt;                     /* p.dt * time_unit (ms), channelsabc.py:54 */
     double z[BT_MAX_IONS];              /* valence (parameters.py:1082-1088) */
     double d_gj[BT_MAX_IONS];           /* D_gj per ion (sim.py:570-574) */
     double c_bound[BT_MAX_IONS];        /* c_env_bound (sim.py:608, 800-806) */
@@ -59,6 +69,8 @@ typedef struct {
     double ko_env, eo, er;              /* ko_env is frozen per phase (sim.py:974, Q13) */
     double cell_radius, true_cell_size;
     double gauss_w[5];                  /* scipy gaussian_filter1d taps, centre first */
+    double smooth_cells;                /* 1/'sharpness cell' (parameters.py:1146), Jn smoothing */
+    double mu;                          /* magnetic permeability (parameters.py:1111), B_field */
 } bt_params;
 
 /* Device buffers. Layouts: per-ion fields are [n_ions][n], grid fields are
@@ -72,7 +84,9 @@ typedef struct {
     const int32_t* mem_sq;      /* [n_mems] map_mem2ecm                           */
     const int32_t* sq_ptr;      /* [ny*nx+1] grid-side CSR of M_divmap_mem2ecm    */
     const int32_t* sq_mems;     /* [n_mems] membranes sorted by square            */
-    const int32_t* tile_ptr;    /* [n_tiles+1] first cell of each 256-membrane tile */
+    const int32_t* mem_pos;     /* [n_mems] position of each membrane in sq_mems  */
+    const int32_t* tile_ptr;    /* [n_tiles+1] first cell of each tile of whole cells
+                                   holding <= BT_TILE_MEMBRANES membranes         */
     const double* mem_sa;       /* [n_mems]                                       */
     const double* gjopen;       /* [n_mems] gj_block*gj_default_weights           */
     const double* cell_vol;     /* [n_cells]                                      */
@@ -88,13 +102,20 @@ typedef struct {
     double* E_x;                /* [ny*nx] E_env_x                                */
     double* E_y;                /* [ny*nx] E_env_y                                */
     double* v_env;              /* [ny*nx] v_env                                  */
+    /* geometry used by the sampled-step observables only */
+    const double* mem_nx;       /* [n_mems] membrane normal x (mem_vects_flat[:,2]) */
+    const double* mem_ny;       /* [n_mems] membrane normal y (mem_vects_flat[:,3]) */
+    const double* cell_sa;      /* [n_cells]                                      */
+    /* voltage-gated channels (may be NULL when n_chan == 0) */
+    double* gates;              /* [n_chan][2][n_mems] m, h gates (in place)      */
+    const double* chan_mod;     /* [n_chan][n_cells] channel modulator per cell   */
     int32_t n_tiles;
-    int32_t reserved;
+    int32_t tile_m;             /* must equal BT_TILE_MEMBRANES                   */
 } bt_buffers;
 
 /* Create a context: validates sizes, copies the parameter block, allocates the
  * workspace (second cell-state buffer, post-diffusion env buffer, per-membrane
- * exchange fluxes, Gaussian scratch, status word) on the current device.
+ * exchange fluxes, surface-charge field, status words) on the current device.
  * `stream` is a hipStream_t (may be NULL for the default stream; graphs need a
  * non-default stream). Replaces the state set-up of init_core/init_dynamics
  * (sim.py:452-1012) on the device side. */
@@ -110,9 +131,50 @@ int bt_update_v(bt_ctx* ctx);
 
 /* Enqueue n_steps iterations of the _run_sim_core_loop body (sim.py:1169-1366):
  * NaK pump, per-ion GHK + gap-junction fluxes, env Nernst-Planck update,
- * Ca pump, cell/env concentration update, update_V, NaN checks. Steps are
+ * Ca pump, voltage-gated channels (networks.py:3115-3215), cell/env
+ * concentration update, update_V, NaN checks. Steps are
  * replayed from a captured hipGraph when use_graph != 0. Asynchronous. */
 int bt_step(bt_ctx* ctx, int32_t n_steps, int32_t use_graph);
+
+/* Observables of one step (get_current, ion_current.py:13-109; update_gj
+ * E_gj, sim.py:2166-2172; update_V vm_ave, sim.py:2038). The reference
+ * evaluates them every step; here they are produced for the last step of a
+ * bt_step_observed call. All pointers are caller-owned device buffers. */
+typedef struct {
+    /* raw fluxes of the observed step, written by bt_step_observed */
+    double* fluxes_mem;    /* [n_ions][n_mems] fluxes_mem (pumps + GHK)          */
+    double* fluxes_gj;     /* [n_ions][n_mems] fluxes_gj                         */
+    double* fluxes_env_x;  /* [n_ions][ny*nx] fluxes_env_x                       */
+    double* fluxes_env_y;  /* [n_ions][ny*nx] fluxes_env_y                       */
+    double* chan_flux;     /* [n_chan][n_mems] channel fluxes (NULL if n_chan==0) */
+    double* rate_NaKATP;   /* [n_mems] -f_Na of the NaK pump                     */
+    /* derived by bt_observables */
+    double* Jn;            /* [n_mems] smoothed transmembrane current density   */
+    double* I_mem;         /* [n_mems]                                           */
+    double* E_gj_x;        /* [n_mems]                                           */
+    double* E_gj_y;        /* [n_mems]                                           */
+    double* J_cell_x;      /* [n_cells]                                          */
+    double* J_cell_y;      /* [n_cells]                                          */
+    double* vm_ave;        /* [n_cells]                                          */
+    double* J_env_x;       /* [ny*nx] divergence-free env current (HH_Decomp)    */
+    double* J_env_y;       /* [ny*nx]                                            */
+    double* Jtx;           /* [ny*nx] J_env + curl-free part                     */
+    double* Jty;           /* [ny*nx]                                            */
+    double* B_field;       /* [ny*nx] mu * stream function                       */
+} bt_obs;
+
+/* bt_step whose last step also writes the raw fluxes into obs. */
+int bt_step_observed(bt_ctx* ctx, int32_t n_steps, int32_t use_graph, const bt_obs* obs);
+
+/* Derive the observables from the fluxes captured in obs. The two Poisson
+ * solves of HH_Decomp (sim_toolbox.py:1236-1290, dense lapENVinv in the
+ * reference) run as DST-I transforms written as FP64 MFMA matrix products. */
+int bt_observables(bt_ctx* ctx, const bt_obs* obs);
+
+/* C = A*B for row-major FP64 matrices whose sizes are multiples of 16, on the
+ * context stream with the MFMA kernel the observables use (exposed for tests). */
+int bt_gemm_f64(bt_ctx* ctx, const double* A, const double* B, double* C,
+                int32_t m, int32_t n, int32_t k);
 
 /* Wait for the stream; return BT_OK or BT_UNSTABLE (a NaN was produced; the
  * state is frozen at the step that produced it and *bad_step receives its
@@ -120,7 +182,7 @@ int bt_step(bt_ctx* ctx, int32_t n_steps, int32_t use_graph);
 int bt_sync(bt_ctx* ctx, int64_t* bad_step);
 
 /* Average device time in ms of one launch of a step kernel (0 membrane/cell,
- * 1 env diffusion, 2 env exchange, 3 Gaussian rows, 4 Gaussian cols+gradient),
+ * 1 env field + diffusion, 2 env exchange, 3 env field only),
  * timed with HIP events over n_launches back-to-back launches on the context
  * stream. Leaves the state undefined: call after the timed work. */
 int bt_time_kernel(bt_ctx* ctx, int32_t kernel, int32_t n_launches, double* ms_out);

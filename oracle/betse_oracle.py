@@ -37,6 +37,9 @@ oracle function             reference
 ``OracleSim.get_current``   ``physics/ion_current.py:13-114``
 ``HH_Decomp``               ``sim_toolbox.py:1236-1290``
 ``sampled_steps``           ``sim.py:2455-2462`` with ``phase/phasecls.py:330``
+``OracleChannel``           ``channels/channelsabc.py:40-60`` (update_mh), ``channels/vg_na.py:34-113,131-186``
+                            (Nav1p2), ``channels/vg_k.py:34-115,133-178`` (Kv1p1)
+``OracleSim.run_loop_channels`` ``chemistry/networks.py:3115-3215`` (channels on all membranes)
 ==========================  ===============================================
 
 Two operator forms are supported:
@@ -229,6 +232,77 @@ def sampled_steps(n_steps: int, t_resample: float):
 
 
 # --------------------------------------------------------------------------
+# voltage-gated channels
+# --------------------------------------------------------------------------
+def nav1p2_rates(V):
+    """Nav1p2._calculate_state (vg_na.py:169-186); V in mV."""
+    mAlpha = (0.182 * ((V - 10.0) - -35.0)) / (1 - (np.exp(-((V - 10.0) - -35.0) / 9)))
+    mBeta = (0.124 * (-(V - 10.0) - 35.0)) / (1 - (np.exp(-(-(V - 10.0) - 35.0) / 9)))
+    mInf = mAlpha / (mAlpha + mBeta)
+    mTau = (1 / (mAlpha + mBeta))
+    hInf = 1.0 / (1 + np.exp((V - -65.0 - 10.0) / 6.2))
+    hTau = (1 / (
+        (0.024 * ((V - 10.0) - -50.0)) /
+        (1 - (np.exp(-((V - 10.0) - -50.0) / 5))) + (
+            0.0091 * (-(V - 10.0) - 75.000123)) / (1 - (np.exp(-(-(V - 10) - 75.000123) / 5)))))
+    return mInf, mTau, hInf, hTau
+
+
+def nav1p2_init(V):
+    """Nav1p2._init_state (vg_na.py:140-166)."""
+    mAlpha = (0.182 * ((V - 10.0) - -35.0)) / (1 - (np.exp(-((V - 10.0) - -35.0) / 9)))
+    mBeta = (0.124 * (-(V - 10.0) - 35.0)) / (1 - (np.exp(-(-(V - 10.0) - 35.0) / 9)))
+    return mAlpha / (mAlpha + mBeta), 1.0 / (1 + np.exp((V - -65.0 - 10.0) / 6.2))
+
+
+def kv1p1_rates(V):
+    """Kv1p1._calculate_state (vg_k.py:167-178); V in mV."""
+    mInf = 1.0000 / (1 + np.exp((V - -30.5000) / -11.3943))
+    mTau = 30.0000 / (1 + np.exp((V - -76.5600) / 26.1479))
+    hInf = 1.0000 / (1 + np.exp((V - -30.0000) / 27.3943))
+    hTau = 15000.0000 / (1 + np.exp((V - -160.5600) / -100.0000))
+    return mInf, mTau, hInf, hTau
+
+
+def kv1p1_init(V):
+    """Kv1p1._init_state (vg_k.py:142-163)."""
+    return (1.0000 / (1 + np.exp((V - -30.5000) / -11.3943)),
+            1.0000 / (1 + np.exp((V - -30.0000) / 27.3943)))
+
+
+CHANNEL_KINDS = {
+    # kind: (ion, rates, init, mpower, hpower, time_unit)
+    "Nav1p2": ("Na", nav1p2_rates, nav1p2_init, 3, 1, 1.0e3),
+    "Kv1p1": ("K", kv1p1_rates, kv1p1_init, 1, 2, 1.0e3),
+}
+
+
+class OracleChannel:
+    """One voltage-gated channel on every membrane (targets = all)."""
+
+    def __init__(self, kind, max_dm, modulator=1.0):
+        self.kind = kind
+        self.ion, self._rates, self._init, self._mpower, self._hpower, self.time_unit = CHANNEL_KINDS[kind]
+        self.maxDm = max_dm
+        self.modulator = modulator
+        self.v_corr = 0.0
+        self.ions = [self.ion]
+        self.rel_perm = [1.0]
+
+    def init(self, vm):
+        V = vm * 1000 + self.v_corr
+        self.m, self.h = self._init(V)
+
+    def run(self, vm, p):
+        V = vm * 1000 + self.v_corr
+        self._mInf, self._mTau, self._hInf, self._hTau = self._rates(V)
+        dt = p.dt * self.time_unit
+        self.m = (self._mTau * self.m + dt * self._mInf) / (self._mTau + dt)
+        self.h = (self._hTau * self.h + dt * self._hInf) / (self._hTau + dt)
+        self.P = (self.m ** self._mpower) * (self.h ** self._hpower)
+
+
+# --------------------------------------------------------------------------
 class OracleSim:
     """Simulator state and step, restated from ``betse/science/sim.py``."""
 
@@ -328,6 +402,29 @@ class OracleSim:
         self.fluxes_gj = np.copy(self.fluxes_mem)
         self.cc_at_mem = np.asarray([cc[cells.mem_to_cells] for cc in self.cc_cells])
         self.Phi_b = np.zeros(self.edl)
+        self.init_channels()
+
+    def init_channels(self):
+        """``Channel.init_channel`` (networks.py:6550-6628) for each configured channel;
+        gates start from ``init(vm)`` or, if the parameters give them, from
+        explicit per-membrane arrays."""
+        p, cells = self.p, self.cells
+        self.channels = []
+        if getattr(p, "channels", ()) and getattr(p, "channel_random_state", False) \
+                and getattr(p, "channel_gates0", None) is None:
+            raise ValueError("the workload's seeded channel state must be drawn before the oracle runs")
+        for k, spec in enumerate(getattr(p, "channels", ())):
+            mod = 1.0
+            mults = getattr(p, "channel_cell_mult", None)
+            if mults is not None and mults[k] is not None:
+                mod = np.asarray(mults[k])[cells.mem_to_cells]
+            ch = OracleChannel(spec.kind, spec.max_dm, mod)
+            ch.init(self.vm)
+            gates = getattr(p, "channel_gates0", None)
+            if gates is not None and gates[k] is not None:
+                ch.m = np.array(gates[k][0], dtype=np.float64)
+                ch.h = np.array(gates[k][1], dtype=np.float64)
+            self.channels.append(ch)
 
     def initDenv(self):
         p, cells = self.p, self.cells
@@ -402,11 +499,16 @@ class OracleSim:
         return sorted(sampled)
 
     def write2storage(self, it):
-        self.storage.append({
+        rec = {
             "step": it, "vm": self.vm.copy(), "vm_ave": self.vm_ave.copy(),
             "cc_cells": self.cc_cells.copy(), "cc_env": self.cc_env.copy(),
             "v_env": self.v_env.copy(), "E_env_x": self.E_env_x.copy(), "E_env_y": self.E_env_y.copy(),
-        })
+        }
+        for k in ("Jn", "I_mem", "J_cell_x", "J_cell_y", "J_env_x", "J_env_y", "Jtx", "Jty", "B_field",
+                  "rate_NaKATP", "E_gj_x", "E_gj_y", "fluxes_mem", "fluxes_gj", "fluxes_env_x", "fluxes_env_y"):
+            if hasattr(self, k):
+                rec[k] = np.array(getattr(self, k), copy=True)
+        self.storage.append(rec)
 
     # ---- one iteration of _run_sim_core_loop -----------------------------
     def step(self, kind: str = "sim", observables: bool = True):
@@ -443,6 +545,9 @@ class OracleSim:
             self.update_intra(i)
         if "Ca" in self.names:
             self.ca_handler()
+        if self.channels and (kind == "sim" or getattr(self.p, "channels_init_active", True)):
+            self.extra_J_mem = np.zeros(self.mdl)       # clear_run_loop (networks.py:2790-2801)
+            self.run_loop_channels()
         self.update_all_concs()
         self.update_V(observables=observables)
         check_v(self.vm)
@@ -450,6 +555,9 @@ class OracleSim:
     def update_gj(self, i):
         p, cells = self.p, self.cells
         self.vgj = self.vm[cells.nn_i] - self.vm[cells.mem_i]
+        self.Egj = -self.vgj / cells.gj_len
+        self.E_gj_x = self.Egj * cells.mem_vects_flat[:, 2]
+        self.E_gj_y = self.Egj * cells.mem_vects_flat[:, 3]
         self.gjopen = self.gj_block * np.ones(self.mdl) * cells.gj_default_weights
         conc_mem = self.cc_at_mem[i]
         fgj_X = electroflux(conc_mem[cells.mem_i], conc_mem[cells.nn_i],
@@ -493,6 +601,34 @@ class OracleSim:
             if p.cluster_open is False:
                 f_CaATP[cells.bflags_mems] = 0
             self.fluxes_mem[self.iCa] += self.rho_pump * (f_CaATP)
+
+    def run_loop_channels(self):
+        """``MasterOfNetworks.run_loop_channels`` (networks.py:3115-3215): gates
+        from the current (nonce-carrying) Vmem, GHK through the channel with the
+        fresh cell and post-diffusion env concentrations, then an immediate
+        ``update_Co`` of cell and env."""
+        p, cells = self.p, self.cells
+        IdM = np.ones(self.mdl)
+        for chan in self.channels:
+            moddy = chan.modulator
+            chan.run(self.vm, p)
+            for ion, rel_perm in zip(chan.ions, chan.rel_perm):
+                ioni = self.names.index(ion)
+                zzz = self.zs[ioni]
+                DChan = chan.P * rel_perm * chan.maxDm * moddy
+                cenv = self.cc_env[ioni]
+                ccell = self.cc_cells[ioni]
+                cmem = self.cc_cells[ioni][cells.mem_to_cells]
+                f_ED = electroflux(cenv[cells.map_mem2ecm], cmem, DChan, IdM * p.tm, zzz * IdM, self.vm,
+                                   self.T, p, rho=self.rho_channel)
+                delta_cells = self.msum(f_ED * cells.mem_sa) / cells.cell_vol
+                cX_cell = ccell + delta_cells * p.dt
+                cX_mem = cX_cell[cells.mem_to_cells]
+                delta_env = self.divmap(-f_ED) / (p.cell_height * cells.delta ** 2)
+                cX_env = cenv + delta_env * p.dt
+                self.cc_cells[ioni], self.cc_at_mem[ioni], self.cc_env[ioni] = cX_cell, cX_mem, cX_env
+                self.extra_J_mem += -f_ED * p.F * zzz
+                chan.chan_flux = f_ED
 
     def update_all_concs(self):
         p, cells = self.p, self.cells

@@ -196,3 +196,95 @@ def test_small_fp32_flux_error(small):
     dv, dc, de, lag, eex = errors(sim, o)
     say(f"fp32 flux: |dVm|={dv:.3e} V rel dc={dc:.3e} rel denv={de:.3e} rel dE={eex:.3e}")
     assert dv < 1e-6
+
+
+def test_channels_fp64_parity():
+    """Nav1p2 + Kv1p1 on every membrane, per-cell multipliers, U[0,1] gates, Ca
+    (the large-cluster physics on the small test cluster), init then sim."""
+    from betse_amd.channels import draw_channel_state
+    p = Parameters.preset("tiny_channels")
+    cells = Cells.make_world(p)
+    draw_channel_state(p, cells.n_cells, cells.n_mems)
+    say("case: channels init 150 + sim 50")
+    o = oracle_run(p, cells, 150, "init")
+    o.run_sim_core("sim", n_steps=50)
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    sim = Simulator(p.copy(), device="cuda:0")
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    sim.prepare(ph)
+    sim.step_n(150)
+    sim.pull()
+    ph2 = SimPhase(SimPhaseKind.SIM, sim.p, cells, sim)
+    sim.prepare(ph2)
+    sim.step_n(50)
+    sim.pull()
+    check(sim, o, "channels")
+    for k, ch in enumerate(o.channels):
+        np.testing.assert_allclose(sim.chan_gates[k, 0], ch.m, rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(sim.chan_gates[k, 1], ch.h, rtol=1e-10, atol=1e-14)
+
+
+def test_mfma_gemm_f64():
+    """The FP64 MFMA kernel behind the Poisson observables against numpy."""
+    import ctypes
+    import torch
+    from betse_amd import _native
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    p = Parameters.preset("tiny")
+    cells = Cells.make_world(p)
+    sim = Simulator(p.copy(), device="cuda:0")
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    sim.prepare(ph)
+    say("case: mfma gemm 32x48x64")
+    rng = np.random.RandomState(2)
+    a = rng.randn(32, 48)
+    b = rng.randn(48, 64)
+    with torch.cuda.stream(sim.stream):
+        A = torch.tensor(a, device="cuda:0")
+        B = torch.tensor(b, device="cuda:0")
+        Cm = torch.full((32, 64), float("nan"), dtype=torch.float64, device="cuda:0")
+    sim.stream.synchronize()
+    rc = sim._lib.bt_gemm_f64(sim._ctx, ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(B.data_ptr()),
+                              ctypes.c_void_p(Cm.data_ptr()), 32, 64, 48)
+    assert rc == _native.BT_OK
+    sim.stream.synchronize()
+    np.testing.assert_allclose(Cm.cpu().numpy(), a @ b, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("preset", ["tiny", "tiny_channels"])
+def test_sampled_observables(preset):
+    """Currents, E_gj, pump rate and the Helmholtz-Hodge env currents at every
+    sampled step against the oracle's get_current (sparse Poisson solve)."""
+    from betse_amd.channels import draw_channel_state
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    p = Parameters.preset(preset)
+    cells = Cells.make_world(p)
+    if p.channels:
+        draw_channel_state(p, cells.n_cells, cells.n_mems)
+    p.init_time_total = 40 * p.init_time_step
+    p.init_time_sampling = 10 * p.init_time_step
+    say(f"case: observables {preset}")
+    sim = Simulator(p.copy(), device="cuda:0")
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    sim.run_sim_core(ph)
+    o = OracleSim(cells, p.copy(), mode="sparse")
+    o.init_core()
+    o.run_sim_core("init")
+    assert len(o.storage) == len(sim.I_mem_time) > 0
+    pairs = {"I_mem_time": "I_mem", "I_cell_x_time": "J_cell_x", "I_cell_y_time": "J_cell_y",
+             "efield_gj_x_time": "E_gj_x", "rate_NaKATP_time": "rate_NaKATP", "I_tot_x_time": "J_env_x",
+             "I_tot_y_time": "J_env_y"}
+    for rec_i, rec in enumerate(o.storage):
+        for mine, theirs in pairs.items():
+            a = np.asarray(getattr(sim, mine)[rec_i]).ravel()
+            b = np.asarray(rec[theirs]).ravel()
+            scale = max(np.abs(b).max(), 1e-300)
+            err = np.abs(a - b).max() / scale
+            assert err < 1e-8, (mine, rec_i, err)
+    np.testing.assert_allclose(sim.B_field.ravel(), o.B_field.ravel(), rtol=0,
+                               atol=1e-8 * np.abs(o.B_field).max())
+    np.testing.assert_allclose(sim.Jtx.ravel(), o.Jtx.ravel(), rtol=0, atol=1e-8 * np.abs(o.Jtx).max())
+    say(f"observables {preset}: ok over {len(o.storage)} sampled steps")
