@@ -49,6 +49,117 @@ def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 
     return b
 
 
+class _Tot:
+    """Totals of a batch of clusters, shaped like Cells for algorithmic_bytes."""
+
+    def __init__(self, worlds):
+        self.n_cells = sum(c.n_cells for _, c in worlds)
+        self.n_mems = sum(c.n_mems for _, c in worlds)
+        self.envInds_inClust = range(sum(len(c.envInds_inClust) for _, c in worlds))
+
+        class _X:
+            size = sum(c.X.size for _, c in worlds)
+        self.X = _X()
+
+
+def bench_batched(a):
+    """Batched workload: 256 independent clusters of 100-5000 cells."""
+    import torch
+    from betse_amd.batch import BatchSimulator, batched_worlds
+    from oracle.betse_oracle import OracleSim
+    t0 = time.time()
+    worlds = batched_worlds(a.clusters)
+    tot = _Tot(worlds)
+    C = tot.n_cells
+    p0 = worlds[0][0]
+    n_ions = len(p0.ion_names())
+    print(f"# batch: {len(worlds)} clusters, {C} cells, {tot.n_mems} membranes, {tot.X.size} squares, "
+          f"built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    modes = ["fp32", "fp64"] if a.flux == "both" else [a.flux]
+    res = {}
+    for mode in modes:
+        b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=mode)
+        b.init("init")
+        b.step_n(a.warmup)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(b.stream)
+        b.step_n(a.steps, sync=False)
+        e1.record(b.stream)
+        b.sync()
+        ev_ms = e0.elapsed_time(e1)
+        kern = {k: b.time_kernel(k, 20) for k in ("mem_cell", "env_field", "env_exchange")}
+        res[mode] = dict(ev_ms=ev_ms, kernels=kern)
+        print(f"# {mode}: {ev_ms / a.steps * 1e3:.1f} us/step, kernels(us): "
+              + ", ".join(f"{k}={v * 1e3:.1f}" for k, v in kern.items()), file=sys.stderr, flush=True)
+        del b
+    cpu, parity = None, {}
+    if not a.no_cpu:
+        sub = worlds[:a.cpu_clusters]
+        # time one step per cluster to size the sample
+        n_cpu = 2
+        t0 = time.perf_counter()
+        oracles = []
+        for p, c in sub:
+            o = OracleSim(c, p.copy(), mode="sparse")
+            o.init_core()
+            o.init_dynamics("init")
+            o.p.set_phase_time("init")
+            o.update_V(observables=False)
+            o.step("init", observables=False)
+            oracles.append(o)
+        one = time.perf_counter() - t0
+        n_cpu = max(2, int(a.cpu_seconds / max(one, 1e-6)))
+        t0 = time.perf_counter()
+        for o in oracles:
+            for _ in range(n_cpu - 1):
+                o.step("init", observables=False)
+        el = one + time.perf_counter() - t0
+        csub = sum(c.n_cells for _, c in sub)
+        cpu = {"value": csub * n_cpu / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 NumPy oracle (sparse operators), the first {len(sub)} clusters "
+                         f"({csub} cells) for {n_cpu} steps each, {el:.1f}s on this host, single thread"}
+        for mode in modes:
+            b = BatchSimulator([(p.copy(), c) for p, c in sub], device="cuda:0", flux_mode=mode)
+            b.init("init")
+            b.step_n(n_cpu)
+            b.pull()
+            parity[mode] = {"steps": n_cpu, "clusters": len(sub),
+                            "max_abs_dVm_V": float(max(np.abs(st.vm - o.vm).max() for st, o in zip(b.states, oracles))),
+                            "max_rel_dc_cell": float(max((np.abs(st.cc_cells - o.cc_cells) / np.abs(o.cc_cells)).max()
+                                                         for st, o in zip(b.states, oracles)))}
+            del b
+        print(f"# cpu oracle sample: {cpu['sample']}; parity {parity}", file=sys.stderr, flush=True)
+    ok = [m for m in modes if (not parity) or parity[m]["max_abs_dVm_V"] < 1e-6]
+    best = min(ok or modes, key=lambda m: res[m]["ev_ms"])
+    r = res[best]
+    ms_step = r["ev_ms"] / a.steps
+    ab = algorithmic_bytes(tot, n_ions, "Ca" in p0.ion_names(), best, len(p0.channels))
+    kern = r["kernels"]
+    dom = max(kern, key=kern.get)
+    achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
+    line = {
+        "metric": "cell-steps/s at |dVmem|<1uV vs FP64 oracle", "value": C * a.steps / (r["ev_ms"] / 1e3),
+        "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "f32-flux/f64-state", "data": "synthetic",
+        "config": {"workload": f"batched: {len(worlds)} independent clusters, {C} cells, {tot.n_mems} membranes, "
+                               f"grids 32-128 ({tot.X.size} squares), ions {'/'.join(p0.ion_names())}, "
+                               f"per-cluster Dm and NaK rate over one decade, dt {p0.dt:g} s",
+                   "clusters": len(worlds), "cells": C, "flux_mode": best},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_us": kern[dom] * 1e3,
+                     "algorithmic_bytes": ab[dom], "step_algorithmic_bytes": ab["step"],
+                     "step_achieved_GBs": ab["step"] / (ms_step * 1e-3) / 1e9},
+        "cpu_baseline": cpu, "parity_prefix": parity,
+        "paths": {m: {"cell_steps_per_s": C * a.steps / (res[m]["ev_ms"] / 1e3),
+                      "us_per_step": res[m]["ev_ms"] / a.steps * 1e3,
+                      "kernels_us": {k: v * 1e3 for k, v in res[m]["kernels"].items()}} for m in modes},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def make_problem(name: str):
     from betse_amd.cells import Cells
     from betse_amd.parameters import Parameters
@@ -114,6 +225,8 @@ def main(argv=None):
     ap.add_argument("--flux", default="both", choices=["fp64", "fp32", "both"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--clusters", type=int, default=256, help="batched config: number of clusters")
+    ap.add_argument("--cpu-clusters", type=int, default=8, help="batched config: clusters in the CPU sample")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args(argv)
@@ -132,6 +245,8 @@ def main(argv=None):
               "(each GPU box has one GPU; the sharding design is in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if a.config == "batched":
+        return bench_batched(a)
     p, cells, t_world = make_problem(a.config)
     n_ions = len(p.ion_names())
     has_ca = "Ca" in p.ion_names()

@@ -35,10 +35,19 @@ _d4 = ctypes.c_double * BT_MAX_CHAN
 _vp = ctypes.c_void_p
 
 
+class BtCluster(ctypes.Structure):
+    _fields_ = [
+        ("grid_ny", ctypes.c_int32), ("grid_nx", ctypes.c_int32), ("sq_off", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("delta", ctypes.c_double), ("mean_sa", ctypes.c_double), ("ko_env", ctypes.c_double),
+        ("alpha_NaK", ctypes.c_double),
+    ]
+
+
 class BtParams(ctypes.Structure):
     _fields_ = [
-        ("n_cells", ctypes.c_int32), ("n_mems", ctypes.c_int32), ("grid_ny", ctypes.c_int32),
-        ("grid_nx", ctypes.c_int32), ("n_ions", ctypes.c_int32),
+        ("n_cells", ctypes.c_int32), ("n_mems", ctypes.c_int32), ("n_squares", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32), ("n_ions", ctypes.c_int32),
         ("i_na", ctypes.c_int32), ("i_k", ctypes.c_int32), ("i_ca", ctypes.c_int32),
         ("flux_mode", ctypes.c_int32), ("n_chan", ctypes.c_int32),
         ("chan_kind", _i4), ("chan_ion", _i4), ("chan_max_dm", _d4), ("chan_dt", ctypes.c_double),
@@ -47,13 +56,12 @@ class BtParams(ctypes.Structure):
         ("F", ctypes.c_double), ("R", ctypes.c_double), ("q", ctypes.c_double), ("kb", ctypes.c_double),
         ("cm", ctypes.c_double), ("tm", ctypes.c_double),
         ("gj_len", ctypes.c_double), ("gj_surface", ctypes.c_double),
-        ("alpha_NaK", ctypes.c_double), ("NaK_block", ctypes.c_double), ("deltaGATP", ctypes.c_double),
+        ("NaK_block", ctypes.c_double), ("deltaGATP", ctypes.c_double),
         ("cATP", ctypes.c_double), ("cADP", ctypes.c_double), ("cPi", ctypes.c_double),
         ("KmNK_Na", ctypes.c_double), ("KmNK_K", ctypes.c_double), ("KmNK_ATP", ctypes.c_double),
         ("alpha_Ca", ctypes.c_double), ("KmCa_Ca", ctypes.c_double), ("KmCa_ATP", ctypes.c_double),
-        ("delta", ctypes.c_double), ("cell_height", ctypes.c_double),
-        ("mean_sa", ctypes.c_double),
-        ("ko_env", ctypes.c_double), ("eo", ctypes.c_double), ("er", ctypes.c_double),
+        ("cell_height", ctypes.c_double),
+        ("eo", ctypes.c_double), ("er", ctypes.c_double),
         ("cell_radius", ctypes.c_double), ("true_cell_size", ctypes.c_double),
         ("gauss_w", _d5),
         ("smooth_cells", ctypes.c_double), ("mu", ctypes.c_double),
@@ -63,6 +71,7 @@ class BtParams(ctypes.Structure):
 class BtBuffers(ctypes.Structure):
     _fields_ = [(n, _vp) for n in (
         "mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "mem_pos", "tile_ptr",
+        "tile_cluster", "sq_cluster",
         "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
         "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "mem_nx", "mem_ny", "cell_sa",
         "gates", "chan_mod")] + [
@@ -80,7 +89,8 @@ class BtObs(ctypes.Structure):
 
 # symbol -> (restype, argtypes); every function include/betse_amd.h declares
 SIGNATURES = {
-    "bt_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(BtParams), ctypes.POINTER(BtBuffers), _vp]),
+    "bt_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(BtParams), ctypes.POINTER(BtCluster),
+                                  ctypes.c_int32, ctypes.POINTER(BtBuffers), _vp]),
     "bt_destroy": (ctypes.c_int, [_vp]),
     "bt_update_v": (ctypes.c_int, [_vp]),
     "bt_step": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
@@ -89,9 +99,26 @@ SIGNATURES = {
     "bt_observables": (ctypes.c_int, [_vp, ctypes.POINTER(BtObs)]),
     "bt_gemm_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "bt_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "bt_residual": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "bt_steps_done": (ctypes.c_int64, [_vp]),
     "bt_last_error": (ctypes.c_char_p, [_vp]),
     "bt_version": (ctypes.c_int, []),
+    # warm-start graph network
+    "bt_gnn_gemm": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int64,
+                                   _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_int32, _vp, ctypes.c_int32, _vp]),
+    "bt_gnn_bias_silu": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "bt_gnn_bias": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "bt_gnn_silu_bwd": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, _vp]),
+    "bt_gnn_mean_agg": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32,
+                                       ctypes.c_int32, _vp]),
+    "bt_gnn_mean_agg_T": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32, _vp]),
+    "bt_gnn_colsum": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _vp, _vp,
+                                     ctypes.c_int32, _vp]),
+    "bt_gnn_mse": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_float, _vp, _vp, ctypes.c_int32, _vp]),
+    "bt_gnn_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_float, ctypes.c_int32, _vp]),
 }
 
 _lib: Optional[ctypes.CDLL] = None

@@ -8,7 +8,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", "betse_step.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "betse_step.hip"), os.path.join(HERE, "csrc", "gnn.hip")]
 INCLUDE = os.path.join(ROOT, "include")
 OUT = os.path.join(HERE, "libbetse_amd.so")
 

@@ -27,6 +27,7 @@ CSR ``mem_ptr`` and the grid-side CSR ``sq_ptr``/``sq_mems`` (the sparse form of
 
 from __future__ import annotations
 
+import itertools
 import math
 from typing import Optional
 
@@ -102,101 +103,114 @@ class Cells:
         vor = Voronoi(points)
         inside = np.hypot(points[:, 0], points[:, 1]) <= p.cluster_radius
         keep = np.nonzero(inside)[0]
-        polys = []
-        regions_v = []
-        for pi in keep:
-            reg = vor.regions[vor.point_region[pi]]
+        regs = [vor.regions[vor.point_region[pi]] for pi in keep]
+        for reg in regs:
             if len(reg) == 0 or -1 in reg:
                 raise RuntimeError("unbounded Voronoi region inside the cluster")
-            v = vor.vertices[reg]
-            a, _ = _polygon_area_centroid(v)
-            if a < 0:               # make every polygon counter-clockwise
-                reg = reg[::-1]
-                v = v[::-1]
-            polys.append(v)
-            regions_v.append(list(reg))
-        ncell = len(polys)
+        ncell = len(regs)
+        counts = np.fromiter((len(r) for r in regs), dtype=np.int64, count=ncell)
+        ptr = np.zeros(ncell + 1, dtype=np.int64)
+        ptr[1:] = np.cumsum(counts)
+        nv = int(ptr[-1])
+        vid = np.fromiter(itertools.chain.from_iterable(regs), dtype=np.int64, count=nv)
+        cell_of = np.repeat(np.arange(ncell), counts)
+        pos = np.arange(nv) - ptr[cell_of]
+        first = ptr[cell_of]
+        last = first + counts[cell_of] - 1
 
-        centres = np.zeros((ncell, 2))
-        for c, v in enumerate(polys):
-            _, centres[c] = _polygon_area_centroid(v)
+        def area_centroid(V):
+            nxt = np.where(pos == counts[cell_of] - 1, first, np.arange(nv) + 1)
+            x, y = V[:, 0], V[:, 1]
+            xs, ys = x[nxt], y[nxt]
+            cross = x * ys - xs * y
+            a = 0.5 * np.add.reduceat(cross, ptr[:-1])
+            cx = np.add.reduceat((x + xs) * cross, ptr[:-1]) / (6.0 * a)
+            cy = np.add.reduceat((y + ys) * cross, ptr[:-1]) / (6.0 * a)
+            return a, np.column_stack((cx, cy))
+
+        a, _ = area_centroid(vor.vertices[vid])
+        # make every polygon counter-clockwise
+        order = np.where(a[cell_of] < 0, first + (counts[cell_of] - 1 - pos), np.arange(nv))
+        vid = vid[order]
+        V = vor.vertices[vid]
+        _, centres = area_centroid(V)
 
         # shrink toward the centroid (cells.py:1009-1016)
-        cell_verts = [p.scale_cell * (v - centres[c]) + centres[c] for c, v in enumerate(polys)]
+        cc = centres[cell_of]
+        Vs = p.scale_cell * (V - cc) + cc
+        self.cell_verts = [Vs[ptr[c]:ptr[c + 1]] for c in range(ncell)]
 
-        mids, nrm, tng, lengths, m2c, edge_key = [], [], [], [], [], []
-        for c, v in enumerate(cell_verts):
-            reg = regions_v[c]
-            n = len(v)
-            for i in range(n):
-                pt1 = v[i - 1]
-                pt2 = v[i]
-                mid = (pt1 + pt2) / 2
-                lgth = np.sqrt((pt2[0] - pt1[0]) ** 2 + (pt2[1] - pt1[1]) ** 2)
-                tang_a = pt2 - pt1
-                tang = tang_a / np.linalg.norm(tang_a)
-                mids.append(mid)
-                nrm.append([tang[1], -tang[0]])
-                tng.append(tang)
-                lengths.append(lgth)
-                m2c.append(c)
-                edge_key.append((min(reg[i - 1], reg[i]), max(reg[i - 1], reg[i])))
+        # membrane k of a cell is the edge (v[k-1], v[k]) (cells.py:1044-1089)
+        prv = np.where(pos == 0, last, np.arange(nv) - 1)
+        pt1, pt2 = Vs[prv], Vs
+        mids = (pt1 + pt2) / 2
+        dx = pt2[:, 0] - pt1[:, 0]
+        dy = pt2[:, 1] - pt1[:, 1]
+        lengths = np.sqrt(dx ** 2 + dy ** 2)
+        tang = np.column_stack((dx, dy)) / np.sqrt(dx * dx + dy * dy)[:, None]
+        nrm = np.column_stack((tang[:, 1], -tang[:, 0]))
 
         self.cell_centres = centres
-        self.cell_verts = cell_verts
-        self.mem_mids_flat = np.asarray(mids)
-        nrm = np.asarray(nrm)
-        tng = np.asarray(tng)
-        self.mem_vects_flat = np.column_stack((self.mem_mids_flat, nrm, tng))
-        self.mem_to_cells = np.asarray(m2c, dtype=np.int64)
-        nmem = len(m2c)
+        self.mem_mids_flat = mids
+        self.mem_vects_flat = np.column_stack((mids, nrm, tang))
+        self.mem_to_cells = cell_of.astype(np.int64)
+        nmem = nv
         self.mem_i = np.arange(nmem)
         self.cell_i = np.arange(ncell)
-        self.mem_sa = np.asarray(lengths) * p.cell_height
-        self.num_mems = np.bincount(self.mem_to_cells, minlength=ncell).astype(np.int64)
-        self.mem_ptr = np.zeros(ncell + 1, dtype=np.int64)
-        self.mem_ptr[1:] = np.cumsum(self.num_mems)
-        self.cell_to_mems = _ragged([np.arange(self.mem_ptr[c], self.mem_ptr[c + 1]) for c in range(ncell)])
+        self.mem_sa = lengths * p.cell_height
+        self.num_mems = counts.copy()
+        self.mem_ptr = ptr.copy()
+        self.cell_to_mems = _ragged([np.arange(ptr[c], ptr[c + 1]) for c in range(ncell)])
 
-        # cell surface area: python sum over each cell's membranes (cells.py:1150-1155)
-        self.cell_sa = np.asarray([sum(self.mem_sa[g]) for g in self.cell_to_mems])
+        # cell surface area: sum over each cell's membranes (cells.py:1150-1155)
+        self.cell_sa = np.add.reduceat(self.mem_sa, ptr[:-1])
 
         rads = self.mem_mids_flat - self.cell_centres[self.mem_to_cells]
         self.R_rads = np.sqrt(rads[:, 0] ** 2 + rads[:, 1] ** 2)
         self.mem_vol = (1 / 2) * self.R_rads * self.mem_sa
         # cell_vol = M_sum_mems . mem_vol (cells.py:1385): contiguous segment sums
-        self.cell_vol = np.add.reduceat(self.mem_vol, self.mem_ptr[:-1])
+        self.cell_vol = np.add.reduceat(self.mem_vol, ptr[:-1])
         self.diviterm = self.cell_vol / self.cell_sa
 
         # gap-junction partners through the shared (unscaled) Voronoi edge
-        partner = {}
-        for m, key in enumerate(edge_key):
-            partner.setdefault(key, []).append(m)
+        lo = np.minimum(vid[prv], vid)
+        hi = np.maximum(vid[prv], vid)
+        key = lo * (len(vor.vertices) + 1) + hi
+        srt = np.argsort(key, kind="stable")
+        ks = key[srt]
+        same = ks[1:] == ks[:-1]
+        if np.any(same[1:] & same[:-1]):
+            raise RuntimeError("edge shared by more than two membranes")
+        ia = srt[:-1][same]
+        ib = srt[1:][same]
         nn = np.arange(nmem, dtype=np.int64)
-        for key, ms in partner.items():
-            if len(ms) == 2:
-                a, b = ms
-                nn[a] = b
-                nn[b] = a
-            elif len(ms) > 2:
-                raise RuntimeError("edge shared by more than two membranes")
+        nn[ia] = ib
+        nn[ib] = ia
         self.nn_i = nn
         self.mem_nn = np.column_stack((self.mem_i, nn))
         self.bflags_mems = np.nonzero(nn == self.mem_i)[0]
         self.bflags_cells = self.mem_to_cells[self.bflags_mems]
         self.cell_nn_i = np.column_stack((self.mem_to_cells, self.mem_to_cells[nn]))
-        cell_nn = []
-        for c in range(ncell):
-            s = []
-            for m in self.cell_to_mems[c]:
-                j = nn[m]
-                if j != m and self.mem_to_cells[j] != c:
-                    s.append(int(self.mem_to_cells[j]))
-            cell_nn.append(s)
-        self.cell_nn = _ragged(cell_nn)
-        self.num_nn = np.asarray([len(s) for s in cell_nn])
+        self._set_cell_nn()
         self.gj_len = p.cell_space                     # cells.py:1004
         self.gj_default_weights = np.ones(nmem)        # cells.py:526
+
+    def _set_cell_nn(self) -> None:
+        """Neighbour cells through gap-junction partners (cells.py:1536-1564), as
+        CSR (``cell_nn_ptr``, ``cell_nn_idx``) and as ragged ``cell_nn``."""
+        m = self.mem_i
+        other = self.mem_to_cells[self.nn_i]
+        ok = (self.nn_i != m) & (other != self.mem_to_cells)
+        src = self.mem_to_cells[ok]
+        dst = other[ok]
+        ncell = len(self.cell_i)
+        cnt = np.bincount(src, minlength=ncell)
+        self.cell_nn_ptr = np.zeros(ncell + 1, dtype=np.int64)
+        self.cell_nn_ptr[1:] = np.cumsum(cnt)
+        self.cell_nn_idx = dst.astype(np.int64)       # src is sorted: membranes are cell-ordered
+        self.num_nn = cnt
+        self.cell_nn = _ragged([list(self.cell_nn_idx[self.cell_nn_ptr[c]:self.cell_nn_ptr[c + 1]])
+                                for c in range(ncell)])
 
     # ------------------------------------------------------------------
     def _build_grid(self, p: Parameters) -> None:
@@ -243,10 +257,11 @@ class Cells:
         self.sq_ptr = np.zeros(n_sq + 1, dtype=np.int64)
         self.sq_ptr[1:] = np.cumsum(counts)
         self.sq_mems = order.astype(np.int64)
-        for k in np.nonzero(counts)[0]:
-            lst = self.sq_mems[self.sq_ptr[k]:self.sq_ptr[k + 1]]
-            self.memSa_per_envSquare[k] = np.sum(self.mem_sa[lst])     # cells.py:1810-1813
-            self.mems_per_envSquare[k] = len(lst)
+        nz = np.nonzero(counts)[0]
+        if len(nz):
+            # sum of mem_sa of the square's membranes in membrane order (cells.py:1810-1813)
+            self.memSa_per_envSquare[nz] = np.add.reduceat(self.mem_sa[self.sq_mems], self.sq_ptr[nz])
+            self.mems_per_envSquare[nz] = counts[nz]
         self.envInds_inClust = np.nonzero(counts)[0]
 
         # tight-junction targets (cells.py:1824-1833)
@@ -301,11 +316,5 @@ class Cells:
         self.mem_i = np.arange(len(self.mem_to_cells))
         self.cell_to_mems = _ragged([np.arange(self.mem_ptr[c], self.mem_ptr[c + 1]) for c in range(ncell)])
         self.mem_nn = np.column_stack((self.mem_i, self.nn_i))
-        cell_nn = []
-        for c in range(ncell):
-            s = [int(self.mem_to_cells[j]) for m in self.cell_to_mems[c]
-                 for j in (self.nn_i[m],) if j != m and self.mem_to_cells[j] != c]
-            cell_nn.append(s)
-        self.cell_nn = _ragged(cell_nn)
-        self.num_nn = np.asarray([len(s) for s in cell_nn])
+        self._set_cell_nn()
         return self

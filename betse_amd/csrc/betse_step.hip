@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "betse_amd.h"
 
@@ -45,7 +46,7 @@ constexpr double FLOAT_NONCE = 1.0e-25;  // sim_toolbox.py:50
 
 // Constants derived on the host, each in the reference's evaluation order.
 struct KParams {
-    int n_cells, n_mems, ny, nx, n_sq, n_ions, i_na, i_k, i_ca, n_chan;
+    int n_cells, n_mems, n_sq, n_ions, i_na, i_k, i_ca, n_chan;
     int chan_kind[BT_MAX_CHAN];
     int chan_ion[BT_MAX_CHAN];
     int ion_chan[BT_MAX_IONS];      // channel acting on each ion, -1 if none
@@ -58,7 +59,6 @@ struct KParams {
     double c_bound[BT_MAX_IONS];
     double dt, tm, gj_len, F, RT, RT_gj, q, kbT, cm_inv;
     // NaK-ATPase (sim_toolbox.py:71-122)
-    double nak_coef;                // -3*block*alpha_NaK
     double k_per_na;                // -(2/3)
     double dG_RT;                   // deltaGATP/(R*T)
     double qn_scal;                 // (cADP*1e-3)*(cPi*1e-3)
@@ -68,10 +68,17 @@ struct KParams {
     double ca_coef;                 // -alpha_Ca
     double qca_num;                 // cADP*cPi
     double cATP, km_ca, atp_kmca, one_p_atp_kmca;
-    // environment
-    double delta, two_delta, env_vol; // env_vol = cell_height*delta**2
-    double surf_scale, mean_sa, kee, screen;
     double gw[5];
+};
+
+// Per-cluster constants of a batch, derived on the host.
+struct KCluster {
+    int ny, nx, sq_off, pad;
+    double delta, two_delta;
+    double env_vol;                 // cell_height*delta**2
+    double surf_scale;              // cell_height*delta**2 (rho_env_surf factor)
+    double mean_sa, kee, screen;    // kee = ko_env*eo*er
+    double nak_coef;                // -3*block*alpha_NaK
 };
 
 __device__ __forceinline__ bool halted(const unsigned* flags) {
@@ -95,7 +102,8 @@ __device__ __forceinline__ float cube(float x) { return powf(x, 3.0f); }
 
 // pumpNaKATP (sim_toolbox.py:71-122): returns f_Na; f_K = -(2/3)*f_Na.
 template <typename FT>
-__device__ __forceinline__ FT pump_nak(FT cNai, FT cNao, FT cKi, FT cKo, FT Vm, const KParams& P) {
+__device__ __forceinline__ FT pump_nak(FT cNai, FT cNao, FT cKi, FT cKo, FT Vm, double nak_coef,
+                                      const KParams& P) {
     FT a = cNao * (FT)1e-3;
     FT b = cKi * (FT)1e-3;
     FT Qnumo = (FT)P.qn_scal * cube(a) * (b * b);
@@ -111,7 +119,7 @@ __device__ __forceinline__ FT pump_nak(FT cNai, FT cNao, FT cKi, FT cKo, FT Vm, 
     FT numo_E = na * k2 * (FT)P.atp_km;
     FT denomo_E = ((FT)1 + na) * ((FT)1 + k2) * (FT)P.one_p_atp_km;
     FT fwd_co = numo_E / denomo_E;
-    return (FT)P.nak_coef * fwd_co * ((FT)1 - (Q / Keq));
+    return (FT)nak_coef * fwd_co * ((FT)1 - (Q / Keq));
 }
 
 // pumpCaATP (sim_toolbox.py:124-182) after no_negs on both concentrations.
@@ -162,7 +170,8 @@ __device__ __forceinline__ double with_nonces(double v, int n) {
 // ---------------------------------------------------------------------------
 struct MemCellArgs {
     const int32_t* mem_ptr; const int32_t* mem_cell; const int32_t* nn_mem; const int32_t* nn_cell;
-    const int32_t* mem_sq; const int32_t* mem_pos; const int32_t* tile_ptr;
+    const int32_t* mem_sq; const int32_t* mem_pos; const int32_t* tile_ptr; const int32_t* tile_cluster;
+    const KCluster* kcl;
     const double* mem_sa; const double* gjopen; const double* cell_vol; const double* diviterm; const double* Dm;
     const double* c_env;      // pre-diffusion env (membrane + NaK gathers, Q2)
     const double* c_mid;      // post-diffusion env (Ca pump and channel gathers)
@@ -195,6 +204,7 @@ k_mem_cell(MemCellArgs A, KParams P) {
     const int j = threadIdx.x % TMB;
     const int m = m0 + j;
     const int M = P.n_mems, C = P.n_cells, G2 = P.n_sq;
+    const double nak_coef = A.kcl[A.tile_cluster[tile]].nak_coef;
     if (threadIdx.x == 0) s_bad = 0;
 
     if (m < m1) {
@@ -211,7 +221,8 @@ k_mem_cell(MemCellArgs A, KParams P) {
         // the Na and the K wavefronts evaluate the same pump
         if (P.i_na >= 0 && (i == P.i_na || i == P.i_k)) {
             const FT fNa = pump_nak<FT>((FT)A.c_lag_in[P.i_na * C + c], (FT)A.c_env[P.i_na * G2 + k],
-                                        (FT)A.c_lag_in[P.i_k * C + c], (FT)A.c_env[P.i_k * G2 + k], (FT)v, P);
+                                        (FT)A.c_lag_in[P.i_k * C + c], (FT)A.c_env[P.i_k * G2 + k], (FT)v,
+                                        nak_coef, P);
             f = f + (double)((i == P.i_na) ? fNa : (FT)P.k_per_na * fNa);
             if (A.obs_rate && i == P.i_na) A.obs_rate[m] = -(double)fNa;
         }
@@ -325,6 +336,8 @@ k_mem_cell(MemCellArgs A, KParams P) {
 // k_env_field: Gaussian + E field from u, then update_ecm for every ion
 // ---------------------------------------------------------------------------
 struct FieldArgs {
+    const int* env_tiles;     // (cluster, first row, first column) of each 16x16 tile
+    const KCluster* kcl;
     const double* u;          // scaled surface charge of the last update_V
     const double* c_env; const double* D_env;
     double* c_mid;            // post-diffusion env (diffuse mode)
@@ -350,15 +363,17 @@ k_env_field(FieldArgs A, KParams P) {
     __shared__ double sD[ER][EC];
     __shared__ double sFx[TY][TX + 2];
     __shared__ double sFy[TY + 2][TX];
-    const int ny = P.ny, nx = P.nx;
-    const int r0 = blockIdx.y * TY, cc0 = blockIdx.x * TX;
+    const int* et = A.env_tiles + 3 * blockIdx.x;
+    const KCluster K = A.kcl[et[0]];
+    const int ny = K.ny, nx = K.nx, so = K.sq_off;
+    const int r0 = et[1], cc0 = et[2];
     const int tid = threadIdx.x;
-    const double dl = P.delta, d2 = P.two_delta;
+    const double dl = K.delta, d2 = K.two_delta;
 
     // u with zero padding outside the grid (gaussian_filter mode='constant')
     for (int e = tid; e < UR * UC; e += TY * TX) {
         const int rr = r0 - UH + e / UC, cc = cc0 - UH + e % UC;
-        sU[e / UC][e % UC] = (rr >= 0 && rr < ny && cc >= 0 && cc < nx) ? A.u[rr * nx + cc] : 0.0;
+        sU[e / UC][e % UC] = (rr >= 0 && rr < ny && cc >= 0 && cc < nx) ? A.u[so + rr * nx + cc] : 0.0;
     }
     __syncthreads();
     // axis-0 pass (scipy correlate1d: centre tap, then pairs outermost first)
@@ -387,12 +402,12 @@ k_env_field(FieldArgs A, KParams P) {
         if (r >= 0 && r < ny && cc >= 0 && cc < nx) {
             const int vr = er + 1, vc = ec + 1;         // (r, cc) in sV
             double gx, gy;
-            if (cc == 0) gx = (P.screen * sV[vr][vc + 1] - P.screen * sV[vr][vc]) / dl;
-            else if (cc == nx - 1) gx = (P.screen * sV[vr][vc] - P.screen * sV[vr][vc - 1]) / dl;
-            else gx = -(P.screen * sV[vr][vc - 1] - P.screen * sV[vr][vc + 1]) / d2;
-            if (r == 0) gy = (P.screen * sV[vr + 1][vc] - P.screen * sV[vr][vc]) / dl;
-            else if (r == ny - 1) gy = (P.screen * sV[vr][vc] - P.screen * sV[vr - 1][vc]) / dl;
-            else gy = -(P.screen * sV[vr - 1][vc] - P.screen * sV[vr + 1][vc]) / d2;
+            if (cc == 0) gx = (K.screen * sV[vr][vc + 1] - K.screen * sV[vr][vc]) / dl;
+            else if (cc == nx - 1) gx = (K.screen * sV[vr][vc] - K.screen * sV[vr][vc - 1]) / dl;
+            else gx = -(K.screen * sV[vr][vc - 1] - K.screen * sV[vr][vc + 1]) / d2;
+            if (r == 0) gy = (K.screen * sV[vr + 1][vc] - K.screen * sV[vr][vc]) / dl;
+            else if (r == ny - 1) gy = (K.screen * sV[vr][vc] - K.screen * sV[vr - 1][vc]) / dl;
+            else gy = -(K.screen * sV[vr - 1][vc] - K.screen * sV[vr + 1][vc]) / d2;
             ex = -gx;
             ey = -gy;
         }
@@ -403,16 +418,16 @@ k_env_field(FieldArgs A, KParams P) {
     if (!DIFFUSE) {
         const int r = r0 + tid / TX, cc = cc0 + tid % TX;
         if (r < ny && cc < nx) {
-            A.v_env[r * nx + cc] = sV[tid / TX + 2][tid % TX + 2];
-            A.Ex[r * nx + cc] = sEx[tid / TX + 1][tid % TX + 1];
-            A.Ey[r * nx + cc] = sEy[tid / TX + 1][tid % TX + 1];
+            A.v_env[so + r * nx + cc] = sV[tid / TX + 2][tid % TX + 2];
+            A.Ex[so + r * nx + cc] = sEx[tid / TX + 1][tid % TX + 1];
+            A.Ey[so + r * nx + cc] = sEy[tid / TX + 1][tid % TX + 1];
         }
         return;
     }
 
     for (int i = 0; i < P.n_ions; ++i) {
-        const double* cenv = A.c_env + (size_t)i * P.n_sq;
-        const double* D = A.D_env + (size_t)i * P.n_sq;
+        const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
+        const double* D = A.D_env + (size_t)i * P.n_sq + so;
         const double cb = P.c_bound[i];
         const double zi = P.zq[i];
         __syncthreads();
@@ -476,10 +491,10 @@ k_env_field(FieldArgs A, KParams P) {
                 else if (r == ny - 1) gy = -((-sFy[lr + 1][lc]) - (-sFy[lr][lc])) / dl;
                 else gy = -((-sFy[lr][lc]) - (-sFy[lr + 2][lc])) / d2;
                 const double div = gx + gy;
-                A.c_mid[(size_t)i * P.n_sq + r * nx + cc] = sC[lr + 2][lc + 2] + div * P.dt;
+                A.c_mid[(size_t)i * P.n_sq + so + r * nx + cc] = sC[lr + 2][lc + 2] + div * P.dt;
                 if (A.obs_fx) {
-                    A.obs_fx[(size_t)i * P.n_sq + r * nx + cc] = sFx[lr][lc + 1];
-                    A.obs_fy[(size_t)i * P.n_sq + r * nx + cc] = sFy[lr + 1][lc];
+                    A.obs_fx[(size_t)i * P.n_sq + so + r * nx + cc] = sFx[lr][lc + 1];
+                    A.obs_fy[(size_t)i * P.n_sq + so + r * nx + cc] = sFy[lr + 1][lc];
                 }
             }
         }
@@ -490,7 +505,7 @@ k_env_field(FieldArgs A, KParams P) {
 // k_env_exchange: membrane -> env exchange, rho_env -> surface charge
 // ---------------------------------------------------------------------------
 struct ExArgs {
-    const int32_t* sq_ptr; const uint8_t* has_mem;
+    const int32_t* sq_ptr; const uint8_t* has_mem; const int32_t* sq_cluster; const KCluster* kcl;
     const double* c_mid; const double* fsa; const double* fch; double* c_env; double* u;
     const unsigned* flags; long long* step_ctr; int count_step;
 };
@@ -502,6 +517,7 @@ k_env_exchange(ExArgs A, KParams P) {
     if (A.count_step && k == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
     if (k >= P.n_sq) return;
     const int a = A.sq_ptr[k], b = A.sq_ptr[k + 1];
+    const KCluster& K = A.kcl[A.sq_cluster[k]];
     double rho = 0.0;
     for (int i = 0; i < P.n_ions; ++i) {
         double cn = A.c_mid[(size_t)i * P.n_sq + k];
@@ -509,22 +525,23 @@ k_env_exchange(ExArgs A, KParams P) {
         if (ch >= 0) {
             double Sc = 0.0;
             for (int j = a; j < b; ++j) Sc = Sc + (-A.fch[(size_t)ch * P.n_mems + j]);
-            cn = cn + (Sc / P.env_vol) * P.dt;
+            cn = cn + (Sc / K.env_vol) * P.dt;
         }
         double S = 0.0;
         for (int j = a; j < b; ++j) S = S + (-A.fsa[(size_t)i * P.n_mems + j]);
-        cn = cn + (S / P.env_vol) * P.dt;
+        cn = cn + (S / K.env_vol) * P.dt;
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         rho = (i == 0) ? P.zF[0] * cn : rho + P.zF[i] * cn;
     }
     double uu = 0.0;
-    if (A.has_mem[k]) uu = ((rho * P.surf_scale) / P.mean_sa) / P.kee;
+    if (A.has_mem[k]) uu = ((rho * K.surf_scale) / K.mean_sa) / K.kee;
     A.u[k] = uu;
 }
 
 // rho_env -> u for the current env state (before a phase's loop)
 __global__ void __launch_bounds__(BLOCK)
-k_env_charge(const double* c_env, const uint8_t* has_mem, double* u, KParams P) {
+k_env_charge(const double* c_env, const uint8_t* has_mem, const int32_t* sq_cluster, const KCluster* kcl,
+             double* u, KParams P) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if (k >= P.n_sq) return;
     double rho = 0.0;
@@ -532,7 +549,8 @@ k_env_charge(const double* c_env, const uint8_t* has_mem, double* u, KParams P) 
         const double cn = c_env[(size_t)i * P.n_sq + k];
         rho = (i == 0) ? P.zF[0] * cn : rho + P.zF[i] * cn;
     }
-    u[k] = has_mem[k] ? ((rho * P.surf_scale) / P.mean_sa) / P.kee : 0.0;
+    const KCluster& K = kcl[sq_cluster[k]];
+    u[k] = has_mem[k] ? ((rho * K.surf_scale) / K.mean_sa) / K.kee : 0.0;
 }
 
 // Vmem of every cell from its concentrations (update_V cell part)
@@ -642,14 +660,14 @@ __device__ __forceinline__ double div_at(const double* Fx, const double* Fy, int
 // rhs of the stream-function solve (-div(-Jy, Jx)) and of the potential solve
 // (div(Jx, Jy)) on the interior, written row-major into ld-wide padded matrices
 __global__ void __launch_bounds__(BLOCK)
-k_obs_rhs(const double* jxo, const double* jyo, double* b1, double* b2, int ld, KParams P) {
-    const int n1 = P.ny - 2, n2 = P.nx - 2;
+k_obs_rhs(const double* jxo, const double* jyo, double* b1, double* b2, int ld, KCluster K, KParams P) {
+    const int n1 = K.ny - 2, n2 = K.nx - 2;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= n1 * n2) return;
     const int r = e / n2 + 1, c = e % n2 + 1;
     // Jxr = -Jy, Jyr = Jx
-    const double dr = div_at(jyo, jxo, r, c, P.ny, P.nx, P.delta, P.two_delta, -1.0, 1.0);
-    const double dd = div_at(jxo, jyo, r, c, P.ny, P.nx, P.delta, P.two_delta, 1.0, 1.0);
+    const double dr = div_at(jyo, jxo, r, c, K.ny, K.nx, K.delta, K.two_delta, -1.0, 1.0);
+    const double dd = div_at(jxo, jyo, r, c, K.ny, K.nx, K.delta, K.two_delta, 1.0, 1.0);
     b1[(r - 1) * ld + (c - 1)] = -dr;
     b2[(r - 1) * ld + (c - 1)] = dd;
 }
@@ -663,28 +681,28 @@ k_scale(double* x, const double* s, int n) {
 // interior solution -> full grid (boundary rows of A are identity/delta^2 and
 // their right-hand sides are zero here)
 __global__ void __launch_bounds__(BLOCK)
-k_scatter_interior(const double* X, int ld, double* out, KParams P) {
+k_scatter_interior(const double* X, int ld, double* out, KCluster K, KParams P) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if (k >= P.n_sq) return;
-    const int r = k / P.nx, c = k % P.nx;
-    const bool bnd = (r == 0 || c == 0 || r == P.ny - 1 || c == P.nx - 1);
+    const int r = k / K.nx, c = k % K.nx;
+    const bool bnd = (r == 0 || c == 0 || r == K.ny - 1 || c == K.nx - 1);
     out[k] = bnd ? 0.0 : X[(r - 1) * ld + (c - 1)];
 }
 
 // gradients of the stream function AA and potential BB -> HH_Decomp outputs
 __global__ void __launch_bounds__(BLOCK)
 k_obs_hh_out(const double* AA, const double* BB, double* Jx, double* Jy, double* Jtx, double* Jty,
-             double* Bf, double mu, KParams P) {
+             double* Bf, double mu, KCluster K, KParams P) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if (k >= P.n_sq) return;
-    const int ny = P.ny, nx = P.nx, r = k / nx, c = k % nx;
+    const int ny = K.ny, nx = K.nx, r = k / nx, c = k % nx;
     auto grad = [&](const double* F, double& gx, double& gy) {
-        if (c == 0) gx = (F[r * nx + 1] - F[r * nx]) / P.delta;
-        else if (c == nx - 1) gx = (F[r * nx + nx - 1] - F[r * nx + nx - 2]) / P.delta;
-        else gx = -(F[r * nx + c - 1] - F[r * nx + c + 1]) / P.two_delta;
-        if (r == 0) gy = (F[nx + c] - F[c]) / P.delta;
-        else if (r == ny - 1) gy = (F[(ny - 1) * nx + c] - F[(ny - 2) * nx + c]) / P.delta;
-        else gy = -(F[(r - 1) * nx + c] - F[(r + 1) * nx + c]) / P.two_delta;
+        if (c == 0) gx = (F[r * nx + 1] - F[r * nx]) / K.delta;
+        else if (c == nx - 1) gx = (F[r * nx + nx - 1] - F[r * nx + nx - 2]) / K.delta;
+        else gx = -(F[r * nx + c - 1] - F[r * nx + c + 1]) / K.two_delta;
+        if (r == 0) gy = (F[nx + c] - F[c]) / K.delta;
+        else if (r == ny - 1) gy = (F[(ny - 1) * nx + c] - F[(ny - 2) * nx + c]) / K.delta;
+        else gy = -(F[(r - 1) * nx + c] - F[(r + 1) * nx + c]) / K.two_delta;
     };
     double gAx, gAy, Gx, Gy;
     grad(AA, gAx, gAy);
@@ -699,7 +717,7 @@ k_obs_hh_out(const double* AA, const double* BB, double* Jx, double* Jy, double*
 
 // C = A*B (row-major, all sizes multiples of 16) with v_mfma_f64_16x16x4f64:
 // one wavefront per 16x16 tile of C. Lane l holds A[l%16][k0+l/16] and
-// B[k0+l/16][l%16]; result element r of lane l is C[4*(l/16)+r][l%16].
+// B[k0+l/16][l%16]; result element r of lane l is C[l/16+4*r][l%16].
 typedef double v4d __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256)
 k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
@@ -716,7 +734,24 @@ k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) C[(size_t)(tm * 16 + 4 * kk + r) * n + col] = acc[r];
+    for (int r = 0; r < 4; ++r) C[(size_t)(tm * 16 + kk + 4 * r) * n + col] = acc[r];
+}
+
+// per-cluster max over cells and ions of |c - c_prev| / |c| (init residual)
+__global__ void __launch_bounds__(BLOCK)
+k_residual(const double* c, const double* c_prev, const int32_t* cell_cluster, unsigned long long* rmax,
+           int n_cells, int n_ions) {
+    const int cidx = blockIdx.x * BLOCK + threadIdx.x;
+    if (cidx >= n_cells) return;
+    double r = 0.0;
+    for (int i = 0; i < n_ions; ++i) {
+        const double x = c[(size_t)i * n_cells + cidx];
+        const double d = fabs(x - c_prev[(size_t)i * n_cells + cidx]);
+        const double q = (x != 0.0) ? d / fabs(x) : d;
+        if (q > r || isnan(q)) r = isnan(q) ? INFINITY : q;
+    }
+    // non-negative doubles order like their bit patterns
+    atomicMax(&rmax[cell_cluster[cidx]], (unsigned long long)__double_as_longlong(r));
 }
 
 }  // namespace
@@ -728,6 +763,11 @@ struct bt_ctx {
     bt_params prm;
     bt_buffers buf;
     KParams kp;
+    std::vector<bt_cluster> clusters;
+    std::vector<KCluster> kcl_host;
+    KCluster* kcl = nullptr;        // device copy
+    int* env_tiles = nullptr;       // device (cluster, r0, c0) per env tile
+    int n_env_tiles = 0;
     hipStream_t stream = nullptr;
     int device = 0;
     int n_tiles = 0;
@@ -762,8 +802,7 @@ static int set_err(bt_ctx* c, int code, const std::string& msg) {
 
 static void derive_params(const bt_params& p, KParams& k) {
     memset(&k, 0, sizeof(k));
-    k.n_cells = p.n_cells; k.n_mems = p.n_mems; k.ny = p.grid_ny; k.nx = p.grid_nx;
-    k.n_sq = p.grid_ny * p.grid_nx; k.n_ions = p.n_ions;
+    k.n_cells = p.n_cells; k.n_mems = p.n_mems; k.n_sq = p.n_squares; k.n_ions = p.n_ions;
     k.i_na = p.i_na; k.i_k = p.i_k; k.i_ca = p.i_ca;
     k.n_chan = p.n_chan;
     for (int i = 0; i < BT_MAX_IONS; ++i) k.ion_chan[i] = -1;
@@ -784,7 +823,6 @@ static void derive_params(const bt_params& p, KParams& k) {
     k.dt = p.dt; k.tm = p.tm; k.gj_len = p.gj_len; k.F = p.F;
     k.RT = p.R * p.T; k.RT_gj = p.R * p.T_gj; k.q = p.q; k.kbT = p.kb * p.T;
     k.cm_inv = 1.0 / p.cm;
-    k.nak_coef = -3.0 * p.NaK_block * p.alpha_NaK;
     k.k_per_na = -(2.0 / 3.0);
     k.dG_RT = p.deltaGATP / (p.R * p.T);
     k.qn_scal = (p.cADP * 1e-3) * (p.cPi * 1e-3);
@@ -797,13 +835,21 @@ static void derive_params(const bt_params& p, KParams& k) {
     k.cATP = p.cATP; k.km_ca = p.KmCa_Ca;
     k.atp_kmca = p.cATP / p.KmCa_ATP;
     k.one_p_atp_kmca = 1.0 + (p.cATP / p.KmCa_ATP);
-    k.delta = p.delta; k.two_delta = 2.0 * p.delta;
-    k.env_vol = p.cell_height * (p.delta * p.delta);
-    k.surf_scale = p.cell_height * (p.delta * p.delta);
-    k.mean_sa = p.mean_sa;
-    k.kee = (p.ko_env * p.eo) * p.er;
-    k.screen = (2.0 / (p.ko_env * p.delta)) * (p.cell_radius / p.true_cell_size);
     for (int j = 0; j < 5; ++j) k.gw[j] = p.gauss_w[j];
+}
+
+static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
+    KCluster k;
+    memset(&k, 0, sizeof(k));
+    k.ny = b.grid_ny; k.nx = b.grid_nx; k.sq_off = b.sq_off;
+    k.delta = b.delta; k.two_delta = 2.0 * b.delta;
+    k.env_vol = p.cell_height * (b.delta * b.delta);
+    k.surf_scale = p.cell_height * (b.delta * b.delta);
+    k.mean_sa = b.mean_sa;
+    k.kee = (b.ko_env * p.eo) * p.er;
+    k.screen = (2.0 / (b.ko_env * b.delta)) * (p.cell_radius / p.true_cell_size);
+    k.nak_coef = -3.0 * p.NaK_block * b.alpha_NaK;
+    return k;
 }
 
 template <int NI>
@@ -831,9 +877,9 @@ static inline int grid_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr) {
     const bt_buffers& B = c->buf;
-    FieldArgs f{c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
+    FieldArgs f{c->env_tiles, c->kcl, c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
                 ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr};
-    dim3 grid((c->kp.nx + TX - 1) / TX, (c->kp.ny + TY - 1) / TY);
+    dim3 grid(c->n_env_tiles);
     if (diffuse)
         hipLaunchKernelGGL(k_env_field<true>, grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else
@@ -856,6 +902,7 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     if (which < 0 || which == 1) launch_field(c, true, ob);
     if (which < 0 || which == 0) {
         MemCellArgs a{B.mem_ptr, B.mem_cell, B.nn_mem, B.nn_cell, B.mem_sq, B.mem_pos, B.tile_ptr,
+                      B.tile_cluster, c->kcl,
                       B.mem_sa, B.gjopen, B.cell_vol, B.diviterm, B.Dm, B.c_env, c->c_mid,
                       ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->fch,
                       c->flags, c->bad_step, c->step_ctr,
@@ -864,7 +911,7 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
         launch_mem_cell(c, a);
     }
     if (which < 0 || which == 2) {
-        ExArgs x{B.sq_ptr, B.sq_has_mem, c->c_mid, c->fsa, c->fch, B.c_env, c->u, c->flags,
+        ExArgs x{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, B.c_env, c->u, c->flags,
                  c->step_ctr, which < 0 ? 1 : 0};
         hipLaunchKernelGGL(k_env_exchange, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, x, c->kp);
     }
@@ -888,13 +935,24 @@ const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : g
 
 int64_t bt_steps_done(const bt_ctx* ctx) { return ctx ? ctx->steps_done : -1; }
 
-int bt_create(bt_ctx** out, const bt_params* p, const bt_buffers* b, void* stream) {
-    if (!out || !p || !b) return set_err(nullptr, BT_EINVAL, "null argument");
+int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_clusters,
+              const bt_buffers* b, void* stream) {
+    if (!out || !p || !b || !cl) return set_err(nullptr, BT_EINVAL, "null argument");
     *out = nullptr;
     if (p->n_ions < 1 || p->n_ions > BT_MAX_IONS) return set_err(nullptr, BT_EINVAL, "n_ions out of range");
     if (p->n_cells < 1 || p->n_mems < 1) return set_err(nullptr, BT_EINVAL, "empty cluster");
-    if (p->grid_ny < 3 || p->grid_nx < 3) return set_err(nullptr, BT_EINVAL, "grid must be at least 3x3");
-    if ((long long)p->grid_ny * p->grid_nx > (1LL << 30)) return set_err(nullptr, BT_EINVAL, "grid too large");
+    if (n_clusters < 1) return set_err(nullptr, BT_EINVAL, "n_clusters < 1");
+    {
+        long long off = 0;
+        for (int q = 0; q < n_clusters; ++q) {
+            if (cl[q].grid_ny < 3 || cl[q].grid_nx < 3) return set_err(nullptr, BT_EINVAL, "grid must be at least 3x3");
+            if (cl[q].sq_off != off) return set_err(nullptr, BT_EINVAL, "cluster grids must be consecutive");
+            if (!(cl[q].delta > 0) || !(cl[q].mean_sa > 0) || !(cl[q].ko_env > 0))
+                return set_err(nullptr, BT_EINVAL, "cluster constants must be positive");
+            off += (long long)cl[q].grid_ny * cl[q].grid_nx;
+        }
+        if (off != p->n_squares || off > (1LL << 30)) return set_err(nullptr, BT_EINVAL, "n_squares mismatch");
+    }
     if (p->i_na >= p->n_ions || p->i_k >= p->n_ions || p->i_ca >= p->n_ions)
         return set_err(nullptr, BT_EINVAL, "ion slot out of range");
     if ((p->i_na < 0) != (p->i_k < 0)) return set_err(nullptr, BT_EINVAL, "NaK pump needs both Na and K");
@@ -914,7 +972,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_buffers* b, void* strea
     if (p->n_chan > 0 && (!b->gates || !b->chan_mod))
         return set_err(nullptr, BT_EINVAL, "channel buffers missing");
     const void* req[] = {b->mem_ptr, b->mem_cell, b->nn_mem, b->nn_cell, b->mem_sq, b->mem_pos, b->sq_ptr,
-                         b->tile_ptr, b->mem_sa, b->gjopen, b->cell_vol, b->diviterm, b->Dm, b->D_env,
+                         b->tile_ptr, b->tile_cluster, b->sq_cluster, b->mem_sa, b->gjopen, b->cell_vol, b->diviterm, b->Dm, b->D_env,
                          b->sq_has_mem, b->c_cell, b->c_lag, b->vm_cell, b->c_env, b->E_x, b->E_y, b->v_env};
     for (const void* q : req)
         if (!q) return set_err(nullptr, BT_EINVAL, "null device buffer");
@@ -925,8 +983,20 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_buffers* b, void* strea
     c->n_tiles = b->n_tiles;
     c->stream = (hipStream_t)stream;
     derive_params(*p, c->kp);
+    c->clusters.assign(cl, cl + n_clusters);
+    std::vector<int> tiles;
+    for (int q = 0; q < n_clusters; ++q) {
+        c->kcl_host.push_back(derive_cluster(*p, cl[q]));
+        for (int r0 = 0; r0 < cl[q].grid_ny; r0 += TY)
+            for (int c0 = 0; c0 < cl[q].grid_nx; c0 += TX) {
+                tiles.push_back(q);
+                tiles.push_back(r0);
+                tiles.push_back(c0);
+            }
+    }
+    c->n_env_tiles = (int)(tiles.size() / 3);
     (void)hipGetDevice(&c->device);
-    const size_t nc = p->n_cells, nm = p->n_mems, ni = p->n_ions, ns = (size_t)p->grid_ny * p->grid_nx;
+    const size_t nc = p->n_cells, nm = p->n_mems, ni = p->n_ions, ns = (size_t)p->n_squares;
     hipError_t e = hipSuccess;
     auto alloc = [&](void** ptr, size_t bytes) {
         if (e == hipSuccess) e = hipMalloc(ptr, bytes < 256 ? 256 : bytes);
@@ -941,6 +1011,11 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_buffers* b, void* strea
     alloc((void**)&c->flags, 256);
     alloc((void**)&c->bad_step, 256);
     alloc((void**)&c->step_ctr, 256);
+    alloc((void**)&c->kcl, sizeof(KCluster) * (size_t)n_clusters);
+    alloc((void**)&c->env_tiles, sizeof(int) * tiles.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(c->kcl, c->kcl_host.data(), sizeof(KCluster) * (size_t)n_clusters, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->env_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->bad_step, 0xff, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->step_ctr, 0, 256, c->stream);
@@ -961,7 +1036,7 @@ int bt_destroy(bt_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u,
-                    c->flags, c->bad_step, c->step_ctr, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
+                    c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
                     c->B1, c->B2, c->T1, c->T2, c->AA, c->BB};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -975,7 +1050,8 @@ int bt_update_v(bt_ctx* c) {
     hipLaunchKernelGGL(k_cell_vm, dim3(grid_blocks(c->kp.n_cells)), dim3(BLOCK), 0, c->stream,
                        (const double*)c->buf.c_cell, c->buf.diviterm, c->buf.vm_cell, c->kp);
     hipLaunchKernelGGL(k_env_charge, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream,
-                       (const double*)c->buf.c_env, c->buf.sq_has_mem, c->u, c->kp);
+                       (const double*)c->buf.c_env, c->buf.sq_has_mem, c->buf.sq_cluster,
+                       (const KCluster*)c->kcl, c->u, c->kp);
     launch_field(c, false);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
@@ -1100,14 +1176,14 @@ static int dst_solve(bt_ctx* c, double* Bm, double* out) {
     if ((rc = launch_gemm(c, c->S1, c->T2, c->T1, c->n1p, c->n2p, c->n1p))) return rc;
     if ((rc = launch_gemm(c, c->T1, c->S2, c->T2, c->n1p, c->n2p, c->n2p))) return rc;
     hipLaunchKernelGGL(k_scatter_interior, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
-                       (const double*)c->T2, c->n2p, out, c->kp);
+                       (const double*)c->T2, c->n2p, out, c->kcl_host[0], c->kp);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }
 
 static int obs_workspace(bt_ctx* c) {
     if (c->S1) return BT_OK;
-    const int n1 = c->kp.ny - 2, n2 = c->kp.nx - 2;
+    const int n1 = c->kcl_host[0].ny - 2, n2 = c->kcl_host[0].nx - 2;
     const int n1p = (n1 + 15) / 16 * 16, n2p = (n2 + 15) / 16 * 16;
     c->n1 = n1; c->n2 = n2; c->n1p = n1p; c->n2p = n2p;
     const size_t ns = (size_t)c->kp.n_sq;
@@ -1132,7 +1208,7 @@ static int obs_workspace(bt_ctx* c) {
         for (int k = 0; k < n1; ++k) h1[(size_t)j * n1p + k] = sin(M_PI * (j + 1) * (k + 1) / (n1 + 1));
     for (int j = 0; j < n2; ++j)
         for (int k = 0; k < n2; ++k) h2[(size_t)j * n2p + k] = sin(M_PI * (j + 1) * (k + 1) / (n2 + 1));
-    const double dl2 = c->prm.delta * c->prm.delta;
+    const double dl2 = c->kcl_host[0].delta * c->kcl_host[0].delta;
     const double scale = 4.0 / ((double)(n1 + 1) * (double)(n2 + 1));
     for (int j = 0; j < n1; ++j)
         for (int k = 0; k < n2; ++k) {
@@ -1151,6 +1227,7 @@ static int obs_workspace(bt_ctx* c) {
 
 int bt_observables(bt_ctx* c, const bt_obs* o) {
     if (!c || !o) return BT_EINVAL;
+    if (c->clusters.size() != 1) return set_err(c, BT_EINVAL, "observables are computed for one cluster");
     const void* need[] = {o->fluxes_mem, o->fluxes_gj, o->fluxes_env_x, o->fluxes_env_y, o->Jn, o->I_mem,
                           o->E_gj_x, o->E_gj_y, o->J_cell_x, o->J_cell_y, o->vm_ave, o->J_env_x, o->J_env_y,
                           o->Jtx, o->Jty, o->B_field};
@@ -1172,12 +1249,23 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
     HIP_TRY(c, hipMemsetAsync(c->B1, 0, pb, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->B2, 0, pb, c->stream));
     hipLaunchKernelGGL(k_obs_rhs, dim3(grid_blocks(c->n1 * c->n2)), dim3(BLOCK), 0, c->stream,
-                       (const double*)c->jxo, (const double*)c->jyo, c->B1, c->B2, c->n2p, c->kp);
+                       (const double*)c->jxo, (const double*)c->jyo, c->B1, c->B2, c->n2p, c->kcl_host[0], c->kp);
     if ((rc = dst_solve(c, c->B1, c->AA))) return rc;
     if ((rc = dst_solve(c, c->B2, c->BB))) return rc;
     hipLaunchKernelGGL(k_obs_hh_out, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
                        (const double*)c->AA, (const double*)c->BB, o->J_env_x, o->J_env_y, o->Jtx, o->Jty,
-                       o->B_field, c->prm.mu, c->kp);
+                       o->B_field, c->prm.mu, c->kcl_host[0], c->kp);
+    HIP_TRY(c, hipGetLastError());
+    return BT_OK;
+}
+
+int bt_residual(bt_ctx* c, const double* c_prev, const int32_t* cell_cluster, double* r_out) {
+    if (!c || !c_prev || !cell_cluster || !r_out) return BT_EINVAL;
+    const size_t nq = c->clusters.size();
+    HIP_TRY(c, hipMemsetAsync(r_out, 0, nq * sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_residual, dim3(grid_blocks(c->kp.n_cells)), dim3(BLOCK), 0, c->stream,
+                       (const double*)c->buf.c_cell, c_prev, cell_cluster, (unsigned long long*)r_out,
+                       c->kp.n_cells, c->kp.n_ions);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }

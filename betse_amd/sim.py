@@ -132,32 +132,9 @@ def make_tiles(mem_ptr: np.ndarray, tile_m: int = TILE_M) -> np.ndarray:
     return np.asarray(tiles, dtype=np.int32)
 
 
-class Simulator:
-    """GPU-backed BETSE ``Simulator`` for the full electrodiffusion solver.
-
-    ``flux_mode`` is ``"fp64"`` (every operation in FP64, the parity path) or
-    ``"fp32"`` (flux laws in FP32; concentrations, charge sums and the
-    charge-to-voltage step in FP64).
-    """
-
-    def __init__(self, p: Optional[Parameters] = None, *, device: Optional[str] = None,
-                 flux_mode: str = "fp64", use_graph: bool = True):
-        import torch
-        if not torch.cuda.is_available():
-            raise BetseSimException("no GPU visible: the GPU simulator has no CPU fallback")
-        if flux_mode not in ("fp64", "fp32"):
-            raise ValueError(flux_mode)
-        self.p = p
-        self.flux_mode = flux_mode
-        self.use_graph = use_graph
-        self.torch = torch
-        self.device = torch.device(device or "cuda")
-        self.stream = torch.cuda.Stream(device=self.device)     # default priority
-        self._lib = _native.load()
-        self._ctx = ctypes.c_void_p()
-        self._geom_key = None
-        self._dev: Dict[str, "torch.Tensor"] = {}
-        self.exception_instability = None
+class HostState:
+    """Host-side state of one cluster, set up as the reference's ``Simulator``
+    does before its loop (``init_core``, ``init_dynamics``); no device work."""
 
     # ------------------------------------------------------------------
     # set-up mirrored from the reference
@@ -334,6 +311,34 @@ class Simulator:
     def rho_cells(self) -> np.ndarray:
         return np.dot(self.zs * self.p.F, self.cc_cells)
 
+
+class Simulator(HostState):
+    """GPU-backed BETSE ``Simulator`` for the full electrodiffusion solver.
+
+    ``flux_mode`` is ``"fp64"`` (every operation in FP64, the parity path) or
+    ``"fp32"`` (flux laws in FP32; concentrations, charge sums and the
+    charge-to-voltage step in FP64).
+    """
+
+    def __init__(self, p: Optional[Parameters] = None, *, device: Optional[str] = None,
+                 flux_mode: str = "fp64", use_graph: bool = True):
+        import torch
+        if not torch.cuda.is_available():
+            raise BetseSimException("no GPU visible: the GPU simulator has no CPU fallback")
+        if flux_mode not in ("fp64", "fp32"):
+            raise ValueError(flux_mode)
+        self.p = p
+        self.flux_mode = flux_mode
+        self.use_graph = use_graph
+        self.torch = torch
+        self.device = torch.device(device or "cuda")
+        self.stream = torch.cuda.Stream(device=self.device)     # default priority
+        self._lib = _native.load()
+        self._ctx = ctypes.c_void_p()
+        self._geom_key = None
+        self._dev: Dict[str, "torch.Tensor"] = {}
+        self.exception_instability = None
+
     # ------------------------------------------------------------------
     # device set-up
     # ------------------------------------------------------------------
@@ -363,6 +368,8 @@ class Simulator:
         tiles = make_tiles(cells.mem_ptr)
         d["tile_ptr"] = self._t(tiles, i32)
         self.n_tiles = len(tiles) - 1
+        d["tile_cluster"] = self._t(np.zeros(self.n_tiles, dtype=np.int32), i32)
+        d["sq_cluster"] = self._t(np.zeros(G2, dtype=np.int32), i32)
         d["mem_sa"] = self._t(cells.mem_sa, f64)
         d["cell_vol"] = self._t(cells.cell_vol, f64)
         d["diviterm"] = self._t(cells.diviterm, f64)
@@ -400,7 +407,14 @@ class Simulator:
 
         prm = _native.BtParams()
         prm.n_cells, prm.n_mems = self.cdl, self.mdl
-        prm.grid_ny, prm.grid_nx = cells.X.shape
+        prm.n_squares = cells.X.size
+        cl = _native.BtCluster()
+        cl.grid_ny, cl.grid_nx = cells.X.shape
+        cl.sq_off = 0
+        cl.delta = cells.delta
+        cl.mean_sa = float(cells.memSa_per_envSquare[cells.map_mem2ecm].mean())
+        cl.ko_env = float(self.ko_env)
+        cl.alpha_NaK = p.alpha_NaK
         prm.n_ions = len(self.names)
         prm.i_na = self.names.index("Na") if ("Na" in self.names and "K" in self.names and p.alpha_NaK > 0.0) else -1
         prm.i_k = self.names.index("K") if prm.i_na >= 0 else -1
@@ -422,13 +436,12 @@ class Simulator:
         prm.dt, prm.T, prm.T_gj = p.dt, self.T, p.T
         prm.F, prm.R, prm.q, prm.kb, prm.cm, prm.tm = p.F, p.R, p.q, p.kb, p.cm, p.tm
         prm.gj_len, prm.gj_surface = cells.gj_len, p.gj_surface
-        prm.alpha_NaK, prm.NaK_block, prm.deltaGATP = p.alpha_NaK, float(self.NaKATP_block), p.deltaGATP
+        prm.NaK_block, prm.deltaGATP = float(self.NaKATP_block), p.deltaGATP
         prm.cATP, prm.cADP, prm.cPi = p.cATP, p.cADP, p.cPi
         prm.KmNK_Na, prm.KmNK_K, prm.KmNK_ATP = p.KmNK_Na, p.KmNK_K, p.KmNK_ATP
         prm.alpha_Ca, prm.KmCa_Ca, prm.KmCa_ATP = p.alpha_Ca, p.KmCa_Ca, p.KmCa_ATP
-        prm.delta, prm.cell_height = cells.delta, p.cell_height
-        prm.mean_sa = float(cells.memSa_per_envSquare[cells.map_mem2ecm].mean())
-        prm.ko_env, prm.eo, prm.er = float(self.ko_env), p.eo, p.er
+        prm.cell_height = p.cell_height
+        prm.eo, prm.er = p.eo, p.er
         prm.cell_radius, prm.true_cell_size = p.cell_radius, p.true_cell_size
         for j, w in enumerate(gaussian_taps(1.0)):
             prm.gauss_w[j] = float(w)
@@ -437,16 +450,17 @@ class Simulator:
 
         buf = _native.BtBuffers()
         for name in ("mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "mem_pos", "tile_ptr",
+                     "tile_cluster", "sq_cluster",
                      "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
                      "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "mem_nx", "mem_ny",
                      "cell_sa", "gates", "chan_mod"):
             setattr(buf, name, d[name].data_ptr())
         buf.n_tiles = self.n_tiles
         buf.tile_m = _native.BT_TILE_MEMBRANES
-        self._prm, self._buf = prm, buf
+        self._prm, self._buf, self._cl = prm, buf, cl
         with torch.cuda.device(self.device):
-            rc = self._lib.bt_create(ctypes.byref(self._ctx), ctypes.byref(prm), ctypes.byref(buf),
-                                     ctypes.c_void_p(self.stream.cuda_stream))
+            rc = self._lib.bt_create(ctypes.byref(self._ctx), ctypes.byref(prm), ctypes.byref(cl), 1,
+                                     ctypes.byref(buf), ctypes.c_void_p(self.stream.cuda_stream))
         if rc != _native.BT_OK:
             raise BetseNativeError(f"bt_create failed ({rc}): {self._lib.bt_last_error(None).decode()}")
 

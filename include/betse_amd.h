@@ -45,9 +45,23 @@ extern "C" {
 
 typedef struct bt_ctx bt_ctx;
 
+/* One cluster of a batch: its environment grid and the constants that differ
+ * between independent clusters (host memory, copied by bt_create). A single
+ * simulation is a batch of one. Cells, membranes and squares of all clusters
+ * are concatenated in the bt_buffers arrays; indices are global. */
+typedef struct {
+    int32_t grid_ny, grid_nx;           /* this cluster's environment grid     */
+    int32_t sq_off;                     /* first square of the grid           */
+    int32_t reserved;
+    double delta;                       /* grid spacing (cells.py:1607)        */
+    double mean_sa;                     /* memSa_per_envSquare[map_mem2ecm].mean() (ion_current.py:93-95) */
+    double ko_env;                      /* frozen per phase (sim.py:974, Q13) */
+    double alpha_NaK;                   /* NaK-ATPase rate (sim_config.yaml:1030) */
+} bt_cluster;
+
 /* Sizes, constants and switches; the names follow betse/science/parameters.py. */
 typedef struct {
-    int32_t n_cells, n_mems, grid_ny, grid_nx, n_ions;
+    int32_t n_cells, n_mems, n_squares, reserved0, n_ions;
     int32_t i_na, i_k, i_ca;            /* ion slots, -1 if the ion is absent */
     int32_t flux_mode;                  /* BT_FLUX_FP64 / BT_FLUX_FP32 */
     int32_t n_chan;                     /* voltage-gated channels on every membrane (0..BT_MAX_CHAN) */
@@ -61,20 +75,20 @@ typedef struct {
     double dt, T, T_gj;                 /* sim.T for membrane/pump/env, p.T for GJ (Q7) */
     double F, R, q, kb, cm, tm;
     double gj_len, gj_surface;
-    double alpha_NaK, NaK_block, deltaGATP, cATP, cADP, cPi;
+    double NaK_block, deltaGATP, cATP, cADP, cPi;
     double KmNK_Na, KmNK_K, KmNK_ATP;
     double alpha_Ca, KmCa_Ca, KmCa_ATP;
-    double delta, cell_height;
-    double mean_sa;                     /* memSa_per_envSquare[map_mem2ecm].mean() (ion_current.py:93-95) */
-    double ko_env, eo, er;              /* ko_env is frozen per phase (sim.py:974, Q13) */
+    double cell_height;
+    double eo, er;
     double cell_radius, true_cell_size;
     double gauss_w[5];                  /* scipy gaussian_filter1d taps, centre first */
     double smooth_cells;                /* 1/'sharpness cell' (parameters.py:1146), Jn smoothing */
     double mu;                          /* magnetic permeability (parameters.py:1111), B_field */
 } bt_params;
 
-/* Device buffers. Layouts: per-ion fields are [n_ions][n], grid fields are
- * row-major [grid_ny][grid_nx] (rows = y = axis 0), indices are int32. */
+/* Device buffers. Layouts: per-ion fields are [n_ions][n], each cluster's grid
+ * is row-major [grid_ny][grid_nx] (rows = y = axis 0) at its sq_off, indices
+ * are int32 and global over the batch. */
 typedef struct {
     /* geometry (read only) */
     const int32_t* mem_ptr;     /* [n_cells+1] CSR of each cell's membranes       */
@@ -86,7 +100,9 @@ typedef struct {
     const int32_t* sq_mems;     /* [n_mems] membranes sorted by square            */
     const int32_t* mem_pos;     /* [n_mems] position of each membrane in sq_mems  */
     const int32_t* tile_ptr;    /* [n_tiles+1] first cell of each tile of whole cells
-                                   holding <= BT_TILE_MEMBRANES membranes         */
+                                   of one cluster holding <= BT_TILE_MEMBRANES membranes */
+    const int32_t* tile_cluster;/* [n_tiles] cluster of each tile                 */
+    const int32_t* sq_cluster;  /* [n_squares] cluster of each square             */
     const double* mem_sa;       /* [n_mems]                                       */
     const double* gjopen;       /* [n_mems] gj_block*gj_default_weights           */
     const double* cell_vol;     /* [n_cells]                                      */
@@ -119,7 +135,8 @@ typedef struct {
  * `stream` is a hipStream_t (may be NULL for the default stream; graphs need a
  * non-default stream). Replaces the state set-up of init_core/init_dynamics
  * (sim.py:452-1012) on the device side. */
-int bt_create(bt_ctx** out, const bt_params* params, const bt_buffers* buffers, void* stream);
+int bt_create(bt_ctx** out, const bt_params* params, const bt_cluster* clusters, int32_t n_clusters,
+              const bt_buffers* buffers, void* stream);
 
 /* Release the workspace, graphs and the context. */
 int bt_destroy(bt_ctx* ctx);
@@ -168,13 +185,50 @@ int bt_step_observed(bt_ctx* ctx, int32_t n_steps, int32_t use_graph, const bt_o
 
 /* Derive the observables from the fluxes captured in obs. The two Poisson
  * solves of HH_Decomp (sim_toolbox.py:1236-1290, dense lapENVinv in the
- * reference) run as DST-I transforms written as FP64 MFMA matrix products. */
+ * reference) run as DST-I transforms written as FP64 MFMA matrix products.
+ * Contexts of one cluster only. */
 int bt_observables(bt_ctx* ctx, const bt_obs* obs);
 
 /* C = A*B for row-major FP64 matrices whose sizes are multiples of 16, on the
  * context stream with the MFMA kernel the observables use (exposed for tests). */
 int bt_gemm_f64(bt_ctx* ctx, const double* A, const double* B, double* C,
                 int32_t m, int32_t n, int32_t k);
+
+/* ------------------------------------------------------------------------
+ * Warm-start graph network (no counterpart in the reference: it predicts the
+ * state the init phase reaches, from which the plain stepper then runs to its
+ * own residual tolerance). FP32 device buffers, asynchronous on `stream`.
+ * ------------------------------------------------------------------------ */
+
+/* C(m x n) = op(A) op(B) (beta=0) or C += op(A) op(B) (beta=1), where
+ * A(i,k) = A[i*sam + k*sak] and B(k,j) = B[k*sbk + j*sbn]; operands are rounded
+ * to BF16 and multiplied on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation.
+ * K is split over ksplit blocks whose partials go to ws (ksplit*m*n floats);
+ * ws is also needed when beta=1. */
+int bt_gnn_gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn,
+                float* C, int64_t ldc, int32_t m, int32_t n, int32_t k, int32_t beta,
+                float* ws, int32_t ksplit, void* stream);
+/* Z += b (per column); H = silu(Z) + Hres (Hres may be NULL). */
+int bt_gnn_bias_silu(float* Z, const float* b, float* H, const float* Hres, int32_t n, int32_t f, void* stream);
+/* Y += b (per column). */
+int bt_gnn_bias(float* Y, const float* b, int32_t n, int32_t f, void* stream);
+/* dZ = dH * silu'(Z). */
+int bt_gnn_silu_bwd(const float* dH, const float* Z, float* dZ, int64_t count, void* stream);
+/* out_i = mean of H_j over the CSR neighbours of i (degree clamped to 1). */
+int bt_gnn_mean_agg(const float* H, int64_t ldh, const int32_t* ptr, const int32_t* idx, float* out, int64_t ldo,
+                    int32_t n, int32_t f, void* stream);
+/* Adjoint of bt_gnn_mean_agg on a symmetric graph: out_j (+)= sum_{i in N(j)} G_i / deg_i. */
+int bt_gnn_mean_agg_T(const float* G, int64_t ldg, const int32_t* ptr, const int32_t* idx, float* out,
+                      int64_t ldo, int32_t n, int32_t f, int32_t accumulate, void* stream);
+/* out_c = sum_r G[r][c], with per-block partials in ws (ws_len floats). */
+int bt_gnn_colsum(const float* G, int64_t ldg, int32_t n, int32_t f, float* out, float* ws, int32_t ws_len,
+                  void* stream);
+/* dY = 2 (Y-T) mask inv_n; part[b] = partial sums of (Y-T)^2 mask inv_n per 256 elements. */
+int bt_gnn_mse(const float* Y, const float* T, const float* mask, int64_t count, float inv_n, float* dY,
+               float* part, int32_t part_len, void* stream);
+/* One Adam update (step t >= 1). */
+int bt_gnn_adam(float* p, const float* g, float* m, float* v, int64_t count, float lr, float b1, float b2,
+                float eps, int32_t t, void* stream);
 
 /* Wait for the stream; return BT_OK or BT_UNSTABLE (a NaN was produced; the
  * state is frozen at the step that produced it and *bad_step receives its
@@ -186,6 +240,13 @@ int bt_sync(bt_ctx* ctx, int64_t* bad_step);
  * timed with HIP events over n_launches back-to-back launches on the context
  * stream. Leaves the state undefined: call after the timed work. */
 int bt_time_kernel(bt_ctx* ctx, int32_t kernel, int32_t n_launches, double* ms_out);
+
+/* Init residual (no counterpart in the reference, whose init runs a fixed
+ * number of steps, parameters.py:525): r_out[q] = max over the cells of
+ * cluster q and the ions of |c - c_prev| / |c|, c the current cc_cells and
+ * c_prev a caller copy ([n_ions][n_cells]); cell_cluster[n_cells] maps cells
+ * to clusters. r_out is device memory of n_clusters doubles. Asynchronous. */
+int bt_residual(bt_ctx* ctx, const double* c_prev, const int32_t* cell_cluster, double* r_out);
 
 /* Steps enqueued since bt_create. */
 int64_t bt_steps_done(const bt_ctx* ctx);

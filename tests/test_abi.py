@@ -40,26 +40,24 @@ def test_load_fails_loudly_when_missing(tmp_path):
 
 
 def test_struct_layout_matches_c(tmp_path):
-    fields_p = [f[0] for f in _native.BtParams._fields_]
-    fields_b = [f[0] for f in _native.BtBuffers._fields_]
-    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "betse_amd.h"', 'int main(void){',
-           'printf("%zu %zu\\n", sizeof(bt_params), sizeof(bt_buffers));']
-    for f in fields_p:
-        src.append(f'printf("p {f} %zu\\n", offsetof(bt_params, {f}));')
-    for f in fields_b:
-        src.append(f'printf("b {f} %zu\\n", offsetof(bt_buffers, {f}));')
+    structs = {"p": ("bt_params", _native.BtParams), "b": ("bt_buffers", _native.BtBuffers),
+               "c": ("bt_cluster", _native.BtCluster), "o": ("bt_obs", _native.BtObs)}
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "betse_amd.h"', 'int main(void){']
+    for tag, (cname, cls) in structs.items():
+        src.append(f'printf("s {tag} %zu\\n", sizeof({cname}));')
+        for f in (x[0] for x in cls._fields_):
+            src.append(f'printf("{tag} {f} %zu\\n", offsetof({cname}, {f}));')
     src.append("return 0;}")
     c = tmp_path / "layout.c"
     c.write_text("\n".join(src))
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
-    sp, sb = map(int, out[0].split())
-    assert sp == ctypes.sizeof(_native.BtParams)
-    assert sb == ctypes.sizeof(_native.BtBuffers)
-    for line in out[1:]:
+    for line in out:
         if not line:
             continue
-        kind, name, off = line.split()
-        cls = _native.BtParams if kind == "p" else _native.BtBuffers
-        assert getattr(cls, name).offset == int(off), name
+        a, b, c = line.split()
+        if a == "s":
+            assert int(c) == ctypes.sizeof(structs[b][1]), b
+        else:
+            assert getattr(structs[a][1], b).offset == int(c), (a, b)

@@ -1,0 +1,63 @@
+"""A batch of independent clusters stepped together, each against its own
+oracle run (per-cluster grids, spacing, diffusivities, pump rate)."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from betse_amd.batch import batched_worlds
+from betse_amd.channels import draw_channel_state
+from oracle.betse_oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_batch(worlds, n, flux_mode):
+    from betse_amd.batch import BatchSimulator
+    b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=flux_mode)
+    b.init("init")
+    b.step_n(n)
+    b.pull()
+    return b
+
+
+@pytest.mark.parametrize("flux_mode,vm_tol,rel_tol", [("fp64", 1e-9, 1e-10), ("fp32", 1e-6, 1e-6)])
+def test_batch_matches_per_cluster_oracle(flux_mode, vm_tol, rel_tol):
+    worlds = batched_worlds(n_clusters=5, cells_range=(30, 400), grid_range=(16, 40), seed=3)
+    n = 60
+    say(f"case: batch of 5 clusters, {flux_mode}, {n} steps: cells "
+        f"{[c.n_cells for _, c in worlds]} grids {[c.X.shape[0] for _, c in worlds]}")
+    b = run_batch(worlds, n, flux_mode)
+    for k, ((p, cells), st) in enumerate(zip(worlds, b.states)):
+        o = OracleSim(cells, p.copy(), mode="sparse")
+        o.init_core()
+        o.run_sim_core("init", n_steps=n)
+        dv = np.abs(st.vm - o.vm).max()
+        dc = (np.abs(st.cc_cells - o.cc_cells) / np.abs(o.cc_cells)).max()
+        de = (np.abs(st.cc_env - o.cc_env) / np.abs(o.cc_env)).max()
+        say(f"cluster {k}: |dVm|={dv:.3e} rel dc={dc:.3e} rel denv={de:.3e}")
+        assert dv <= vm_tol and dc <= rel_tol and de <= rel_tol
+
+
+def test_batch_with_channels():
+    worlds = batched_worlds(n_clusters=3, cells_range=(30, 200), grid_range=(16, 24), seed=5,
+                            base="tiny_channels")
+    for p, c in worlds:
+        draw_channel_state(p, c.n_cells, c.n_mems)
+    say("case: batch of 3 clusters with Nav/Kv, fp64, 40 steps")
+    b = run_batch(worlds, 40, "fp64")
+    for k, ((p, cells), st) in enumerate(zip(worlds, b.states)):
+        o = OracleSim(cells, p.copy(), mode="sparse")
+        o.init_core()
+        o.run_sim_core("init", n_steps=40)
+        dv = np.abs(st.vm - o.vm).max()
+        say(f"cluster {k}: |dVm|={dv:.3e}")
+        assert dv <= 1e-9
+        np.testing.assert_allclose(st.cc_cells, o.cc_cells, rtol=1e-10)
+        np.testing.assert_allclose(st.chan_gates[0, 0], o.channels[0].m, rtol=1e-10, atol=1e-14)
