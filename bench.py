@@ -160,12 +160,64 @@ def bench_batched(a):
     print(json.dumps(line), flush=True)
 
 
+def bench_warmstart(a):
+    """Learned warm start: init-phase steady states of random clusters from the
+    plain method (labels), the network trained on the GPU, then init time to
+    the residual tolerance on held-out clusters, cold versus warm."""
+    from betse_amd import warmstart as ws
+
+    def log(msg):
+        print("# " + msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    data, stats, base = ws.make_dataset(a.train_clusters, seed=1000, batch=a.clusters, max_steps=a.max_init_steps,
+                                        flux_mode="fp64", log=log)
+    t_data = time.perf_counter() - t0
+    n_ok = sum(d["ok"] for d in data)
+    log(f"dataset: {len(data)} clusters, {n_ok} converged, {t_data:.1f}s")
+    starter, hist, t_train = ws.train(data, epochs=a.epochs, log=log)
+    if a.save_model:
+        starter.save(a.save_model)
+    ev = ws.evaluate(starter, a.eval_clusters, seed=999_000, max_steps=a.max_init_steps, flux_mode="fp64", log=log)
+    log(f"evaluation: {ev}")
+    line = {
+        "metric": "init seconds to steady state (residual < 1e-6), warm start vs default start",
+        "value": ev["warm_seconds"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": int(ev["warm_max_steps"]),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 step; bf16-mfma/f32 network",
+        "data": "synthetic",
+        "config": {"workload": f"learned warm start: {len(data)} training clusters of 500-2000 cells "
+                               f"({n_ok} converged within {a.max_init_steps} steps), {a.eval_clusters} held-out "
+                               f"clusters, Dm(Na/K/Cl) and NaK rate log-uniform over two decades",
+                   "train_clusters": len(data), "eval_clusters": a.eval_clusters,
+                   "max_init_steps": a.max_init_steps, "epochs": a.epochs},
+        "cold_seconds": ev["cold_seconds"],
+        "speedup": ev["cold_seconds"] / max(ev["warm_seconds"], 1e-9),
+        "evaluation": ev,
+        "dataset_seconds": t_data, "train_seconds": t_train,
+        "train_loss_first_last": [hist[0], hist[-1]],
+        "dataset_batches": stats,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def make_problem(name: str):
     from betse_amd.cells import Cells
     from betse_amd.parameters import Parameters
+    from betse_amd.channels import draw_channel_state
     p = Parameters.preset(name)
     t0 = time.time()
     cells = Cells.make_world(p)
+    if p.channels and p.channel_random_state and p.channel_gates0 is None:
+        # one seeded draw of gates and per-cell multipliers shared by the GPU runs and the oracle
+        draw_channel_state(p, cells.n_cells, cells.n_mems)
     return p, cells, time.time() - t0
 
 
@@ -227,6 +279,11 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--clusters", type=int, default=256, help="batched config: number of clusters")
     ap.add_argument("--cpu-clusters", type=int, default=8, help="batched config: clusters in the CPU sample")
+    ap.add_argument("--train-clusters", type=int, default=1024, help="warmstart config: training clusters")
+    ap.add_argument("--eval-clusters", type=int, default=256, help="warmstart config: held-out clusters")
+    ap.add_argument("--max-init-steps", type=int, default=100_000, help="warmstart config: init step cap")
+    ap.add_argument("--epochs", type=int, default=40, help="warmstart config: training epochs")
+    ap.add_argument("--save-model", default="", help="warmstart config: save the trained network (.npz)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args(argv)
@@ -247,6 +304,8 @@ def main(argv=None):
 
     if a.config == "batched":
         return bench_batched(a)
+    if a.config == "warmstart":
+        return bench_warmstart(a)
     p, cells, t_world = make_problem(a.config)
     n_ions = len(p.ion_names())
     has_ca = "Ca" in p.ion_names()

@@ -20,19 +20,50 @@ BT_EINVAL = -1
 BT_EHIP = -2
 BT_MAX_IONS = 8
 BT_MAX_CHAN = 4
-BT_CHAN_KINDS = {"Nav1p2": 0, "Kv1p1": 1}
+BT_CHAN_KINDS = {"Nav1p2": 0, "Kv1p1": 1, "Cav1p2": 2}
 BT_FLUX_FP64 = 0
 BT_FLUX_FP32 = 1
 
 BT_TILE_MEMBRANES = 64
 
-KERNEL_IDS = {"mem_cell": 0, "env_field": 1, "env_exchange": 2, "env_field_only": 3}
+KERNEL_IDS = {"mem_cell": 0, "env_field": 1, "env_exchange": 2, "env_field_only": 3, "network": 4}
+
+BT_NET_MAX_SPECIES = 16
+BT_NET_MAX_RATES = 32
+BT_NET_MAX_HILL = 4
+BT_NET_MAX_TERMS = 4
+BT_NET_MAX_STOICH = 8
+BT_RATE_GROWTH = 0
+BT_RATE_REACTION = 1
 
 _d8 = ctypes.c_double * BT_MAX_IONS
 _d5 = ctypes.c_double * 5
 _i4 = ctypes.c_int32 * BT_MAX_CHAN
 _d4 = ctypes.c_double * BT_MAX_CHAN
 _vp = ctypes.c_void_p
+
+
+class BtRate(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("species", ctypes.c_int32), ("reversible", ctypes.c_int32),
+        ("n_act", ctypes.c_int32), ("n_inh", ctypes.c_int32), ("n_reac", ctypes.c_int32),
+        ("n_prod", ctypes.c_int32), ("n_stoich", ctypes.c_int32),
+        ("act", ctypes.c_int32 * BT_NET_MAX_HILL), ("inh", ctypes.c_int32 * BT_NET_MAX_HILL),
+        ("reac", ctypes.c_int32 * BT_NET_MAX_TERMS), ("prod", ctypes.c_int32 * BT_NET_MAX_TERMS),
+        ("stoich_species", ctypes.c_int32 * BT_NET_MAX_STOICH),
+        ("act_km", ctypes.c_double * BT_NET_MAX_HILL), ("act_n", ctypes.c_double * BT_NET_MAX_HILL),
+        ("inh_km", ctypes.c_double * BT_NET_MAX_HILL), ("inh_n", ctypes.c_double * BT_NET_MAX_HILL),
+        ("reac_coeff", ctypes.c_double * BT_NET_MAX_TERMS), ("reac_km", ctypes.c_double * BT_NET_MAX_TERMS),
+        ("prod_coeff", ctypes.c_double * BT_NET_MAX_TERMS), ("prod_km", ctypes.c_double * BT_NET_MAX_TERMS),
+        ("stoich_coeff", ctypes.c_double * BT_NET_MAX_STOICH),
+        ("r_prod", ctypes.c_double), ("r_decay", ctypes.c_double), ("decay_max", ctypes.c_double),
+        ("gad_scale", ctypes.c_double), ("vmax", ctypes.c_double), ("keq", ctypes.c_double),
+    ]
+
+
+class BtNetwork(ctypes.Structure):
+    _fields_ = [("n_species", ctypes.c_int32), ("n_rates", ctypes.c_int32),
+                ("rates", BtRate * BT_NET_MAX_RATES)]
 
 
 class BtCluster(ctypes.Structure):
@@ -99,6 +130,7 @@ SIGNATURES = {
     "bt_observables": (ctypes.c_int, [_vp, ctypes.POINTER(BtObs)]),
     "bt_gemm_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "bt_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "bt_set_network": (ctypes.c_int, [_vp, _vp, _vp]),
     "bt_residual": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "bt_steps_done": (ctypes.c_int64, [_vp]),
     "bt_last_error": (ctypes.c_char_p, [_vp]),

@@ -71,6 +71,11 @@ class BatchSimulator:
             st.init_dynamics(SimPhase(k, p, cells))
         self._build()
 
+    def rebuild(self) -> None:
+        """Upload the host states again (after they were changed on the host)."""
+        self._check_compatible()
+        self._build()
+
     def _check_compatible(self) -> None:
         s0, p0 = self.states[0], self.worlds[0][0]
         for st, (p, _) in zip(self.states, self.worlds):

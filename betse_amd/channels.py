@@ -5,7 +5,7 @@ channels up, as ``Channel.init_channel`` does (``chemistry/networks.py:6550-6628
 
 * ``init_gates(kind, vm)`` -- the channel's ``_init_state`` at the current
   Vmem (``channels/vg_na.py:140-166`` for Nav1p2, ``channels/vg_k.py:142-163``
-  for Kv1p1);
+  for Kv1p1, ``channels/vg_ca.py:302-326`` for Cav1p2);
 * ``draw_channel_state(p, cells)`` -- the seeded per-cell density multipliers
   (log-uniform) and per-membrane gates (uniform in [0, 1]) of a workload that
   asks for them.
@@ -25,6 +25,7 @@ import numpy as np
 CHANNEL_KINDS = {
     "Nav1p2": ("Na", 3, 1, 1.0e3),
     "Kv1p1": ("K", 1, 2, 1.0e3),
+    "Cav1p2": ("Ca", 2, 1, 1.0e3),
 }
 
 
@@ -38,6 +39,10 @@ def init_gates(kind: str, vm: np.ndarray):
     if kind == "Kv1p1":
         return (1.0000 / (1 + np.exp((V - -30.5000) / -11.3943)),
                 1.0000 / (1 + np.exp((V - -30.0000) / 27.3943)))
+    if kind == "Cav1p2":
+        W = V - 10
+        return (1.0000 / (1 + np.exp((W - -30.000) / -6)),
+                1.0000 / (1 + np.exp((W - -80.000) / 6.4)))
     raise ValueError(f"unsupported channel kind {kind!r}")
 
 

@@ -159,6 +159,16 @@ __device__ __forceinline__ void kv1p1_rates(FT V, FT& mInf, FT& mTau, FT& hInf, 
     hTau = (FT)15000.0 / ((FT)1 + exp((V - (FT)-160.56) / (FT)-100.0));
 }
 
+// Cav1p2._calculate_state (channels/vg_ca.py:328-340); V in mV, shifted by -10 mV.
+template <typename FT>
+__device__ __forceinline__ void cav1p2_rates(FT V, FT& mInf, FT& mTau, FT& hInf, FT& hTau) {
+    const FT W = V - (FT)10;
+    mInf = (FT)1.0 / ((FT)1 + exp((W - (FT)-30.0) / (FT)-6));
+    mTau = ((FT)5.0 + (FT)20.0 / ((FT)1 + exp((W - (FT)-25.0) / (FT)5)));
+    hInf = (FT)1.0 / ((FT)1 + exp((W - (FT)-80.0) / (FT)6.4));
+    hTau = ((FT)20.0 + (FT)50.0 / ((FT)1 + exp((W - (FT)-40.0) / (FT)7)));
+}
+
 // Vmem carrying n sequential +1e-25 nonces (electroflux mutates sim.vm in place, Q5)
 __device__ __forceinline__ double with_nonces(double v, int n) {
     for (int q = 0; q < n; ++q) v = v + FLOAT_NONCE;
@@ -260,16 +270,19 @@ k_mem_cell(MemCellArgs A, KParams P) {
             const double vg = with_nonces(v, NI + ch);
             const double V = vg * 1000.0 + 0.0;
             double mInf, mTau, hInf, hTau;
-            const bool nav = (P.chan_kind[ch] == BT_CHAN_NAV1P2);
-            if (nav) nav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
-            else kv1p1_rates<double>(V, mInf, mTau, hInf, hTau);
+            const int kind = P.chan_kind[ch];
+            if (kind == BT_CHAN_NAV1P2) nav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
+            else if (kind == BT_CHAN_KV1P1) kv1p1_rates<double>(V, mInf, mTau, hInf, hTau);
+            else cav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
             const double dtm = P.chan_dt;
             double mm = gm[m], hh = gh[m];
             mm = (mTau * mm + dtm * mInf) / (mTau + dtm);
             hh = (hTau * hh + dtm * hInf) / (hTau + dtm);
             gm[m] = mm;
             gh[m] = hh;
-            const double Pch = nav ? cube(mm) * hh : mm * (hh * hh);
+            // m**p * h**q with numpy's power: p = 2 squares, p = 3 calls pow
+            const double Pch = (kind == BT_CHAN_NAV1P2) ? cube(mm) * hh
+                             : (kind == BT_CHAN_KV1P1) ? mm * (hh * hh) : (mm * mm) * hh;
             const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * A.chan_mod[(size_t)ch * C + c]);
             const FT fc = ghk<FT>((FT)A.c_mid[i * G2 + k], (FT)A.c_cell_in[i * C + c], Dch, (FT)P.tm,
                                   (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE), (FT)P.F, (FT)P.RT);
@@ -567,19 +580,119 @@ k_cell_vm(const double* c_cell, const double* diviterm, double* vm, KParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// per-cell reaction network (MasterOfNetworks.run_loop, networks.py:2805-2982)
+// ---------------------------------------------------------------------------
+constexpr int NET_BLOCK = 128;
+
+// (c/Km)**n with numpy's power: exponents 1 and 2 take the exact fast paths
+__device__ __forceinline__ double hill_pow(double x, double n) {
+    if (n == 1.0) return x;
+    if (n == 2.0) return x * x;
+    return pow(x, n);
+}
+
+// activator term (((c/Km)**n)/(1 + (c/Km)**n)) and inhibitor term (1/(1 + (c/Km)**n))
+// (get_influencers, networks.py:5184-5200, 5396-5420)
+__device__ __forceinline__ double hill_act(double c, double km, double n) {
+    const double x = hill_pow(c / km, n);
+    return x / (1.0 + hill_pow(c / km, n));
+}
+__device__ __forceinline__ double hill_inh(double c, double km, double n) {
+    return 1.0 / (1.0 + hill_pow(c / km, n));
+}
+
+// One thread per cell; the cell's concentrations and accumulated deltas live
+// in LDS so that the network's species indices can be runtime values.
+// Rates are evaluated from the pre-step concentrations in the reference's
+// order (growth/decay terms of every substance, then reactions) and each
+// rate's reaction_matrix column is added as it is produced, which sums
+// delta[s] over the rates in ascending order (np.dot(reaction_matrix, rates)).
+__global__ void __launch_bounds__(NET_BLOCK)
+k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cells, double dt) {
+    __shared__ double s_c[BT_NET_MAX_SPECIES][NET_BLOCK];
+    __shared__ double s_d[BT_NET_MAX_SPECIES][NET_BLOCK];
+    const int t = threadIdx.x;
+    const int cell = blockIdx.x * NET_BLOCK + t;
+    if (cell >= n_cells) return;
+    const int S = net->n_species;
+    for (int s = 0; s < S; ++s) {
+        s_c[s][t] = cn[(size_t)s * n_cells + cell];
+        s_d[s][t] = 0.0;
+    }
+    const int R = net->n_rates;
+    for (int j = 0; j < R; ++j) {
+        const bt_rate& r = net->rates[j];
+        // alpha = (product of activator terms) * (product of inhibitor terms), each
+        // product taken left to right; an empty product is np.ones
+        double act = 1.0, inh = 1.0;
+        for (int q = 0; q < r.n_act; ++q) {
+            const double h = hill_act(s_c[r.act[q]][t], r.act_km[q], r.act_n[q]);
+            act = (q == 0) ? h : act * h;
+        }
+        for (int q = 0; q < r.n_inh; ++q) {
+            const double h = hill_inh(s_c[r.inh[q]][t], r.inh_km[q], r.inh_n[q]);
+            inh = (q == 0) ? h : inh * h;
+        }
+        double rate;
+        if (r.kind == BT_RATE_GROWTH) {
+            // mod*r_prod*alpha - r_decay*cc - decay_max*1*cc, cc = c/gad_scale (networks.py:1233-1249)
+            const double cc = s_c[r.species][t] / r.gad_scale;
+            rate = ((1.0 * r.r_prod) * act) * inh;
+            rate = rate - r.r_decay * cc;
+            rate = rate - ((r.decay_max * 1.0) * 1.0) * cc;
+        } else {
+            double x;
+            if (r.reversible) {
+                // Q = prod(1e-3*c**k) / prod(1e-3*c**k) (mol/L), rate ~ (1.0 - Q/Keq) (networks.py:1346-1391, 1520-1524)
+                double num = 1.0, den = 1.0;
+                for (int q = 0; q < r.n_prod; ++q) {
+                    const double f = 1.0e-3 * hill_pow(s_c[r.prod[q]][t], r.prod_coeff[q]);
+                    num = (q == 0) ? f : num * f;
+                }
+                for (int q = 0; q < r.n_reac; ++q) {
+                    const double f = 1.0e-3 * hill_pow(s_c[r.reac[q]][t], r.reac_coeff[q]);
+                    den = (q == 0) ? f : den * f;
+                }
+                x = 1.0 - ((num / den) / r.keq);
+            } else {
+                // forward coefficient: product over reactants of ((c/Km)**k)/(1 + (c/Km)**k)
+                double fwd = 1.0;
+                for (int q = 0; q < r.n_reac; ++q) {
+                    const double f = hill_act(s_c[r.reac[q]][t], r.reac_km[q], r.reac_coeff[q]);
+                    fwd = (q == 0) ? f : fwd * f;
+                }
+                x = fwd - (0.0 * 0.0);
+            }
+            rate = ((r.vmax * act) * inh) * x;
+        }
+        for (int q = 0; q < r.n_stoich; ++q) {
+            const int s = r.stoich_species[q];
+            s_d[s][t] = s_d[s][t] + r.stoich_coeff[q] * rate;
+        }
+    }
+    for (int s = 0; s < S; ++s) {
+        double v = s_c[s][t] + s_d[s][t] * dt;
+        if (v < 0.0) v = 0.0;                    // stb.no_negs
+        cn[(size_t)s * n_cells + cell] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // sampled-step observables
 // ---------------------------------------------------------------------------
 struct ObsMemArgs {
     const int32_t* mem_ptr; const int32_t* nn_mem; const double* mem_sa; const double* mem_nx;
-    const double* mem_ny; const double* cell_sa; const double* vm_cell; const int32_t* mem_cell;
+    const double* mem_ny; const double* cell_sa; const double* vm_cell; const double* vm_gj;
+    const int32_t* mem_cell;
     const double* fm; const double* fgj; const double* fch;
     double* Jn; double* I_mem; double* E_gj_x; double* E_gj_y;
     double* J_cell_x; double* J_cell_y; double* vm_ave;
     double smooth_cells;
 };
 
-// get_current cell/membrane part (ion_current.py:18-47), E_gj (sim.py:2166-2172),
-// vm_ave (sim.py:2038); one thread per cell, membranes in order
+// get_current cell/membrane part (ion_current.py:18-47), E_gj (sim.py:2166-2172,
+// from the Vmem the step started with, vm_gj), vm_ave (sim.py:2038); one thread
+// per cell, membranes in order
 __global__ void __launch_bounds__(BLOCK)
 k_obs_mem(ObsMemArgs A, KParams P) {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
@@ -607,6 +720,7 @@ k_obs_mem(ObsMemArgs A, KParams P) {
     const double w_mem = (nm - 1) / nm, w_o = 1 / nm;
     double sx = 0.0, sy = 0.0, svm = 0.0;
     const double v = A.vm_cell[c];
+    const double vg = A.vm_gj[c];
     for (int m = a; m < b; ++m) {
         const double jn = w_mem * jn_raw(m) + jn_ave * w_o;
         A.Jn[m] = jn;
@@ -616,7 +730,7 @@ k_obs_mem(ObsMemArgs A, KParams P) {
         sx = (m == a) ? jx : sx + jx;
         sy = (m == a) ? jy : sy + jy;
         svm = (m == a) ? v : svm + v;
-        const double vgj = A.vm_cell[A.mem_cell[A.nn_mem[m]]] - v;
+        const double vgj = A.vm_gj[A.mem_cell[A.nn_mem[m]]] - vg;
         const double egj = -vgj / P.gj_len;
         A.E_gj_x[m] = egj * A.mem_nx[m];
         A.E_gj_y[m] = egj * A.mem_ny[m];
@@ -783,6 +897,10 @@ struct bt_ctx {
     double* S1 = nullptr; double* S2 = nullptr; double* invlam = nullptr;
     double* jxo = nullptr; double* jyo = nullptr; double* B1 = nullptr; double* B2 = nullptr;
     double* T1 = nullptr; double* T2 = nullptr; double* AA = nullptr; double* BB = nullptr;
+    // Vmem per cell before the last observed step: update_gj's E_gj uses it (sim.py:2166-2172)
+    double* vm_prev = nullptr; bool vm_prev_ok = false;
+    // reaction network (device copy of the description) and its state buffer
+    bt_network* net = nullptr; double* c_net = nullptr;
     std::string err;
 };
 
@@ -916,6 +1034,9 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
         hipLaunchKernelGGL(k_env_exchange, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, x, c->kp);
     }
     if (which == 3) launch_field(c, false);
+    if ((which < 0 || which == 4) && c->net && c->c_net)
+        hipLaunchKernelGGL(k_network, dim3((c->prm.n_cells + NET_BLOCK - 1) / NET_BLOCK), dim3(NET_BLOCK), 0,
+                           c->stream, (const bt_network*)c->net, c->c_net, c->prm.n_cells, c->prm.dt);
 }
 
 static int copy_back(bt_ctx* c) {
@@ -962,7 +1083,8 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     if (b->tile_m != BT_TILE_MEMBRANES) return set_err(nullptr, BT_EINVAL, "tiles must hold <= 64 membranes");
     if (p->n_chan < 0 || p->n_chan > BT_MAX_CHAN) return set_err(nullptr, BT_EINVAL, "n_chan out of range");
     for (int c = 0; c < p->n_chan; ++c) {
-        if (p->chan_kind[c] != BT_CHAN_NAV1P2 && p->chan_kind[c] != BT_CHAN_KV1P1)
+        if (p->chan_kind[c] != BT_CHAN_NAV1P2 && p->chan_kind[c] != BT_CHAN_KV1P1 &&
+            p->chan_kind[c] != BT_CHAN_CAV1P2)
             return set_err(nullptr, BT_EINVAL, "unknown channel kind");
         if (p->chan_ion[c] < 0 || p->chan_ion[c] >= p->n_ions)
             return set_err(nullptr, BT_EINVAL, "channel ion slot out of range");
@@ -1037,7 +1159,7 @@ int bt_destroy(bt_ctx* c) {
     if (c->graph) (void)hipGraphDestroy(c->graph);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u,
                     c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
-                    c->B1, c->B2, c->T1, c->T2, c->AA, c->BB};
+                    c->B1, c->B2, c->T1, c->T2, c->AA, c->BB, c->vm_prev, c->net};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     delete c;
@@ -1103,7 +1225,8 @@ int bt_sync(bt_ctx* c, int64_t* bad_step) {
 }
 
 int bt_time_kernel(bt_ctx* c, int32_t kernel, int32_t n_launches, double* ms_out) {
-    if (!c || !ms_out || n_launches < 1 || kernel < 0 || kernel > 3) return BT_EINVAL;
+    if (!c || !ms_out || n_launches < 1 || kernel < 0 || kernel > 4) return BT_EINVAL;
+    if (kernel == 4 && !c->c_net) return set_err(c, BT_EINVAL, "no reaction network attached");
     hipEvent_t e0, e1;
     HIP_TRY(c, hipEventCreate(&e0));
     HIP_TRY(c, hipEventCreate(&e1));
@@ -1120,6 +1243,42 @@ int bt_time_kernel(bt_ctx* c, int32_t kernel, int32_t n_launches, double* ms_out
     return BT_OK;
 }
 
+int bt_set_network(bt_ctx* c, const bt_network* net, double* c_net) {
+    if (!c) return BT_EINVAL;
+    if (net) {
+        if (!c_net) return set_err(c, BT_EINVAL, "reaction network state buffer missing");
+        if (net->n_species < 1 || net->n_species > BT_NET_MAX_SPECIES || net->n_rates < 0 ||
+            net->n_rates > BT_NET_MAX_RATES)
+            return set_err(c, BT_EINVAL, "reaction network size out of range");
+        for (int j = 0; j < net->n_rates; ++j) {
+            const bt_rate& r = net->rates[j];
+            const int S = net->n_species;
+            bool ok = (r.kind == BT_RATE_GROWTH || r.kind == BT_RATE_REACTION) &&
+                      r.n_act >= 0 && r.n_act <= BT_NET_MAX_HILL && r.n_inh >= 0 && r.n_inh <= BT_NET_MAX_HILL &&
+                      r.n_reac >= 0 && r.n_reac <= BT_NET_MAX_TERMS && r.n_prod >= 0 &&
+                      r.n_prod <= BT_NET_MAX_TERMS && r.n_stoich >= 0 && r.n_stoich <= BT_NET_MAX_STOICH;
+            if (ok && r.kind == BT_RATE_GROWTH) ok = r.species >= 0 && r.species < S;
+            if (ok && r.kind == BT_RATE_REACTION && r.reversible) ok = r.n_reac >= 1 && r.n_prod >= 1;
+            for (int q = 0; ok && q < r.n_act; ++q) ok = r.act[q] >= 0 && r.act[q] < S;
+            for (int q = 0; ok && q < r.n_inh; ++q) ok = r.inh[q] >= 0 && r.inh[q] < S;
+            for (int q = 0; ok && q < r.n_reac; ++q) ok = r.reac[q] >= 0 && r.reac[q] < S;
+            for (int q = 0; ok && q < r.n_prod; ++q) ok = r.prod[q] >= 0 && r.prod[q] < S;
+            for (int q = 0; ok && q < r.n_stoich; ++q) ok = r.stoich_species[q] >= 0 && r.stoich_species[q] < S;
+            if (!ok) return set_err(c, BT_EINVAL, "reaction network rate " + std::to_string(j) + " is invalid");
+        }
+        if (!c->net) HIP_TRY(c, hipMalloc((void**)&c->net, sizeof(bt_network)));
+        HIP_TRY(c, hipMemcpyAsync(c->net, net, sizeof(bt_network), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->c_net = c_net;
+    } else {
+        c->c_net = nullptr;
+    }
+    // captured steps do not match the new step body any more
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    return BT_OK;
+}
+
 int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs* obs) {
     if (!c || !obs) return BT_EINVAL;
     if (n_steps < 1) return set_err(c, BT_EINVAL, "an observed run needs at least one step");
@@ -1128,6 +1287,8 @@ int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs
         return set_err(c, BT_EINVAL, "observation buffers missing");
     if (c->prm.i_na < 0)
         HIP_TRY(c, hipMemsetAsync(obs->rate_NaKATP, 0, (size_t)c->prm.n_mems * 8, c->stream));
+    const size_t vbytes = (size_t)c->prm.n_cells * 8;
+    if (!c->vm_prev) HIP_TRY(c, hipMalloc((void**)&c->vm_prev, vbytes));
     const int ahead = n_steps - 1;
     if (ahead > 0) {
         // keep the graph on the caller buffers: an even number of graph steps, then eager ones
@@ -1137,10 +1298,12 @@ int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs
     int rest = (ahead & 1) + 1;           // 1 or 2 eager steps, the last one observed
     if (rest == 2) {
         enqueue_step(c, 0);
+        HIP_TRY(c, hipMemcpyAsync(c->vm_prev, c->vm_b, vbytes, hipMemcpyDeviceToDevice, c->stream));
         c->capture = obs;
         enqueue_step(c, 1);
         c->capture = nullptr;
     } else {
+        HIP_TRY(c, hipMemcpyAsync(c->vm_prev, c->buf.vm_cell, vbytes, hipMemcpyDeviceToDevice, c->stream));
         c->capture = obs;
         enqueue_step(c, 0);
         c->capture = nullptr;
@@ -1149,6 +1312,7 @@ int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs
     }
     enqueue_step(c, 0, 3);
     HIP_TRY(c, hipGetLastError());
+    c->vm_prev_ok = true;
     c->steps_done += rest;
     return BT_OK;
 }
@@ -1239,7 +1403,8 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
     int rc = obs_workspace(c);
     if (rc != BT_OK) return rc;
     const bt_buffers& B = c->buf;
-    ObsMemArgs a{B.mem_ptr, B.nn_mem, B.mem_sa, B.mem_nx, B.mem_ny, B.cell_sa, B.vm_cell, B.mem_cell,
+    ObsMemArgs a{B.mem_ptr, B.nn_mem, B.mem_sa, B.mem_nx, B.mem_ny, B.cell_sa, B.vm_cell,
+                 c->vm_prev_ok ? c->vm_prev : B.vm_cell, B.mem_cell,
                  o->fluxes_mem, o->fluxes_gj, o->chan_flux, o->Jn, o->I_mem, o->E_gj_x, o->E_gj_y,
                  o->J_cell_x, o->J_cell_y, o->vm_ave, c->prm.smooth_cells};
     hipLaunchKernelGGL(k_obs_mem, dim3(grid_blocks(c->kp.n_cells)), dim3(BLOCK), 0, c->stream, a, c->kp);

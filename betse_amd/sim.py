@@ -185,6 +185,10 @@ class HostState:
         self.v_env = np.zeros(self.edl)
         self.Phi_b = np.zeros(self.edl)
         self.init_channels(cells, p)
+        # reaction network substances (MasterOfNetworks cell_concs), uniform start
+        self.network = getattr(p, "network", None)
+        self.c_net = (np.tile(np.asarray(self.network.c0, dtype=np.float64)[:, None], (1, self.cdl))
+                      if self.network is not None else None)
 
     def init_channels(self, cells: Cells, p: Parameters) -> None:
         """``Channel.init_channel`` (networks.py:6550-6628) for every configured
@@ -403,6 +407,8 @@ class Simulator(HostState):
             nch = len(self.channel_specs)
             d["gates"] = self._t(self.chan_gates if nch else np.zeros((1, 2, 1)), f64)
             d["chan_mod"] = self._t(self.chan_mod if nch else np.ones((1, 1)), f64)
+            if self.network is not None:
+                d["c_net"] = self._t(self.c_net, f64)
             self.stream.synchronize()
 
         prm = _native.BtParams()
@@ -463,6 +469,10 @@ class Simulator(HostState):
                                      ctypes.byref(buf), ctypes.c_void_p(self.stream.cuda_stream))
         if rc != _native.BT_OK:
             raise BetseNativeError(f"bt_create failed ({rc}): {self._lib.bt_last_error(None).decode()}")
+        if self.network is not None:
+            self._net = self.network.pack(p.R, self.T)
+            self._check(self._lib.bt_set_network(self._ctx, ctypes.byref(self._net),
+                                                 ctypes.c_void_p(d["c_net"].data_ptr())))
 
     def _destroy_ctx(self) -> None:
         if self._ctx:
@@ -492,6 +502,8 @@ class Simulator(HostState):
         self.v_env = d["v_env"].cpu().numpy().copy()
         if len(self.channel_specs):
             self.chan_gates = d["gates"].cpu().numpy().copy()
+        if self.network is not None:
+            self.c_net = d["c_net"].cpu().numpy().copy()
 
     def _sync(self, phase_step0: int = 0) -> None:
         bad = ctypes.c_int64(-1)

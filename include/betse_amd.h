@@ -33,9 +33,10 @@ extern "C" {
 #define BT_MAX_CHAN  4
 #define BT_TILE_MEMBRANES 64   /* membranes per cell tile (whole cells, one wavefront per ion) */
 
-/* Voltage-gated channel models (betse/science/channels/vg_na.py, vg_k.py). */
+/* Voltage-gated channel models (betse/science/channels/vg_na.py, vg_k.py, vg_ca.py). */
 #define BT_CHAN_NAV1P2 0
 #define BT_CHAN_KV1P1  1
+#define BT_CHAN_CAV1P2 2
 
 /* Flux arithmetic of the membrane and environment flux laws.
  * Cell/environment state, charge sums and the charge->voltage step are FP64
@@ -129,6 +130,48 @@ typedef struct {
     int32_t tile_m;             /* must equal BT_TILE_MEMBRANES                   */
 } bt_buffers;
 
+/* Per-cell reaction network of intracellular substances
+ * (MasterOfNetworks.run_loop, betse/science/chemistry/networks.py:2805-2982).
+ * Every rate is either a substance's growth/decay term (write_growth_and_decay,
+ * networks.py:1187-1305) or a reaction (write_reactions, networks.py:1312-1570),
+ * each scaled by Hill activators and inhibitors (get_influencers,
+ * networks.py:5084-5600). The substances stay inside their cells (no membrane
+ * or gap-junction transport), so a step is a per-cell explicit Euler update
+ * c += (reaction_matrix . rates) dt followed by no_negs. */
+#define BT_NET_MAX_SPECIES 16
+#define BT_NET_MAX_RATES   32
+#define BT_NET_MAX_HILL     4
+#define BT_NET_MAX_TERMS    4
+#define BT_NET_MAX_STOICH   8
+#define BT_RATE_GROWTH      0
+#define BT_RATE_REACTION    1
+
+typedef struct {
+    int32_t kind;                       /* BT_RATE_GROWTH or BT_RATE_REACTION */
+    int32_t species;                    /* growth: its substance               */
+    int32_t reversible;                 /* reaction with delta_Go: vmax*alpha*(1 - Q/Keq) */
+    int32_t n_act, n_inh;               /* Hill activators and inhibitors      */
+    int32_t n_reac, n_prod;             /* reaction terms                      */
+    int32_t n_stoich;                   /* nonzero entries of this rate's reaction_matrix column */
+    int32_t act[BT_NET_MAX_HILL];
+    int32_t inh[BT_NET_MAX_HILL];
+    int32_t reac[BT_NET_MAX_TERMS];
+    int32_t prod[BT_NET_MAX_TERMS];
+    int32_t stoich_species[BT_NET_MAX_STOICH];
+    double act_km[BT_NET_MAX_HILL], act_n[BT_NET_MAX_HILL];
+    double inh_km[BT_NET_MAX_HILL], inh_n[BT_NET_MAX_HILL];
+    double reac_coeff[BT_NET_MAX_TERMS], reac_km[BT_NET_MAX_TERMS];
+    double prod_coeff[BT_NET_MAX_TERMS], prod_km[BT_NET_MAX_TERMS];
+    double stoich_coeff[BT_NET_MAX_STOICH];
+    double r_prod, r_decay, decay_max, gad_scale;   /* growth/decay (time factor folded in) */
+    double vmax, keq;                               /* reaction; keq = exp(-delta_Go/(R T)) */
+} bt_rate;
+
+typedef struct {
+    int32_t n_species, n_rates;
+    bt_rate rates[BT_NET_MAX_RATES];
+} bt_network;
+
 /* Create a context: validates sizes, copies the parameter block, allocates the
  * workspace (second cell-state buffer, post-diffusion env buffer, per-membrane
  * exchange fluxes, surface-charge field, status words) on the current device.
@@ -152,6 +195,13 @@ int bt_update_v(bt_ctx* ctx);
  * concentration update, update_V, NaN checks. Steps are
  * replayed from a captured hipGraph when use_graph != 0. Asynchronous. */
 int bt_step(bt_ctx* ctx, int32_t n_steps, int32_t use_graph);
+
+/* Attach a reaction network to the step (NULL net detaches it). c_net is the
+ * caller's device buffer [n_species][n_cells] of substance concentrations
+ * (mM), updated in place by every following step after the channels and
+ * before the ion concentration update, as in sim.py:1286-1300. The network
+ * is copied; captured step graphs are rebuilt. */
+int bt_set_network(bt_ctx* ctx, const bt_network* net, double* c_net);
 
 /* Observables of one step (get_current, ion_current.py:13-109; update_gj
  * E_gj, sim.py:2166-2172; update_V vm_ave, sim.py:2038). The reference

@@ -270,10 +270,27 @@ def kv1p1_init(V):
             1.0000 / (1 + np.exp((V - -30.0000) / 27.3943)))
 
 
+def cav1p2_rates(V):
+    """Cav1p2._calculate_state (vg_ca.py:328-340); V in mV, shifted by -10 mV first."""
+    V = V - 10
+    mInf = 1.0000 / (1 + np.exp((V - -30.000) / -6))
+    mTau = (5.0000 + 20.0000 / (1 + np.exp((V - -25.000) / 5)))
+    hInf = 1.0000 / (1 + np.exp((V - -80.000) / 6.4))
+    hTau = (20.0000 + 50.0000 / (1 + np.exp((V - -40.000) / 7)))
+    return mInf, mTau, hInf, hTau
+
+
+def cav1p2_init(V):
+    """Cav1p2._init_state (vg_ca.py:302-326)."""
+    V = V - 10
+    return 1.0000 / (1 + np.exp((V - -30.000) / -6)), 1.0000 / (1 + np.exp((V - -80.000) / 6.4))
+
+
 CHANNEL_KINDS = {
     # kind: (ion, rates, init, mpower, hpower, time_unit)
     "Nav1p2": ("Na", nav1p2_rates, nav1p2_init, 3, 1, 1.0e3),
     "Kv1p1": ("K", kv1p1_rates, kv1p1_init, 1, 2, 1.0e3),
+    "Cav1p2": ("Ca", cav1p2_rates, cav1p2_init, 2, 1, 1.0e3),
 }
 
 
@@ -361,6 +378,10 @@ class OracleSim:
         self.extra_J_mem = np.zeros(self.mdl)
         self.extra_Jenv_x = np.zeros(self.edl)
         self.extra_Jenv_y = np.zeros(self.edl)
+        # intracellular reaction network (MasterOfNetworks), substances uniform at c0
+        self.network = getattr(p, "network", None)
+        self.c_net = (np.tile(np.asarray(self.network.c0, dtype=np.float64)[:, None], (1, self.cdl))
+                      if self.network is not None else None)
         self.vgj = np.zeros(self.mdl)
         self.gj_block = 1
         self.T = p.T
@@ -548,6 +569,9 @@ class OracleSim:
         if self.channels and (kind == "sim" or getattr(self.p, "channels_init_active", True)):
             self.extra_J_mem = np.zeros(self.mdl)       # clear_run_loop (networks.py:2790-2801)
             self.run_loop_channels()
+        if self.network is not None:                    # molecules run_loop (sim.py:1286-1300)
+            from oracle import network_oracle
+            self.c_net = network_oracle.step(self.network, self.c_net, p.dt, p.R, self.T)
         self.update_all_concs()
         self.update_V(observables=observables)
         check_v(self.vm)
