@@ -29,7 +29,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0) -> dict:
+def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0, n_sub: int = 0) -> dict:
     """Compulsory bytes per launch of each step kernel (each array element the
     kernel must touch, counted once at its storage width)."""
     C, M = cells.n_cells, cells.n_mems
@@ -45,6 +45,8 @@ def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 
                      + n_in * 8 * (ni + (1 if ca else 0) + n_ch))  # env gathers (pre- and post-diffusion)
     b["env_field"] = G2 * (8 + 8 * ni + 8 * ni + 8 * ni)    # u r, c_env r, D_env r, c_mid w
     b["env_exchange"] = G2 * (8 * ni + 8 * ni + 8 + 1 + 4) + M * 8 * (ni + n_ch)
+    if n_sub:
+        b["network"] = C * n_sub * 16                          # substances read and written
     b["step"] = sum(b.values())
     return b
 
@@ -171,14 +173,15 @@ def bench_warmstart(a):
 
     t0 = time.perf_counter()
     data, stats, base = ws.make_dataset(a.train_clusters, seed=1000, batch=a.clusters, max_steps=a.max_init_steps,
-                                        flux_mode="fp64", log=log)
+                                        flux_mode=a.ws_flux, log=log)
     t_data = time.perf_counter() - t0
     n_ok = sum(d["ok"] for d in data)
     log(f"dataset: {len(data)} clusters, {n_ok} converged, {t_data:.1f}s")
     starter, hist, t_train = ws.train(data, epochs=a.epochs, log=log)
     if a.save_model:
         starter.save(a.save_model)
-    ev = ws.evaluate(starter, a.eval_clusters, seed=999_000, max_steps=a.max_init_steps, flux_mode="fp64", log=log)
+    ev = ws.evaluate(starter, a.eval_clusters, seed=999_000, max_steps=a.max_init_steps, flux_mode=a.ws_flux,
+                     window=100, log=log)
     log(f"evaluation: {ev}")
     line = {
         "metric": "init seconds to steady state (residual < 1e-6), warm start vs default start",
@@ -191,7 +194,7 @@ def bench_warmstart(a):
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 step; bf16-mfma/f32 network",
+        "dtype": ("f64" if a.ws_flux == "fp64" else "f32-flux/f64-state") + " step; bf16-mfma/f32 network",
         "data": "synthetic",
         "config": {"workload": f"learned warm start: {len(data)} training clusters of 500-2000 cells "
                                f"({n_ok} converged within {a.max_init_steps} steps), {a.eval_clusters} held-out "
@@ -245,8 +248,27 @@ def time_gpu(p, cells, flux_mode, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev_ms = e0.elapsed_time(e1)
-    kernels = {k: sim.time_kernel(k, 100) for k in ("mem_cell", "env_field", "env_exchange")}
+    names = ("mem_cell", "env_field", "env_exchange") + (("network",) if sim.network is not None else ())
+    kernels = {k: sim.time_kernel(k, 100) for k in names}
     return sim, wall, ev_ms, kernels
+
+
+def time_sampled_loop(p, cells, flux_mode, steps):
+    """The init phase through run_sim_core: sampled steps run bt_step_observed,
+    derive the get_current observables (HH decomposition by DST on FP64 MFMA)
+    and go through write2storage on the host."""
+    import torch
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    q = p.copy()
+    q.init_time_total = steps * q.init_time_step
+    sim = Simulator(q, device="cuda:0", flux_mode=flux_mode, use_graph=True)
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run_sim_core(ph)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, len(sim.time)
 
 
 def cpu_baseline(p, cells, target_s: float):
@@ -284,6 +306,8 @@ def main(argv=None):
     ap.add_argument("--max-init-steps", type=int, default=100_000, help="warmstart config: init step cap")
     ap.add_argument("--epochs", type=int, default=40, help="warmstart config: training epochs")
     ap.add_argument("--save-model", default="", help="warmstart config: save the trained network (.npz)")
+    ap.add_argument("--ws-flux", default="fp32", choices=["fp64", "fp32"],
+                    help="warmstart config: flux arithmetic of the init runs")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args(argv)
@@ -322,6 +346,14 @@ def main(argv=None):
               f"kernels(us): " + ", ".join(f"{k}={v*1e3:.2f}" for k, v in kern.items()),
               file=sys.stderr, flush=True)
 
+    # the same steps through run_sim_core with the preset's sampling (observables + storage)
+    sampled = {}
+    for mode in modes:
+        wall_s, n_samp = time_sampled_loop(p, cells, mode, a.steps)
+        sampled[mode] = {"cell_steps_per_s": C * a.steps / wall_s, "wall_s": wall_s, "sampled_steps": n_samp}
+        print(f"# {mode} run_sim_core with {n_samp} sampled steps: {wall_s / a.steps * 1e6:.1f} us/step",
+              file=sys.stderr, flush=True)
+
     # CPU baseline + parity on the same prefix
     cpu = None
     parity = {}
@@ -339,7 +371,20 @@ def main(argv=None):
                 "max_abs_dVm_V": float(np.abs(sim.vm - o.vm).max()),
                 "max_rel_dc_cell": float((np.abs(sim.cc_cells - o.cc_cells) / np.abs(o.cc_cells)).max()),
                 "max_rel_dc_env": float((np.abs(sim.cc_env - o.cc_env) / np.abs(o.cc_env)).max()),
+                "per_ion_rel_dc_cell": {n: float((np.abs(sim.cc_cells[i] - o.cc_cells[i]) / np.abs(o.cc_cells[i])).max())
+                                        for i, n in enumerate(sim.names)},
+                "per_ion_rel_dc_env": {n: float((np.abs(sim.cc_env[i] - o.cc_env[i]) / np.abs(o.cc_env[i])).max())
+                                       for i, n in enumerate(sim.names)},
+                "max_abs_dE_env_V_per_m": float(max(np.abs(sim.E_env_x - o.E_env_x).max(),
+                                                    np.abs(sim.E_env_y - o.E_env_y).max())),
             }
+            if len(p.channels):
+                parity[mode]["max_abs_dgate"] = {ch.kind: float(max(np.abs(sim.chan_gates[k, 0] - ch.m).max(),
+                                                                    np.abs(sim.chan_gates[k, 1] - ch.h).max()))
+                                                 for k, ch in enumerate(o.channels)}
+            if getattr(p, "network", None) is not None:
+                parity[mode]["max_rel_dc_substance"] = float(
+                    (np.abs(sim.c_net - o.c_net) / np.maximum(np.abs(o.c_net), 1e-300)).max())
         print(f"# cpu oracle: {n_cpu} steps in {el:.2f}s; parity {parity}", file=sys.stderr, flush=True)
 
     # headline: fastest path that holds |dVm| < 1 uV against the oracle prefix
@@ -348,7 +393,8 @@ def main(argv=None):
     r = res[best]
     ms_step = r["ev_ms"] / a.steps
     value = C * a.steps / (r["ev_ms"] / 1e3)
-    ab = algorithmic_bytes(cells, n_ions, has_ca, best, len(p.channels))
+    n_sub = p.network.n_species if getattr(p, "network", None) is not None else 0
+    ab = algorithmic_bytes(cells, n_ions, has_ca, best, len(p.channels), n_sub)
     kern = r["kernels"]
     dom = max(kern, key=kern.get)
     achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
@@ -377,6 +423,7 @@ def main(argv=None):
                      "step_achieved_GBs": ab["step"] / (ms_step * 1e-3) / 1e9},
         "cpu_baseline": cpu,
         "parity_prefix": parity,
+        "with_sampling": sampled,
         "paths": {m: {"cell_steps_per_s": C * a.steps / (res[m]["ev_ms"] / 1e3),
                       "us_per_step": res[m]["ev_ms"] / a.steps * 1e3,
                       "kernels_us": {k: v * 1e3 for k, v in res[m]["kernels"].items()}} for m in modes},

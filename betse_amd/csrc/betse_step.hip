@@ -4,23 +4,27 @@
 // One step of Simulator._run_sim_core_loop (betse/science/sim.py:1169-1366) is
 // three kernels on one stream:
 //
-//   k_env_field     one block per 16x16 tile of the environment grid, staged
-//                   through LDS: the surface-charge field u of the previous
-//                   update_V -> SciPy Gaussian (sigma 1, constant 0) -> v_env ->
-//                   E = -grad(screen*v_env) (ion_current.py:93-109), then for
-//                   every ion the boundary reset (Q3), gradient, Nernst-Planck
+//   k_env_field     one block per (16x16 tile of the environment grid, ion),
+//                   staged through LDS: the surface-charge field u of the
+//                   previous update_V -> SciPy Gaussian (sigma 1, constant 0) ->
+//                   v_env -> E = -grad(screen*v_env) (ion_current.py:93-109),
+//                   then the ion's boundary reset (Q3), gradient, Nernst-Planck
 //                   flux and sign-flipped boundary divergence (Q4) of update_ecm
 //                   (sim.py:2209-2254)                          c_env -> c_mid
-//   k_mem_cell      one block per tile of whole cells (<= 64 membranes), one
-//                   wavefront per ion: NaK pump, GHK, GHK gap junction, Ca pump,
-//                   voltage-gated channel per (membrane, ion); per cell the CSR
-//                   segment sums from LDS, update_Co + GJ update, no_negs clamp,
-//                   rho_cells -> Vmem (sim.py:1196-1283, 2086-2111, 2007-2038,
+//   k_mem_cell      one wavefront per tile of whole cells (<= 64 membranes), a
+//                   lane per membrane for every ion (NaK pump, GHK, GHK gap
+//                   junction, Ca pump) and voltage-gated channel (gating,
+//                   channel GHK); per cell the CSR segment sums from LDS,
+//                   update_Co + GJ update, no_negs clamp, rho_cells -> Vmem
+//                   (sim.py:1196-1283, 2086-2111, 2007-2038,
 //                   networks.py:3115-3215)                      cell A -> cell B
-//   k_env_exchange  per square: the square's membrane fluxes (written by
-//                   k_mem_cell in square-sorted order, the sparse form of
-//                   M_divmap_mem2ecm) -> env update -> rho_env -> u
-//                   (sim_toolbox.py:1191-1229, ion_current.py:75-97) c_mid -> c_env
+//   k_env_exchange  one wavefront per ion over 64 squares: the square's membrane
+//                   fluxes (written by k_mem_cell in square-sorted order, the
+//                   sparse form of M_divmap_mem2ecm) -> env update; then
+//                   rho_env -> u (sim_toolbox.py:1191-1229, ion_current.py:75-97)
+//                                                               c_mid -> c_env
+//   k_network       (when a reaction network is attached) one thread per cell:
+//                   the substances' explicit Euler update (networks.py:2805-2982)
 //
 // Floating-point expressions keep the reference's operation order (the code is
 // compiled with -ffp-contract=off) so the FP64 path tracks the NumPy reference
@@ -39,9 +43,11 @@
 
 namespace {
 
-constexpr int TMB = BT_TILE_MEMBRANES;   // membranes per k_mem_cell tile (one wavefront per ion)
+constexpr int TMB = BT_TILE_MEMBRANES;   // membranes per k_mem_cell tile (one wavefront)
 constexpr int BLOCK = 256;               // threads per per-square kernel block
 constexpr int TY = 16, TX = 16;          // k_env_field tile
+constexpr int ENV_SPLIT_TILES = 2048;    // below this many env tiles k_env_field splits blocks by ion
+constexpr int EX_SPLIT_SQUARES = 1 << 19; // below this many squares k_env_exchange splits by ion
 constexpr double FLOAT_NONCE = 1.0e-25;  // sim_toolbox.py:50
 
 // Constants derived on the host, each in the reference's evaluation order.
@@ -197,74 +203,102 @@ struct MemCellArgs {
 };
 
 template <int NI, typename FT>
-__global__ void __launch_bounds__(NI * TMB)
+__global__ void __launch_bounds__(TMB)
 k_mem_cell(MemCellArgs A, KParams P) {
     if (halted(A.flags)) return;
+    // one wavefront per tile; lane j owns membrane m0 + j and, in the per-cell
+    // stage, cell c0 + j. Every ion (and channel) of a membrane is evaluated by
+    // its lane, so a wavefront keeps NI independent gathers in flight.
     __shared__ double s_mem[NI][TMB];
     __shared__ double s_gj[NI][TMB];
     __shared__ double s_ch[BT_MAX_CHAN][TMB];
-    __shared__ double s_c2[NI][TMB];
     __shared__ int s_bad;
     const int tile = blockIdx.x;
     const int c0 = A.tile_ptr[tile];
     const int c1 = A.tile_ptr[tile + 1];
     const int m0 = A.mem_ptr[c0];
     const int m1 = A.mem_ptr[c1];
-    const int i = threadIdx.x / TMB;      // ion of this wavefront
-    const int j = threadIdx.x % TMB;
+    const int j = threadIdx.x;
     const int m = m0 + j;
     const int M = P.n_mems, C = P.n_cells, G2 = P.n_sq;
     const double nak_coef = A.kcl[A.tile_cluster[tile]].nak_coef;
-    if (threadIdx.x == 0) s_bad = 0;
+    if (j == 0) s_bad = 0;
+
+    // per-cell inputs of the second stage, fetched up front
+    const int c = c0 + j;
+    const bool own_cell = (c < c1);
+    int ca = 0, cb = 0;
+    double vol = 1.0, divi = 0.0;
+    double cin[NI];
+    if (own_cell) {
+        ca = A.mem_ptr[c] - m0;
+        cb = A.mem_ptr[c + 1] - m0;
+        vol = A.cell_vol[c];
+        divi = A.diviterm[c];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) cin[i] = A.c_cell_in[i * C + c];
+    }
 
     if (m < m1) {
-        const int c = A.mem_cell[m];
+        const int mc = A.mem_cell[m];
         const int cn = A.nn_cell[m];
         const bool bnd = (A.nn_mem[m] == m);
         const int k = A.mem_sq[m];
         const int pos = A.mem_pos[m];
         const double sa = A.mem_sa[m];
-        const double v = A.vm_in[c];
+        const double gjo = A.gjopen[m];
+        const double v = A.vm_in[mc];
         const double vn = A.vm_in[cn];
-        double f = 0.0;                                  // fluxes_mem.fill(0)
-        // NaK-ATPase on the lagged membrane concentrations (sim.py:1196-1234, Q1);
-        // the Na and the K wavefronts evaluate the same pump
-        if (P.i_na >= 0 && (i == P.i_na || i == P.i_k)) {
-            const FT fNa = pump_nak<FT>((FT)A.c_lag_in[P.i_na * C + c], (FT)A.c_env[P.i_na * G2 + k],
-                                        (FT)A.c_lag_in[P.i_k * C + c], (FT)A.c_env[P.i_k * G2 + k], (FT)v,
-                                        nak_coef, P);
-            f = f + (double)((i == P.i_na) ? fNa : (FT)P.k_per_na * fNa);
-            if (A.obs_rate && i == P.i_na) A.obs_rate[m] = -(double)fNa;
+        // NaK-ATPase on the lagged membrane concentrations (sim.py:1196-1234, Q1)
+        FT fNa = (FT)0;
+        if (P.i_na >= 0) {
+            fNa = pump_nak<FT>((FT)A.c_lag_in[P.i_na * C + mc], (FT)A.c_env[P.i_na * G2 + k],
+                               (FT)A.c_lag_in[P.i_k * C + mc], (FT)A.c_env[P.i_k * G2 + k], (FT)v, nak_coef, P);
+            if (A.obs_rate) A.obs_rate[m] = -(double)fNa;
         }
-        // GHK across the membrane and across the gap junction (sim.py:1253-1269, 2162-2206)
-        const double vmut = with_nonces(v, i + 1);
-        const double vnmut = with_nonces(vn, i + 1);
-        const double clag = A.c_lag_in[i * C + c];
-        const FT fED = ghk<FT>((FT)A.c_env[i * G2 + k], (FT)clag, (FT)A.Dm[i * M + m], (FT)P.tm,
-                               (FT)P.zplus[i], (FT)vmut, (FT)P.F, (FT)P.RT);
-        f = f + (double)fED;
-        double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
-        if (!bnd) {
-            const double vBA = (vnmut - vmut) + FLOAT_NONCE;
-            fgj = (double)ghk<FT>((FT)clag, (FT)A.c_lag_in[i * C + cn], (FT)(P.dgj_surf[i] * A.gjopen[m]),
-                                  (FT)P.gj_len, (FT)P.zplus[i], (FT)vBA, (FT)P.F, (FT)P.RT_gj);
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            double f = 0.0;                                  // fluxes_mem.fill(0)
+            if (i == P.i_na) f = f + (double)fNa;
+            if (i == P.i_k) f = f + (double)((FT)P.k_per_na * fNa);
+            // GHK across the membrane and across the gap junction (sim.py:1253-1269, 2162-2206)
+            const double vmut = with_nonces(v, i + 1);
+            const double vnmut = with_nonces(vn, i + 1);
+            const double clag = A.c_lag_in[i * C + mc];
+            const FT fED = ghk<FT>((FT)A.c_env[i * G2 + k], (FT)clag, (FT)A.Dm[i * M + m], (FT)P.tm,
+                                   (FT)P.zplus[i], (FT)vmut, (FT)P.F, (FT)P.RT);
+            f = f + (double)fED;
+            double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
+            if (!bnd) {
+                const double vBA = (vnmut - vmut) + FLOAT_NONCE;
+                fgj = (double)ghk<FT>((FT)clag, (FT)A.c_lag_in[i * C + cn], (FT)(P.dgj_surf[i] * gjo),
+                                      (FT)P.gj_len, (FT)P.zplus[i], (FT)vBA, (FT)P.F, (FT)P.RT_gj);
+            }
+            // Ca-ATPase after the ion loop: refreshed cell Ca, post-diffusion env Ca,
+            // Vmem carrying all n_ions nonces (sim.py:2126-2159)
+            if (i == P.i_ca) {
+                double cai = A.c_cell_in[i * C + mc];
+                double cao = A.c_mid[i * G2 + k];
+                if (cai < 0.0) cai = 0.0;
+                if (cao < 0.0) cao = 0.0;
+                f = f + (double)pump_ca<FT>((FT)cai, (FT)cao, (FT)with_nonces(v, NI), P);
+            }
+            if (A.obs_fm) {
+                A.obs_fm[(size_t)i * M + m] = f;
+                A.obs_fgj[(size_t)i * M + m] = fgj;
+            }
+            const double x = f * sa;
+            s_mem[i][j] = x;
+            A.fsa[(size_t)i * M + pos] = x;
+            s_gj[i][j] = (-fgj) * sa;
         }
-        // Ca-ATPase after the ion loop: refreshed cell Ca, post-diffusion env Ca,
-        // Vmem carrying all n_ions nonces (sim.py:2126-2159)
-        if (i == P.i_ca) {
-            double cai = A.c_cell_in[i * C + c];
-            double cao = A.c_mid[i * G2 + k];
-            if (cai < 0.0) cai = 0.0;
-            if (cao < 0.0) cao = 0.0;
-            f = f + (double)pump_ca<FT>((FT)cai, (FT)cao, (FT)with_nonces(v, NI), P);
-        }
-        // voltage-gated channel on this ion (networks.py:3115-3215): gates from
-        // Vmem carrying the nonces so far, update_mh, open probability, GHK with
-        // the current cell and post-diffusion env concentrations. Gating stays
-        // FP64 in both flux modes: the Hodgkin-Huxley rates have removable 0/0
-        // points (V = -25 mV, -40 mV for Nav1p2) that FP32 rounding would hit.
-        const int ch = P.ion_chan[i];
-        if (ch >= 0) {
+        // voltage-gated channels (networks.py:3115-3215): gates from Vmem carrying
+        // the nonces so far, update_mh, open probability, GHK with the current cell
+        // and post-diffusion env concentrations. Gating stays FP64 in both flux
+        // modes: the Hodgkin-Huxley rates have removable 0/0 points (V = -25 mV,
+        // -40 mV for Nav1p2) that FP32 rounding would hit.
+        for (int ch = 0; ch < P.n_chan; ++ch) {
+            const int i = P.chan_ion[ch];
             double* gm = A.gates + (size_t)(2 * ch) * M;
             double* gh = gm + M;
             const double vg = with_nonces(v, NI + ch);
@@ -283,63 +317,52 @@ k_mem_cell(MemCellArgs A, KParams P) {
             // m**p * h**q with numpy's power: p = 2 squares, p = 3 calls pow
             const double Pch = (kind == BT_CHAN_NAV1P2) ? cube(mm) * hh
                              : (kind == BT_CHAN_KV1P1) ? mm * (hh * hh) : (mm * mm) * hh;
-            const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * A.chan_mod[(size_t)ch * C + c]);
-            const FT fc = ghk<FT>((FT)A.c_mid[i * G2 + k], (FT)A.c_cell_in[i * C + c], Dch, (FT)P.tm,
+            const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * A.chan_mod[(size_t)ch * C + mc]);
+            const FT fc = ghk<FT>((FT)A.c_mid[i * G2 + k], (FT)A.c_cell_in[i * C + mc], Dch, (FT)P.tm,
                                   (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE), (FT)P.F, (FT)P.RT);
             const double x = (double)fc * sa;
             s_ch[ch][j] = x;
             A.fch[(size_t)ch * M + pos] = x;
             if (A.obs_fch) A.obs_fch[(size_t)ch * M + m] = (double)fc;
         }
-        if (A.obs_fm) {
-            A.obs_fm[(size_t)i * M + m] = f;
-            A.obs_fgj[(size_t)i * M + m] = fgj;
-        }
-        const double x = f * sa;
-        s_mem[i][j] = x;
-        A.fsa[(size_t)i * M + pos] = x;
-        s_gj[i][j] = (-fgj) * sa;
     }
     __syncthreads();
 
-    // per-cell segment sums in membrane order (np.dot(M_sum_mems, .)), one ion per wavefront
-    const int c = c0 + j;
-    if (c < c1) {
-        const int a = A.mem_ptr[c] - m0;
-        const int b = A.mem_ptr[c + 1] - m0;
-        const double vol = A.cell_vol[c];
-        double base = A.c_cell_in[i * C + c];
-        const int ch = P.ion_chan[i];
-        if (ch >= 0) {             // immediate update_Co of the channel flux
-            double Sc = s_ch[ch][a];
-            for (int q = a + 1; q < b; ++q) Sc = Sc + s_ch[ch][q];
-            base = base + (Sc / vol) * P.dt;
-        }
-        double S = s_mem[i][a];
-        double Sg = s_gj[i][a];
-        for (int q = a + 1; q < b; ++q) {
-            S = S + s_mem[i][q];
-            Sg = Sg + s_gj[i][q];
-        }
-        const double c1v = base + (S / vol) * P.dt;
-        A.c_lag_out[i * C + c] = c1v;
-        double c2 = c1v + P.dt * (Sg / vol);
-        if (c2 < 0.0) c2 = 0.0;                    // no_negs clamp (Q9)
-        if (isnan(c2)) s_bad = 1;
-        A.c_cell_out[i * C + c] = c2;
-        s_c2[i][j] = c2;
-    }
-    __syncthreads();
-    if (i == 0 && c < c1) {
-        double rho = P.zF[0] * s_c2[0][j];
+    // per-cell segment sums in membrane order (np.dot(M_sum_mems, .)), then
+    // update_Co, the GJ update, no_negs, and rho_cells -> Vmem (ion order)
+    if (own_cell) {
+        bool bad = false;
+        double rho = 0.0;
 #pragma unroll
-        for (int ii = 1; ii < NI; ++ii) rho = rho + P.zF[ii] * s_c2[ii][j];
-        const double vm = P.cm_inv * (rho * A.diviterm[c]);
-        if (isnan(vm)) s_bad = 1;
+        for (int i = 0; i < NI; ++i) {
+            double base = cin[i];
+            const int ch = P.ion_chan[i];
+            if (ch >= 0) {             // immediate update_Co of the channel flux
+                double Sc = s_ch[ch][ca];
+                for (int q = ca + 1; q < cb; ++q) Sc = Sc + s_ch[ch][q];
+                base = base + (Sc / vol) * P.dt;
+            }
+            double S = s_mem[i][ca];
+            double Sg = s_gj[i][ca];
+            for (int q = ca + 1; q < cb; ++q) {
+                S = S + s_mem[i][q];
+                Sg = Sg + s_gj[i][q];
+            }
+            const double c1v = base + (S / vol) * P.dt;
+            A.c_lag_out[i * C + c] = c1v;
+            double c2 = c1v + P.dt * (Sg / vol);
+            if (c2 < 0.0) c2 = 0.0;                    // no_negs clamp (Q9)
+            if (isnan(c2)) bad = true;
+            A.c_cell_out[i * C + c] = c2;
+            rho = (i == 0) ? P.zF[0] * c2 : rho + P.zF[i] * c2;
+        }
+        const double vm = P.cm_inv * (rho * divi);
+        if (isnan(vm)) bad = true;
         A.vm_out[c] = vm;
+        if (bad) s_bad = 1;
     }
     __syncthreads();
-    if (threadIdx.x == 0 && s_bad) {
+    if (j == 0 && s_bad) {
         atomicOr(A.flags, 1u);
         atomicMin(A.bad_step, (unsigned long long)(*A.step_ctr));
     }
@@ -438,7 +461,11 @@ k_env_field(FieldArgs A, KParams P) {
         return;
     }
 
-    for (int i = 0; i < P.n_ions; ++i) {
+    // with gridDim.y == n_ions each block takes one ion (and rebuilds its tile's
+    // field); with gridDim.y == 1 a block runs every ion over its tile
+    const int i_lo = (gridDim.y == 1) ? 0 : (int)blockIdx.y;
+    const int i_hi = (gridDim.y == 1) ? P.n_ions : (int)blockIdx.y + 1;
+    for (int i = i_lo; i < i_hi; ++i) {
         const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
         const double* D = A.D_env + (size_t)i * P.n_sq + so;
         const double cb = P.c_bound[i];
@@ -523,8 +550,48 @@ struct ExArgs {
     const unsigned* flags; long long* step_ctr; int count_step;
 };
 
-__global__ void __launch_bounds__(BLOCK)
+// One wavefront per ion over EXQ squares; the charge of each square is then
+// summed over the ions in ion order from LDS.
+constexpr int EXQ = 64;
+
+__global__ void __launch_bounds__(BT_MAX_IONS * EXQ)
 k_env_exchange(ExArgs A, KParams P) {
+    if (halted(A.flags)) return;
+    __shared__ double s_cn[BT_MAX_IONS][EXQ];
+    const int i = threadIdx.x / EXQ, j = threadIdx.x % EXQ;
+    const int k = blockIdx.x * EXQ + j;
+    if (A.count_step && blockIdx.x == 0 && threadIdx.x == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
+    const bool live = (k < P.n_sq);
+    if (live) {
+        const int a = A.sq_ptr[k], b = A.sq_ptr[k + 1];
+        const double env_vol = A.kcl[A.sq_cluster[k]].env_vol;
+        double cn = A.c_mid[(size_t)i * P.n_sq + k];
+        const int ch = P.ion_chan[i];
+        if (ch >= 0) {
+            double Sc = 0.0;
+            for (int q = a; q < b; ++q) Sc = Sc + (-A.fch[(size_t)ch * P.n_mems + q]);
+            cn = cn + (Sc / env_vol) * P.dt;
+        }
+        double S = 0.0;
+        for (int q = a; q < b; ++q) S = S + (-A.fsa[(size_t)i * P.n_mems + q]);
+        cn = cn + (S / env_vol) * P.dt;
+        A.c_env[(size_t)i * P.n_sq + k] = cn;
+        s_cn[i][j] = cn;
+    }
+    __syncthreads();
+    if (i == 0 && live) {
+        double rho = P.zF[0] * s_cn[0][j];
+        for (int ii = 1; ii < P.n_ions; ++ii) rho = rho + P.zF[ii] * s_cn[ii][j];
+        const KCluster& K = A.kcl[A.sq_cluster[k]];
+        double uu = 0.0;
+        if (A.has_mem[k]) uu = ((rho * K.surf_scale) / K.mean_sa) / K.kee;
+        A.u[k] = uu;
+    }
+}
+
+// one thread per square over all ions (large batches: enough squares to fill the GPU)
+__global__ void __launch_bounds__(BLOCK)
+k_env_exchange_sq(ExArgs A, KParams P) {
     if (halted(A.flags)) return;
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if (A.count_step && k == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
@@ -537,11 +604,11 @@ k_env_exchange(ExArgs A, KParams P) {
         const int ch = P.ion_chan[i];
         if (ch >= 0) {
             double Sc = 0.0;
-            for (int j = a; j < b; ++j) Sc = Sc + (-A.fch[(size_t)ch * P.n_mems + j]);
+            for (int q = a; q < b; ++q) Sc = Sc + (-A.fch[(size_t)ch * P.n_mems + q]);
             cn = cn + (Sc / K.env_vol) * P.dt;
         }
         double S = 0.0;
-        for (int j = a; j < b; ++j) S = S + (-A.fsa[(size_t)i * P.n_mems + j]);
+        for (int q = a; q < b; ++q) S = S + (-A.fsa[(size_t)i * P.n_mems + q]);
         cn = cn + (S / K.env_vol) * P.dt;
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         rho = (i == 0) ? P.zF[0] * cn : rho + P.zF[i] * cn;
@@ -973,9 +1040,9 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
 template <int NI>
 static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a) {
     if (c->prm.flux_mode == BT_FLUX_FP32)
-        hipLaunchKernelGGL((k_mem_cell<NI, float>), dim3(c->n_tiles), dim3(NI * TMB), 0, c->stream, a, c->kp);
+        hipLaunchKernelGGL((k_mem_cell<NI, float>), dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
     else
-        hipLaunchKernelGGL((k_mem_cell<NI, double>), dim3(c->n_tiles), dim3(NI * TMB), 0, c->stream, a, c->kp);
+        hipLaunchKernelGGL((k_mem_cell<NI, double>), dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
 }
 
 static void launch_mem_cell(bt_ctx* c, const MemCellArgs& a) {
@@ -997,7 +1064,10 @@ static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr) {
     const bt_buffers& B = c->buf;
     FieldArgs f{c->env_tiles, c->kcl, c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
                 ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr};
-    dim3 grid(c->n_env_tiles);
+    // diffusion: one block per (tile, ion) while tiles alone would leave CUs idle,
+    // else one block per tile looping over the ions; field only: one block per tile
+    const bool split = diffuse && c->n_env_tiles < ENV_SPLIT_TILES;
+    dim3 grid(c->n_env_tiles, split ? c->prm.n_ions : 1);
     if (diffuse)
         hipLaunchKernelGGL(k_env_field<true>, grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else
@@ -1031,7 +1101,11 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     if (which < 0 || which == 2) {
         ExArgs x{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, B.c_env, c->u, c->flags,
                  c->step_ctr, which < 0 ? 1 : 0};
-        hipLaunchKernelGGL(k_env_exchange, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, x, c->kp);
+        if (nsq < EX_SPLIT_SQUARES)
+            hipLaunchKernelGGL(k_env_exchange, dim3((nsq + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0,
+                               c->stream, x, c->kp);
+        else
+            hipLaunchKernelGGL(k_env_exchange_sq, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, x, c->kp);
     }
     if (which == 3) launch_field(c, false);
     if ((which < 0 || which == 4) && c->net && c->c_net)

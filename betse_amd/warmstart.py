@@ -125,9 +125,10 @@ class InitResult:
 
 
 def run_to_residual(bs: BatchSimulator, tol: float = 1e-6, window: int = 50, max_steps: int = 200_000,
-                    log=None) -> InitResult:
+                    log=None, run_all: bool = False) -> InitResult:
     """Step the whole batch in windows until every cluster's residual is below
-    ``tol`` (or ``max_steps``); returns per-cluster step counts."""
+    ``tol`` (or ``max_steps``); returns per-cluster step counts. With
+    ``run_all`` the batch runs the full ``max_steps`` regardless."""
     n_cl = len(bs.states)
     steps = np.full(n_cl, -1, dtype=np.int64)
     secs = np.full(n_cl, np.nan)
@@ -143,7 +144,7 @@ def run_to_residual(bs: BatchSimulator, tol: float = 1e-6, window: int = 50, max
         steps[new] = n
         secs[new] = now
         last = r
-        if np.all(steps >= 0):
+        if np.all(steps >= 0) and not run_all:
             break
         if log is not None and time.perf_counter() > next_log:
             next_log = time.perf_counter() + 30.0
@@ -239,11 +240,14 @@ def warm_start_batch(bs: BatchSimulator, starter: Optional[WarmStarter], tol: fl
 # training data and training
 # ----------------------------------------------------------------------
 def make_dataset(n_clusters: int, seed: int, *, cells_range=(500, 2000), grid_range=(48, 88),
-                 decades: float = 2.0, batch: int = 256, tol: float = 1e-6, label_tol: float = 0.5e-6,
-                 window: int = 50, max_steps: int = 200_000, flux_mode: str = "fp64", device=None, log=None):
-    """Init-phase steady states of ``n_clusters`` random clusters, produced by
-    the plain method on the GPU in batches. Returns per-cluster
-    (features, neighbour CSR, targets, converged flag) and run statistics."""
+                 decades: float = 2.0, batch: int = 256, tol: float = 1e-6,
+                 window: int = 100, max_steps: int = 200_000, flux_mode: str = "fp64", device=None, log=None):
+    """Init-phase states of ``n_clusters`` random clusters, produced by the plain
+    method on the GPU in batches: every cluster runs the full ``max_steps``
+    and its final state is the label (the furthest along the relaxation the
+    init phase gets); ``ok`` records whether its residual fell below ``tol``
+    within the horizon. Returns per-cluster (features, neighbour CSR,
+    targets, ok, steps) and run statistics."""
     base = Parameters.preset("small")
     data = []
     stats = []
@@ -255,12 +259,13 @@ def make_dataset(n_clusters: int, seed: int, *, cells_range=(500, 2000), grid_ra
         t_w = time.perf_counter() - t0
         bs = BatchSimulator(worlds, device=device, flux_mode=flux_mode)
         bs.init("init")
-        res = run_to_residual(bs, label_tol, window, max_steps, log)
+        res = run_to_residual(bs, tol, window, max_steps, log, run_all=True)
         bs.pull()
         for i, (st, (p, cells)) in enumerate(zip(bs.states, worlds)):
             data.append(dict(X=cell_features(p, cells, base), ptr=cells.cell_nn_ptr.copy(),
                              idx=cells.cell_nn_idx.copy(), Y=state_targets(st, st.names),
-                             ok=bool(res.steps[i] >= 0), steps=int(res.steps[i]), names=st.names))
+                             ok=bool(res.steps[i] >= 0), steps=int(res.steps[i]), names=st.names,
+                             residual=float(res.residual[i])))
         stats.append(dict(clusters=nb, cells=int(sum(c.n_cells for _, c in worlds)), world_seconds=t_w,
                           init_seconds=res.seconds, converged=int((res.steps >= 0).sum()),
                           median_steps=float(np.median(np.where(res.steps >= 0, res.steps, max_steps)))))
@@ -271,13 +276,14 @@ def make_dataset(n_clusters: int, seed: int, *, cells_range=(500, 2000), grid_ra
 
 
 def train(data, *, hidden: int = 64, n_layers: int = 6, epochs: int = 30, clusters_per_batch: int = 32,
-          lr: float = 2e-3, seed: int = 0, device=None, base: Optional[Parameters] = None, log=None):
-    """Train ``WarmStartGNN`` on converged clusters of ``data`` (GPU kernels
-    only: BF16 MFMA products, FP32 accumulation, Adam)."""
+          lr: float = 2e-3, seed: int = 0, device=None, base: Optional[Parameters] = None, log=None,
+          converged_only: bool = False):
+    """Train ``WarmStartGNN`` on the clusters of ``data`` (GPU kernels only:
+    BF16 MFMA products, FP32 accumulation, Adam)."""
     from .gnn import WarmStartGNN
-    good = [d for d in data if d["ok"]]
+    good = [d for d in data if d["ok"] or not converged_only]
     if not good:
-        raise ValueError("no converged cluster to train on")
+        raise ValueError("no cluster to train on")
     X = np.concatenate([d["X"] for d in good])
     Y = np.concatenate([d["Y"] for d in good])
     x_mean, x_std = X.mean(axis=0), X.std(axis=0) + 1e-6
@@ -355,6 +361,9 @@ def evaluate(starter: WarmStarter, n_clusters: int, seed: int, *, cells_range=(5
         cold_median_steps=float(np.median(np.where(cold.steps >= 0, cold.steps, max_steps))),
         warm_median_steps=float(np.median(np.where(warm.steps >= 0, warm.steps, max_steps))),
         cold_max_steps=int(np.where(cold.steps >= 0, cold.steps, max_steps).max()),
+        cold_total_steps=int(np.where(cold.steps >= 0, cold.steps, max_steps).sum()),
+        warm_total_steps=int(np.where(warm.steps >= 0, warm.steps, max_steps).sum()),
+        cold_steps=cold.steps.tolist(), warm_steps=warm.steps.tolist(),
         warm_max_steps=int(np.where(warm.steps >= 0, warm.steps, max_steps).max()),
         speedup_median=float(np.median(cold.steps[both] / np.maximum(warm.steps[both], 1))) if both.any() else 0.0,
         pred_err_logc_max=err_pred[:, :-1].max(axis=0).tolist() if len(err_pred) else [],

@@ -38,8 +38,11 @@ oracle function             reference
 ``HH_Decomp``               ``sim_toolbox.py:1236-1290``
 ``sampled_steps``           ``sim.py:2455-2462`` with ``phase/phasecls.py:330``
 ``OracleChannel``           ``channels/channelsabc.py:40-60`` (update_mh), ``channels/vg_na.py:34-113,131-186``
-                            (Nav1p2), ``channels/vg_k.py:34-115,133-178`` (Kv1p1)
+                            (Nav1p2), ``channels/vg_k.py:34-115,133-178`` (Kv1p1),
+                            ``channels/vg_ca.py:60-113,292-340`` (Cav1p2)
 ``OracleSim.run_loop_channels`` ``chemistry/networks.py:3115-3215`` (channels on all membranes)
+reaction network            ``oracle/network_oracle.py`` (``chemistry/networks.py:2805-2982``),
+                            called where ``sim.py:1286-1300`` runs the molecules network
 ==========================  ===============================================
 
 Two operator forms are supported:
@@ -54,7 +57,7 @@ Two operator forms are supported:
 
 Parity pins (see ``tests/test_oracle_*.py``): closed forms of the GHK and pump
 fluxes (Fick limit, antisymmetry, equilibrium), golden vectors of the
-finite-difference stencils written by ``tests/golden/make_fd_vectors.py``,
+finite-difference stencils written by ``tests/golden/make_golden_vectors.py``,
 ``scipy.ndimage.gaussian_filter`` itself, dense == sparse operators, exact
 mass conservation of membrane/environment exchange, the electroneutral
 start (Vmem == 0), and the residual of the Poisson solve.

@@ -283,8 +283,14 @@ def test_network_kernel_alone_matches_oracle():
     got = sim._dev["c_net"].cpu().numpy()
     for _ in range(2000):
         C = no.step(net, C, p.dt, p.R, sim.T)
-    rel = np.abs(got - C) / np.maximum(np.abs(C), 1e-30)
-    say(f"network alone: max rel err {rel.max():.3e}")
+    # a reversible reaction whose reactant was clamped to zero gives Q = x/0 in the
+    # reference as well: the non-finite cells must be the same ones
+    bad = ~np.isfinite(C)
+    assert np.array_equal(bad, ~np.isfinite(got))
+    ok = ~bad
+    rel = np.abs(got[ok] - C[ok]) / np.maximum(np.abs(C[ok]), 1e-30)
+    say(f"network alone: max rel err {rel.max():.3e} over {ok.sum()} finite of {C.size}")
+    assert ok.mean() > 0.5
     assert rel.max() < 1e-10
 
 
