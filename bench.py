@@ -172,11 +172,11 @@ def bench_warmstart(a):
         print("# " + msg, file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    data, stats, base = ws.make_dataset(a.train_clusters, seed=1000, batch=a.clusters, max_steps=a.max_init_steps,
+    data, stats, base = ws.make_dataset(a.train_clusters, seed=1000, batch=a.clusters, horizon=a.label_horizon,
                                         flux_mode=a.ws_flux, log=log)
     t_data = time.perf_counter() - t0
     n_ok = sum(d["ok"] for d in data)
-    log(f"dataset: {len(data)} clusters, {n_ok} converged, {t_data:.1f}s")
+    log(f"dataset: {len(data)} clusters, {n_ok} labels accepted, {t_data:.1f}s")
     starter, hist, t_train = ws.train(data, epochs=a.epochs, log=log)
     if a.save_model:
         starter.save(a.save_model)
@@ -197,10 +197,11 @@ def bench_warmstart(a):
         "dtype": ("f64" if a.ws_flux == "fp64" else "f32-flux/f64-state") + " step; bf16-mfma/f32 network",
         "data": "synthetic",
         "config": {"workload": f"learned warm start: {len(data)} training clusters of 500-2000 cells "
-                               f"({n_ok} converged within {a.max_init_steps} steps), {a.eval_clusters} held-out "
-                               f"clusters, Dm(Na/K/Cl) and NaK rate log-uniform over two decades",
+                               f"({n_ok} steady-state labels accepted by the stepper), {a.eval_clusters} held-out "
+                               f"clusters, Dm(Na/K/Cl) and NaK rate log-uniform over two decades, init capped at "
+                               f"{a.max_init_steps} steps",
                    "train_clusters": len(data), "eval_clusters": a.eval_clusters,
-                   "max_init_steps": a.max_init_steps, "epochs": a.epochs},
+                   "max_init_steps": a.max_init_steps, "label_horizon": a.label_horizon, "epochs": a.epochs},
         "cold_seconds": ev["cold_seconds"],
         "speedup": ev["cold_seconds"] / max(ev["warm_seconds"], 1e-9),
         "evaluation": ev,
@@ -251,6 +252,24 @@ def time_gpu(p, cells, flux_mode, steps, warmup):
     names = ("mem_cell", "env_field", "env_exchange") + (("network",) if sim.network is not None else ())
     kernels = {k: sim.time_kernel(k, 100) for k in names}
     return sim, wall, ev_ms, kernels
+
+
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def measured_traffic(workload: str, kernel: str, flux_mode: str):
+    """HBM bytes per launch of a step kernel from the committed rocprofv3 PMC
+    summary (profiles/hbm_traffic_<workload>.txt: FETCH_SIZE and WRITE_SIZE per
+    dispatch, already scaled by the plain-copy calibration), or None."""
+    path = os.path.join(PROFILES, f"hbm_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 6 and t[0] == "kernel" and t[1] == kernel and t[3] == flux_mode:
+                return float(t[5])
+    return None
 
 
 def time_sampled_loop(p, cells, flux_mode, steps):
@@ -304,6 +323,8 @@ def main(argv=None):
     ap.add_argument("--train-clusters", type=int, default=1024, help="warmstart config: training clusters")
     ap.add_argument("--eval-clusters", type=int, default=256, help="warmstart config: held-out clusters")
     ap.add_argument("--max-init-steps", type=int, default=100_000, help="warmstart config: init step cap")
+    ap.add_argument("--label-horizon", type=int, default=20_000,
+                    help="warmstart config: stepper steps before the first extrapolation of a label")
     ap.add_argument("--epochs", type=int, default=40, help="warmstart config: training epochs")
     ap.add_argument("--save-model", default="", help="warmstart config: save the trained network (.npz)")
     ap.add_argument("--ws-flux", default="fp32", choices=["fp64", "fp32"],
@@ -417,7 +438,8 @@ def main(argv=None):
                    "cells": C, "membranes": cells.n_mems, "grid": list(cells.X.shape), "n_ions": n_ions,
                    "flux_mode": best},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(a.config, dom, best),
                      "launch_us": kern[dom] * 1e3, "algorithmic_bytes": ab[dom],
                      "step_algorithmic_bytes": ab["step"],
                      "step_achieved_GBs": ab["step"] / (ms_step * 1e-3) / 1e9},

@@ -25,6 +25,7 @@ BT_FLUX_FP64 = 0
 BT_FLUX_FP32 = 1
 
 BT_TILE_MEMBRANES = 64
+BT_TILE_CELLS = 32
 
 KERNEL_IDS = {"mem_cell": 0, "env_field": 1, "env_exchange": 2, "env_field_only": 3, "network": 4}
 

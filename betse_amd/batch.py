@@ -266,6 +266,12 @@ class BatchSimulator:
             if len(st.channel_specs):
                 st.chan_gates = h["gates"][:, :, moff[i]:moff[i + 1]].copy()
 
+    def snapshot(self):
+        """(c_cell [n_ions][C], vm_cell [C]) of the whole batch, on the host."""
+        self.sync()
+        self.stream.synchronize()
+        return self._dev["c_cell"].cpu().numpy().copy(), self._dev["vm_cell"].cpu().numpy().copy()
+
     def residual_window(self, window: int) -> np.ndarray:
         """Run ``window`` steps and return, per cluster, max |c - c_prev| / (window |c|)
         over cells and ions (the per-step relative residual of the init phase)."""

@@ -103,8 +103,10 @@ __device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT d, FT zc, FT vBA, FT F
     return -((Dc * alpha) / d) * ((cB - cA * exp_alpha) / deno);
 }
 
-__device__ __forceinline__ double cube(double x) { return pow(x, 3.0); }
-__device__ __forceinline__ float cube(float x) { return powf(x, 3.0f); }
+// x**3: two multiplications (NumPy calls pow; the results differ by at most an
+// ulp, far inside the parity bars, and pow costs an order of magnitude more)
+__device__ __forceinline__ double cube(double x) { return (x * x) * x; }
+__device__ __forceinline__ float cube(float x) { return (x * x) * x; }
 
 // pumpNaKATP (sim_toolbox.py:71-122): returns f_Na; f_K = -(2/3)*f_Na.
 template <typename FT>
@@ -314,7 +316,7 @@ k_mem_cell(MemCellArgs A, KParams P) {
             hh = (hTau * hh + dtm * hInf) / (hTau + dtm);
             gm[m] = mm;
             gh[m] = hh;
-            // m**p * h**q with numpy's power: p = 2 squares, p = 3 calls pow
+            // m**p * h**q: p = 2 squares as NumPy does, p = 3 is cube()
             const double Pch = (kind == BT_CHAN_NAV1P2) ? cube(mm) * hh
                              : (kind == BT_CHAN_KV1P1) ? mm * (hh * hh) : (mm * mm) * hh;
             const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * A.chan_mod[(size_t)ch * C + mc]);
@@ -1172,6 +1174,22 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
                          b->sq_has_mem, b->c_cell, b->c_lag, b->vm_cell, b->c_env, b->E_x, b->E_y, b->v_env};
     for (const void* q : req)
         if (!q) return set_err(nullptr, BT_EINVAL, "null device buffer");
+    {
+        // k_mem_cell indexes LDS by a tile's membranes and cells: check every tile
+        std::vector<int32_t> tp((size_t)b->n_tiles + 1), mp((size_t)p->n_cells + 1);
+        hipError_t e = hipMemcpy(tp.data(), b->tile_ptr, tp.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(mp.data(), b->mem_ptr, mp.size() * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return set_err(nullptr, BT_EHIP, std::string("reading tiles: ") + hipGetErrorString(e));
+        if (tp[0] != 0 || tp[b->n_tiles] != p->n_cells || mp[0] != 0 || mp[p->n_cells] != p->n_mems)
+            return set_err(nullptr, BT_EINVAL, "tile_ptr/mem_ptr do not cover the cells and membranes");
+        for (int t = 0; t < b->n_tiles; ++t) {
+            const int a0 = tp[t], a1 = tp[t + 1];
+            if (a1 <= a0 || a1 - a0 > BT_TILE_CELLS || mp[a1] - mp[a0] > BT_TILE_MEMBRANES)
+                return set_err(nullptr, BT_EINVAL, "tile " + std::to_string(t) + " exceeds the tile limits");
+        }
+        for (int q = 0; q < p->n_cells; ++q)
+            if (mp[q + 1] <= mp[q]) return set_err(nullptr, BT_EINVAL, "a cell has no membranes");
+    }
 
     bt_ctx* c = new bt_ctx();
     c->prm = *p;

@@ -113,8 +113,9 @@ def phase_time_steps(n_steps: int, dt: float, t_resample: float):
     return time_steps, sampled
 
 
-def make_tiles(mem_ptr: np.ndarray, tile_m: int = TILE_M) -> np.ndarray:
-    """Greedy packing of consecutive whole cells into tiles of <= tile_m membranes."""
+def make_tiles(mem_ptr: np.ndarray, tile_m: int = TILE_M, tile_c: int = _native.BT_TILE_CELLS) -> np.ndarray:
+    """Greedy packing of consecutive whole cells into tiles of <= tile_m
+    membranes and <= tile_c cells."""
     n = len(mem_ptr) - 1
     counts = np.diff(mem_ptr)
     if counts.max(initial=0) > tile_m:
@@ -124,7 +125,7 @@ def make_tiles(mem_ptr: np.ndarray, tile_m: int = TILE_M) -> np.ndarray:
     tiles = [0]
     acc = 0
     for c in range(n):
-        if acc + counts[c] > tile_m:
+        if acc + counts[c] > tile_m or c - tiles[-1] == tile_c:
             tiles.append(c)
             acc = 0
         acc += counts[c]
@@ -489,21 +490,47 @@ class Simulator(HostState):
         if rc != _native.BT_OK:
             raise BetseNativeError(f"libbetse_amd error {rc}: {self._lib.bt_last_error(self._ctx).decode()}")
 
-    def _pull_state(self) -> None:
-        d = self._dev
+    def _fetch(self, tensors: dict) -> dict:
+        """Device tensors -> host arrays through cached pinned buffers: all copies
+        are queued on the simulation stream, then one synchronisation."""
+        torch = self.torch
+        pinned = self.__dict__.setdefault("_pinned", {})
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            for k, t in tensors.items():
+                h = pinned.get(k)
+                if h is None or h.shape != t.shape or h.dtype != t.dtype:
+                    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+                    pinned[k] = h
+                h.copy_(t, non_blocking=True)
         self.stream.synchronize()
-        self.cc_cells = d["c_cell"].cpu().numpy().copy()
-        self.c_lag = d["c_lag"].cpu().numpy().copy()
-        self.vm_cell = d["vm_cell"].cpu().numpy().copy()
-        self.cc_env = d["c_env"].cpu().numpy().copy()
-        shp = self.cells.X.shape
-        self.E_env_x = d["E_x"].cpu().numpy().reshape(shp).copy()
-        self.E_env_y = d["E_y"].cpu().numpy().reshape(shp).copy()
-        self.v_env = d["v_env"].cpu().numpy().copy()
-        if len(self.channel_specs):
-            self.chan_gates = d["gates"].cpu().numpy().copy()
+        return {k: pinned[k].numpy().copy() for k in tensors}
+
+    def _pull_state(self, full: bool = True) -> None:
+        """Device state -> host. ``full=False`` (sampled steps) skips what
+        write2storage does not store: the lagged membrane concentrations and
+        the channel gates."""
+        d = self._dev
+        names = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"]
+        if full:
+            names.append("c_lag")
+            if len(self.channel_specs):
+                names.append("gates")
         if self.network is not None:
-            self.c_net = d["c_net"].cpu().numpy().copy()
+            names.append("c_net")
+        h = self._fetch({k: d[k] for k in names})
+        shp = self.cells.X.shape
+        self.cc_cells = h["c_cell"]
+        self.vm_cell = h["vm_cell"]
+        self.cc_env = h["c_env"]
+        self.E_env_x = h["E_x"].reshape(shp)
+        self.E_env_y = h["E_y"].reshape(shp)
+        self.v_env = h["v_env"]
+        if full:
+            self.c_lag = h["c_lag"]
+            if len(self.channel_specs):
+                self.chan_gates = h["gates"]
+        if self.network is not None:
+            self.c_net = h["c_net"]
 
     def _sync(self, phase_step0: int = 0) -> None:
         bad = ctypes.c_int64(-1)
@@ -547,13 +574,12 @@ class Simulator(HostState):
         """Observables of the last observed step (get_current, ion_current.py:13-109)."""
         obs, t = self._obs_buffers()
         self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
-        self.stream.synchronize()
         shp = self.cells.X.shape
-        h = {k: v.cpu().numpy().copy() for k, v in t.items()}
-        self.fluxes_mem, self.fluxes_gj = h["fluxes_mem"], h["fluxes_gj"]
-        self.fluxes_env_x, self.fluxes_env_y = h["fluxes_env_x"], h["fluxes_env_y"]
+        # the fields write2storage keeps; the raw fluxes stay on the device (pull_raw_fluxes)
+        keep = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "J_env_x",
+                "J_env_y", "Jtx", "Jty", "B_field")
+        h = self._fetch({k: t[k] for k in keep})
         self.rate_NaKATP = h["rate_NaKATP"]
-        self.chan_flux = h["chan_flux"][:len(self.channel_specs)]
         self.Jn, self.I_mem = h["Jn"], h["I_mem"]
         self.E_gj_x, self.E_gj_y = h["E_gj_x"], h["E_gj_y"]
         self.J_cell_x, self.J_cell_y = h["J_cell_x"], h["J_cell_y"]
@@ -561,6 +587,15 @@ class Simulator(HostState):
         self.J_env_x, self.J_env_y = h["J_env_x"].reshape(shp), h["J_env_y"].reshape(shp)
         self.Jtx, self.Jty = h["Jtx"].reshape(shp), h["Jty"].reshape(shp)
         self.B_field = h["B_field"].reshape(shp)
+
+    def pull_raw_fluxes(self) -> None:
+        """The last observed step's raw fluxes (fluxes_mem, fluxes_gj,
+        fluxes_env_x/y, channel fluxes) to the host."""
+        _, t = self._obs_buffers()
+        h = self._fetch({k: t[k] for k in ("fluxes_mem", "fluxes_gj", "fluxes_env_x", "fluxes_env_y", "chan_flux")})
+        self.fluxes_mem, self.fluxes_gj = h["fluxes_mem"], h["fluxes_gj"]
+        self.fluxes_env_x, self.fluxes_env_y = h["fluxes_env_x"], h["fluxes_env_y"]
+        self.chan_flux = h["chan_flux"][:len(self.channel_specs)]
 
     def step_observed(self, n: int) -> None:
         """Run n steps; the last one's observables are pulled to the host."""
@@ -635,7 +670,7 @@ class Simulator(HostState):
             self._step_observed(s + 1 - pos)
             pos = s + 1
             self._sync(step0)
-            self._pull_state()
+            self._pull_state(full=False)
             self._pull_observables()
             phase.callbacks.progressed_next()
             self.write2storage(time_steps[s], phase.cells, phase.p)

@@ -239,15 +239,98 @@ def warm_start_batch(bs: BatchSimulator, starter: Optional[WarmStarter], tol: fl
 # ----------------------------------------------------------------------
 # training data and training
 # ----------------------------------------------------------------------
+def _extrapolate(snaps, coff):
+    """One vector extrapolation step per cluster from three equally spaced
+    snapshots (c, vm): with d1 = x2 - x1, d2 = x3 - x2 and the cluster's
+    contraction ratio q = <d1, d2>/<d1, d1> (relative changes over all cells
+    and ions), the limit of x_n = x* + A q^n is x3 + d2 q/(1 - q). Clusters
+    without a contracting ratio keep x3."""
+    (c1, v1), (c2, v2), (c3, v3) = snaps
+    cx, vx = c3.copy(), v3.copy()
+    qs = np.zeros(len(coff) - 1)
+    for i in range(len(coff) - 1):
+        a, b = coff[i], coff[i + 1]
+        den = np.where(c3[:, a:b] != 0, np.abs(c3[:, a:b]), 1.0)
+        d1 = (c2[:, a:b] - c1[:, a:b]) / den
+        d2 = (c3[:, a:b] - c2[:, a:b]) / den
+        q = float(np.sum(d1 * d2) / max(np.sum(d1 * d1), 1e-300))
+        qs[i] = q
+        if 0.0 < q < 0.999:
+            g = q / (1.0 - q)
+            cx[:, a:b] = c3[:, a:b] + (c3[:, a:b] - c2[:, a:b]) * g
+            vx[a:b] = v3[a:b] + (v3[a:b] - v2[a:b]) * g
+    return cx, vx, qs
+
+
+def _targets_from(st, c, vm) -> np.ndarray:
+    """state_targets of a cluster state given as arrays."""
+    names = st.names
+    ions = target_ions(names)
+    Y = np.zeros((c.shape[1], len(ions) + 1))
+    for j, ion in enumerate(ions):
+        Y[:, j] = np.log(np.maximum(c[names.index(ion)], 1e-300) / st.p.cell_concs[ion])
+    Y[:, -1] = vm / V_SCALE
+    return Y
+
+
+def _run_snapshots(bs: BatchSimulator, lead: int, gap: int):
+    """Run ``lead`` steps, then take three snapshots ``gap`` steps apart."""
+    if lead > 0:
+        bs.step_n(lead)
+    snaps = [bs.snapshot()]
+    for _ in range(2):
+        bs.step_n(gap)
+        snaps.append(bs.snapshot())
+    return snaps
+
+
+def steady_state_labels(bs: BatchSimulator, horizon: int = 20_000, rounds: int = 2, tol: float = 1e-6,
+                        window: int = 100, log=None):
+    """Init-phase steady states of every cluster of ``bs``: the plain stepper
+    runs ``horizon`` steps from the default start; its trajectory is
+    extrapolated to its limit (``_extrapolate``); each further round restarts
+    the stepper from the extrapolated state (charge made consistent through M,
+    as ``inject``) and extrapolates again over ``horizon/2`` steps. A label is
+    accepted only if the plain stepper, started from it, is below ``tol`` on
+    its own residual. Returns (targets per cluster, accepted flags, residuals,
+    contraction ratios, stepper steps spent)."""
+    coff = bs.offsets[0]
+    gap = max(horizon // 4, window)
+    bs.init("init")
+    snaps = _run_snapshots(bs, horizon - 2 * gap, gap)
+    steps = horizon
+    cx, vx, qs = _extrapolate(snaps, coff)
+    Y = [_targets_from(st, cx[:, coff[i]:coff[i + 1]], vx[coff[i]:coff[i + 1]]) for i, st in enumerate(bs.states)]
+    for rnd in range(1, rounds):
+        bs.init("init")
+        usable = np.array([inject(st, y) for st, y in zip(bs.states, Y)])
+        bs.rebuild()
+        snaps = _run_snapshots(bs, 0, gap)
+        steps += 2 * gap
+        cx, vx, qs2 = _extrapolate(snaps, coff)
+        for i, st in enumerate(bs.states):
+            if usable[i]:
+                Y[i] = _targets_from(st, cx[:, coff[i]:coff[i + 1]], vx[coff[i]:coff[i + 1]])
+                qs[i] = qs2[i]
+        if log is not None:
+            log(f"  extrapolation round {rnd + 1}: {int(usable.sum())}/{len(usable)} restarted from the limit")
+    # verification by the plain stepper itself
+    bs.init("init")
+    usable = np.array([inject(st, y) for st, y in zip(bs.states, Y)])
+    bs.rebuild()
+    bs.residual_window(window)                       # let the injected state settle one window
+    r = bs.residual_window(window)
+    steps += 2 * window
+    ok = usable & (r < tol)
+    return Y, ok, r, qs, steps
+
+
 def make_dataset(n_clusters: int, seed: int, *, cells_range=(500, 2000), grid_range=(48, 88),
-                 decades: float = 2.0, batch: int = 256, tol: float = 1e-6,
-                 window: int = 100, max_steps: int = 200_000, flux_mode: str = "fp64", device=None, log=None):
-    """Init-phase states of ``n_clusters`` random clusters, produced by the plain
-    method on the GPU in batches: every cluster runs the full ``max_steps``
-    and its final state is the label (the furthest along the relaxation the
-    init phase gets); ``ok`` records whether its residual fell below ``tol``
-    within the horizon. Returns per-cluster (features, neighbour CSR,
-    targets, ok, steps) and run statistics."""
+                 decades: float = 2.0, batch: int = 256, tol: float = 1e-6, window: int = 100,
+                 horizon: int = 20_000, rounds: int = 2, flux_mode: str = "fp64", device=None, log=None):
+    """Init-phase steady states of ``n_clusters`` random clusters
+    (``steady_state_labels``), produced on the GPU in batches. Returns
+    per-cluster (features, neighbour CSR, targets, ok) and run statistics."""
     base = Parameters.preset("small")
     data = []
     stats = []
@@ -258,17 +341,16 @@ def make_dataset(n_clusters: int, seed: int, *, cells_range=(500, 2000), grid_ra
                                 base="small", decade=decades)
         t_w = time.perf_counter() - t0
         bs = BatchSimulator(worlds, device=device, flux_mode=flux_mode)
-        bs.init("init")
-        res = run_to_residual(bs, tol, window, max_steps, log, run_all=True)
-        bs.pull()
-        for i, (st, (p, cells)) in enumerate(zip(bs.states, worlds)):
+        t1 = time.perf_counter()
+        Y, ok, r, qs, steps = steady_state_labels(bs, horizon, rounds, tol, window, log)
+        t_l = time.perf_counter() - t1
+        for i, (p, cells) in enumerate(worlds):
             data.append(dict(X=cell_features(p, cells, base), ptr=cells.cell_nn_ptr.copy(),
-                             idx=cells.cell_nn_idx.copy(), Y=state_targets(st, st.names),
-                             ok=bool(res.steps[i] >= 0), steps=int(res.steps[i]), names=st.names,
-                             residual=float(res.residual[i])))
+                             idx=cells.cell_nn_idx.copy(), Y=Y[i], ok=bool(ok[i]), residual=float(r[i]),
+                             q=float(qs[i]), names=bs.states[i].names))
         stats.append(dict(clusters=nb, cells=int(sum(c.n_cells for _, c in worlds)), world_seconds=t_w,
-                          init_seconds=res.seconds, converged=int((res.steps >= 0).sum()),
-                          median_steps=float(np.median(np.where(res.steps >= 0, res.steps, max_steps)))))
+                          label_seconds=t_l, stepper_steps=int(steps), accepted=int(ok.sum()),
+                          median_residual=float(np.median(r))))
         if log is not None:
             log(f"dataset batch {b0 // batch}: {stats[-1]}")
         del bs
@@ -276,14 +358,13 @@ def make_dataset(n_clusters: int, seed: int, *, cells_range=(500, 2000), grid_ra
 
 
 def train(data, *, hidden: int = 64, n_layers: int = 6, epochs: int = 30, clusters_per_batch: int = 32,
-          lr: float = 2e-3, seed: int = 0, device=None, base: Optional[Parameters] = None, log=None,
-          converged_only: bool = False):
-    """Train ``WarmStartGNN`` on the clusters of ``data`` (GPU kernels only:
-    BF16 MFMA products, FP32 accumulation, Adam)."""
+          lr: float = 2e-3, seed: int = 0, device=None, base: Optional[Parameters] = None, log=None):
+    """Train ``WarmStartGNN`` on the accepted labels of ``data`` (GPU kernels
+    only: BF16 MFMA products, FP32 accumulation, Adam)."""
     from .gnn import WarmStartGNN
-    good = [d for d in data if d["ok"] or not converged_only]
+    good = [d for d in data if d["ok"]]
     if not good:
-        raise ValueError("no cluster to train on")
+        raise ValueError("no accepted label to train on")
     X = np.concatenate([d["X"] for d in good])
     Y = np.concatenate([d["Y"] for d in good])
     x_mean, x_std = X.mean(axis=0), X.std(axis=0) + 1e-6

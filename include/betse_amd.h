@@ -31,7 +31,8 @@ extern "C" {
 
 #define BT_MAX_IONS  8
 #define BT_MAX_CHAN  4
-#define BT_TILE_MEMBRANES 64   /* membranes per cell tile (whole cells, one wavefront per ion) */
+#define BT_TILE_MEMBRANES 64   /* at most this many membranes per cell tile (whole cells, one wavefront) */
+#define BT_TILE_CELLS     32   /* at most this many cells per cell tile */
 
 /* Voltage-gated channel models (betse/science/channels/vg_na.py, vg_k.py, vg_ca.py). */
 #define BT_CHAN_NAV1P2 0
@@ -101,7 +102,8 @@ typedef struct {
     const int32_t* sq_mems;     /* [n_mems] membranes sorted by square            */
     const int32_t* mem_pos;     /* [n_mems] position of each membrane in sq_mems  */
     const int32_t* tile_ptr;    /* [n_tiles+1] first cell of each tile of whole cells
-                                   of one cluster holding <= BT_TILE_MEMBRANES membranes */
+                                   of one cluster holding <= BT_TILE_MEMBRANES membranes
+                                   and <= BT_TILE_CELLS cells */
     const int32_t* tile_cluster;/* [n_tiles] cluster of each tile                 */
     const int32_t* sq_cluster;  /* [n_squares] cluster of each square             */
     const double* mem_sa;       /* [n_mems]                                       */
