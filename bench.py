@@ -273,19 +273,24 @@ def measured_traffic(workload: str, kernel: str, flux_mode: str):
 
 
 def time_sampled_loop(p, cells, flux_mode, steps):
-    """The init phase through run_sim_core: sampled steps run bt_step_observed,
-    derive the get_current observables (HH decomposition by DST on FP64 MFMA)
-    and go through write2storage on the host."""
+    """The init phase's step loop (_run_sim_core_loop) with the preset's
+    sampling: sampled steps run bt_step_observed, derive the get_current
+    observables (HH decomposition by DST on FP64 MFMA) and go through
+    write2storage on the host. Set-up (run_sim_core's init_dynamics, upload,
+    first update_V, observables workspace) is outside the timed loop."""
     import torch
-    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator, phase_time_steps
     q = p.copy()
     q.init_time_total = steps * q.init_time_step
     sim = Simulator(q, device="cuda:0", flux_mode=flux_mode, use_graph=True)
     ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
     sim.init_core(ph)
+    sim.prepare(ph)
+    sim.step_observed(1)                 # allocates the observables workspace
+    time_steps, sampled = phase_time_steps(q.init_tsteps, q.dt, q.t_resample)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sim.run_sim_core(ph)
+    sim._run_sim_core_loop(ph, time_steps, sampled)
     torch.cuda.synchronize()
     return time.perf_counter() - t0, len(sim.time)
 

@@ -294,9 +294,9 @@ def steady_state_labels(bs: BatchSimulator, horizon: int = 20_000, rounds: int =
     accepted only if the plain stepper, started from it, is below ``tol`` on
     its own residual. Returns (targets per cluster, accepted flags, residuals,
     contraction ratios, stepper steps spent)."""
-    coff = bs.offsets[0]
     gap = max(horizon // 4, window)
     bs.init("init")
+    coff = bs.offsets[0]
     snaps = _run_snapshots(bs, horizon - 2 * gap, gap)
     steps = horizon
     cx, vx, qs = _extrapolate(snaps, coff)
