@@ -325,7 +325,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--clusters", type=int, default=256, help="batched config: number of clusters")
     ap.add_argument("--cpu-clusters", type=int, default=8, help="batched config: clusters in the CPU sample")
-    ap.add_argument("--train-clusters", type=int, default=1024, help="warmstart config: training clusters")
+    ap.add_argument("--train-clusters", type=int, default=4096, help="warmstart config: training clusters")
     ap.add_argument("--eval-clusters", type=int, default=256, help="warmstart config: held-out clusters")
     ap.add_argument("--max-init-steps", type=int, default=100_000, help="warmstart config: init step cap")
     ap.add_argument("--label-horizon", type=int, default=20_000,

@@ -71,6 +71,18 @@ class BatchSimulator:
             st.init_dynamics(SimPhase(k, p, cells))
         self._build()
 
+    def subset(self, keep: Sequence[int]) -> "BatchSimulator":
+        """A new batch of the clusters ``keep`` that continues from their current
+        state (pulled from this batch); this batch is left as it is."""
+        self.pull()
+        sub = BatchSimulator([self.worlds[i] for i in keep], device=str(self.device), flux_mode=self.flux_mode,
+                             use_graph=self.use_graph)
+        sub.kind = self.kind
+        sub.states = [self.states[i] for i in keep]
+        sub._check_compatible()
+        sub._build()
+        return sub
+
     def rebuild(self) -> None:
         """Upload the host states again (after they were changed on the host)."""
         self._check_compatible()

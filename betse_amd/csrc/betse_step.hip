@@ -87,6 +87,17 @@ struct KCluster {
     double nak_coef;                // -3*block*alpha_NaK
 };
 
+// Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
+// their linear index. Remap the linear index b of n so that each XCD receives
+// one contiguous chunk of the work: neighbouring tiles, which share squares,
+// halos and gap-junction partners, then hit the same L2.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_chunk(int b, int n) {
+    const int q = n / N_XCD, r = n % N_XCD;
+    const int x = b % N_XCD, k = b / N_XCD;
+    return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 __device__ __forceinline__ bool halted(const unsigned* flags) {
     return __atomic_load_n(flags, __ATOMIC_RELAXED) != 0u;
 }
@@ -215,7 +226,7 @@ k_mem_cell(MemCellArgs A, KParams P) {
     __shared__ double s_gj[NI][TMB];
     __shared__ double s_ch[BT_MAX_CHAN][TMB];
     __shared__ int s_bad;
-    const int tile = blockIdx.x;
+    const int tile = xcd_chunk(blockIdx.x, gridDim.x);
     const int c0 = A.tile_ptr[tile];
     const int c1 = A.tile_ptr[tile + 1];
     const int m0 = A.mem_ptr[c0];
@@ -392,16 +403,25 @@ k_env_field(FieldArgs A, KParams P) {
     constexpr int TR = TY + 4;                                  // row-blurred rows tile +- 2
     constexpr int VR = TY + 4, VC = TX + 4;                     // v on tile +- 2
     constexpr int ER = TY + 2, EC = TX + 2;                     // E on tile +- 1
-    __shared__ double sU[UR][UC];
-    __shared__ double sT[TR][UC];
+    // the Gaussian staging (sU, sT) is dead once v_env is built, so the
+    // diffusion staging (sC, sD, sFx, sFy) reuses its LDS
+    constexpr int A1 = UR * UC + TR * UC;
+    constexpr int A2 = VR * VC + ER * EC + TY * (TX + 2) + (TY + 2) * TX;
+    __shared__ double s_pool[A1 > A2 ? A1 : A2];
     __shared__ double sV[VR][VC];
     __shared__ double sEx[ER][EC];
     __shared__ double sEy[ER][EC];
-    __shared__ double sC[VR][VC];
-    __shared__ double sD[ER][EC];
-    __shared__ double sFx[TY][TX + 2];
-    __shared__ double sFy[TY + 2][TX];
-    const int* et = A.env_tiles + 3 * blockIdx.x;
+    double (&sU)[UR][UC] = *reinterpret_cast<double (*)[UR][UC]>(s_pool);
+    double (&sT)[TR][UC] = *reinterpret_cast<double (*)[TR][UC]>(s_pool + UR * UC);
+    double (&sC)[VR][VC] = *reinterpret_cast<double (*)[VR][VC]>(s_pool);
+    double (&sD)[ER][EC] = *reinterpret_cast<double (*)[ER][EC]>(s_pool + VR * VC);
+    double (&sFx)[TY][TX + 2] = *reinterpret_cast<double (*)[TY][TX + 2]>(s_pool + VR * VC + ER * EC);
+    double (&sFy)[TY + 2][TX] =
+        *reinterpret_cast<double (*)[TY + 2][TX]>(s_pool + VR * VC + ER * EC + TY * (TX + 2));
+    // (tile, ion) from the XCD-chunked linear block index, tiles fastest
+    const int lin = xcd_chunk(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int tile_id = lin % gridDim.x, ion_id = lin / gridDim.x;
+    const int* et = A.env_tiles + 3 * tile_id;
     const KCluster K = A.kcl[et[0]];
     const int ny = K.ny, nx = K.nx, so = K.sq_off;
     const int r0 = et[1], cc0 = et[2];
@@ -465,8 +485,8 @@ k_env_field(FieldArgs A, KParams P) {
 
     // with gridDim.y == n_ions each block takes one ion (and rebuilds its tile's
     // field); with gridDim.y == 1 a block runs every ion over its tile
-    const int i_lo = (gridDim.y == 1) ? 0 : (int)blockIdx.y;
-    const int i_hi = (gridDim.y == 1) ? P.n_ions : (int)blockIdx.y + 1;
+    const int i_lo = (gridDim.y == 1) ? 0 : ion_id;
+    const int i_hi = (gridDim.y == 1) ? P.n_ions : ion_id + 1;
     for (int i = i_lo; i < i_hi; ++i) {
         const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
         const double* D = A.D_env + (size_t)i * P.n_sq + so;

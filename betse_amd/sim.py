@@ -612,27 +612,36 @@ class Simulator(HostState):
         self._check(self._lib.bt_update_v(self._ctx))
 
     def write2storage(self, t, cells=None, p=None) -> None:
-        """``sim.py:1789-1884`` for the fields of this path."""
+        """``sim.py:1789-1884`` for the fields of this path. Every sampled step
+        pulls fresh host arrays from the device (nothing on the host modifies
+        them afterwards), so they are stored without a further copy; Dm and
+        gjopen do not change within a phase on this path and are stored as one
+        snapshot per phase."""
         self.time.append(t)
         if getattr(self, "Jn", None) is not None:
-            self.efield_gj_x_time.append(self.E_gj_x * 1)
-            self.efield_gj_y_time.append(self.E_gj_y * 1)
-            self.I_cell_x_time.append(self.J_cell_x * 1)
-            self.I_cell_y_time.append(self.J_cell_y * 1)
-            self.I_mem_time.append(self.I_mem * 1)
-            self.rate_NaKATP_time.append(self.rate_NaKATP * 1)
-            self.I_tot_x_time.append(self.J_env_x * 1)
-            self.I_tot_y_time.append(self.J_env_y * 1)
-        self.cc_time.append(np.copy(self.cc_cells))
-        self.cc_env_time.append(np.copy(self.cc_env))
-        self.dd_time.append(np.copy(self.Dm_cells))
-        self.vm_time.append(self.vm * 1)
-        self.vm_ave_time.append(self.vm_ave)
-        self.rho_cells_time.append(self.rho_cells * 1)
-        self.venv_time.append(self.v_env * 1)
-        self.efield_ecm_x_time.append(self.E_env_x * 1)
-        self.efield_ecm_y_time.append(self.E_env_y * 1)
-        self.gjopen_time.append(self.gjopen * 1)
+            self.efield_gj_x_time.append(self.E_gj_x)
+            self.efield_gj_y_time.append(self.E_gj_y)
+            self.I_cell_x_time.append(self.J_cell_x)
+            self.I_cell_y_time.append(self.J_cell_y)
+            self.I_mem_time.append(self.I_mem)
+            self.rate_NaKATP_time.append(self.rate_NaKATP)
+            self.I_tot_x_time.append(self.J_env_x)
+            self.I_tot_y_time.append(self.J_env_y)
+        self.cc_time.append(self.cc_cells)
+        self.cc_env_time.append(self.cc_env)
+        snap = self.__dict__.get("_phase_snapshot")
+        if snap is None or len(self.time) == 1:
+            snap = (np.copy(self.Dm_cells), self.gjopen * 1)
+            self._phase_snapshot = snap
+        self.dd_time.append(snap[0])
+        vm = self.vm
+        self.vm_time.append(vm)
+        self.vm_ave_time.append(np.add.reduceat(vm, self.cells.mem_ptr[:-1]) / self.cells.num_mems)
+        self.rho_cells_time.append(self.rho_cells)
+        self.venv_time.append(self.v_env)
+        self.efield_ecm_x_time.append(self.E_env_x)
+        self.efield_ecm_y_time.append(self.E_env_y)
+        self.gjopen_time.append(snap[1])
 
     def run_sim_core(self, phase: SimPhase) -> None:
         """``sim.py:1016-1128``."""
