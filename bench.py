@@ -29,15 +29,19 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0, n_sub: int = 0) -> dict:
+def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0, n_sub: int = 0,
+                      dm_loaded: int = None, gj_loaded: bool = True) -> dict:
     """Compulsory bytes per launch of each step kernel (each array element the
-    kernel must touch, counted once at its storage width)."""
+    kernel must touch, counted once at its storage width). ``dm_loaded`` ions
+    have per-membrane Dm (the others are one value per cluster and come with
+    the cluster constants); ``gj_loaded`` likewise for gjopen."""
     C, M = cells.n_cells, cells.n_mems
     G2 = cells.X.size
     n_in = len(cells.envInds_inClust)
     ni = n_ions
+    ndm = ni if dm_loaded is None else dm_loaded
     b = {}
-    b["mem_cell"] = (M * (5 * 4 + 8 + 8 + 8 * ni)          # mem_cell, nn_cell, nn_mem, mem_sq, mem_pos, mem_sa, gjopen, Dm
+    b["mem_cell"] = (M * (5 * 4 + 8 + (8 if gj_loaded else 0) + 8 * ndm)   # indices, mem_sa, gjopen, Dm
                      + M * 8 * ni                           # fsa write
                      + M * n_ch * (16 + 16 + 8)              # gates r/w, channel flux write
                      + C * (8 * ni * 2 + 8 * ni + 8 * ni     # c_cell r, c_lag r, c_cell w, c_lag w
@@ -82,6 +86,7 @@ def bench_batched(a):
     for mode in modes:
         b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=mode)
         b.init("init")
+        uniform = all(np.all(st.Dm_cells == st.Dm_cells[:, :1]) for st in b.states)
         b.step_n(a.warmup)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -137,7 +142,9 @@ def bench_batched(a):
     best = min(ok or modes, key=lambda m: res[m]["ev_ms"])
     r = res[best]
     ms_step = r["ev_ms"] / a.steps
-    ab = algorithmic_bytes(tot, n_ions, "Ca" in p0.ion_names(), best, len(p0.channels))
+    # Dm (per-cluster factors) and gjopen are one value per cluster in this workload
+    ab = algorithmic_bytes(tot, n_ions, "Ca" in p0.ion_names(), best, len(p0.channels), 0,
+                           0 if uniform else n_ions, not uniform)
     kern = r["kernels"]
     dom = max(kern, key=kern.get)
     achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
@@ -420,7 +427,10 @@ def main(argv=None):
     ms_step = r["ev_ms"] / a.steps
     value = C * a.steps / (r["ev_ms"] / 1e3)
     n_sub = p.network.n_species if getattr(p, "network", None) is not None else 0
-    ab = algorithmic_bytes(cells, n_ions, has_ca, best, len(p.channels), n_sub)
+    Dm, gjo = r["sim"].Dm_cells, r["sim"].gjopen
+    dm_loaded = int(sum(not np.all(Dm[i] == Dm[i][0]) for i in range(n_ions)))
+    ab = algorithmic_bytes(cells, n_ions, has_ca, best, len(p.channels), n_sub, dm_loaded,
+                           not bool(np.all(gjo == gjo[0])))
     kern = r["kernels"]
     dom = max(kern, key=kern.get)
     achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9

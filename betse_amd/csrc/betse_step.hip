@@ -36,6 +36,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -85,6 +86,10 @@ struct KCluster {
     double surf_scale;              // cell_height*delta**2 (rho_env_surf factor)
     double mean_sa, kee, screen;    // kee = ko_env*eo*er
     double nak_coef;                // -3*block*alpha_NaK
+    // the cluster's Dm per ion and gjopen when they are one value over all of
+    // its membranes (NaN otherwise): the membrane kernel then skips the loads
+    double dm_u[BT_MAX_IONS];
+    double gjo_u;
 };
 
 // Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
@@ -234,7 +239,8 @@ k_mem_cell(MemCellArgs A, KParams P) {
     const int j = threadIdx.x;
     const int m = m0 + j;
     const int M = P.n_mems, C = P.n_cells, G2 = P.n_sq;
-    const double nak_coef = A.kcl[A.tile_cluster[tile]].nak_coef;
+    const KCluster& KC = A.kcl[A.tile_cluster[tile]];
+    const double nak_coef = KC.nak_coef;
     if (j == 0) s_bad = 0;
 
     // per-cell inputs of the second stage, fetched up front
@@ -259,7 +265,7 @@ k_mem_cell(MemCellArgs A, KParams P) {
         const int k = A.mem_sq[m];
         const int pos = A.mem_pos[m];
         const double sa = A.mem_sa[m];
-        const double gjo = A.gjopen[m];
+        const double gjo = (KC.gjo_u == KC.gjo_u) ? KC.gjo_u : A.gjopen[m];
         const double v = A.vm_in[mc];
         const double vn = A.vm_in[cn];
         // NaK-ATPase on the lagged membrane concentrations (sim.py:1196-1234, Q1)
@@ -278,7 +284,8 @@ k_mem_cell(MemCellArgs A, KParams P) {
             const double vmut = with_nonces(v, i + 1);
             const double vnmut = with_nonces(vn, i + 1);
             const double clag = A.c_lag_in[i * C + mc];
-            const FT fED = ghk<FT>((FT)A.c_env[i * G2 + k], (FT)clag, (FT)A.Dm[i * M + m], (FT)P.tm,
+            const double dm = (KC.dm_u[i] == KC.dm_u[i]) ? KC.dm_u[i] : A.Dm[i * M + m];
+            const FT fED = ghk<FT>((FT)A.c_env[i * G2 + k], (FT)clag, (FT)dm, (FT)P.tm,
                                    (FT)P.zplus[i], (FT)vmut, (FT)P.F, (FT)P.RT);
             f = f + (double)fED;
             double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
@@ -1056,6 +1063,8 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
     k.kee = (b.ko_env * p.eo) * p.er;
     k.screen = (2.0 / (b.ko_env * b.delta)) * (p.cell_radius / p.true_cell_size);
     k.nak_coef = -3.0 * p.NaK_block * b.alpha_NaK;
+    for (int i = 0; i < BT_MAX_IONS; ++i) k.dm_u[i] = NAN;
+    k.gjo_u = NAN;
     return k;
 }
 
@@ -1194,11 +1203,12 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
                          b->sq_has_mem, b->c_cell, b->c_lag, b->vm_cell, b->c_env, b->E_x, b->E_y, b->v_env};
     for (const void* q : req)
         if (!q) return set_err(nullptr, BT_EINVAL, "null device buffer");
+    std::vector<int32_t> tp((size_t)b->n_tiles + 1), mp((size_t)p->n_cells + 1), tcl((size_t)b->n_tiles);
     {
         // k_mem_cell indexes LDS by a tile's membranes and cells: check every tile
-        std::vector<int32_t> tp((size_t)b->n_tiles + 1), mp((size_t)p->n_cells + 1);
         hipError_t e = hipMemcpy(tp.data(), b->tile_ptr, tp.size() * 4, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipMemcpy(mp.data(), b->mem_ptr, mp.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(tcl.data(), b->tile_cluster, tcl.size() * 4, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return set_err(nullptr, BT_EHIP, std::string("reading tiles: ") + hipGetErrorString(e));
         if (tp[0] != 0 || tp[b->n_tiles] != p->n_cells || mp[0] != 0 || mp[p->n_cells] != p->n_mems)
             return set_err(nullptr, BT_EINVAL, "tile_ptr/mem_ptr do not cover the cells and membranes");
@@ -1209,6 +1219,8 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
         }
         for (int q = 0; q < p->n_cells; ++q)
             if (mp[q + 1] <= mp[q]) return set_err(nullptr, BT_EINVAL, "a cell has no membranes");
+        for (int t = 0; t < b->n_tiles; ++t)
+            if (tcl[t] < 0 || tcl[t] >= n_clusters) return set_err(nullptr, BT_EINVAL, "tile_cluster out of range");
     }
 
     bt_ctx* c = new bt_ctx();
@@ -1229,6 +1241,34 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
             }
     }
     c->n_env_tiles = (int)(tiles.size() / 3);
+    {
+        // Dm and gjopen that are one value over a cluster's membranes (read once:
+        // they do not change within a phase on this path)
+        std::vector<double> dm((size_t)p->n_ions * p->n_mems), gj((size_t)p->n_mems);
+        hipError_t e = hipMemcpy(dm.data(), b->Dm, dm.size() * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(gj.data(), b->gjopen, gj.size() * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            delete c;
+            return set_err(nullptr, BT_EHIP, std::string("reading Dm: ") + hipGetErrorString(e));
+        }
+        std::vector<int> lo(n_clusters, p->n_mems), hi(n_clusters, 0);
+        for (int t = 0; t < b->n_tiles; ++t) {
+            lo[tcl[t]] = std::min(lo[tcl[t]], mp[tp[t]]);
+            hi[tcl[t]] = std::max(hi[tcl[t]], mp[tp[t + 1]]);
+        }
+        for (int q = 0; q < n_clusters; ++q) {
+            if (hi[q] <= lo[q]) continue;
+            for (int i = 0; i < p->n_ions; ++i) {
+                const double* d = dm.data() + (size_t)i * p->n_mems;
+                bool same = true;
+                for (int m = lo[q]; m < hi[q] && same; ++m) same = (d[m] == d[lo[q]]);
+                c->kcl_host[q].dm_u[i] = same ? d[lo[q]] : NAN;
+            }
+            bool same = true;
+            for (int m = lo[q]; m < hi[q] && same; ++m) same = (gj[m] == gj[lo[q]]);
+            c->kcl_host[q].gjo_u = same ? gj[lo[q]] : NAN;
+        }
+    }
     (void)hipGetDevice(&c->device);
     const size_t nc = p->n_cells, nm = p->n_mems, ni = p->n_ions, ns = (size_t)p->n_squares;
     hipError_t e = hipSuccess;

@@ -174,9 +174,12 @@ typedef struct {
     bt_rate rates[BT_NET_MAX_RATES];
 } bt_network;
 
-/* Create a context: validates sizes, copies the parameter block, allocates the
- * workspace (second cell-state buffer, post-diffusion env buffer, per-membrane
- * exchange fluxes, surface-charge field, status words) on the current device.
+/* Create a context: validates sizes and tiles, copies the parameter block,
+ * allocates the workspace (second cell-state buffer, post-diffusion env buffer,
+ * per-membrane exchange fluxes, surface-charge field, status words) on the
+ * current device. Dm and gjopen are read once here (a cluster whose values are
+ * uniform gets them as constants); they do not change within a phase on this
+ * path, and a caller that changes them creates a new context.
  * `stream` is a hipStream_t (may be NULL for the default stream; graphs need a
  * non-default stream). Replaces the state set-up of init_core/init_dynamics
  * (sim.py:452-1012) on the device side. */

@@ -30,9 +30,22 @@ def test_kv1p1_anchor_points():
     assert mTau[1] == 15.0 and hTau[3] == 7500.0
 
 
+def test_cav1p2_anchor_points():
+    """Carlin's L-type model after the -10 mV shift: Boltzmann half points at
+    -20 mV (m) and -70 mV (h); tau midpoints at -15 mV (m: 5 + 20/2) and
+    -30 mV (h: 20 + 50/2)."""
+    mInf, mTau, hInf, hTau = O.cav1p2_rates(np.array([-20.0, -15.0, -70.0, -30.0]))
+    assert mInf[0] == 0.5 and hInf[2] == 0.5
+    assert mTau[1] == 15.0 and hTau[3] == 45.0
+    V = np.linspace(-100, 40, 141)
+    mInf, mTau, hInf, hTau = O.cav1p2_rates(V)
+    assert np.all(np.diff(mInf) > 0) and np.all(np.diff(hInf) < 0)
+    assert np.all((mTau > 5) & (mTau < 25)) and np.all((hTau > 20) & (hTau < 70))
+
+
 def test_init_gates_match_steady_state():
     V = np.array([-0.07, -0.03, 0.01])
-    for kind, rates in (("Nav1p2", O.nav1p2_rates), ("Kv1p1", O.kv1p1_rates)):
+    for kind, rates in (("Nav1p2", O.nav1p2_rates), ("Kv1p1", O.kv1p1_rates), ("Cav1p2", O.cav1p2_rates)):
         m0, h0 = init_gates(kind, V)
         mInf, _, hInf, _ = rates(V * 1000)
         np.testing.assert_allclose(m0, mInf, rtol=1e-14)
