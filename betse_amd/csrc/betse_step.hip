@@ -37,6 +37,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -110,13 +111,18 @@ __device__ __forceinline__ bool halted(const unsigned* flags) {
 // ---------------------------------------------------------------------------
 // flux laws
 // ---------------------------------------------------------------------------
+// IEEE division in both paths (a reciprocal-based FP32 quotient was measured:
+// no gain, larger FP32-path error)
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a / b; }
+
 // electroflux (sim_toolbox.py:18-69); vBA already carries its +1e-25.
 template <typename FT>
 __device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT d, FT zc, FT vBA, FT F, FT RT) {
-    FT alpha = (zc * vBA * F) / RT;
+    FT alpha = dv(zc * vBA * F, RT);
     FT exp_alpha = exp(-alpha);
     FT deno = -expm1(-alpha);
-    return -((Dc * alpha) / d) * ((cB - cA * exp_alpha) / deno);
+    return -dv(Dc * alpha, d) * dv(cB - cA * exp_alpha, deno);
 }
 
 // x**3: two multiplications (NumPy calls pow; the results differ by at most an
@@ -135,15 +141,15 @@ __device__ __forceinline__ FT pump_nak(FT cNai, FT cNao, FT cKi, FT cKo, FT Vm, 
     FT d = cKo * (FT)1e-3;
     FT Qdenomo = (FT)P.qd_scal * cube(c) * (d * d);
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-15;
-    FT Q = Qnumo / Qdenomo;
-    FT Keq = exp(-((FT)P.dG_RT - (((FT)P.F * Vm) / (FT)P.RT)));
-    FT na = cube(cNai / (FT)P.km_na);
-    FT kk = cKo / (FT)P.km_k;
+    FT Q = dv(Qnumo, Qdenomo);
+    FT Keq = exp(-((FT)P.dG_RT - dv((FT)P.F * Vm, (FT)P.RT)));
+    FT na = cube(dv(cNai, (FT)P.km_na));
+    FT kk = dv(cKo, (FT)P.km_k);
     FT k2 = kk * kk;
     FT numo_E = na * k2 * (FT)P.atp_km;
     FT denomo_E = ((FT)1 + na) * ((FT)1 + k2) * (FT)P.one_p_atp_km;
-    FT fwd_co = numo_E / denomo_E;
-    return (FT)nak_coef * fwd_co * ((FT)1 - (Q / Keq));
+    FT fwd_co = dv(numo_E, denomo_E);
+    return (FT)nak_coef * fwd_co * ((FT)1 - dv(Q, Keq));
 }
 
 // pumpCaATP (sim_toolbox.py:124-182) after no_negs on both concentrations.
@@ -152,12 +158,12 @@ __device__ __forceinline__ FT pump_ca(FT cCai, FT cCao, FT Vm, const KParams& P)
     FT Qnumo = (FT)P.qca_num * cCao;
     FT Qdenomo = (FT)P.cATP * cCai;
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-16;
-    FT Q = Qnumo / Qdenomo;
-    FT Keq = exp(-((FT)P.dG_RT - (FT)2 * (((FT)P.F * Vm) / (FT)P.RT)));
-    FT numo_E = (cCai / (FT)P.km_ca) * (FT)P.atp_kmca;
-    FT denomo_E = ((FT)1 + (cCai / (FT)P.km_ca)) * (FT)P.one_p_atp_kmca;
-    FT frwd = numo_E / denomo_E;
-    return (FT)P.ca_coef * frwd * ((FT)1 - (Q / Keq));
+    FT Q = dv(Qnumo, Qdenomo);
+    FT Keq = exp(-((FT)P.dG_RT - (FT)2 * dv((FT)P.F * Vm, (FT)P.RT)));
+    FT numo_E = dv(cCai, (FT)P.km_ca) * (FT)P.atp_kmca;
+    FT denomo_E = ((FT)1 + dv(cCai, (FT)P.km_ca)) * (FT)P.one_p_atp_kmca;
+    FT frwd = dv(numo_E, denomo_E);
+    return (FT)P.ca_coef * frwd * ((FT)1 - dv(Q, Keq));
 }
 
 // Nav1p2._calculate_state (channels/vg_na.py:169-186); V in mV.
@@ -315,8 +321,9 @@ k_mem_cell(MemCellArgs A, KParams P) {
         // voltage-gated channels (networks.py:3115-3215): gates from Vmem carrying
         // the nonces so far, update_mh, open probability, GHK with the current cell
         // and post-diffusion env concentrations. Gating stays FP64 in both flux
-        // modes: the Hodgkin-Huxley rates have removable 0/0 points (V = -25 mV,
-        // -40 mV for Nav1p2) that FP32 rounding would hit.
+        // modes: the gates carry state from step to step, and FP32 rates (even in
+        // cancellation-free form) bias them enough to move Vmem by ~1e-7 V within
+        // a few hundred steps (measured), against the 1 uV bar.
         for (int ch = 0; ch < P.n_chan; ++ch) {
             const int i = P.chan_ion[ch];
             double* gm = A.gates + (size_t)(2 * ch) * M;

@@ -225,6 +225,39 @@ def test_channels_fp64_parity():
         np.testing.assert_allclose(sim.chan_gates[k, 1], ch.h, rtol=1e-10, atol=1e-14)
 
 
+def test_channels_fp32_flux_parity():
+    """The mixed-precision path with Nav1p2 + Kv1p1 (FP32 fluxes and FP32
+    gating in cancellation-free form, FP64 state): |dVm| < 1 uV and gates
+    within 1e-5 of the FP64 oracle after init 150 + sim 100 steps."""
+    from betse_amd.channels import draw_channel_state
+    p = Parameters.preset("tiny_channels")
+    cells = Cells.make_world(p)
+    draw_channel_state(p, cells.n_cells, cells.n_mems)
+    say("case: channels fp32 flux init 150 + sim 100")
+    o = oracle_run(p, cells, 150, "init")
+    o.run_sim_core("sim", n_steps=100)
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    sim = Simulator(p.copy(), device="cuda:0", flux_mode="fp32")
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    sim.prepare(ph)
+    sim.step_n(150)
+    sim.pull()
+    ph2 = SimPhase(SimPhaseKind.SIM, sim.p, cells, sim)
+    sim.prepare(ph2)
+    sim.step_n(100)
+    sim.pull()
+    dv, dc, de, lag, eex = errors(sim, o)
+    say(f"channels fp32: |dVm|={dv:.3e} V rel dc={dc:.3e} rel denv={de:.3e}")
+    assert dv < 1e-6
+    assert dc < 1e-6 and de < 1e-6
+    for k, ch in enumerate(o.channels):
+        gm = np.abs(sim.chan_gates[k, 0] - ch.m).max()
+        gh = np.abs(sim.chan_gates[k, 1] - ch.h).max()
+        say(f"  gate {ch.kind}: max |dm| {gm:.2e} |dh| {gh:.2e}")
+        assert gm < 1e-5 and gh < 1e-5
+
+
 def test_full_physics_fp64_parity():
     """Nav1p2 + Kv1p1 + Cav1p2 and the 8-substance reaction network on the test
     cluster, init then sim, against the oracle (fp64 bars, substances 1e-10)."""
