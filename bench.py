@@ -219,6 +219,65 @@ def bench_warmstart(a):
     print(json.dumps(line), flush=True)
 
 
+def bench_full_pipeline(a):
+    """The full workload as a pipeline: a warm-start network trained for its
+    physics (random clusters of 500-2000 cells with the same ions, channels
+    and multiplier statistics), the init phase of the 100k-cell cluster to the
+    1e-6 residual from the default start and from the prediction, then the
+    sim phase (channels, reaction network, mixed precision)."""
+    from betse_amd import warmstart as ws
+    from betse_amd.batch import BatchSimulator
+
+    def log(msg):
+        print("# " + msg, file=sys.stderr, flush=True)
+
+    p, cells, t_world = make_problem("full")
+    log(f"world: {cells.n_cells} cells, {cells.n_mems} membranes, built in {t_world:.1f}s")
+    t0 = time.perf_counter()
+    data, stats, _ = ws.make_dataset(a.train_clusters, seed=3000, base="full", cells_range=(500, 2000),
+                                     grid_range=(36, 72), decades=0.0, batch=a.clusters, horizon=a.label_horizon,
+                                     flux_mode="fp32", log=log)
+    t_data = time.perf_counter() - t0
+    # features are relative to the workload's own parameters (factors 0)
+    starter, hist, t_train = ws.train(data, epochs=a.epochs, base=p, log=log)
+    bs = BatchSimulator([(p, cells)], device="cuda:0", flux_mode="fp32")
+    t0 = time.perf_counter()
+    bs.init("init")
+    t_setup = time.perf_counter() - t0
+    cold = ws.run_to_residual(bs, 1e-6, 100, a.max_init_steps, log)
+    warm, t_pred = ws.warm_start_batch(bs, starter, 1e-6, 100, a.max_init_steps, log)
+    log(f"init: cold {cold.seconds + t_setup:.2f}s ({int(cold.steps[0])} steps), "
+        f"warm {warm.seconds + t_pred:.2f}s ({int(warm.steps[0])} steps, injected {bool(warm.warm[0])})")
+    # sim phase from the warm-started init state
+    bs.prepare("sim")
+    import torch
+    bs.step_n(a.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.step_n(a.sim_steps)
+    torch.cuda.synchronize()
+    t_sim = time.perf_counter() - t0
+    C = cells.n_cells
+    line = {
+        "metric": "full pipeline: init seconds to steady state (warm vs default start) and sim cell-steps/s",
+        "value": warm.seconds + t_pred + t_sim, "unit": "s", "n_gpus": 1, "steps": a.sim_steps,
+        "warmup": a.warmup, "ms_per_step": t_sim / a.sim_steps * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32-flux/f64-state", "data": "synthetic",
+        "config": {"workload": f"full: {C} cells, {cells.n_mems} membranes, {cells.X.shape[0]}x{cells.X.shape[1]} "
+                               f"grid, Nav1p2/Kv1p1/Cav1p2, {p.network.n_species}-substance reaction network; "
+                               f"surrogate from {len(data)} clusters of 500-2000 cells, then {a.sim_steps} sim steps",
+                   "cells": C},
+        "init_cold_seconds": cold.seconds + t_setup, "init_cold_steps": int(cold.steps[0]),
+        "init_warm_seconds": warm.seconds + t_pred, "init_warm_steps": int(warm.steps[0]),
+        "warm_injected": bool(warm.warm[0]),
+        "sim_seconds": t_sim, "sim_cell_steps_per_s": C * a.sim_steps / t_sim,
+        "surrogate": {"train_clusters": len(data), "accepted": int(sum(d["ok"] for d in data)),
+                      "dataset_seconds": t_data, "train_seconds": t_train,
+                      "train_loss_first_last": [hist[0], hist[-1]]},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def make_problem(name: str):
     from betse_amd.cells import Cells
     from betse_amd.parameters import Parameters
@@ -339,6 +398,7 @@ def main(argv=None):
                     help="warmstart config: stepper steps before the first extrapolation of a label")
     ap.add_argument("--epochs", type=int, default=40, help="warmstart config: training epochs")
     ap.add_argument("--save-model", default="", help="warmstart config: save the trained network (.npz)")
+    ap.add_argument("--sim-steps", type=int, default=20_000, help="full_pipeline config: sim-phase steps")
     ap.add_argument("--ws-flux", default="fp32", choices=["fp64", "fp32"],
                     help="warmstart config: flux arithmetic of the init runs")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -363,6 +423,8 @@ def main(argv=None):
         return bench_batched(a)
     if a.config == "warmstart":
         return bench_warmstart(a)
+    if a.config == "full_pipeline":
+        return bench_full_pipeline(a)
     p, cells, t_world = make_problem(a.config)
     n_ions = len(p.ion_names())
     has_ca = "Ca" in p.ion_names()

@@ -21,7 +21,7 @@ import numpy as np
 
 from . import _native
 from .cells import Cells
-from .channels import CHANNEL_KINDS
+from .channels import CHANNEL_KINDS, draw_channel_state
 from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableException
 from .parameters import Parameters
 from .sim import HostState, SimPhase, SimPhaseKind, gaussian_taps, make_tiles
@@ -99,6 +99,9 @@ class BatchSimulator:
                 raise BetseSimException("clusters of a batch need the same channels")
             if st.c_env_bound != s0.c_env_bound:
                 raise BetseSimException("clusters of a batch need the same boundary concentrations")
+            if (st.network is None) != (s0.network is None) or (
+                    st.network is not None and bytes(st.network.pack(p.R, st.T)) != bytes(s0.network.pack(p0.R, s0.T))):
+                raise BetseSimException("clusters of a batch need the same reaction network")
             if (p.alpha_NaK > 0.0) != (p0.alpha_NaK > 0.0):
                 raise BetseSimException("the NaK pump must be on in all clusters or in none")
 
@@ -166,6 +169,10 @@ class BatchSimulator:
             "chan_mod": (np.concatenate([st.chan_mod for st in sts], axis=1) if nch else np.ones((1, 1)), f64),
         }
         host["cell_cluster"] = (np.concatenate([np.full(n, i, dtype=np.int32) for i, n in enumerate(C)]), i32)
+        net = sts[0].network
+        if net is not None:
+            host["c_net"] = (np.concatenate([st.c_net for st in sts], axis=1), f64)
+        self._dev = {}
         d = self._dev
         with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
             for k, (a, dt) in host.items():
@@ -225,6 +232,10 @@ class BatchSimulator:
                                      ctypes.byref(buf), ctypes.c_void_p(self.stream.cuda_stream))
         if rc != _native.BT_OK:
             raise BetseNativeError(f"bt_create failed ({rc}): {self._lib.bt_last_error(None).decode()}")
+        if net is not None:
+            self._net = net.pack(p0.R, s0.T)
+            self._check(self._lib.bt_set_network(self._ctx, ctypes.byref(self._net),
+                                                 ctypes.c_void_p(d["c_net"].data_ptr())))
         self._check(self._lib.bt_update_v(self._ctx))
 
     # ------------------------------------------------------------------
@@ -264,7 +275,8 @@ class BatchSimulator:
         self.stream.synchronize()
         d = self._dev
         coff, moff, soff = self.offsets
-        h = {k: d[k].cpu().numpy() for k in ("c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "gates")}
+        keys = ("c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "gates") + (("c_net",) if "c_net" in d else ())
+        h = {k: d[k].cpu().numpy() for k in keys}
         for i, (st, (p, c)) in enumerate(zip(self.states, self.worlds)):
             a, b = coff[i], coff[i + 1]
             sa, sb = soff[i], soff[i + 1]
@@ -277,6 +289,8 @@ class BatchSimulator:
             st.v_env = h["v_env"][sa:sb].copy()
             if len(st.channel_specs):
                 st.chan_gates = h["gates"][:, :, moff[i]:moff[i + 1]].copy()
+            if "c_net" in h:
+                st.c_net = h["c_net"][:, a:b].copy()
 
     def snapshot(self):
         """(c_cell [n_ions][C], vm_cell [C]) of the whole batch, on the host."""
@@ -338,5 +352,10 @@ def batched_worlds(n_clusters: int = 256, cells_range=(100, 5000), grid_range=(3
         for j, ion in enumerate(("Na", "K", "Cl")):
             p.mem_perms[ion] = p.mem_perms[ion] * f[j]
         p.alpha_NaK = p.alpha_NaK * f[3]
-        out.append((p, Cells.make_world(p)))
+        cells = Cells.make_world(p)
+        if p.channels and p.channel_random_state:
+            # every cluster its own seeded channel multipliers and gates
+            p.channel_seed = int(rng.randint(0, 2 ** 31 - 1))
+            draw_channel_state(p, cells.n_cells, cells.n_mems)
+        out.append((p, cells))
     return out

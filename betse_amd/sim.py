@@ -490,9 +490,8 @@ class Simulator(HostState):
         if rc != _native.BT_OK:
             raise BetseNativeError(f"libbetse_amd error {rc}: {self._lib.bt_last_error(self._ctx).decode()}")
 
-    def _fetch(self, tensors: dict) -> dict:
-        """Device tensors -> host arrays through cached pinned buffers: all copies
-        are queued on the simulation stream, then one synchronisation."""
+    def _fetch_start(self, tensors: dict) -> None:
+        """Queue device -> pinned host copies on the simulation stream."""
         torch = self.torch
         pinned = self.__dict__.setdefault("_pinned", {})
         with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
@@ -502,14 +501,20 @@ class Simulator(HostState):
                     h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
                     pinned[k] = h
                 h.copy_(t, non_blocking=True)
-        self.stream.synchronize()
-        return {k: pinned[k].numpy().copy() for k in tensors}
 
-    def _pull_state(self, full: bool = True) -> None:
-        """Device state -> host. ``full=False`` (sampled steps) skips what
-        write2storage does not store: the lagged membrane concentrations and
-        the channel gates."""
-        d = self._dev
+    def _fetch_finish(self, keys) -> dict:
+        """Host arrays of pinned copies that have completed."""
+        pinned = self._pinned
+        return {k: pinned[k].numpy().copy() for k in keys}
+
+    def _fetch(self, tensors: dict) -> dict:
+        """Device tensors -> host arrays through cached pinned buffers: all copies
+        are queued on the simulation stream, then one synchronisation."""
+        self._fetch_start(tensors)
+        self.stream.synchronize()
+        return self._fetch_finish(tensors.keys())
+
+    def _state_names(self, full: bool):
         names = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"]
         if full:
             names.append("c_lag")
@@ -517,7 +522,16 @@ class Simulator(HostState):
                 names.append("gates")
         if self.network is not None:
             names.append("c_net")
-        h = self._fetch({k: d[k] for k in names})
+        return names
+
+    def _pull_state(self, full: bool = True) -> None:
+        """Device state -> host. ``full=False`` (sampled steps) skips what
+        write2storage does not store: the lagged membrane concentrations and
+        the channel gates."""
+        d = self._dev
+        self._set_state(self._fetch({k: d[k] for k in self._state_names(full)}), full)
+
+    def _set_state(self, h: dict, full: bool) -> None:
         shp = self.cells.X.shape
         self.cc_cells = h["c_cell"]
         self.vm_cell = h["vm_cell"]
@@ -574,11 +588,14 @@ class Simulator(HostState):
         """Observables of the last observed step (get_current, ion_current.py:13-109)."""
         obs, t = self._obs_buffers()
         self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
+        self._set_observables(self._fetch({k: t[k] for k in self._OBS_KEEP}))
+
+    # the observables write2storage keeps; the raw fluxes stay on the device (pull_raw_fluxes)
+    _OBS_KEEP = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "J_env_x",
+                 "J_env_y", "Jtx", "Jty", "B_field")
+
+    def _set_observables(self, h: dict) -> None:
         shp = self.cells.X.shape
-        # the fields write2storage keeps; the raw fluxes stay on the device (pull_raw_fluxes)
-        keep = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "J_env_x",
-                "J_env_y", "Jtx", "Jty", "B_field")
-        h = self._fetch({k: t[k] for k in keep})
         self.rate_NaKATP = h["rate_NaKATP"]
         self.Jn, self.I_mem = h["Jn"], h["I_mem"]
         self.E_gj_x, self.E_gj_y = h["E_gj_x"], h["E_gj_y"]
@@ -674,17 +691,36 @@ class Simulator(HostState):
             raise BetseSimException("in-loop animation is not supported by the GPU step")
         step0 = int(self._lib.bt_steps_done(self._ctx))
         sampled_idx = [i for i, t in enumerate(time_steps) if t in time_steps_sampled]
+        obs, t_obs = self._obs_buffers() if sampled_idx else (None, None)
+        d = self._dev
+        state_keys = self._state_names(False)
+        pending = None
+
+        def finish(t_sample):
+            # host side of a sampled step: runs while the device steps on
+            h = self._fetch_finish(state_keys + list(self._OBS_KEEP))
+            self._set_state(h, False)
+            self._set_observables(h)
+            phase.callbacks.progressed_next()
+            self.write2storage(t_sample, phase.cells, phase.p)
+
         pos = 0
         for s in sampled_idx:
             self._step_observed(s + 1 - pos)
             pos = s + 1
+            if pending is not None:
+                finish(pending)
+                pending = None
             self._sync(step0)
-            self._pull_state(full=False)
-            self._pull_observables()
-            phase.callbacks.progressed_next()
-            self.write2storage(time_steps[s], phase.cells, phase.p)
+            # observables of the sampled step, then one batch of copies to pinned host memory
+            self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
+            self._fetch_start({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in self._OBS_KEEP}})
+            self.stream.synchronize()
+            pending = time_steps[s]
         if pos < len(time_steps):
             self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, 1 if self.use_graph else 0))
+        if pending is not None:
+            finish(pending)
         self._sync(step0)
         self._pull_state()
 

@@ -40,7 +40,8 @@ from .batch import BatchSimulator, batched_worlds
 from .exceptions import BetseSimUnstableException
 from .parameters import Parameters
 
-N_FEATURES = 10
+CHANNEL_FEATURES = ("Nav1p2", "Kv1p1", "Cav1p2")
+N_FEATURES = 10 + len(CHANNEL_FEATURES)
 V_SCALE = 0.05        # Vmem target in units of 50 mV
 
 
@@ -49,8 +50,10 @@ def target_ions(names: Sequence[str]) -> List[str]:
 
 
 def cell_features(p: Parameters, cells, base: Parameters) -> np.ndarray:
-    """Per-cell inputs: geometry, position in the cluster and the cluster's
-    membrane diffusivities and pump rate relative to ``base``."""
+    """Per-cell inputs: geometry, position in the cluster, the cluster's
+    membrane diffusivities and pump rate relative to ``base``, and the cell's
+    voltage-gated channel density multipliers (log; 0 without the channel).
+    The cluster-size input saturates at 10^3.5 cells."""
     n = cells.n_cells
     X = np.zeros((n, N_FEATURES))
     vol = cells.cell_vol
@@ -65,8 +68,12 @@ def cell_features(p: Parameters, cells, base: Parameters) -> np.ndarray:
     for j, ion in enumerate(("Na", "K", "Cl")):
         X[:, 4 + j] = np.log10(p.mem_perms[ion] / base.mem_perms[ion])
     X[:, 7] = np.log10(p.alpha_NaK / base.alpha_NaK)
-    X[:, 8] = np.log10(n) / 4.0 - 0.75
+    X[:, 8] = min(np.log10(n), 3.5) / 4.0 - 0.75
     X[:, 9] = cells.diviterm / cells.diviterm.mean() - 1.0
+    mults = getattr(p, "channel_cell_mult", None)
+    for k, spec in enumerate(getattr(p, "channels", ())):
+        if spec.kind in CHANNEL_FEATURES and mults is not None and mults[k] is not None:
+            X[:, 10 + CHANNEL_FEATURES.index(spec.kind)] = np.log(mults[k])
     return X
 
 
@@ -342,18 +349,20 @@ def steady_state_labels(bs: BatchSimulator, horizon: int = 20_000, rounds: int =
 
 def make_dataset(n_clusters: int, seed: int, *, cells_range=(500, 2000), grid_range=(48, 88),
                  decades: float = 2.0, batch: int = 256, tol: float = 1e-6, window: int = 100,
-                 horizon: int = 20_000, rounds: int = 2, flux_mode: str = "fp64", device=None, log=None):
+                 horizon: int = 20_000, rounds: int = 2, flux_mode: str = "fp64", device=None, log=None,
+                 base: str = "small"):
     """Init-phase steady states of ``n_clusters`` random clusters
     (``steady_state_labels``), produced on the GPU in batches. Returns
     per-cluster (features, neighbour CSR, targets, ok) and run statistics."""
-    base = Parameters.preset("small")
+    base_name = base
+    base = Parameters.preset(base_name)
     data = []
     stats = []
     for b0 in range(0, n_clusters, batch):
         nb = min(batch, n_clusters - b0)
         t0 = time.perf_counter()
         worlds = batched_worlds(nb, cells_range=cells_range, grid_range=grid_range, seed=seed + b0,
-                                base="small", decade=decades)
+                                base=base_name, decade=decades)
         t_w = time.perf_counter() - t0
         bs = BatchSimulator(worlds, device=device, flux_mode=flux_mode)
         t1 = time.perf_counter()
