@@ -236,7 +236,7 @@ def bench_full_pipeline(a):
     t0 = time.perf_counter()
     data, stats, _ = ws.make_dataset(a.train_clusters, seed=3000, base="full", cells_range=(500, 2000),
                                      grid_range=(36, 72), decades=0.0, batch=a.clusters, horizon=a.label_horizon,
-                                     flux_mode="fp32", log=log)
+                                     rounds=4, flux_mode="fp32", log=log)
     t_data = time.perf_counter() - t0
     # features are relative to the workload's own parameters (factors 0)
     starter, hist, t_train = ws.train(data, epochs=a.epochs, base=p, log=log)

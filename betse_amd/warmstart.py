@@ -19,7 +19,9 @@ and a graph network that predicts where the init phase ends up:
   predicted Vmem are made consistent by solving for the charge-balancing
   anion M, the ion the reference itself balances with
   (``parameters.py:1569-1605``). The lagged membrane concentrations start
-  equal to the cell values, as after ``init_core`` (``sim.py:744-745``).
+  equal to the cell values, as after ``init_core`` (``sim.py:744-745``);
+  voltage-gated channel gates start at their steady state for the predicted
+  Vmem (the channels' ``_init_state``).
 * **Fallback.** A prediction that is not finite, gives a non-positive M or
   an implausible Vmem is not injected: that cluster starts cold. A batch
   that turns unstable restarts with the clusters that went non-finite
@@ -37,6 +39,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 from .batch import BatchSimulator, batched_worlds
+from .channels import init_gates
 from .exceptions import BetseSimUnstableException
 from .parameters import Parameters
 
@@ -115,6 +118,12 @@ def inject(st, Y: np.ndarray, vm_limit: float = 0.3) -> bool:
     cc[iM] = cM
     st.cc_cells = cc
     st.c_lag = cc.copy()
+    # channel gates at their steady state for the predicted Vmem (the gates'
+    # _init_state, channels/vg_*.py), which is where they relax to
+    for k, spec in enumerate(getattr(st, "channel_specs", ())):
+        m, h = init_gates(spec.kind, vm[cells.mem_to_cells])
+        st.chan_gates[k, 0] = m
+        st.chan_gates[k, 1] = h
     return True
 
 

@@ -61,3 +61,24 @@ def test_batch_with_channels():
         assert dv <= 1e-9
         np.testing.assert_allclose(st.cc_cells, o.cc_cells, rtol=1e-10)
         np.testing.assert_allclose(st.chan_gates[0, 0], o.channels[0].m, rtol=1e-10, atol=1e-14)
+
+
+def test_batch_with_full_physics():
+    """Nav/Kv/Cav and the reaction network in a batch of clusters of
+    different sizes, each against its own oracle run."""
+    worlds = batched_worlds(n_clusters=3, cells_range=(30, 200), grid_range=(16, 24), seed=9, base="tiny_full",
+                            decade=0.0)
+    say("case: batch of 3 clusters with Nav/Kv/Cav and the network, fp64, 60 steps")
+    b = run_batch(worlds, 60, "fp64")
+    for k, ((p, cells), st) in enumerate(zip(worlds, b.states)):
+        o = OracleSim(cells, p.copy(), mode="sparse")
+        o.init_core()
+        o.run_sim_core("init", n_steps=60)
+        dv = np.abs(st.vm - o.vm).max()
+        dn = (np.abs(st.c_net - o.c_net) / np.maximum(np.abs(o.c_net), 1e-30)).max()
+        say(f"cluster {k}: |dVm|={dv:.3e} rel dc_net={dn:.3e}")
+        assert dv <= 1e-9
+        np.testing.assert_allclose(st.cc_cells, o.cc_cells, rtol=1e-10)
+        for ch in range(len(o.channels)):
+            np.testing.assert_allclose(st.chan_gates[ch, 0], o.channels[ch].m, rtol=1e-10, atol=1e-14)
+        assert dn < 1e-10

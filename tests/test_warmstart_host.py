@@ -83,3 +83,22 @@ def test_inject_refuses_unusable_predictions(worlds):
     Y[:, 2] = 5.0                       # Cl x e^5: M would go negative
     assert not ws.inject(st, Y)
     assert np.array_equal(st.cc_cells, cc0)
+
+
+def test_inject_sets_channel_gates_to_their_steady_state():
+    from betse_amd.channels import draw_channel_state, init_gates
+    p = Parameters.preset("tiny_channels")
+    cells = __import__("betse_amd.cells", fromlist=["Cells"]).Cells.make_world(p)
+    draw_channel_state(p, cells.n_cells, cells.n_mems)
+    st = _state(p, cells)
+    Y = np.zeros((cells.n_cells, 5))                  # Na, K, Cl, Ca, Vm
+    Y[:, 4] = -0.065 / ws.V_SCALE
+    assert ws.inject(st, Y)
+    vm_mem = np.full(cells.n_mems, -0.065)
+    for k, spec in enumerate(st.channel_specs):
+        m, h = init_gates(spec.kind, vm_mem)
+        assert np.allclose(st.chan_gates[k, 0], m) and np.allclose(st.chan_gates[k, 1], h)
+    # features carry the per-cell channel multipliers
+    X = ws.cell_features(p, cells, Parameters.preset("tiny_channels"))
+    assert np.allclose(X[:, 10], np.log(p.channel_cell_mult[0]))
+    assert np.allclose(X[:, 11], np.log(p.channel_cell_mult[1]))
