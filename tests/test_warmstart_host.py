@@ -102,3 +102,21 @@ def test_inject_sets_channel_gates_to_their_steady_state():
     X = ws.cell_features(p, cells, Parameters.preset("tiny_channels"))
     assert np.allclose(X[:, 10], np.log(p.channel_cell_mult[0]))
     assert np.allclose(X[:, 11], np.log(p.channel_cell_mult[1]))
+
+
+def test_extrapolation_recovers_the_limit_of_a_geometric_relaxation():
+    """x_n = x* + A q^n per cluster (one ratio per cluster, any amplitudes):
+    three snapshots give x* exactly; a non-contracting cluster keeps x3."""
+    rng = np.random.RandomState(1)
+    coff = np.array([0, 5, 12, 20])
+    xs = rng.uniform(1.0, 10.0, (3, 20))
+    A = rng.uniform(-0.5, 0.5, (3, 20))
+    vs, Av = rng.uniform(-0.07, -0.05, 20), rng.uniform(-0.01, 0.01, 20)
+    q = np.array([0.9, 0.5, 1.2])                        # last cluster grows: no extrapolation
+    qc = np.repeat(q, np.diff(coff))
+    snaps = [(xs + A * qc ** n, vs + Av * qc ** n) for n in (1, 2, 3)]
+    cx, vx, qs = ws._extrapolate(snaps, coff)
+    assert np.allclose(qs, q)
+    assert np.allclose(cx[:, :12], xs[:, :12], rtol=1e-12)
+    assert np.allclose(vx[:12], vs[:12], rtol=1e-12)
+    assert np.array_equal(cx[:, 12:], snaps[2][0][:, 12:])
