@@ -226,28 +226,31 @@ struct MemCellArgs {
     double* obs_fm; double* obs_fgj; double* obs_fch; double* obs_rate;
 };
 
-template <int NI, typename FT>
-__global__ void __launch_bounds__(TMB)
+template <int NI, typename FT, int WG>
+__global__ void __launch_bounds__(TMB * WG)
 k_mem_cell(MemCellArgs A, KParams P) {
     if (halted(A.flags)) return;
-    // one wavefront per tile; lane j owns membrane m0 + j and, in the per-cell
-    // stage, cell c0 + j. Every ion (and channel) of a membrane is evaluated by
-    // its lane, so a wavefront keeps NI independent gathers in flight.
+    // WG wavefronts per tile; lane j of each owns membrane m0 + j and, in the
+    // per-cell stage, cell c0 + j. Wavefront w evaluates the ions i with
+    // i % WG == w (and the channels likewise), so a wavefront keeps several
+    // independent gathers in flight and small problems still fill the GPU.
     __shared__ double s_mem[NI][TMB];
     __shared__ double s_gj[NI][TMB];
     __shared__ double s_ch[BT_MAX_CHAN][TMB];
+    __shared__ double s_c2[WG > 1 ? NI : 1][TMB];
     __shared__ int s_bad;
     const int tile = xcd_chunk(blockIdx.x, gridDim.x);
     const int c0 = A.tile_ptr[tile];
     const int c1 = A.tile_ptr[tile + 1];
     const int m0 = A.mem_ptr[c0];
     const int m1 = A.mem_ptr[c1];
-    const int j = threadIdx.x;
+    const int w = (WG > 1) ? (int)(threadIdx.x / TMB) : 0;
+    const int j = threadIdx.x % TMB;
     const int m = m0 + j;
     const int M = P.n_mems, C = P.n_cells, G2 = P.n_sq;
     const KCluster& KC = A.kcl[A.tile_cluster[tile]];
     const double nak_coef = KC.nak_coef;
-    if (j == 0) s_bad = 0;
+    if (threadIdx.x == 0) s_bad = 0;
 
     // per-cell inputs of the second stage, fetched up front
     const int c = c0 + j;
@@ -261,7 +264,8 @@ k_mem_cell(MemCellArgs A, KParams P) {
         vol = A.cell_vol[c];
         divi = A.diviterm[c];
 #pragma unroll
-        for (int i = 0; i < NI; ++i) cin[i] = A.c_cell_in[i * C + c];
+        for (int i = 0; i < NI; ++i)
+            if (i % WG == w) cin[i] = A.c_cell_in[i * C + c];
     }
 
     if (m < m1) {
@@ -276,13 +280,14 @@ k_mem_cell(MemCellArgs A, KParams P) {
         const double vn = A.vm_in[cn];
         // NaK-ATPase on the lagged membrane concentrations (sim.py:1196-1234, Q1)
         FT fNa = (FT)0;
-        if (P.i_na >= 0) {
+        if (P.i_na >= 0 && (P.i_na % WG == w || P.i_k % WG == w)) {
             fNa = pump_nak<FT>((FT)A.c_lag_in[P.i_na * C + mc], (FT)A.c_env[P.i_na * G2 + k],
                                (FT)A.c_lag_in[P.i_k * C + mc], (FT)A.c_env[P.i_k * G2 + k], (FT)v, nak_coef, P);
-            if (A.obs_rate) A.obs_rate[m] = -(double)fNa;
+            if (A.obs_rate && P.i_na % WG == w) A.obs_rate[m] = -(double)fNa;
         }
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
+            if (i % WG != w) continue;
             double f = 0.0;                                  // fluxes_mem.fill(0)
             if (i == P.i_na) f = f + (double)fNa;
             if (i == P.i_k) f = f + (double)((FT)P.k_per_na * fNa);
@@ -324,7 +329,7 @@ k_mem_cell(MemCellArgs A, KParams P) {
         // modes: the gates carry state from step to step, and FP32 rates (even in
         // cancellation-free form) bias them enough to move Vmem by ~1e-7 V within
         // a few hundred steps (measured), against the 1 uV bar.
-        for (int ch = 0; ch < P.n_chan; ++ch) {
+        for (int ch = w; ch < P.n_chan; ch += WG) {
             const int i = P.chan_ion[ch];
             double* gm = A.gates + (size_t)(2 * ch) * M;
             double* gh = gm + M;
@@ -362,6 +367,7 @@ k_mem_cell(MemCellArgs A, KParams P) {
         double rho = 0.0;
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
+            if (i % WG != w) continue;
             double base = cin[i];
             const int ch = P.ion_chan[i];
             if (ch >= 0) {             // immediate update_Co of the channel flux
@@ -381,15 +387,29 @@ k_mem_cell(MemCellArgs A, KParams P) {
             if (c2 < 0.0) c2 = 0.0;                    // no_negs clamp (Q9)
             if (isnan(c2)) bad = true;
             A.c_cell_out[i * C + c] = c2;
-            rho = (i == 0) ? P.zF[0] * c2 : rho + P.zF[i] * c2;
+            if (WG > 1) s_c2[i][j] = c2;
+            else rho = (i == 0) ? P.zF[0] * c2 : rho + P.zF[i] * c2;
         }
-        const double vm = P.cm_inv * (rho * divi);
-        if (isnan(vm)) bad = true;
-        A.vm_out[c] = vm;
+        if (WG == 1) {
+            const double vm = P.cm_inv * (rho * divi);
+            if (isnan(vm)) bad = true;
+            A.vm_out[c] = vm;
+        }
         if (bad) s_bad = 1;
     }
+    if (WG > 1) {
+        __syncthreads();
+        if (w == 0 && own_cell) {
+            double rho = P.zF[0] * s_c2[0][j];
+#pragma unroll
+            for (int i = 1; i < NI; ++i) rho = rho + P.zF[i] * s_c2[i][j];
+            const double vm = P.cm_inv * (rho * divi);
+            if (isnan(vm)) s_bad = 1;
+            A.vm_out[c] = vm;
+        }
+    }
     __syncthreads();
-    if (j == 0 && s_bad) {
+    if (threadIdx.x == 0 && s_bad) {
         atomicOr(A.flags, 1u);
         atomicMin(A.bad_step, (unsigned long long)(*A.step_ctr));
     }
@@ -1075,12 +1095,22 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
     return k;
 }
 
+// Wavefronts per tile of k_mem_cell: two (ions split between them) while one
+// wavefront per tile would leave the SIMDs with few waves to switch between.
+#ifndef BT_MEM_WG_TILES
+#define BT_MEM_WG_TILES 4096
+#endif
+template <int NI, int WG>
+static void launch_mem_cell_w(bt_ctx* c, const MemCellArgs& a) {
+    if (c->prm.flux_mode == BT_FLUX_FP32)
+        hipLaunchKernelGGL((k_mem_cell<NI, float, WG>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a, c->kp);
+    else
+        hipLaunchKernelGGL((k_mem_cell<NI, double, WG>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a, c->kp);
+}
 template <int NI>
 static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a) {
-    if (c->prm.flux_mode == BT_FLUX_FP32)
-        hipLaunchKernelGGL((k_mem_cell<NI, float>), dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
-    else
-        hipLaunchKernelGGL((k_mem_cell<NI, double>), dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
+    if (c->n_tiles < BT_MEM_WG_TILES) launch_mem_cell_w<NI, 2>(c, a);
+    else launch_mem_cell_w<NI, 1>(c, a);
 }
 
 static void launch_mem_cell(bt_ctx* c, const MemCellArgs& a) {

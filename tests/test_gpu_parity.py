@@ -225,6 +225,26 @@ def test_channels_fp64_parity():
         np.testing.assert_allclose(sim.chan_gates[k, 1], ch.h, rtol=1e-10, atol=1e-14)
 
 
+def test_many_tiles_single_wavefront_path():
+    """A cluster with more than 4096 cell tiles runs the membrane kernel with one
+    wavefront per tile (smaller ones split the ions over two): both FP64 and
+    FP32 flux paths against the oracle over 20 steps."""
+    from betse_amd.channels import draw_channel_state
+    from betse_amd.sim import make_tiles
+    p = Parameters.preset("large")
+    p.cluster_radius = 1.1e-3
+    cells = Cells.make_world(p)
+    assert len(make_tiles(cells.mem_ptr)) - 1 > 4096
+    draw_channel_state(p, cells.n_cells, cells.n_mems)
+    say(f"case: {cells.n_cells} cells, single-wavefront tiles, 20 steps")
+    o = oracle_run(p, cells, 20)
+    for mode, vm_tol, rel in (("fp64", VM_TOL, REL_TOL), ("fp32", 1e-6, 1e-6)):
+        sim = gpu_run(p, cells, 20, flux_mode=mode)
+        dv, dc, de, lag, eex = errors(sim, o)
+        say(f"  {mode}: |dVm|={dv:.3e} rel dc={dc:.3e} rel denv={de:.3e}")
+        assert dv <= vm_tol and dc <= rel and de <= rel
+
+
 def test_channels_fp32_flux_parity():
     """The mixed-precision path with Nav1p2 + Kv1p1 (FP32 fluxes and FP32
     gating in cancellation-free form, FP64 state): |dVm| < 1 uV and gates
