@@ -45,9 +45,11 @@ def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 
                      + M * 8 * ni                           # fsa write
                      + M * n_ch * (16 + 16 + 8)              # gates r/w, channel flux write
                      + C * (8 * ni * 2 + 8 * ni + 8 * ni     # c_cell r, c_lag r, c_cell w, c_lag w
-                            + 16 + 16 + 4 + 8 * n_ch)        # vm r/w, cell_vol, diviterm, mem_ptr, modulators
+                            + 16 + 16 + 4 + 8 * n_ch         # vm r/w, cell_vol, diviterm, mem_ptr, modulators
+                            + 32 * n_ch)                     # gating rates r
                      + n_in * 8 * (ni + (1 if ca else 0) + n_ch))  # env gathers (pre- and post-diffusion)
-    b["env_field"] = G2 * (8 + 8 * ni + 8 * ni + 8 * ni)    # u r, c_env r, D_env r, c_mid w
+    b["env_field"] = (G2 * (8 + 8 * ni + 8 * ni + 8 * ni)   # u r, c_env r, D_env r, c_mid w
+                      + (C * (8 + 32 * n_ch) if n_ch else 0))  # vm r, gating rates w
     b["env_exchange"] = G2 * (8 * ni + 8 * ni + 8 + 1 + 4) + M * 8 * (ni + n_ch)
     if n_sub:
         b["network"] = C * n_sub * 16                          # substances read and written

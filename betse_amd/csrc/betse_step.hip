@@ -93,6 +93,9 @@ struct KCluster {
     double gjo_u;
 };
 
+// x clamped into [0, n - 1]
+__device__ __forceinline__ int clampi(int x, int n) { return min(max(x, 0), n - 1); }
+
 // Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
 // their linear index. Remap the linear index b of n so that each XCD receives
 // one contiguous chunk of the work: neighbouring tiles, which share squares,
@@ -205,6 +208,14 @@ __device__ __forceinline__ double with_nonces(double v, int n) {
     return v;
 }
 
+// steady states and time constants of one channel kind at V (mV), FP64
+__device__ __forceinline__ void gate_rates(int kind, double V, double& mInf, double& mTau, double& hInf,
+                                           double& hTau) {
+    if (kind == BT_CHAN_NAV1P2) nav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
+    else if (kind == BT_CHAN_KV1P1) kv1p1_rates<double>(V, mInf, mTau, hInf, hTau);
+    else cav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
+}
+
 // ---------------------------------------------------------------------------
 // k_mem_cell: membrane fluxes + per-cell update + Vmem
 // ---------------------------------------------------------------------------
@@ -220,52 +231,60 @@ struct MemCellArgs {
     double* fsa;              // [n_ions][n_mems] flux_mem * mem_sa, square-sorted
     double* gates;            // [n_chan][2][n_mems]
     const double* chan_mod;   // [n_chan][n_cells]
+    const double* rates;      // [n_chan][4][n_cells] gating rates of this step (k_env_field)
     double* fch;              // [n_chan][n_mems] channel flux * mem_sa, square-sorted
     unsigned* flags; unsigned long long* bad_step; const long long* step_ctr;
     // observed step only (NULL otherwise): raw fluxes in membrane order
     double* obs_fm; double* obs_fgj; double* obs_fch; double* obs_rate;
 };
 
-template <int NI, typename FT, int WG>
-__global__ void __launch_bounds__(TMB * WG)
-k_mem_cell(MemCellArgs A, KParams P) {
-    if (halted(A.flags)) return;
-    // WG wavefronts per tile; lane j of each owns membrane m0 + j and, in the
-    // per-cell stage, cell c0 + j. Wavefront w evaluates the ions i with
-    // i % WG == w (and the channels likewise), so a wavefront keeps several
-    // independent gathers in flight and small problems still fill the GPU.
-    __shared__ double s_mem[NI][TMB];
-    __shared__ double s_gj[NI][TMB];
-    __shared__ double s_ch[BT_MAX_CHAN][TMB];
-    __shared__ double s_c2[WG > 1 ? NI : 1][TMB];
-    __shared__ int s_bad;
-    const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+// wavefronts per tile of k_mem_cell on problems of fewer than BT_MEM_WG_TILES tiles
+#ifndef BT_MEM_WG
+#define BT_MEM_WG 2
+#endif
+
+// LDS of one k_mem_cell block, shared by its wavefronts
+struct MemLds {
+    double (*mem)[TMB];       // [NI] membrane flux * mem_sa, membrane order
+    double (*gj)[TMB];        // [NI] -GJ flux * mem_sa
+    double (*ch)[TMB];        // [BT_MAX_CHAN] channel flux * mem_sa
+    double (*c2)[TMB];        // [NI] updated cell concentrations (WG > 1)
+    int* bad;
+};
+
+// The work of wavefront W of a k_mem_cell block. W is a template argument so
+// that every ion index below is a compile-time constant (the per-ion kernel
+// parameters then load once, up front).
+template <int NI, typename FT, int WG, int W>
+__device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParams& P, int tile, const MemLds& L) {
+    constexpr int NS = (NI + WG - 1) / WG;             // ion slots per wavefront
+    constexpr int NQ = (BT_MAX_CHAN + WG - 1) / WG;    // channel slots per wavefront
     const int c0 = A.tile_ptr[tile];
     const int c1 = A.tile_ptr[tile + 1];
     const int m0 = A.mem_ptr[c0];
     const int m1 = A.mem_ptr[c1];
-    const int w = (WG > 1) ? (int)(threadIdx.x / TMB) : 0;
     const int j = threadIdx.x % TMB;
     const int m = m0 + j;
     const int M = P.n_mems, C = P.n_cells, G2 = P.n_sq;
-    const KCluster& KC = A.kcl[A.tile_cluster[tile]];
-    const double nak_coef = KC.nak_coef;
-    if (threadIdx.x == 0) s_bad = 0;
+    const KCluster* KC = A.kcl + A.tile_cluster[tile];
+    const double nak_coef = KC->nak_coef;
+    const double gjo_u = KC->gjo_u;
+    if (threadIdx.x == 0) *L.bad = 0;
 
     // per-cell inputs of the second stage, fetched up front
     const int c = c0 + j;
     const bool own_cell = (c < c1);
     int ca = 0, cb = 0;
     double vol = 1.0, divi = 0.0;
-    double cin[NI];
+    double cin[NS];
     if (own_cell) {
         ca = A.mem_ptr[c] - m0;
         cb = A.mem_ptr[c + 1] - m0;
         vol = A.cell_vol[c];
         divi = A.diviterm[c];
 #pragma unroll
-        for (int i = 0; i < NI; ++i)
-            if (i % WG == w) cin[i] = A.c_cell_in[i * C + c];
+        for (int s = 0; s < NS; ++s)
+            if (W + s * WG < NI) cin[s] = A.c_cell_in[(W + s * WG) * C + c];
     }
 
     if (m < m1) {
@@ -275,85 +294,127 @@ k_mem_cell(MemCellArgs A, KParams P) {
         const int k = A.mem_sq[m];
         const int pos = A.mem_pos[m];
         const double sa = A.mem_sa[m];
-        const double gjo = (KC.gjo_u == KC.gjo_u) ? KC.gjo_u : A.gjopen[m];
+        const double gjo = (gjo_u == gjo_u) ? gjo_u : A.gjopen[m];
         const double v = A.vm_in[mc];
         const double vn = A.vm_in[cn];
+        // the gathers of one ion slot / channel slot, issued together
+        double g_env[NS], g_lag[NS], g_lagn[NS], g_dm[NS];
+        auto gather_ion = [&](int s) {
+            const int i = W + s * WG;
+            const double du = KC->dm_u[i];
+            // the cluster's uniform Dm when it has one, else the membrane's
+            const double* dmp = (du == du) ? &KC->dm_u[i] : &A.Dm[i * M + m];
+            g_env[s] = A.c_env[i * G2 + k];
+            g_lag[s] = A.c_lag_in[i * C + mc];
+            g_lagn[s] = A.c_lag_in[i * C + cn];
+            g_dm[s] = *dmp;
+        };
+        double h_m[NQ], h_h[NQ], h_mod[NQ], h_out[NQ], h_in[NQ];
+        double r_mi[NQ], r_mt[NQ], r_hi[NQ], r_ht[NQ];   // mInf, mTau, hInf, hTau
+        auto gather_chan = [&](int q) {
+            const int ch = W + q * WG;
+            const int i = P.chan_ion[ch];
+            h_m[q] = A.gates[(size_t)(2 * ch) * M + m];
+            h_h[q] = A.gates[(size_t)(2 * ch + 1) * M + m];
+            h_mod[q] = A.chan_mod[(size_t)ch * C + mc];
+            h_out[q] = A.c_mid[i * G2 + k];
+            h_in[q] = A.c_cell_in[i * C + mc];
+            const double* R = A.rates + (size_t)(4 * ch) * C + mc;
+            r_mi[q] = R[0];
+            r_mt[q] = R[C];
+            r_hi[q] = R[2 * C];
+            r_ht[q] = R[3 * C];
+        };
+        const bool do_nak = P.i_na >= 0 && (P.i_na % WG == W || P.i_k % WG == W);
+        double nai = 0.0, nao = 0.0, ki = 0.0, ko = 0.0, cai = 0.0, cao = 0.0;
+        if (do_nak) {
+            nai = A.c_lag_in[P.i_na * C + mc];
+            nao = A.c_env[P.i_na * G2 + k];
+            ki = A.c_lag_in[P.i_k * C + mc];
+            ko = A.c_env[P.i_k * G2 + k];
+        }
+        if (P.i_ca >= 0 && P.i_ca % WG == W) {
+            cai = A.c_cell_in[P.i_ca * C + mc];
+            cao = A.c_mid[P.i_ca * G2 + k];
+        }
+
         // NaK-ATPase on the lagged membrane concentrations (sim.py:1196-1234, Q1)
         FT fNa = (FT)0;
-        if (P.i_na >= 0 && (P.i_na % WG == w || P.i_k % WG == w)) {
-            fNa = pump_nak<FT>((FT)A.c_lag_in[P.i_na * C + mc], (FT)A.c_env[P.i_na * G2 + k],
-                               (FT)A.c_lag_in[P.i_k * C + mc], (FT)A.c_env[P.i_k * G2 + k], (FT)v, nak_coef, P);
-            if (A.obs_rate && P.i_na % WG == w) A.obs_rate[m] = -(double)fNa;
+        if (do_nak) {
+            fNa = pump_nak<FT>((FT)nai, (FT)nao, (FT)ki, (FT)ko, (FT)v, nak_coef, P);
+            if (A.obs_rate && P.i_na % WG == W) A.obs_rate[m] = -(double)fNa;
         }
+        // Vmem and the neighbour's carrying the i + 1 nonces of ion i, built up
+        // slot by slot in the same sequence of additions
+        double vmut = with_nonces(v, W + 1);
+        double vnmut = with_nonces(vn, W + 1);
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            if (i % WG != w) continue;
+        for (int s = 0; s < NS; ++s) {
+            const int i = W + s * WG;
+            if (i >= NI) break;
+            gather_ion(s);
+            if (s > 0) {
+                vmut = with_nonces(vmut, WG);
+                vnmut = with_nonces(vnmut, WG);
+            }
             double f = 0.0;                                  // fluxes_mem.fill(0)
             if (i == P.i_na) f = f + (double)fNa;
             if (i == P.i_k) f = f + (double)((FT)P.k_per_na * fNa);
             // GHK across the membrane and across the gap junction (sim.py:1253-1269, 2162-2206)
-            const double vmut = with_nonces(v, i + 1);
-            const double vnmut = with_nonces(vn, i + 1);
-            const double clag = A.c_lag_in[i * C + mc];
-            const double dm = (KC.dm_u[i] == KC.dm_u[i]) ? KC.dm_u[i] : A.Dm[i * M + m];
-            const FT fED = ghk<FT>((FT)A.c_env[i * G2 + k], (FT)clag, (FT)dm, (FT)P.tm,
+            const double clag = g_lag[s];
+            const FT fED = ghk<FT>((FT)g_env[s], (FT)clag, (FT)g_dm[s], (FT)P.tm,
                                    (FT)P.zplus[i], (FT)vmut, (FT)P.F, (FT)P.RT);
             f = f + (double)fED;
             double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
             if (!bnd) {
                 const double vBA = (vnmut - vmut) + FLOAT_NONCE;
-                fgj = (double)ghk<FT>((FT)clag, (FT)A.c_lag_in[i * C + cn], (FT)(P.dgj_surf[i] * gjo),
+                fgj = (double)ghk<FT>((FT)clag, (FT)g_lagn[s], (FT)(P.dgj_surf[i] * gjo),
                                       (FT)P.gj_len, (FT)P.zplus[i], (FT)vBA, (FT)P.F, (FT)P.RT_gj);
             }
             // Ca-ATPase after the ion loop: refreshed cell Ca, post-diffusion env Ca,
             // Vmem carrying all n_ions nonces (sim.py:2126-2159)
             if (i == P.i_ca) {
-                double cai = A.c_cell_in[i * C + mc];
-                double cao = A.c_mid[i * G2 + k];
-                if (cai < 0.0) cai = 0.0;
-                if (cao < 0.0) cao = 0.0;
-                f = f + (double)pump_ca<FT>((FT)cai, (FT)cao, (FT)with_nonces(v, NI), P);
+                const double cai0 = (cai < 0.0) ? 0.0 : cai;
+                const double cao0 = (cao < 0.0) ? 0.0 : cao;
+                f = f + (double)pump_ca<FT>((FT)cai0, (FT)cao0, (FT)with_nonces(v, NI), P);
             }
             if (A.obs_fm) {
                 A.obs_fm[(size_t)i * M + m] = f;
                 A.obs_fgj[(size_t)i * M + m] = fgj;
             }
             const double x = f * sa;
-            s_mem[i][j] = x;
+            L.mem[i][j] = x;
             A.fsa[(size_t)i * M + pos] = x;
-            s_gj[i][j] = (-fgj) * sa;
+            L.gj[i][j] = (-fgj) * sa;
         }
         // voltage-gated channels (networks.py:3115-3215): gates from Vmem carrying
-        // the nonces so far, update_mh, open probability, GHK with the current cell
-        // and post-diffusion env concentrations. Gating stays FP64 in both flux
-        // modes: the gates carry state from step to step, and FP32 rates (even in
-        // cancellation-free form) bias them enough to move Vmem by ~1e-7 V within
-        // a few hundred steps (measured), against the 1 uV bar.
-        for (int ch = w; ch < P.n_chan; ch += WG) {
+        // the nonces so far (their rates, per cell, come from k_env_field),
+        // update_mh, open probability, GHK with the current cell and post-diffusion
+        // env concentrations. Gating stays FP64 in both flux modes: the gates carry
+        // state from step to step, and FP32 rates (even in cancellation-free form)
+        // bias them enough to move Vmem by ~1e-7 V within a few hundred steps
+        // (measured), against the 1 uV bar.
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int ch = W + q * WG;
+            if (ch >= P.n_chan) break;
+            gather_chan(q);
             const int i = P.chan_ion[ch];
-            double* gm = A.gates + (size_t)(2 * ch) * M;
-            double* gh = gm + M;
             const double vg = with_nonces(v, NI + ch);
-            const double V = vg * 1000.0 + 0.0;
-            double mInf, mTau, hInf, hTau;
             const int kind = P.chan_kind[ch];
-            if (kind == BT_CHAN_NAV1P2) nav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
-            else if (kind == BT_CHAN_KV1P1) kv1p1_rates<double>(V, mInf, mTau, hInf, hTau);
-            else cav1p2_rates<double>(V, mInf, mTau, hInf, hTau);
             const double dtm = P.chan_dt;
-            double mm = gm[m], hh = gh[m];
-            mm = (mTau * mm + dtm * mInf) / (mTau + dtm);
-            hh = (hTau * hh + dtm * hInf) / (hTau + dtm);
-            gm[m] = mm;
-            gh[m] = hh;
+            const double mm = (r_mt[q] * h_m[q] + dtm * r_mi[q]) / (r_mt[q] + dtm);
+            const double hh = (r_ht[q] * h_h[q] + dtm * r_hi[q]) / (r_ht[q] + dtm);
+            A.gates[(size_t)(2 * ch) * M + m] = mm;
+            A.gates[(size_t)(2 * ch + 1) * M + m] = hh;
             // m**p * h**q: p = 2 squares as NumPy does, p = 3 is cube()
             const double Pch = (kind == BT_CHAN_NAV1P2) ? cube(mm) * hh
                              : (kind == BT_CHAN_KV1P1) ? mm * (hh * hh) : (mm * mm) * hh;
-            const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * A.chan_mod[(size_t)ch * C + mc]);
-            const FT fc = ghk<FT>((FT)A.c_mid[i * G2 + k], (FT)A.c_cell_in[i * C + mc], Dch, (FT)P.tm,
+            const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * h_mod[q]);
+            const FT fc = ghk<FT>((FT)h_out[q], (FT)h_in[q], Dch, (FT)P.tm,
                                   (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE), (FT)P.F, (FT)P.RT);
             const double x = (double)fc * sa;
-            s_ch[ch][j] = x;
+            L.ch[ch][j] = x;
             A.fch[(size_t)ch * M + pos] = x;
             if (A.obs_fch) A.obs_fch[(size_t)ch * M + m] = (double)fc;
         }
@@ -366,20 +427,21 @@ k_mem_cell(MemCellArgs A, KParams P) {
         bool bad = false;
         double rho = 0.0;
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            if (i % WG != w) continue;
-            double base = cin[i];
+        for (int s = 0; s < NS; ++s) {
+            const int i = W + s * WG;
+            if (i >= NI) break;
+            double base = cin[s];
             const int ch = P.ion_chan[i];
             if (ch >= 0) {             // immediate update_Co of the channel flux
-                double Sc = s_ch[ch][ca];
-                for (int q = ca + 1; q < cb; ++q) Sc = Sc + s_ch[ch][q];
+                double Sc = L.ch[ch][ca];
+                for (int q = ca + 1; q < cb; ++q) Sc = Sc + L.ch[ch][q];
                 base = base + (Sc / vol) * P.dt;
             }
-            double S = s_mem[i][ca];
-            double Sg = s_gj[i][ca];
+            double S = L.mem[i][ca];
+            double Sg = L.gj[i][ca];
             for (int q = ca + 1; q < cb; ++q) {
-                S = S + s_mem[i][q];
-                Sg = Sg + s_gj[i][q];
+                S = S + L.mem[i][q];
+                Sg = Sg + L.gj[i][q];
             }
             const double c1v = base + (S / vol) * P.dt;
             A.c_lag_out[i * C + c] = c1v;
@@ -387,7 +449,7 @@ k_mem_cell(MemCellArgs A, KParams P) {
             if (c2 < 0.0) c2 = 0.0;                    // no_negs clamp (Q9)
             if (isnan(c2)) bad = true;
             A.c_cell_out[i * C + c] = c2;
-            if (WG > 1) s_c2[i][j] = c2;
+            if (WG > 1) L.c2[i][j] = c2;
             else rho = (i == 0) ? P.zF[0] * c2 : rho + P.zF[i] * c2;
         }
         if (WG == 1) {
@@ -395,24 +457,45 @@ k_mem_cell(MemCellArgs A, KParams P) {
             if (isnan(vm)) bad = true;
             A.vm_out[c] = vm;
         }
-        if (bad) s_bad = 1;
+        if (bad) *L.bad = 1;
     }
     if (WG > 1) {
         __syncthreads();
-        if (w == 0 && own_cell) {
-            double rho = P.zF[0] * s_c2[0][j];
+        if (W == 0 && own_cell) {
+            double rho = P.zF[0] * L.c2[0][j];
 #pragma unroll
-            for (int i = 1; i < NI; ++i) rho = rho + P.zF[i] * s_c2[i][j];
+            for (int i = 1; i < NI; ++i) rho = rho + P.zF[i] * L.c2[i][j];
             const double vm = P.cm_inv * (rho * divi);
-            if (isnan(vm)) s_bad = 1;
+            if (isnan(vm)) *L.bad = 1;
             A.vm_out[c] = vm;
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && s_bad) {
+    if (threadIdx.x == 0 && *L.bad) {
         atomicOr(A.flags, 1u);
         atomicMin(A.bad_step, (unsigned long long)(*A.step_ctr));
     }
+}
+
+template <int NI, typename FT, int WG>
+__global__ void __launch_bounds__(TMB * WG)
+k_mem_cell(MemCellArgs A, KParams P) {
+    static_assert(WG == 1 || WG == 2, "one or two wavefronts per tile");
+    if (halted(A.flags)) return;
+    // WG wavefronts per tile; lane j of each owns membrane m0 + j and, in the
+    // per-cell stage, cell c0 + j. Wavefront w evaluates the ions w, w + WG,
+    // w + 2 WG, ... (its ion slots) and the channels likewise, so small
+    // problems still fill the GPU. Both wavefronts pass the same barriers.
+    __shared__ double s_mem[NI][TMB];
+    __shared__ double s_gj[NI][TMB];
+    __shared__ double s_ch[BT_MAX_CHAN][TMB];
+    __shared__ double s_c2[WG > 1 ? NI : 1][TMB];
+    __shared__ int s_bad;
+    const MemLds L{s_mem, s_gj, s_ch, s_c2, &s_bad};
+    const int tile = xcd_chunk(blockIdx.x, gridDim.x);
+    const int w = (WG > 1) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TMB)) : 0;
+    if (w == 0) mem_cell_wave<NI, FT, WG, 0>(A, P, tile, L);
+    else mem_cell_wave<NI, FT, WG, WG - 1>(A, P, tile, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -427,9 +510,12 @@ struct FieldArgs {
     double* v_env; double* Ex; double* Ey;   // written in field-only mode
     const unsigned* flags;
     double* obs_fx; double* obs_fy;          // fluxes_env_x/y of the observed step, or NULL
+    const double* vm;         // Vmem the step's k_mem_cell reads (diffuse mode)
+    double* rates;            // [n_chan][4][n_cells] gating rates out (diffuse mode), or NULL
 };
 
-template <bool DIFFUSE>
+// RATES: also the gating rates (diffuse mode with channels)
+template <bool DIFFUSE, bool RATES>
 __global__ void __launch_bounds__(TY * TX)
 k_env_field(FieldArgs A, KParams P) {
     if (halted(A.flags)) return;
@@ -463,9 +549,25 @@ k_env_field(FieldArgs A, KParams P) {
     const double dl = K.delta, d2 = K.two_delta;
 
     // u with zero padding outside the grid (gaussian_filter mode='constant')
-    for (int e = tid; e < UR * UC; e += TY * TX) {
-        const int rr = r0 - UH + e / UC, cc = cc0 - UH + e % UC;
-        sU[e / UC][e % UC] = (rr >= 0 && rr < ny && cc >= 0 && cc < nx) ? A.u[so + rr * nx + cc] : 0.0;
+    // (each thread issues all of its loads, from clamped in-grid addresses,
+    // before the first LDS write, so they overlap)
+    {
+        constexpr int NIT = (UR * UC + TY * TX - 1) / (TY * TX);
+        double t[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int e = min(tid + it * TY * TX, UR * UC - 1);
+            const int rr = r0 - UH + e / UC, cc = cc0 - UH + e % UC;
+            t[it] = A.u[so + clampi(rr, ny) * nx + clampi(cc, nx)];
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int e = tid + it * TY * TX;
+            if (e < UR * UC) {
+                const int rr = r0 - UH + e / UC, cc = cc0 - UH + e % UC;
+                sU[e / UC][e % UC] = (rr >= 0 && rr < ny && cc >= 0 && cc < nx) ? t[it] : 0.0;
+            }
+        }
     }
     __syncthreads();
     // axis-0 pass (scipy correlate1d: centre tap, then pairs outermost first)
@@ -528,16 +630,40 @@ k_env_field(FieldArgs A, KParams P) {
         const double zi = P.zq[i];
         __syncthreads();
         // concentrations with the world boundary reset to c_env_bound (Q3)
-        for (int e = tid; e < VR * VC; e += TY * TX) {
+        // (all loads first, from clamped in-grid addresses, then the LDS writes)
+        constexpr int NC = (VR * VC + TY * TX - 1) / (TY * TX);
+        constexpr int ND = (ER * EC + TY * TX - 1) / (TY * TX);
+        double tc[NC], td[ND];
+#pragma unroll
+        for (int it = 0; it < NC; ++it) {
+            const int e = min(tid + it * TY * TX, VR * VC - 1);
             const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
-            double val = 0.0;
-            if (r >= 0 && r < ny && cc >= 0 && cc < nx)
-                val = (r == 0 || r == ny - 1 || cc == 0 || cc == nx - 1) ? cb : cenv[r * nx + cc];
-            sC[e / VC][e % VC] = val;
+            tc[it] = cenv[clampi(r, ny) * nx + clampi(cc, nx)];
         }
-        for (int e = tid; e < ER * EC; e += TY * TX) {
+#pragma unroll
+        for (int it = 0; it < ND; ++it) {
+            const int e = min(tid + it * TY * TX, ER * EC - 1);
             const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
-            sD[e / EC][e % EC] = (r >= 0 && r < ny && cc >= 0 && cc < nx) ? D[r * nx + cc] : 0.0;
+            td[it] = D[clampi(r, ny) * nx + clampi(cc, nx)];
+        }
+#pragma unroll
+        for (int it = 0; it < NC; ++it) {
+            const int e = tid + it * TY * TX;
+            if (e < VR * VC) {
+                const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
+                double val = 0.0;
+                if (r >= 0 && r < ny && cc >= 0 && cc < nx)
+                    val = (r == 0 || r == ny - 1 || cc == 0 || cc == nx - 1) ? cb : tc[it];
+                sC[e / VC][e % VC] = val;
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < ND; ++it) {
+            const int e = tid + it * TY * TX;
+            if (e < ER * EC) {
+                const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
+                sD[e / EC][e % EC] = (r >= 0 && r < ny && cc >= 0 && cc < nx) ? td[it] : 0.0;
+            }
         }
         __syncthreads();
         // Nernst-Planck fluxes (sim_toolbox.py:409-411, gv = -E, u = 0, mu = 0)
@@ -595,6 +721,26 @@ k_env_field(FieldArgs A, KParams P) {
             }
         }
     }
+    // The gating rates of every (channel, cell) for this step's k_mem_cell.
+    // They depend on the cell's Vmem alone, so evaluating them here, one
+    // (channel, cell) per thread, replaces one FP64 evaluation per membrane.
+    if (RATES) {
+        const int nc = P.n_cells;
+        const int items = P.n_chan * nc;
+        const int nthr = gridDim.x * gridDim.y * (TY * TX);
+        for (int t = (blockIdx.y * gridDim.x + blockIdx.x) * (TY * TX) + tid; t < items; t += nthr) {
+            const int ch = t / nc, cell = t - ch * nc;
+            // Vmem carrying the n_ions + ch nonces of the membrane stage
+            const double V = with_nonces(A.vm[cell], P.n_ions + ch) * 1000.0 + 0.0;
+            double mInf, mTau, hInf, hTau;
+            gate_rates(P.chan_kind[ch], V, mInf, mTau, hInf, hTau);
+            double* R = A.rates + (size_t)(4 * ch) * nc + cell;
+            R[0] = mInf;
+            R[nc] = mTau;
+            R[2 * nc] = hInf;
+            R[3 * nc] = hTau;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -606,9 +752,26 @@ struct ExArgs {
     const unsigned* flags; long long* step_ctr; int count_step;
 };
 
+// sum of -x[q] over q in [a, b), added in order from a; the loads are issued
+// eight at a time (clamped to b - 1, past-the-end values unused) so one
+// square's membranes cost one memory latency instead of one each
+__device__ __forceinline__ double neg_seg_sum(const double* x, int a, int b) {
+    double S = 0.0;
+    for (int q0 = a; q0 < b; q0 += 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = x[min(q0 + u, b - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (q0 + u < b) S = S + (-t[u]);
+    }
+    return S;
+}
+
+constexpr int EXQ = 64;
+
 // One wavefront per ion over EXQ squares; the charge of each square is then
 // summed over the ions in ion order from LDS.
-constexpr int EXQ = 64;
 
 __global__ void __launch_bounds__(BT_MAX_IONS * EXQ)
 k_env_exchange(ExArgs A, KParams P) {
@@ -618,18 +781,25 @@ k_env_exchange(ExArgs A, KParams P) {
     const int k = blockIdx.x * EXQ + j;
     if (A.count_step && blockIdx.x == 0 && threadIdx.x == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
     const bool live = (k < P.n_sq);
+    double surf_scale = 0.0, mean_sa = 1.0, kee = 1.0;
+    bool hm = false;
     if (live) {
         const int a = A.sq_ptr[k], b = A.sq_ptr[k + 1];
-        const double env_vol = A.kcl[A.sq_cluster[k]].env_vol;
+        const KCluster* K = A.kcl + A.sq_cluster[k];
+        const double env_vol = K->env_vol;
+        if (i == 0) {          // the surface charge's inputs, fetched with the rest
+            surf_scale = K->surf_scale;
+            mean_sa = K->mean_sa;
+            kee = K->kee;
+            hm = A.has_mem[k] != 0;
+        }
         double cn = A.c_mid[(size_t)i * P.n_sq + k];
         const int ch = P.ion_chan[i];
         if (ch >= 0) {
-            double Sc = 0.0;
-            for (int q = a; q < b; ++q) Sc = Sc + (-A.fch[(size_t)ch * P.n_mems + q]);
+            const double Sc = neg_seg_sum(A.fch + (size_t)ch * P.n_mems, a, b);
             cn = cn + (Sc / env_vol) * P.dt;
         }
-        double S = 0.0;
-        for (int q = a; q < b; ++q) S = S + (-A.fsa[(size_t)i * P.n_mems + q]);
+        const double S = neg_seg_sum(A.fsa + (size_t)i * P.n_mems, a, b);
         cn = cn + (S / env_vol) * P.dt;
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         s_cn[i][j] = cn;
@@ -638,9 +808,8 @@ k_env_exchange(ExArgs A, KParams P) {
     if (i == 0 && live) {
         double rho = P.zF[0] * s_cn[0][j];
         for (int ii = 1; ii < P.n_ions; ++ii) rho = rho + P.zF[ii] * s_cn[ii][j];
-        const KCluster& K = A.kcl[A.sq_cluster[k]];
         double uu = 0.0;
-        if (A.has_mem[k]) uu = ((rho * K.surf_scale) / K.mean_sa) / K.kee;
+        if (hm) uu = ((rho * surf_scale) / mean_sa) / kee;
         A.u[k] = uu;
     }
 }
@@ -659,12 +828,10 @@ k_env_exchange_sq(ExArgs A, KParams P) {
         double cn = A.c_mid[(size_t)i * P.n_sq + k];
         const int ch = P.ion_chan[i];
         if (ch >= 0) {
-            double Sc = 0.0;
-            for (int q = a; q < b; ++q) Sc = Sc + (-A.fch[(size_t)ch * P.n_mems + q]);
+            const double Sc = neg_seg_sum(A.fch + (size_t)ch * P.n_mems, a, b);
             cn = cn + (Sc / K.env_vol) * P.dt;
         }
-        double S = 0.0;
-        for (int q = a; q < b; ++q) S = S + (-A.fsa[(size_t)i * P.n_mems + q]);
+        const double S = neg_seg_sum(A.fsa + (size_t)i * P.n_mems, a, b);
         cn = cn + (S / K.env_vol) * P.dt;
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         rho = (i == 0) ? P.zF[0] * cn : rho + P.zF[i] * cn;
@@ -1011,6 +1178,7 @@ struct bt_ctx {
     // workspace
     double* c_cell_b = nullptr; double* c_lag_b = nullptr; double* vm_b = nullptr;
     double* c_mid = nullptr; double* fsa = nullptr; double* fch = nullptr; double* u = nullptr;
+    double* rates = nullptr;  // [n_chan][4][n_cells] gating rates of the current step
     unsigned* flags = nullptr; unsigned long long* bad_step = nullptr; long long* step_ctr = nullptr;
     hipGraph_t graph = nullptr; hipGraphExec_t gexec = nullptr;
     long long steps_done = 0;
@@ -1100,6 +1268,7 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
 #ifndef BT_MEM_WG_TILES
 #define BT_MEM_WG_TILES 4096
 #endif
+
 template <int NI, int WG>
 static void launch_mem_cell_w(bt_ctx* c, const MemCellArgs& a) {
     if (c->prm.flux_mode == BT_FLUX_FP32)
@@ -1109,7 +1278,7 @@ static void launch_mem_cell_w(bt_ctx* c, const MemCellArgs& a) {
 }
 template <int NI>
 static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a) {
-    if (c->n_tiles < BT_MEM_WG_TILES) launch_mem_cell_w<NI, 2>(c, a);
+    if (c->n_tiles < BT_MEM_WG_TILES) launch_mem_cell_w<NI, BT_MEM_WG>(c, a);
     else launch_mem_cell_w<NI, 1>(c, a);
 }
 
@@ -1128,18 +1297,23 @@ static void launch_mem_cell(bt_ctx* c, const MemCellArgs& a) {
 
 static inline int grid_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
-static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr) {
+// diffuse: the step's env field and diffusion, and the gating rates from vm;
+// else the field alone (v_env, E) from u
+static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, const double* vm = nullptr) {
     const bt_buffers& B = c->buf;
     FieldArgs f{c->env_tiles, c->kcl, c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
-                ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr};
+                ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
+                vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr};
     // diffusion: one block per (tile, ion) while tiles alone would leave CUs idle,
     // else one block per tile looping over the ions; field only: one block per tile
     const bool split = diffuse && c->n_env_tiles < ENV_SPLIT_TILES;
     dim3 grid(c->n_env_tiles, split ? c->prm.n_ions : 1);
-    if (diffuse)
-        hipLaunchKernelGGL(k_env_field<true>, grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+    if (diffuse && f.rates)
+        hipLaunchKernelGGL((k_env_field<true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+    else if (diffuse)
+        hipLaunchKernelGGL((k_env_field<true, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else
-        hipLaunchKernelGGL(k_env_field<false>, grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
 }
 
 // One step. parity 0: cell state caller -> workspace; parity 1: workspace -> caller.
@@ -1155,12 +1329,12 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     double* vo = parity ? B.vm_cell : c->vm_b;
 
     const bt_obs* ob = c->capture;
-    if (which < 0 || which == 1) launch_field(c, true, ob);
+    if (which < 0 || which == 1) launch_field(c, true, ob, vi);
     if (which < 0 || which == 0) {
         MemCellArgs a{B.mem_ptr, B.mem_cell, B.nn_mem, B.nn_cell, B.mem_sq, B.mem_pos, B.tile_ptr,
                       B.tile_cluster, c->kcl,
                       B.mem_sa, B.gjopen, B.cell_vol, B.diviterm, B.Dm, B.c_env, c->c_mid,
-                      ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->fch,
+                      ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->rates, c->fch,
                       c->flags, c->bad_step, c->step_ctr,
                       ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
                       ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr};
@@ -1318,6 +1492,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     alloc((void**)&c->c_mid, ni * ns * 8);
     alloc((void**)&c->fsa, ni * nm * 8);
     alloc((void**)&c->fch, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * nm * 8);
+    alloc((void**)&c->rates, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * 4 * nc * 8);
     alloc((void**)&c->u, ns * 8);
     alloc((void**)&c->flags, 256);
     alloc((void**)&c->bad_step, 256);
@@ -1331,6 +1506,8 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     if (e == hipSuccess) e = hipMemsetAsync(c->bad_step, 0xff, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->step_ctr, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->u, 0, ns * 8, c->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(c->rates, 0, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * 4 * nc * 8, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         std::string msg = std::string("workspace allocation: ") + hipGetErrorString(e);
@@ -1346,7 +1523,7 @@ int bt_destroy(bt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u,
+    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates,
                     c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
                     c->B1, c->B2, c->T1, c->T2, c->AA, c->BB, c->vm_prev, c->net};
     for (void* q : ptrs)
