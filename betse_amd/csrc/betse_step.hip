@@ -4,17 +4,19 @@
 // One step of Simulator._run_sim_core_loop (betse/science/sim.py:1169-1366) is
 // three kernels on one stream:
 //
-//   k_env_field     one block per (16x16 tile of the environment grid, ion),
-//                   staged through LDS: the surface-charge field u of the
-//                   previous update_V -> SciPy Gaussian (sigma 1, constant 0) ->
-//                   v_env -> E = -grad(screen*v_env) (ion_current.py:93-109),
-//                   then the ion's boundary reset (Q3), gradient, Nernst-Planck
-//                   flux and sign-flipped boundary divergence (Q4) of update_ecm
-//                   (sim.py:2209-2254)                          c_env -> c_mid
-//   k_mem_cell      one wavefront per tile of whole cells (<= 64 membranes), a
-//                   lane per membrane for every ion (NaK pump, GHK, GHK gap
-//                   junction, Ca pump) and voltage-gated channel (gating,
-//                   channel GHK); per cell the CSR segment sums from LDS,
+//   k_env_field     one block per (16x16 tile of the environment grid, pair
+//                   of ions) (per tile over all ions on large batches), staged
+//                   through LDS: the surface-charge field u of the previous
+//                   update_V -> SciPy Gaussian (sigma 1, constant 0) -> v_env ->
+//                   E = -grad(screen*v_env) (ion_current.py:93-109), then each
+//                   ion's boundary reset (Q3), gradient, Nernst-Planck flux and
+//                   sign-flipped boundary divergence (Q4) of update_ecm
+//                   (sim.py:2209-2254); with channels, also the gating rates
+//                   of every (channel, cell) from Vmem           c_env -> c_mid
+//   k_mem_cell      one or two wavefronts per tile of whole cells (<= 64
+//                   membranes), a lane per membrane for every ion (NaK pump,
+//                   GHK, GHK gap junction, Ca pump) and voltage-gated channel
+//                   (gate update, channel GHK); per cell the CSR segment sums from LDS,
 //                   update_Co + GJ update, no_negs clamp, rho_cells -> Vmem
 //                   (sim.py:1196-1283, 2086-2111, 2007-2038,
 //                   networks.py:3115-3215)                      cell A -> cell B
@@ -48,7 +50,12 @@ namespace {
 constexpr int TMB = BT_TILE_MEMBRANES;   // membranes per k_mem_cell tile (one wavefront)
 constexpr int BLOCK = 256;               // threads per per-square kernel block
 constexpr int TY = 16, TX = 16;          // k_env_field tile
-constexpr int ENV_SPLIT_TILES = 2048;    // below this many env tiles k_env_field splits blocks by ion
+#ifndef ENV_SPLIT_TILES
+#define ENV_SPLIT_TILES 2048               // below this many env tiles k_env_field splits blocks by ion
+#endif
+#ifndef ENV_IONS_PER_BLOCK
+#define ENV_IONS_PER_BLOCK 2               // ions per k_env_field block when it splits
+#endif
 constexpr int EX_SPLIT_SQUARES = 1 << 19; // below this many squares k_env_exchange splits by ion
 constexpr double FLOAT_NONCE = 1.0e-25;  // sim_toolbox.py:50
 
@@ -77,6 +84,9 @@ struct KParams {
     double qca_num;                 // cADP*cPi
     double cATP, km_ca, atp_kmca, one_p_atp_kmca;
     double gw[5];
+    // FP32-flux path: F/RT, F/RT_gj, 1/tm, 1/gj_len, 1/RT and the pumps'
+    // 1/Km rounded once to FP32
+    float f_rt32, f_rtgj32, inv_tm32, inv_gjlen32, inv_rt32, inv_kmna32, inv_kmk32, inv_kmca32;
 };
 
 // Per-cluster constants of a batch, derived on the host.
@@ -114,8 +124,8 @@ __device__ __forceinline__ bool halted(const unsigned* flags) {
 // ---------------------------------------------------------------------------
 // flux laws
 // ---------------------------------------------------------------------------
-// IEEE division in both paths (a reciprocal-based FP32 quotient was measured:
-// no gain, larger FP32-path error)
+// IEEE division by computed values in both paths (a reciprocal-based FP32
+// quotient of those was measured: no gain, larger FP32-path error)
 __device__ __forceinline__ double dv(double a, double b) { return a / b; }
 __device__ __forceinline__ float dv(float a, float b) { return a / b; }
 
@@ -128,10 +138,36 @@ __device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT d, FT zc, FT vBA, FT F
     return -dv(Dc * alpha, d) * dv(cB - cA * exp_alpha, deno);
 }
 
+// The same law on the FP32-flux path with the two constant quotients as
+// products by FP32 reciprocals (F/RT and 1/d): one division instead of three.
+__device__ __forceinline__ float ghk32(float cA, float cB, float Dc, float zc, float vBA, float f_rt, float inv_d) {
+    const float alpha = (zc * vBA) * f_rt;
+    const float exp_alpha = exp(-alpha);
+    const float deno = -expm1(-alpha);
+    return -((Dc * alpha) * inv_d) * dv(cB - cA * exp_alpha, deno);
+}
+
+// membrane / gap-junction / channel GHK of either path: d and RT as the
+// reference divides by them (FP64), or their FP32 reciprocals
+template <typename FT>
+__device__ __forceinline__ FT ghk_path(FT cA, FT cB, FT Dc, double d, float inv_d32, FT zc, FT vBA, double F,
+                                       double RT, float f_rt32) {
+    if constexpr (sizeof(FT) == 4) return ghk32(cA, cB, Dc, zc, vBA, f_rt32, inv_d32);
+    else return ghk<FT>(cA, cB, Dc, (FT)d, zc, vBA, (FT)F, (FT)RT);
+}
+
 // x**3: two multiplications (NumPy calls pow; the results differ by at most an
 // ulp, far inside the parity bars, and pow costs an order of magnitude more)
 __device__ __forceinline__ double cube(double x) { return (x * x) * x; }
 __device__ __forceinline__ float cube(float x) { return (x * x) * x; }
+
+// x / c with c a constant of the law: the quotient itself on the FP64 path,
+// the product by the FP32 reciprocal inv32 on the FP32-flux path
+template <typename FT>
+__device__ __forceinline__ FT dvc(FT x, double c, float inv32) {
+    if constexpr (sizeof(FT) == 4) return x * inv32;
+    else return dv(x, (FT)c);
+}
 
 // pumpNaKATP (sim_toolbox.py:71-122): returns f_Na; f_K = -(2/3)*f_Na.
 template <typename FT>
@@ -145,9 +181,9 @@ __device__ __forceinline__ FT pump_nak(FT cNai, FT cNao, FT cKi, FT cKo, FT Vm, 
     FT Qdenomo = (FT)P.qd_scal * cube(c) * (d * d);
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-15;
     FT Q = dv(Qnumo, Qdenomo);
-    FT Keq = exp(-((FT)P.dG_RT - dv((FT)P.F * Vm, (FT)P.RT)));
-    FT na = cube(dv(cNai, (FT)P.km_na));
-    FT kk = dv(cKo, (FT)P.km_k);
+    FT Keq = exp(-((FT)P.dG_RT - dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
+    FT na = cube(dvc<FT>(cNai, P.km_na, P.inv_kmna32));
+    FT kk = dvc<FT>(cKo, P.km_k, P.inv_kmk32);
     FT k2 = kk * kk;
     FT numo_E = na * k2 * (FT)P.atp_km;
     FT denomo_E = ((FT)1 + na) * ((FT)1 + k2) * (FT)P.one_p_atp_km;
@@ -162,9 +198,9 @@ __device__ __forceinline__ FT pump_ca(FT cCai, FT cCao, FT Vm, const KParams& P)
     FT Qdenomo = (FT)P.cATP * cCai;
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-16;
     FT Q = dv(Qnumo, Qdenomo);
-    FT Keq = exp(-((FT)P.dG_RT - (FT)2 * dv((FT)P.F * Vm, (FT)P.RT)));
-    FT numo_E = dv(cCai, (FT)P.km_ca) * (FT)P.atp_kmca;
-    FT denomo_E = ((FT)1 + dv(cCai, (FT)P.km_ca)) * (FT)P.one_p_atp_kmca;
+    FT Keq = exp(-((FT)P.dG_RT - (FT)2 * dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
+    FT numo_E = dvc<FT>(cCai, P.km_ca, P.inv_kmca32) * (FT)P.atp_kmca;
+    FT denomo_E = ((FT)1 + dvc<FT>(cCai, P.km_ca, P.inv_kmca32)) * (FT)P.one_p_atp_kmca;
     FT frwd = dv(numo_E, denomo_E);
     return (FT)P.ca_coef * frwd * ((FT)1 - dv(Q, Keq));
 }
@@ -362,14 +398,14 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             if (i == P.i_k) f = f + (double)((FT)P.k_per_na * fNa);
             // GHK across the membrane and across the gap junction (sim.py:1253-1269, 2162-2206)
             const double clag = g_lag[s];
-            const FT fED = ghk<FT>((FT)g_env[s], (FT)clag, (FT)g_dm[s], (FT)P.tm,
-                                   (FT)P.zplus[i], (FT)vmut, (FT)P.F, (FT)P.RT);
+            const FT fED = ghk_path<FT>((FT)g_env[s], (FT)clag, (FT)g_dm[s], P.tm, P.inv_tm32,
+                                        (FT)P.zplus[i], (FT)vmut, P.F, P.RT, P.f_rt32);
             f = f + (double)fED;
             double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
             if (!bnd) {
                 const double vBA = (vnmut - vmut) + FLOAT_NONCE;
-                fgj = (double)ghk<FT>((FT)clag, (FT)g_lagn[s], (FT)(P.dgj_surf[i] * gjo),
-                                      (FT)P.gj_len, (FT)P.zplus[i], (FT)vBA, (FT)P.F, (FT)P.RT_gj);
+                fgj = (double)ghk_path<FT>((FT)clag, (FT)g_lagn[s], (FT)(P.dgj_surf[i] * gjo), P.gj_len,
+                                           P.inv_gjlen32, (FT)P.zplus[i], (FT)vBA, P.F, P.RT_gj, P.f_rtgj32);
             }
             // Ca-ATPase after the ion loop: refreshed cell Ca, post-diffusion env Ca,
             // Vmem carrying all n_ions nonces (sim.py:2126-2159)
@@ -411,8 +447,8 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             const double Pch = (kind == BT_CHAN_NAV1P2) ? cube(mm) * hh
                              : (kind == BT_CHAN_KV1P1) ? mm * (hh * hh) : (mm * mm) * hh;
             const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * h_mod[q]);
-            const FT fc = ghk<FT>((FT)h_out[q], (FT)h_in[q], Dch, (FT)P.tm,
-                                  (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE), (FT)P.F, (FT)P.RT);
+            const FT fc = ghk_path<FT>((FT)h_out[q], (FT)h_in[q], Dch, P.tm, P.inv_tm32,
+                                       (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE), P.F, P.RT, P.f_rt32);
             const double x = (double)fc * sa;
             L.ch[ch][j] = x;
             A.fch[(size_t)ch * M + pos] = x;
@@ -538,7 +574,7 @@ k_env_field(FieldArgs A, KParams P) {
     double (&sFx)[TY][TX + 2] = *reinterpret_cast<double (*)[TY][TX + 2]>(s_pool + VR * VC + ER * EC);
     double (&sFy)[TY + 2][TX] =
         *reinterpret_cast<double (*)[TY + 2][TX]>(s_pool + VR * VC + ER * EC + TY * (TX + 2));
-    // (tile, ion) from the XCD-chunked linear block index, tiles fastest
+    // (tile, ion group) from the XCD-chunked linear block index, tiles fastest
     const int lin = xcd_chunk(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int tile_id = lin % gridDim.x, ion_id = lin / gridDim.x;
     const int* et = A.env_tiles + 3 * tile_id;
@@ -619,10 +655,11 @@ k_env_field(FieldArgs A, KParams P) {
         return;
     }
 
-    // with gridDim.y == n_ions each block takes one ion (and rebuilds its tile's
-    // field); with gridDim.y == 1 a block runs every ion over its tile
-    const int i_lo = (gridDim.y == 1) ? 0 : ion_id;
-    const int i_hi = (gridDim.y == 1) ? P.n_ions : ion_id + 1;
+    // block row ion_id takes the ions [ion_id * ipb, ion_id * ipb + ipb), with
+    // ipb = ceil(n_ions / gridDim.y), and rebuilds its tile's field for them
+    const int ipb = (P.n_ions + gridDim.y - 1) / gridDim.y;
+    const int i_lo = ion_id * ipb;
+    const int i_hi = min(i_lo + ipb, P.n_ions);
     for (int i = i_lo; i < i_hi; ++i) {
         const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
         const double* D = A.D_env + (size_t)i * P.n_sq + so;
@@ -1232,6 +1269,14 @@ static void derive_params(const bt_params& p, KParams& k) {
     k.dt = p.dt; k.tm = p.tm; k.gj_len = p.gj_len; k.F = p.F;
     k.RT = p.R * p.T; k.RT_gj = p.R * p.T_gj; k.q = p.q; k.kbT = p.kb * p.T;
     k.cm_inv = 1.0 / p.cm;
+    k.f_rt32 = (float)(p.F / (p.R * p.T));
+    k.f_rtgj32 = (float)(p.F / (p.R * p.T_gj));
+    k.inv_tm32 = (float)(1.0 / p.tm);
+    k.inv_gjlen32 = (float)(1.0 / p.gj_len);
+    k.inv_rt32 = (float)(1.0 / (p.R * p.T));
+    k.inv_kmna32 = (float)(1.0 / p.KmNK_Na);
+    k.inv_kmk32 = (float)(1.0 / p.KmNK_K);
+    k.inv_kmca32 = (float)(1.0 / p.KmCa_Ca);
     k.k_per_na = -(2.0 / 3.0);
     k.dG_RT = p.deltaGATP / (p.R * p.T);
     k.qn_scal = (p.cADP * 1e-3) * (p.cPi * 1e-3);
@@ -1304,10 +1349,12 @@ static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, co
     FieldArgs f{c->env_tiles, c->kcl, c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
                 ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
                 vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr};
-    // diffusion: one block per (tile, ion) while tiles alone would leave CUs idle,
-    // else one block per tile looping over the ions; field only: one block per tile
+    // diffusion: one block per (tile, ENV_IONS_PER_BLOCK ions) while tiles alone
+    // would leave CUs idle, else one block per tile looping over the ions; field
+    // only: one block per tile
     const bool split = diffuse && c->n_env_tiles < ENV_SPLIT_TILES;
-    dim3 grid(c->n_env_tiles, split ? c->prm.n_ions : 1);
+    const int ipb = ENV_IONS_PER_BLOCK;
+    dim3 grid(c->n_env_tiles, split ? (c->prm.n_ions + ipb - 1) / ipb : 1);
     if (diffuse && f.rates)
         hipLaunchKernelGGL((k_env_field<true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else if (diffuse)
