@@ -140,10 +140,21 @@ __device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT d, FT zc, FT vBA, FT F
 
 // The same law on the FP32-flux path with the two constant quotients as
 // products by FP32 reciprocals (F/RT and 1/d): one division instead of three.
+// One transcendental per evaluation: exp(-alpha) where |alpha| > 1/2 (then
+// |1 - exp(-alpha)| >= 0.39 loses nothing to cancellation), else expm1(-alpha)
+// and exp(-alpha) = 1 + expm1(-alpha) (then in [0.6, 1.7], so the sum is
+// within an ulp or two).
 __device__ __forceinline__ float ghk32(float cA, float cB, float Dc, float zc, float vBA, float f_rt, float inv_d) {
     const float alpha = (zc * vBA) * f_rt;
-    const float exp_alpha = exp(-alpha);
-    const float deno = -expm1(-alpha);
+    float exp_alpha, deno;
+    if (fabsf(alpha) > 0.5f) {
+        exp_alpha = exp(-alpha);
+        deno = 1.0f - exp_alpha;
+    } else {
+        const float em1 = expm1(-alpha);
+        exp_alpha = 1.0f + em1;
+        deno = -em1;
+    }
     return -((Dc * alpha) * inv_d) * dv(cB - cA * exp_alpha, deno);
 }
 
@@ -320,7 +331,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         divi = A.diviterm[c];
 #pragma unroll
         for (int s = 0; s < NS; ++s)
-            if (W + s * WG < NI) cin[s] = A.c_cell_in[(W + s * WG) * C + c];
+            if (W * NS + s < NI) cin[s] = A.c_cell_in[(W * NS + s) * C + c];
     }
 
     if (m < m1) {
@@ -336,7 +347,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         // the gathers of one ion slot / channel slot, issued together
         double g_env[NS], g_lag[NS], g_lagn[NS], g_dm[NS];
         auto gather_ion = [&](int s) {
-            const int i = W + s * WG;
+            const int i = W * NS + s;
             const double du = KC->dm_u[i];
             // the cluster's uniform Dm when it has one, else the membrane's
             const double* dmp = (du == du) ? &KC->dm_u[i] : &A.Dm[i * M + m];
@@ -361,7 +372,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             r_hi[q] = R[2 * C];
             r_ht[q] = R[3 * C];
         };
-        const bool do_nak = P.i_na >= 0 && (P.i_na % WG == W || P.i_k % WG == W);
+        const bool do_nak = P.i_na >= 0 && (P.i_na / NS == W || P.i_k / NS == W);
         double nai = 0.0, nao = 0.0, ki = 0.0, ko = 0.0, cai = 0.0, cao = 0.0;
         if (do_nak) {
             nai = A.c_lag_in[P.i_na * C + mc];
@@ -369,7 +380,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             ki = A.c_lag_in[P.i_k * C + mc];
             ko = A.c_env[P.i_k * G2 + k];
         }
-        if (P.i_ca >= 0 && P.i_ca % WG == W) {
+        if (P.i_ca >= 0 && P.i_ca / NS == W) {
             cai = A.c_cell_in[P.i_ca * C + mc];
             cao = A.c_mid[P.i_ca * G2 + k];
         }
@@ -378,20 +389,20 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         FT fNa = (FT)0;
         if (do_nak) {
             fNa = pump_nak<FT>((FT)nai, (FT)nao, (FT)ki, (FT)ko, (FT)v, nak_coef, P);
-            if (A.obs_rate && P.i_na % WG == W) A.obs_rate[m] = -(double)fNa;
+            if (A.obs_rate && P.i_na / NS == W) A.obs_rate[m] = -(double)fNa;
         }
         // Vmem and the neighbour's carrying the i + 1 nonces of ion i, built up
         // slot by slot in the same sequence of additions
-        double vmut = with_nonces(v, W + 1);
-        double vnmut = with_nonces(vn, W + 1);
+        double vmut = with_nonces(v, W * NS + 1);
+        double vnmut = with_nonces(vn, W * NS + 1);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const int i = W + s * WG;
+            const int i = W * NS + s;
             if (i >= NI) break;
             gather_ion(s);
             if (s > 0) {
-                vmut = with_nonces(vmut, WG);
-                vnmut = with_nonces(vnmut, WG);
+                vmut = with_nonces(vmut, 1);
+                vnmut = with_nonces(vnmut, 1);
             }
             double f = 0.0;                                  // fluxes_mem.fill(0)
             if (i == P.i_na) f = f + (double)fNa;
@@ -462,16 +473,20 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
     if (own_cell) {
         bool bad = false;
         double rho = 0.0;
+        // x / vol: the quotient on the FP64 path; on the FP32-flux path the
+        // product by 1/vol (an FP64 ulp, far below that path's FP32 fluxes)
+        const double rvol = 1.0 / vol;
+        auto per_vol = [&](double x) { return (sizeof(FT) == 4) ? x * rvol : x / vol; };
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const int i = W + s * WG;
+            const int i = W * NS + s;
             if (i >= NI) break;
             double base = cin[s];
             const int ch = P.ion_chan[i];
             if (ch >= 0) {             // immediate update_Co of the channel flux
                 double Sc = L.ch[ch][ca];
                 for (int q = ca + 1; q < cb; ++q) Sc = Sc + L.ch[ch][q];
-                base = base + (Sc / vol) * P.dt;
+                base = base + per_vol(Sc) * P.dt;
             }
             double S = L.mem[i][ca];
             double Sg = L.gj[i][ca];
@@ -479,9 +494,9 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
                 S = S + L.mem[i][q];
                 Sg = Sg + L.gj[i][q];
             }
-            const double c1v = base + (S / vol) * P.dt;
+            const double c1v = base + per_vol(S) * P.dt;
             A.c_lag_out[i * C + c] = c1v;
-            double c2 = c1v + P.dt * (Sg / vol);
+            double c2 = c1v + P.dt * per_vol(Sg);
             if (c2 < 0.0) c2 = 0.0;                    // no_negs clamp (Q9)
             if (isnan(c2)) bad = true;
             A.c_cell_out[i * C + c] = c2;
@@ -519,9 +534,11 @@ k_mem_cell(MemCellArgs A, KParams P) {
     static_assert(WG == 1 || WG == 2, "one or two wavefronts per tile");
     if (halted(A.flags)) return;
     // WG wavefronts per tile; lane j of each owns membrane m0 + j and, in the
-    // per-cell stage, cell c0 + j. Wavefront w evaluates the ions w, w + WG,
-    // w + 2 WG, ... (its ion slots) and the channels likewise, so small
-    // problems still fill the GPU. Both wavefronts pass the same barriers.
+    // per-cell stage, cell c0 + j. Wavefront w evaluates the ions
+    // [w * NS, w * NS + NS) with NS = ceil(n_ions / WG) (its ion slots; Na and
+    // K, the first two, share one pump evaluation) and the channels w, w + WG,
+    // ..., so small problems still fill the GPU. Both wavefronts pass the same
+    // barriers.
     __shared__ double s_mem[NI][TMB];
     __shared__ double s_gj[NI][TMB];
     __shared__ double s_ch[BT_MAX_CHAN][TMB];
