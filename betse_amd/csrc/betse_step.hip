@@ -357,7 +357,8 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             g_dm[s] = *dmp;
         };
         double h_m[NQ], h_h[NQ], h_mod[NQ], h_out[NQ], h_in[NQ];
-        double r_mi[NQ], r_mt[NQ], r_hi[NQ], r_ht[NQ];   // mInf, mTau, hInf, hTau
+        // mInf, mTau, hInf, hTau (FP64 path) or a_m, b_m, a_h, b_h (FP32-flux path)
+        double r_mi[NQ], r_mt[NQ], r_hi[NQ], r_ht[NQ];
         auto gather_chan = [&](int q) {
             const int ch = W + q * WG;
             const int i = P.chan_ion[ch];
@@ -449,9 +450,16 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             const int i = P.chan_ion[ch];
             const double vg = with_nonces(v, NI + ch);
             const int kind = P.chan_kind[ch];
-            const double dtm = P.chan_dt;
-            const double mm = (r_mt[q] * h_m[q] + dtm * r_mi[q]) / (r_mt[q] + dtm);
-            const double hh = (r_ht[q] * h_h[q] + dtm * r_hi[q]) / (r_ht[q] + dtm);
+            double mm, hh;
+            if constexpr (sizeof(FT) == 4) {
+                // the FP32-flux path reads the update coefficients a, b of each gate
+                mm = r_mi[q] * h_m[q] + r_mt[q];
+                hh = r_hi[q] * h_h[q] + r_ht[q];
+            } else {
+                const double dtm = P.chan_dt;
+                mm = (r_mt[q] * h_m[q] + dtm * r_mi[q]) / (r_mt[q] + dtm);
+                hh = (r_ht[q] * h_h[q] + dtm * r_hi[q]) / (r_ht[q] + dtm);
+            }
             A.gates[(size_t)(2 * ch) * M + m] = mm;
             A.gates[(size_t)(2 * ch + 1) * M + m] = hh;
             // m**p * h**q: p = 2 squares as NumPy does, p = 3 is cube()
@@ -565,6 +573,7 @@ struct FieldArgs {
     double* obs_fx; double* obs_fy;          // fluxes_env_x/y of the observed step, or NULL
     const double* vm;         // Vmem the step's k_mem_cell reads (diffuse mode)
     double* rates;            // [n_chan][4][n_cells] gating rates out (diffuse mode), or NULL
+    int gate_coef;            // 1: store the gate-update coefficients instead (FP32-flux path)
 };
 
 // RATES: also the gating rates (diffuse mode with channels)
@@ -789,10 +798,20 @@ k_env_field(FieldArgs A, KParams P) {
             double mInf, mTau, hInf, hTau;
             gate_rates(P.chan_kind[ch], V, mInf, mTau, hInf, hTau);
             double* R = A.rates + (size_t)(4 * ch) * nc + cell;
-            R[0] = mInf;
-            R[nc] = mTau;
-            R[2 * nc] = hInf;
-            R[3 * nc] = hTau;
+            if (A.gate_coef) {
+                // update_mh as x' = a x + b: a = tau/(tau+dt), b = dt*inf/(tau+dt)
+                // (FP64, an ulp from the quotient form; the FP32-flux path only)
+                const double dtm = P.chan_dt;
+                R[0] = mTau / (mTau + dtm);
+                R[nc] = (dtm * mInf) / (mTau + dtm);
+                R[2 * nc] = hTau / (hTau + dtm);
+                R[3 * nc] = (dtm * hInf) / (hTau + dtm);
+            } else {
+                R[0] = mInf;
+                R[nc] = mTau;
+                R[2 * nc] = hInf;
+                R[3 * nc] = hTau;
+            }
         }
     }
 }
@@ -1365,7 +1384,8 @@ static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, co
     const bt_buffers& B = c->buf;
     FieldArgs f{c->env_tiles, c->kcl, c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
                 ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
-                vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr};
+                vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr,
+                c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0};
     // diffusion: one block per (tile, ENV_IONS_PER_BLOCK ions) while tiles alone
     // would leave CUs idle, else one block per tile looping over the ions; field
     // only: one block per tile
