@@ -25,7 +25,8 @@
 //                   sparse form of M_divmap_mem2ecm) -> env update; then
 //                   rho_env -> u (sim_toolbox.py:1191-1229, ion_current.py:75-97)
 //                                                               c_mid -> c_env
-//   k_network       (when a reaction network is attached) one thread per cell:
+//   k_network       (when a reaction network is attached) a lane per cell in
+//                   each of four wavefronts that split the rates:
 //                   the substances' explicit Euler update (networks.py:2805-2982)
 //
 // Floating-point expressions keep the reference's operation order (the code is
@@ -945,7 +946,6 @@ k_cell_vm(const double* c_cell, const double* diviterm, double* vm, KParams P) {
 // ---------------------------------------------------------------------------
 // per-cell reaction network (MasterOfNetworks.run_loop, networks.py:2805-2982)
 // ---------------------------------------------------------------------------
-constexpr int NET_BLOCK = 128;
 
 // (c/Km)**n with numpy's power: exponents 1 and 2 take the exact fast paths
 __device__ __forceinline__ double hill_pow(double x, double n) {
@@ -964,26 +964,33 @@ __device__ __forceinline__ double hill_inh(double c, double km, double n) {
     return 1.0 / (1.0 + hill_pow(c / km, n));
 }
 
-// One thread per cell; the cell's concentrations and accumulated deltas live
-// in LDS so that the network's species indices can be runtime values.
-// Rates are evaluated from the pre-step concentrations in the reference's
-// order (growth/decay terms of every substance, then reactions) and each
-// rate's reaction_matrix column is added as it is produced, which sums
-// delta[s] over the rates in ascending order (np.dot(reaction_matrix, rates)).
+// A block holds NET_CELLS cells and NET_WAVES wavefronts: lane l of every
+// wavefront works on cell l, and wavefront w evaluates the rates w, w +
+// NET_WAVES, ... from the pre-step concentrations (rate-uniform per wavefront,
+// so the network description is read through scalar loads and no lane
+// diverges from its wavefront). After a barrier, wavefront w updates the
+// substances w, w + NET_WAVES, ...: delta[s] sums coeff * rate over the rates
+// in ascending order (np.dot(reaction_matrix, rates)) in the reference's
+// order (growth/decay terms of every substance, then reactions), then
+// c + delta*dt and no_negs.
+constexpr int NET_CELLS = 64;
+constexpr int NET_WAVES = 4;
+constexpr int NET_BLOCK = NET_CELLS * NET_WAVES;
+
 __global__ void __launch_bounds__(NET_BLOCK)
 k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cells, double dt) {
-    __shared__ double s_c[BT_NET_MAX_SPECIES][NET_BLOCK];
-    __shared__ double s_d[BT_NET_MAX_SPECIES][NET_BLOCK];
-    const int t = threadIdx.x;
-    const int cell = blockIdx.x * NET_BLOCK + t;
-    if (cell >= n_cells) return;
+    __shared__ double s_c[BT_NET_MAX_SPECIES][NET_CELLS];
+    __shared__ double s_r[BT_NET_MAX_RATES][NET_CELLS];
+    const int t = threadIdx.x % NET_CELLS;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / NET_CELLS));
+    const int cell = blockIdx.x * NET_CELLS + t;
+    const bool live = cell < n_cells;
     const int S = net->n_species;
-    for (int s = 0; s < S; ++s) {
-        s_c[s][t] = cn[(size_t)s * n_cells + cell];
-        s_d[s][t] = 0.0;
-    }
+    // cells past the end evaluate on ones and store nothing
+    for (int s = w; s < S; s += NET_WAVES) s_c[s][t] = live ? cn[(size_t)s * n_cells + cell] : 1.0;
+    __syncthreads();
     const int R = net->n_rates;
-    for (int j = 0; j < R; ++j) {
+    for (int j = w; j < R; j += NET_WAVES) {
         const bt_rate& r = net->rates[j];
         // alpha = (product of activator terms) * (product of inhibitor terms), each
         // product taken left to right; an empty product is np.ones
@@ -1028,13 +1035,18 @@ k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cel
             }
             rate = ((r.vmax * act) * inh) * x;
         }
-        for (int q = 0; q < r.n_stoich; ++q) {
-            const int s = r.stoich_species[q];
-            s_d[s][t] = s_d[s][t] + r.stoich_coeff[q] * rate;
-        }
+        s_r[j][t] = rate;
     }
-    for (int s = 0; s < S; ++s) {
-        double v = s_c[s][t] + s_d[s][t] * dt;
+    __syncthreads();
+    if (!live) return;
+    for (int s = w; s < S; s += NET_WAVES) {
+        double d = 0.0;
+        for (int j = 0; j < R; ++j) {
+            const bt_rate& r = net->rates[j];
+            for (int q = 0; q < r.n_stoich; ++q)
+                if (r.stoich_species[q] == s) d = d + r.stoich_coeff[q] * s_r[j][t];
+        }
+        double v = s_c[s][t] + d * dt;
         if (v < 0.0) v = 0.0;                    // stb.no_negs
         cn[(size_t)s * n_cells + cell] = v;
     }
@@ -1435,7 +1447,7 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     }
     if (which == 3) launch_field(c, false);
     if ((which < 0 || which == 4) && c->net && c->c_net)
-        hipLaunchKernelGGL(k_network, dim3((c->prm.n_cells + NET_BLOCK - 1) / NET_BLOCK), dim3(NET_BLOCK), 0,
+        hipLaunchKernelGGL(k_network, dim3((c->prm.n_cells + NET_CELLS - 1) / NET_CELLS), dim3(NET_BLOCK), 0,
                            c->stream, (const bt_network*)c->net, c->c_net, c->prm.n_cells, c->prm.dt);
 }
 

@@ -246,9 +246,9 @@ def test_many_tiles_single_wavefront_path():
 
 
 def test_channels_fp32_flux_parity():
-    """The mixed-precision path with Nav1p2 + Kv1p1 (FP32 fluxes and FP32
-    gating in cancellation-free form, FP64 state): |dVm| < 1 uV and gates
-    within 1e-5 of the FP64 oracle after init 150 + sim 100 steps."""
+    """The mixed-precision path with Nav1p2 + Kv1p1 (FP32 flux laws, FP64
+    gating and state): |dVm| < 1 uV and gates within 1e-5 of the FP64 oracle
+    after init 150 + sim 100 steps."""
     from betse_amd.channels import draw_channel_state
     p = Parameters.preset("tiny_channels")
     cells = Cells.make_world(p)
@@ -421,3 +421,22 @@ def test_sampled_observables(preset):
                                atol=1e-8 * np.abs(o.B_field).max())
     np.testing.assert_allclose(sim.Jtx.ravel(), o.Jtx.ravel(), rtol=0, atol=1e-8 * np.abs(o.Jtx).max())
     say(f"observables {preset}: ok over {len(o.storage)} sampled steps")
+
+
+def test_large_fp32_flux_full_horizon():
+    """The headline workload over its whole run: the large cluster (Nav1p2 +
+    Kv1p1, Ca) for 10,000 steps on the FP32-flux path stays within 1 uV of the
+    FP64 path (which the tests above hold to the oracle; the oracle itself
+    would take minutes here), and both stay finite and charge-consistent."""
+    from betse_amd.channels import draw_channel_state
+    p = Parameters.preset("large")
+    cells = Cells.make_world(p)
+    draw_channel_state(p, cells.n_cells, cells.n_mems)
+    say(f"case: large {cells.n_cells} cells, fp32 vs fp64, 10000 steps")
+    s64 = gpu_run(p, cells, 10000, flux_mode="fp64")
+    s32 = gpu_run(p, cells, 10000, flux_mode="fp32")
+    dv = float(np.abs(s32.vm - s64.vm).max())
+    dc = float((np.abs(s32.cc_cells - s64.cc_cells) / np.abs(s64.cc_cells).clip(1e-300)).max())
+    say(f"  fp32 vs fp64 after 10000 steps: |dVm|={dv:.3e} V rel dc={dc:.3e}")
+    assert np.isfinite(s64.vm).all() and np.isfinite(s32.vm).all()
+    assert dv < 1e-6
