@@ -1217,7 +1217,20 @@ k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
     const int tm = tile / tiles_n, tn = tile % tiles_n;
     const int row = tm * 16 + (lane % 16), col = tn * 16 + (lane % 16), kk = lane / 16;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < kd; k0 += 4) {
+    // eight k-steps of operands are loaded before their eight MFMAs, so the
+    // loads overlap instead of each step waiting on its own (same k order)
+    int k0 = 0;
+    for (; k0 + 32 <= kd; k0 += 32) {
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = A[(size_t)row * kd + k0 + 4 * u + kk];
+            b[u] = B[(size_t)(k0 + 4 * u + kk) * n + col];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+    }
+    for (; k0 < kd; k0 += 4) {
         const double a = A[(size_t)row * kd + k0 + kk];
         const double b = B[(size_t)(k0 + kk) * n + col];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);

@@ -291,7 +291,8 @@ int bt_gnn_adam(float* p, const float* g, float* m, float* v, int64_t count, flo
 int bt_sync(bt_ctx* ctx, int64_t* bad_step);
 
 /* Average device time in ms of one launch of a step kernel (0 membrane/cell,
- * 1 env field + diffusion, 2 env exchange, 3 env field only),
+ * 1 env field + diffusion, 2 env exchange, 3 env field only, 4 reaction
+ * network; 4 needs an attached network),
  * timed with HIP events over n_launches back-to-back launches on the context
  * stream. Leaves the state undefined: call after the timed work. */
 int bt_time_kernel(bt_ctx* ctx, int32_t kernel, int32_t n_launches, double* ms_out);
