@@ -577,8 +577,10 @@ struct FieldArgs {
     int gate_coef;            // 1: store the gate-update coefficients instead (FP32-flux path)
 };
 
-// RATES: also the gating rates (diffuse mode with channels)
-template <bool DIFFUSE, bool RATES>
+// RATES: also the gating rates (diffuse mode with channels). RECIP (the
+// FP32-flux path): divisions by the grid spacing and by kT as products by
+// their FP64 reciprocals (an FP64 ulp each; the FP64 path divides).
+template <bool DIFFUSE, bool RATES, bool RECIP>
 __global__ void __launch_bounds__(TY * TX)
 k_env_field(FieldArgs A, KParams P) {
     if (halted(A.flags)) return;
@@ -610,6 +612,9 @@ k_env_field(FieldArgs A, KParams P) {
     const int r0 = et[1], cc0 = et[2];
     const int tid = threadIdx.x;
     const double dl = K.delta, d2 = K.two_delta;
+    const double rdl = 1.0 / dl, rd2 = 1.0 / d2;
+    auto per_dl = [&](double x) { return RECIP ? x * rdl : x / dl; };
+    auto per_d2 = [&](double x) { return RECIP ? x * rd2 : x / d2; };
 
     // u with zero padding outside the grid (gaussian_filter mode='constant')
     // (each thread issues all of its loads, from clamped in-grid addresses,
@@ -659,12 +664,12 @@ k_env_field(FieldArgs A, KParams P) {
         if (r >= 0 && r < ny && cc >= 0 && cc < nx) {
             const int vr = er + 1, vc = ec + 1;         // (r, cc) in sV
             double gx, gy;
-            if (cc == 0) gx = (K.screen * sV[vr][vc + 1] - K.screen * sV[vr][vc]) / dl;
-            else if (cc == nx - 1) gx = (K.screen * sV[vr][vc] - K.screen * sV[vr][vc - 1]) / dl;
-            else gx = -(K.screen * sV[vr][vc - 1] - K.screen * sV[vr][vc + 1]) / d2;
-            if (r == 0) gy = (K.screen * sV[vr + 1][vc] - K.screen * sV[vr][vc]) / dl;
-            else if (r == ny - 1) gy = (K.screen * sV[vr][vc] - K.screen * sV[vr - 1][vc]) / dl;
-            else gy = -(K.screen * sV[vr - 1][vc] - K.screen * sV[vr + 1][vc]) / d2;
+            if (cc == 0) gx = per_dl((K.screen * sV[vr][vc + 1] - K.screen * sV[vr][vc]));
+            else if (cc == nx - 1) gx = per_dl((K.screen * sV[vr][vc] - K.screen * sV[vr][vc - 1]));
+            else gx = per_d2(-(K.screen * sV[vr][vc - 1] - K.screen * sV[vr][vc + 1]));
+            if (r == 0) gy = per_dl((K.screen * sV[vr + 1][vc] - K.screen * sV[vr][vc]));
+            else if (r == ny - 1) gy = per_dl((K.screen * sV[vr][vc] - K.screen * sV[vr - 1][vc]));
+            else gy = per_d2(-(K.screen * sV[vr - 1][vc] - K.screen * sV[vr + 1][vc]));
             ex = -gx;
             ey = -gy;
         }
@@ -692,6 +697,7 @@ k_env_field(FieldArgs A, KParams P) {
         const double* D = A.D_env + (size_t)i * P.n_sq + so;
         const double cb = P.c_bound[i];
         const double zi = P.zq[i];
+        const double zq_kt = (zi * P.q) / P.kbT;        // RECIP: alpha = D * zq_kt
         __syncthreads();
         // concentrations with the world boundary reset to c_env_bound (Q3)
         // (all loads first, from clamped in-grid addresses, then the LDS writes)
@@ -738,11 +744,11 @@ k_env_field(FieldArgs A, KParams P) {
             if (r < ny && cc >= 0 && cc < nx) {
                 const int sr = lr + 2, sc = lc + 1;            // in sC
                 double g;
-                if (cc == 0) g = (sC[sr][sc + 1] - sC[sr][sc]) / dl;
-                else if (cc == nx - 1) g = (sC[sr][sc] - sC[sr][sc - 1]) / dl;
-                else g = -(sC[sr][sc - 1] - sC[sr][sc + 1]) / d2;
+                if (cc == 0) g = per_dl((sC[sr][sc + 1] - sC[sr][sc]));
+                else if (cc == nx - 1) g = per_dl((sC[sr][sc] - sC[sr][sc - 1]));
+                else g = per_d2(-(sC[sr][sc - 1] - sC[sr][sc + 1]));
                 const double Dv = sD[lr + 1][lc];
-                const double alpha = (Dv * zi * P.q) / P.kbT;
+                const double alpha = RECIP ? Dv * zq_kt : (Dv * zi * P.q) / P.kbT;
                 fx = (-Dv) * g - (alpha * (-sEx[lr + 1][lc])) * sC[sr][sc];
             }
             sFx[lr][lc] = fx;
@@ -754,11 +760,11 @@ k_env_field(FieldArgs A, KParams P) {
             if (r >= 0 && r < ny && cc < nx) {
                 const int sr = lr + 1, sc = lc + 2;
                 double g;
-                if (r == 0) g = (sC[sr + 1][sc] - sC[sr][sc]) / dl;
-                else if (r == ny - 1) g = (sC[sr][sc] - sC[sr - 1][sc]) / dl;
-                else g = -(sC[sr - 1][sc] - sC[sr + 1][sc]) / d2;
+                if (r == 0) g = per_dl((sC[sr + 1][sc] - sC[sr][sc]));
+                else if (r == ny - 1) g = per_dl((sC[sr][sc] - sC[sr - 1][sc]));
+                else g = per_d2(-(sC[sr - 1][sc] - sC[sr + 1][sc]));
                 const double Dv = sD[lr][lc + 1];
-                const double alpha = (Dv * zi * P.q) / P.kbT;
+                const double alpha = RECIP ? Dv * zq_kt : (Dv * zi * P.q) / P.kbT;
                 fy = (-Dv) * g - (alpha * (-sEy[lr][lc + 1])) * sC[sr][sc];
             }
             sFy[lr][lc] = fy;
@@ -770,12 +776,12 @@ k_env_field(FieldArgs A, KParams P) {
             const int r = r0 + lr, cc = cc0 + lc;
             if (r < ny && cc < nx) {
                 double gx, gy;
-                if (cc == 0) gx = ((-sFx[lr][lc + 1]) - (-sFx[lr][lc + 2])) / dl;
-                else if (cc == nx - 1) gx = ((-sFx[lr][lc]) - (-sFx[lr][lc + 1])) / dl;
-                else gx = -((-sFx[lr][lc]) - (-sFx[lr][lc + 2])) / d2;
-                if (r == 0) gy = -((-sFy[lr + 2][lc]) - (-sFy[lr + 1][lc])) / dl;
-                else if (r == ny - 1) gy = -((-sFy[lr + 1][lc]) - (-sFy[lr][lc])) / dl;
-                else gy = -((-sFy[lr][lc]) - (-sFy[lr + 2][lc])) / d2;
+                if (cc == 0) gx = per_dl(((-sFx[lr][lc + 1]) - (-sFx[lr][lc + 2])));
+                else if (cc == nx - 1) gx = per_dl(((-sFx[lr][lc]) - (-sFx[lr][lc + 1])));
+                else gx = per_d2(-((-sFx[lr][lc]) - (-sFx[lr][lc + 2])));
+                if (r == 0) gy = per_dl(-((-sFy[lr + 2][lc]) - (-sFy[lr + 1][lc])));
+                else if (r == ny - 1) gy = per_dl(-((-sFy[lr + 1][lc]) - (-sFy[lr][lc])));
+                else gy = per_d2(-((-sFy[lr][lc]) - (-sFy[lr + 2][lc])));
                 const double div = gx + gy;
                 A.c_mid[(size_t)i * P.n_sq + so + r * nx + cc] = sC[lr + 2][lc + 2] + div * P.dt;
                 if (A.obs_fx) {
@@ -1417,12 +1423,17 @@ static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, co
     const bool split = diffuse && c->n_env_tiles < ENV_SPLIT_TILES;
     const int ipb = ENV_IONS_PER_BLOCK;
     dim3 grid(c->n_env_tiles, split ? (c->prm.n_ions + ipb - 1) / ipb : 1);
-    if (diffuse && f.rates)
-        hipLaunchKernelGGL((k_env_field<true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+    const bool recip = c->prm.flux_mode == BT_FLUX_FP32;
+    if (diffuse && f.rates && recip)
+        hipLaunchKernelGGL((k_env_field<true, true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+    else if (diffuse && f.rates)
+        hipLaunchKernelGGL((k_env_field<true, true, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+    else if (diffuse && recip)
+        hipLaunchKernelGGL((k_env_field<true, false, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else if (diffuse)
-        hipLaunchKernelGGL((k_env_field<true, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<true, false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else
-        hipLaunchKernelGGL((k_env_field<false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<false, false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
 }
 
 // One step. parity 0: cell state caller -> workspace; parity 1: workspace -> caller.

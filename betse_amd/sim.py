@@ -502,10 +502,46 @@ class Simulator(HostState):
                     pinned[k] = h
                 h.copy_(t, non_blocking=True)
 
+    def _fetch_start_async(self, tensors: dict) -> None:
+        """Sampled steps: snapshot the tensors on the simulation stream (device
+        copies), then copy the snapshots to pinned host memory on a side stream
+        (default priority), so the host transfer overlaps the next steps.
+        ``_fetch_finish`` waits for it. The snapshots and pinned buffers are
+        reused: a caller finishes one fetch before it starts the next."""
+        torch = self.torch
+        snaps = self.__dict__.setdefault("_snaps", {})
+        pinned = self.__dict__.setdefault("_pinned", {})
+        if self.__dict__.get("_copy_stream") is None:
+            self._copy_stream = torch.cuda.Stream(device=self.device)     # default priority
+        with torch.cuda.device(self.device):
+            with torch.cuda.stream(self.stream):
+                for k, t in tensors.items():
+                    sn = snaps.get(k)
+                    if sn is None or sn.shape != t.shape or sn.dtype != t.dtype:
+                        sn = torch.empty_like(t)
+                        snaps[k] = sn
+                    sn.copy_(t, non_blocking=True)
+                taken = torch.cuda.Event()
+                taken.record(self.stream)
+            with torch.cuda.stream(self._copy_stream):
+                self._copy_stream.wait_event(taken)
+                for k in tensors:
+                    sn, h = snaps[k], pinned.get(k)
+                    if h is None or h.shape != sn.shape or h.dtype != sn.dtype:
+                        h = torch.empty(sn.shape, dtype=sn.dtype, pin_memory=True)
+                        pinned[k] = h
+                    h.copy_(sn, non_blocking=True)
+                self._copied = torch.cuda.Event()
+                self._copied.record(self._copy_stream)
+
     def _fetch_finish(self, keys) -> dict:
-        """Host arrays of pinned copies that have completed."""
+        """Host arrays of pinned copies that have completed (torch's CPU clone
+        copies with all host threads; the pinned buffers are reused)."""
+        copied = self.__dict__.pop("_copied", None)
+        if copied is not None:
+            copied.synchronize()
         pinned = self._pinned
-        return {k: pinned[k].numpy().copy() for k in keys}
+        return {k: pinned[k].clone().numpy() for k in keys}
 
     def _fetch(self, tensors: dict) -> dict:
         """Device tensors -> host arrays through cached pinned buffers: all copies
@@ -712,10 +748,10 @@ class Simulator(HostState):
                 finish(pending)
                 pending = None
             self._sync(step0)
-            # observables of the sampled step, then one batch of copies to pinned host memory
+            # observables of the sampled step, then one batch of copies to pinned host
+            # memory that runs beside the next steps
             self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
-            self._fetch_start({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in self._OBS_KEEP}})
-            self.stream.synchronize()
+            self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in self._OBS_KEEP}})
             pending = time_steps[s]
         if pos < len(time_steps):
             self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, 1 if self.use_graph else 0))
