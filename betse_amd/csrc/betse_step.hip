@@ -303,7 +303,7 @@ struct MemLds {
 // The work of wavefront W of a k_mem_cell block. W is a template argument so
 // that every ion index below is a compile-time constant (the per-ion kernel
 // parameters then load once, up front).
-template <int NI, typename FT, int WG, int W>
+template <int NI, typename FT, int WG, int W, bool OBS>
 __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParams& P, int tile, const MemLds& L) {
     constexpr int NS = (NI + WG - 1) / WG;             // ion slots per wavefront
     constexpr int NQ = (BT_MAX_CHAN + WG - 1) / WG;    // channel slots per wavefront
@@ -391,7 +391,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         FT fNa = (FT)0;
         if (do_nak) {
             fNa = pump_nak<FT>((FT)nai, (FT)nao, (FT)ki, (FT)ko, (FT)v, nak_coef, P);
-            if (A.obs_rate && P.i_na / NS == W) A.obs_rate[m] = -(double)fNa;
+            if (OBS && P.i_na / NS == W) A.obs_rate[m] = -(double)fNa;
         }
         // Vmem and the neighbour's carrying the i + 1 nonces of ion i, built up
         // slot by slot in the same sequence of additions
@@ -427,7 +427,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
                 const double cao0 = (cao < 0.0) ? 0.0 : cao;
                 f = f + (double)pump_ca<FT>((FT)cai0, (FT)cao0, (FT)with_nonces(v, NI), P);
             }
-            if (A.obs_fm) {
+            if (OBS) {
                 A.obs_fm[(size_t)i * M + m] = f;
                 A.obs_fgj[(size_t)i * M + m] = fgj;
             }
@@ -472,7 +472,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             const double x = (double)fc * sa;
             L.ch[ch][j] = x;
             A.fch[(size_t)ch * M + pos] = x;
-            if (A.obs_fch) A.obs_fch[(size_t)ch * M + m] = (double)fc;
+            if (OBS) A.obs_fch[(size_t)ch * M + m] = (double)fc;
         }
     }
     __syncthreads();
@@ -537,7 +537,8 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
     }
 }
 
-template <int NI, typename FT, int WG>
+// OBS: an observed step (the raw fluxes also go to the observation buffers)
+template <int NI, typename FT, int WG, bool OBS>
 __global__ void __launch_bounds__(TMB * WG)
 k_mem_cell(MemCellArgs A, KParams P) {
     static_assert(WG == 1 || WG == 2, "one or two wavefronts per tile");
@@ -556,8 +557,8 @@ k_mem_cell(MemCellArgs A, KParams P) {
     const MemLds L{s_mem, s_gj, s_ch, s_c2, &s_bad};
     const int tile = xcd_chunk(blockIdx.x, gridDim.x);
     const int w = (WG > 1) ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TMB)) : 0;
-    if (w == 0) mem_cell_wave<NI, FT, WG, 0>(A, P, tile, L);
-    else mem_cell_wave<NI, FT, WG, WG - 1>(A, P, tile, L);
+    if (w == 0) mem_cell_wave<NI, FT, WG, 0, OBS>(A, P, tile, L);
+    else mem_cell_wave<NI, FT, WG, WG - 1, OBS>(A, P, tile, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1381,12 +1382,18 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
 #define BT_MEM_WG_TILES 4096
 #endif
 
+template <int NI, int WG, bool OBS>
+static void launch_mem_cell_o(bt_ctx* c, const MemCellArgs& a) {
+    if (c->prm.flux_mode == BT_FLUX_FP32)
+        hipLaunchKernelGGL((k_mem_cell<NI, float, WG, OBS>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a, c->kp);
+    else
+        hipLaunchKernelGGL((k_mem_cell<NI, double, WG, OBS>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a,
+                           c->kp);
+}
 template <int NI, int WG>
 static void launch_mem_cell_w(bt_ctx* c, const MemCellArgs& a) {
-    if (c->prm.flux_mode == BT_FLUX_FP32)
-        hipLaunchKernelGGL((k_mem_cell<NI, float, WG>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a, c->kp);
-    else
-        hipLaunchKernelGGL((k_mem_cell<NI, double, WG>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a, c->kp);
+    if (a.obs_fm) launch_mem_cell_o<NI, WG, true>(c, a);
+    else launch_mem_cell_o<NI, WG, false>(c, a);
 }
 template <int NI>
 static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a) {
