@@ -29,6 +29,12 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
+def _squares_with_membranes(cells) -> int:
+    if hasattr(cells, "n_live"):
+        return int(cells.n_live)
+    return int(np.count_nonzero(np.diff(cells.sq_ptr)))
+
+
 def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0, n_sub: int = 0,
                       dm_loaded: int = None, gj_loaded: bool = True) -> dict:
     """Compulsory bytes per launch of each step kernel (each array element the
@@ -50,7 +56,10 @@ def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 
                      + n_in * 8 * (ni + (1 if ca else 0) + n_ch))  # env gathers (pre- and post-diffusion)
     b["env_field"] = (G2 * (8 + 8 * ni + 8 * ni + 8 * ni)   # u r, c_env r, D_env r, c_mid w
                       + (C * (8 + 32 * n_ch) if n_ch else 0))  # vm r, gating rates w
-    b["env_exchange"] = G2 * (8 * ni + 8 * ni + 8 + 1 + 4) + M * 8 * (ni + n_ch)
+    # the exchange runs over the squares with membranes (the others take their
+    # new state from k_env_field): c_mid r, c_env w, u w, has_mem, sq_cluster, sq_ptr, list
+    n_live = _squares_with_membranes(cells)
+    b["env_exchange"] = n_live * (8 * ni + 8 * ni + 8 + 1 + 4 + 4 + 4) + M * 8 * (ni + n_ch)
     if n_sub:
         b["network"] = C * n_sub * 16                          # substances read and written
     b["step"] = sum(b.values())
@@ -64,6 +73,7 @@ class _Tot:
         self.n_cells = sum(c.n_cells for _, c in worlds)
         self.n_mems = sum(c.n_mems for _, c in worlds)
         self.envInds_inClust = range(sum(len(c.envInds_inClust) for _, c in worlds))
+        self.n_live = sum(int(np.count_nonzero(np.diff(c.sq_ptr))) for _, c in worlds)
 
         class _X:
             size = sum(c.X.size for _, c in worlds)

@@ -38,7 +38,9 @@ def test_algorithmic_bytes_counts_each_array_once():
     mem += C * (8 * ni * 4 + 16 + 16 + 4) + n_in * 8 * ni
     assert b["mem_cell"] == mem
     assert b["env_field"] == G2 * (8 + 24 * ni)
-    assert b["env_exchange"] == G2 * (16 * ni + 13) + M * 8 * ni
+    n_live = int(np.count_nonzero(np.diff(cells.sq_ptr)))        # squares with membranes
+    assert 0 < n_live < G2
+    assert b["env_exchange"] == n_live * (16 * ni + 21) + M * 8 * ni
     assert b["step"] == b["mem_cell"] + b["env_field"] + b["env_exchange"]
     # channels add the gates (read and write), the channel fluxes, the per-cell
     # multipliers and rates, and the rates the env kernel writes from Vmem
