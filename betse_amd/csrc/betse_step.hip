@@ -12,7 +12,9 @@
 //                   ion's boundary reset (Q3), gradient, Nernst-Planck flux and
 //                   sign-flipped boundary divergence (Q4) of update_ecm
 //                   (sim.py:2209-2254); with channels, also the gating rates
-//                   of every (channel, cell) from Vmem           c_env -> c_mid
+//                   of every (channel, cell) from Vmem
+//                                       c_env -> c_mid (squares with membranes)
+//                                       c_env -> next c_env (the other squares)
 //   k_mem_cell      one or two wavefronts per tile of whole cells (<= 64
 //                   membranes), a lane per membrane for every ion (NaK pump,
 //                   GHK, GHK gap junction, Ca pump) and voltage-gated channel
@@ -24,10 +26,13 @@
 //                   fluxes (written by k_mem_cell in square-sorted order, the
 //                   sparse form of M_divmap_mem2ecm) -> env update; then
 //                   rho_env -> u (sim_toolbox.py:1191-1229, ion_current.py:75-97)
-//                                                               c_mid -> c_env
+//                   over the squares with membranes only      c_mid -> next c_env
 //   k_network       (when a reaction network is attached) a lane per cell in
 //                   each of four wavefronts that split the rates:
 //                   the substances' explicit Euler update (networks.py:2805-2982)
+//
+// The env state ping-pongs between the caller's c_env and a workspace copy, like
+// the cell state, since k_env_field reads neighbours that other blocks update.
 //
 // Floating-point expressions keep the reference's operation order (the code is
 // compiled with -ffp-contract=off) so the FP64 path tracks the NumPy reference
@@ -576,6 +581,11 @@ struct FieldArgs {
     const double* vm;         // Vmem the step's k_mem_cell reads (diffuse mode)
     double* rates;            // [n_chan][4][n_cells] gating rates out (diffuse mode), or NULL
     int gate_coef;            // 1: store the gate-update coefficients instead (FP32-flux path)
+    // diffuse mode: a square without membranes has no exchange, so its
+    // post-diffusion value is already the step's new env state and goes to
+    // c_env_out; squares with membranes go to c_mid for k_env_exchange
+    const uint8_t* live;      // per square: 1 if k_env_exchange updates it
+    double* c_env_out;        // the step's env state buffer (never the one read here)
 };
 
 // RATES: also the gating rates (diffuse mode with channels). RECIP (the
@@ -693,6 +703,12 @@ k_env_field(FieldArgs A, KParams P) {
     const int ipb = (P.n_ions + gridDim.y - 1) / gridDim.y;
     const int i_lo = ion_id * ipb;
     const int i_hi = min(i_lo + ipb, P.n_ions);
+    // whether this thread's square goes to the exchange (c_mid) or is final (c_env_out)
+    bool out_live = false;
+    {
+        const int r = r0 + tid / TX, cc = cc0 + tid % TX;
+        if (r < ny && cc < nx) out_live = A.live[so + r * nx + cc] != 0;
+    }
     for (int i = i_lo; i < i_hi; ++i) {
         const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
         const double* D = A.D_env + (size_t)i * P.n_sq + so;
@@ -784,7 +800,10 @@ k_env_field(FieldArgs A, KParams P) {
                 else if (r == ny - 1) gy = per_dl(-((-sFy[lr + 1][lc]) - (-sFy[lr][lc])));
                 else gy = per_d2(-((-sFy[lr][lc]) - (-sFy[lr + 2][lc])));
                 const double div = gx + gy;
-                A.c_mid[(size_t)i * P.n_sq + so + r * nx + cc] = sC[lr + 2][lc + 2] + div * P.dt;
+                const size_t e = (size_t)i * P.n_sq + so + r * nx + cc;
+                const double cn = sC[lr + 2][lc + 2] + div * P.dt;
+                if (out_live) A.c_mid[e] = cn;
+                else A.c_env_out[e] = cn;
                 if (A.obs_fx) {
                     A.obs_fx[(size_t)i * P.n_sq + so + r * nx + cc] = sFx[lr][lc + 1];
                     A.obs_fy[(size_t)i * P.n_sq + so + r * nx + cc] = sFy[lr + 1][lc];
@@ -831,6 +850,9 @@ struct ExArgs {
     const int32_t* sq_ptr; const uint8_t* has_mem; const int32_t* sq_cluster; const KCluster* kcl;
     const double* c_mid; const double* fsa; const double* fch; double* c_env; double* u;
     const unsigned* flags; long long* step_ctr; int count_step;
+    // the squares the exchange updates (membranes or a surface charge), ascending;
+    // the others keep u = 0 and take their env state from k_env_field
+    const int32_t* sq_live; int n_live;
 };
 
 // sum of -x[q] over q in [a, b), added in order from a; the loads are issued
@@ -859,9 +881,10 @@ k_env_exchange(ExArgs A, KParams P) {
     if (halted(A.flags)) return;
     __shared__ double s_cn[BT_MAX_IONS][EXQ];
     const int i = threadIdx.x / EXQ, j = threadIdx.x % EXQ;
-    const int k = blockIdx.x * EXQ + j;
+    const int q = blockIdx.x * EXQ + j;
     if (A.count_step && blockIdx.x == 0 && threadIdx.x == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
-    const bool live = (k < P.n_sq);
+    const bool live = (q < A.n_live);
+    const int k = live ? A.sq_live[q] : 0;
     double surf_scale = 0.0, mean_sa = 1.0, kee = 1.0;
     bool hm = false;
     if (live) {
@@ -899,9 +922,10 @@ k_env_exchange(ExArgs A, KParams P) {
 __global__ void __launch_bounds__(BLOCK)
 k_env_exchange_sq(ExArgs A, KParams P) {
     if (halted(A.flags)) return;
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (A.count_step && k == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
-    if (k >= P.n_sq) return;
+    const int q = blockIdx.x * BLOCK + threadIdx.x;
+    if (A.count_step && q == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
+    if (q >= A.n_live) return;
+    const int k = A.sq_live[q];
     const int a = A.sq_ptr[k], b = A.sq_ptr[k + 1];
     const KCluster& K = A.kcl[A.sq_cluster[k]];
     double rho = 0.0;
@@ -1284,6 +1308,10 @@ struct bt_ctx {
     double* c_cell_b = nullptr; double* c_lag_b = nullptr; double* vm_b = nullptr;
     double* c_mid = nullptr; double* fsa = nullptr; double* fch = nullptr; double* u = nullptr;
     double* rates = nullptr;  // [n_chan][4][n_cells] gating rates of the current step
+    double* c_env_b = nullptr;           // env state of odd steps
+    int32_t* sq_live = nullptr;          // squares the exchange updates, ascending
+    uint8_t* live_flag = nullptr;        // per square: 1 if in sq_live
+    int n_live = 0;
     unsigned* flags = nullptr; unsigned long long* bad_step = nullptr; long long* step_ctr = nullptr;
     hipGraph_t graph = nullptr; hipGraphExec_t gexec = nullptr;
     long long steps_done = 0;
@@ -1418,12 +1446,13 @@ static inline int grid_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 // diffuse: the step's env field and diffusion, and the gating rates from vm;
 // else the field alone (v_env, E) from u
-static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, const double* vm = nullptr) {
+static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, const double* vm = nullptr,
+                         const double* env_in = nullptr, double* env_out = nullptr) {
     const bt_buffers& B = c->buf;
-    FieldArgs f{c->env_tiles, c->kcl, c->u, B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y, c->flags,
-                ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
+    FieldArgs f{c->env_tiles, c->kcl, c->u, env_in ? env_in : B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y,
+                c->flags, ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
                 vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr,
-                c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0};
+                c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0, c->live_flag, env_out};
     // diffusion: one block per (tile, ENV_IONS_PER_BLOCK ions) while tiles alone
     // would leave CUs idle, else one block per tile looping over the ions; field
     // only: one block per tile
@@ -1454,13 +1483,15 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     double* co = parity ? B.c_cell : c->c_cell_b;
     double* lo = parity ? B.c_lag : c->c_lag_b;
     double* vo = parity ? B.vm_cell : c->vm_b;
+    double* ei = parity ? c->c_env_b : B.c_env;     // env state in and out, ping-pong like the cells
+    double* eo = parity ? B.c_env : c->c_env_b;
 
     const bt_obs* ob = c->capture;
-    if (which < 0 || which == 1) launch_field(c, true, ob, vi);
+    if (which < 0 || which == 1) launch_field(c, true, ob, vi, ei, eo);
     if (which < 0 || which == 0) {
         MemCellArgs a{B.mem_ptr, B.mem_cell, B.nn_mem, B.nn_cell, B.mem_sq, B.mem_pos, B.tile_ptr,
                       B.tile_cluster, c->kcl,
-                      B.mem_sa, B.gjopen, B.cell_vol, B.diviterm, B.Dm, B.c_env, c->c_mid,
+                      B.mem_sa, B.gjopen, B.cell_vol, B.diviterm, B.Dm, ei, c->c_mid,
                       ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->rates, c->fch,
                       c->flags, c->bad_step, c->step_ctr,
                       ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
@@ -1468,13 +1499,14 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
         launch_mem_cell(c, a);
     }
     if (which < 0 || which == 2) {
-        ExArgs x{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, B.c_env, c->u, c->flags,
-                 c->step_ctr, which < 0 ? 1 : 0};
+        ExArgs x{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u, c->flags,
+                 c->step_ctr, which < 0 ? 1 : 0, c->sq_live, c->n_live};
+        const int nl = c->n_live > 0 ? c->n_live : 1;    // block 0 also counts the step
         if (nsq < EX_SPLIT_SQUARES)
-            hipLaunchKernelGGL(k_env_exchange, dim3((nsq + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0,
+            hipLaunchKernelGGL(k_env_exchange, dim3((nl + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0,
                                c->stream, x, c->kp);
         else
-            hipLaunchKernelGGL(k_env_exchange_sq, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream, x, c->kp);
+            hipLaunchKernelGGL(k_env_exchange_sq, dim3(grid_blocks(nl)), dim3(BLOCK), 0, c->stream, x, c->kp);
     }
     if (which == 3) launch_field(c, false);
     if ((which < 0 || which == 4) && c->net && c->c_net)
@@ -1488,6 +1520,8 @@ static int copy_back(bt_ctx* c) {
     HIP_TRY(c, hipMemcpyAsync(c->buf.c_cell, c->c_cell_b, ni * nc * 8, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->buf.c_lag, c->c_lag_b, ni * nc * 8, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->buf.vm_cell, c->vm_b, nc * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->buf.c_env, c->c_env_b, ni * (size_t)c->kp.n_sq * 8, hipMemcpyDeviceToDevice,
+                              c->stream));
     return BT_OK;
 }
 
@@ -1560,6 +1594,26 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
         for (int t = 0; t < b->n_tiles; ++t)
             if (tcl[t] < 0 || tcl[t] >= n_clusters) return set_err(nullptr, BT_EINVAL, "tile_cluster out of range");
     }
+    // the squares k_env_exchange updates: those with membranes or a surface
+    // charge (the others' exchange is the identity and their u stays 0)
+    std::vector<int32_t> sq_live;
+    std::vector<uint8_t> live_flag((size_t)p->n_squares, 0);
+    {
+        std::vector<int32_t> sp((size_t)p->n_squares + 1);
+        std::vector<uint8_t> hm((size_t)p->n_squares);
+        hipError_t e = hipMemcpy(sp.data(), b->sq_ptr, sp.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hm.data(), b->sq_has_mem, hm.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return set_err(nullptr, BT_EHIP, std::string("reading squares: ") + hipGetErrorString(e));
+        if (sp[0] != 0 || sp[p->n_squares] != p->n_mems)
+            return set_err(nullptr, BT_EINVAL, "sq_ptr does not cover the membranes");
+        for (int k = 0; k < p->n_squares; ++k) {
+            if (sp[k + 1] < sp[k]) return set_err(nullptr, BT_EINVAL, "sq_ptr decreases");
+            if (sp[k + 1] > sp[k] || hm[k]) {
+                sq_live.push_back(k);
+                live_flag[k] = 1;
+            }
+        }
+    }
 
     bt_ctx* c = new bt_ctx();
     c->prm = *p;
@@ -1626,6 +1680,13 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     alloc((void**)&c->step_ctr, 256);
     alloc((void**)&c->kcl, sizeof(KCluster) * (size_t)n_clusters);
     alloc((void**)&c->env_tiles, sizeof(int) * tiles.size());
+    alloc((void**)&c->c_env_b, ni * ns * 8);
+    alloc((void**)&c->sq_live, sizeof(int32_t) * (sq_live.empty() ? 1 : sq_live.size()));
+    alloc((void**)&c->live_flag, ns);
+    c->n_live = (int)sq_live.size();
+    if (e == hipSuccess && !sq_live.empty())
+        e = hipMemcpy(c->sq_live, sq_live.data(), sizeof(int32_t) * sq_live.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->live_flag, live_flag.data(), ns, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c->kcl, c->kcl_host.data(), sizeof(KCluster) * (size_t)n_clusters, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->env_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice);
@@ -1650,8 +1711,8 @@ int bt_destroy(bt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
-    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates,
-                    c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
+    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
+                    c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
                     c->B1, c->B2, c->T1, c->T2, c->AA, c->BB, c->vm_prev, c->net};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
