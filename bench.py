@@ -170,7 +170,8 @@ def bench_batched(a):
                                f"per-cluster Dm and NaK rate over one decade, dt {p0.dt:g} s",
                    "clusters": len(worlds), "cells": C, "flux_mode": best},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "launch_us": kern[dom] * 1e3,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batched", dom, best),
+                     "launch_us": kern[dom] * 1e3,
                      "algorithmic_bytes": ab[dom], "step_algorithmic_bytes": ab["step"],
                      "step_achieved_GBs": ab["step"] / (ms_step * 1e-3) / 1e9},
         "cpu_baseline": cpu, "parity_prefix": parity,
