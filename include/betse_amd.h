@@ -174,12 +174,17 @@ typedef struct {
     bt_rate rates[BT_NET_MAX_RATES];
 } bt_network;
 
-/* Create a context: validates sizes and tiles, copies the parameter block,
- * allocates the workspace (second cell-state buffer, post-diffusion env buffer,
- * per-membrane exchange fluxes, surface-charge field, status words) on the
- * current device. Dm and gjopen are read once here (a cluster whose values are
- * uniform gets them as constants); they do not change within a phase on this
- * path, and a caller that changes them creates a new context.
+/* Create a context: validates sizes, tiles and the grid-side CSR, copies the
+ * parameter block, allocates the workspace (second cell-state and env-state
+ * buffers, post-diffusion env buffer, per-membrane exchange fluxes, per-cell
+ * gating rates, surface-charge field, the list of squares with membranes,
+ * status words) on the current device. It reads tile_ptr, mem_ptr,
+ * tile_cluster, sq_ptr, sq_has_mem, Dm and gjopen with blocking copies, so
+ * their contents must be complete when it is called. Dm and gjopen are read
+ * once here (a cluster whose values are uniform gets them as constants); they
+ * do not change within a phase on this path, and a caller that changes them
+ * creates a new context. Between calls the caller's state buffers hold the
+ * state (the workspace copies are internal).
  * `stream` is a hipStream_t (may be NULL for the default stream; graphs need a
  * non-default stream). Replaces the state set-up of init_core/init_dynamics
  * (sim.py:452-1012) on the device side. */
