@@ -440,3 +440,18 @@ def test_large_fp32_flux_full_horizon():
     say(f"  fp32 vs fp64 after 10000 steps: |dVm|={dv:.3e} V rel dc={dc:.3e}")
     assert np.isfinite(s64.vm).all() and np.isfinite(s32.vm).all()
     assert dv < 1e-6
+
+
+def test_coarse_grid_many_membranes_per_square():
+    """An 8x8 grid under the 35-cell cluster: up to 17 membranes share a square,
+    so the exchange sums a square's fluxes over several batches of loads, and
+    most squares carry membranes. FP64 against the oracle over 300 steps."""
+    p = Parameters.preset("tiny")
+    p.grid_size = 8
+    cells = Cells.make_world(p)
+    counts = np.bincount(cells.map_mem2ecm, minlength=cells.X.size)
+    assert counts.max() > 16
+    say(f"case: coarse grid {cells.X.shape}, up to {counts.max()} membranes per square, 300 steps")
+    sim = gpu_run(p, cells, 300)
+    o = oracle_run(p, cells, 300)
+    check(sim, o, "coarse grid")
