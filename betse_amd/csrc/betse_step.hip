@@ -1313,7 +1313,8 @@ struct bt_ctx {
     uint8_t* live_flag = nullptr;        // per square: 1 if in sq_live
     int n_live = 0;
     unsigned* flags = nullptr; unsigned long long* bad_step = nullptr; long long* step_ctr = nullptr;
-    hipGraph_t graph = nullptr; hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr; hipGraphExec_t gexec = nullptr;       // one pair of steps
+    hipGraph_t graph_n = nullptr; hipGraphExec_t gexec_n = nullptr;   // BT_GRAPH_PAIRS pairs
     long long steps_done = 0;
     const bt_obs* capture = nullptr;   // set while enqueueing an observed step
     // observables workspace (allocated on first use)
@@ -1406,6 +1407,9 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
 
 // Wavefronts per tile of k_mem_cell: two (ions split between them) while one
 // wavefront per tile would leave the SIMDs with few waves to switch between.
+#ifndef BT_GRAPH_PAIRS
+#define BT_GRAPH_PAIRS 8        // pairs of steps per replayed graph (fewer graph launches)
+#endif
 #ifndef BT_MEM_WG_TILES
 #define BT_MEM_WG_TILES 4096
 #endif
@@ -1711,6 +1715,8 @@ int bt_destroy(bt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->gexec_n) (void)hipGraphExecDestroy(c->gexec_n);
+    if (c->graph_n) (void)hipGraphDestroy(c->graph_n);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
                     c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
                     c->B1, c->B2, c->T1, c->T2, c->AA, c->BB, c->vm_prev, c->net};
@@ -1739,15 +1745,33 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     if (n_steps == 0) return BT_OK;
     int pairs = n_steps / 2;
     if (use_graph && pairs > 0) {
-        if (!c->gexec) {
+        // a graph of BT_GRAPH_PAIRS pairs of steps for the bulk, one of a single
+        // pair for the rest (each captured on first use)
+        auto capture = [&](int npairs, hipGraph_t* g, hipGraphExec_t* x) -> int {
             HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(c, 0);
-            enqueue_step(c, 1);
-            hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+            for (int q = 0; q < npairs; ++q) {
+                enqueue_step(c, 0);
+                enqueue_step(c, 1);
+            }
+            hipError_t e = hipStreamEndCapture(c->stream, g);
             if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-            HIP_TRY(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+            HIP_TRY(c, hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
+            return BT_OK;
+        };
+        int done = 0;
+        if (BT_GRAPH_PAIRS > 1 && pairs >= BT_GRAPH_PAIRS) {
+            if (!c->gexec_n) {
+                int rc = capture(BT_GRAPH_PAIRS, &c->graph_n, &c->gexec_n);
+                if (rc != BT_OK) return rc;
+            }
+            for (; done + BT_GRAPH_PAIRS <= pairs; done += BT_GRAPH_PAIRS)
+                HIP_TRY(c, hipGraphLaunch(c->gexec_n, c->stream));
         }
-        for (int s = 0; s < pairs; ++s) HIP_TRY(c, hipGraphLaunch(c->gexec, c->stream));
+        if (done < pairs && !c->gexec) {
+            int rc = capture(1, &c->graph, &c->gexec);
+            if (rc != BT_OK) return rc;
+        }
+        for (; done < pairs; ++done) HIP_TRY(c, hipGraphLaunch(c->gexec, c->stream));
     } else {
         for (int s = 0; s < pairs; ++s) {
             enqueue_step(c, 0);
@@ -1830,6 +1854,8 @@ int bt_set_network(bt_ctx* c, const bt_network* net, double* c_net) {
     // captured steps do not match the new step body any more
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (c->gexec_n) { (void)hipGraphExecDestroy(c->gexec_n); c->gexec_n = nullptr; }
+    if (c->graph_n) { (void)hipGraphDestroy(c->graph_n); c->graph_n = nullptr; }
     return BT_OK;
 }
 
