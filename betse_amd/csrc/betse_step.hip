@@ -18,11 +18,13 @@
 //   k_mem_cell      one or two wavefronts per tile of whole cells (<= 64
 //                   membranes), a lane per membrane for every ion (NaK pump,
 //                   GHK, GHK gap junction, Ca pump) and voltage-gated channel
-//                   (gate update, channel GHK); per cell the CSR segment sums from LDS,
+//                   (gate update from the per-cell rates, channel GHK); per
+//                   cell the CSR segment sums from LDS,
 //                   update_Co + GJ update, no_negs clamp, rho_cells -> Vmem
 //                   (sim.py:1196-1283, 2086-2111, 2007-2038,
 //                   networks.py:3115-3215)                      cell A -> cell B
-//   k_env_exchange  one wavefront per ion over 64 squares: the square's membrane
+//   k_env_exchange  one wavefront per ion over 64 squares (one thread per
+//                   square over all ions on very large batches): the square's membrane
 //                   fluxes (written by k_mem_cell in square-sorted order, the
 //                   sparse form of M_divmap_mem2ecm) -> env update; then
 //                   rho_env -> u (sim_toolbox.py:1191-1229, ion_current.py:75-97)
@@ -36,7 +38,10 @@
 //
 // Floating-point expressions keep the reference's operation order (the code is
 // compiled with -ffp-contract=off) so the FP64 path tracks the NumPy reference
-// to a few ulps per step.
+// to a few ulps per step. The FP32-flux path evaluates the flux laws in FP32
+// and trades exact operation order for speed where an FP64 ulp or an FP32 ulp
+// is all it costs (reciprocal constants, one transcendental per GHK law,
+// gate-update coefficients per cell); its bar is |dVmem| < 1 uV.
 
 #include <hip/hip_runtime.h>
 #include <math.h>
