@@ -36,11 +36,12 @@ def _squares_with_membranes(cells) -> int:
 
 
 def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0, n_sub: int = 0,
-                      dm_loaded: int = None, gj_loaded: bool = True) -> dict:
+                      dm_loaded: int = None, gj_loaded: bool = True, denv_loaded: int = None) -> dict:
     """Compulsory bytes per launch of each step kernel (each array element the
     kernel must touch, counted once at its storage width). ``dm_loaded`` ions
     have per-membrane Dm (the others are one value per cluster and come with
-    the cluster constants); ``gj_loaded`` likewise for gjopen."""
+    the cluster constants); ``gj_loaded`` likewise for gjopen, ``denv_loaded``
+    for D_env over the grid."""
     C, M = cells.n_cells, cells.n_mems
     G2 = cells.X.size
     n_in = len(cells.envInds_inClust)
@@ -54,7 +55,8 @@ def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 
                             + 16 + 16 + 4 + 8 * n_ch         # vm r/w, cell_vol, diviterm, mem_ptr, modulators
                             + 32 * n_ch)                     # gating rates r
                      + n_in * 8 * (ni + (1 if ca else 0) + n_ch))  # env gathers (pre- and post-diffusion)
-    b["env_field"] = (G2 * (8 + 8 * ni + 8 * ni + 8 * ni)   # u r, c_env r, D_env r, c_mid w
+    nde = ni if denv_loaded is None else denv_loaded
+    b["env_field"] = (G2 * (8 + 8 * ni + 8 * nde + 8 * ni)  # u r, c_env r, D_env r, c_mid / next c_env w
                       + (C * (8 + 32 * n_ch) if n_ch else 0))  # vm r, gating rates w
     # the exchange runs over the squares with membranes (the others take their
     # new state from k_env_field): c_mid r, c_env w, u w, has_mem, sq_cluster, sq_ptr, list
@@ -99,6 +101,8 @@ def bench_batched(a):
         b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=mode)
         b.init("init")
         uniform = all(np.all(st.Dm_cells == st.Dm_cells[:, :1]) for st in b.states)
+        de_uniform = all(np.all((st.D_env * st.TJ_modulator) == (st.D_env * st.TJ_modulator)[:, :1])
+                         for st in b.states)
         b.step_n(a.warmup)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -156,7 +160,7 @@ def bench_batched(a):
     ms_step = r["ev_ms"] / a.steps
     # Dm (per-cluster factors) and gjopen are one value per cluster in this workload
     ab = algorithmic_bytes(tot, n_ions, "Ca" in p0.ion_names(), best, len(p0.channels), 0,
-                           0 if uniform else n_ions, not uniform)
+                           0 if uniform else n_ions, not uniform, 0 if de_uniform else n_ions)
     kern = r["kernels"]
     dom = max(kern, key=kern.get)
     achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
@@ -511,9 +515,11 @@ def main(argv=None):
     value = C * a.steps / (r["ev_ms"] / 1e3)
     n_sub = p.network.n_species if getattr(p, "network", None) is not None else 0
     Dm, gjo = r["sim"].Dm_cells, r["sim"].gjopen
+    De = r["sim"].D_env * r["sim"].TJ_modulator
     dm_loaded = int(sum(not np.all(Dm[i] == Dm[i][0]) for i in range(n_ions)))
+    de_loaded = int(sum(not np.all(De[i] == De[i][0]) for i in range(n_ions)))
     ab = algorithmic_bytes(cells, n_ions, has_ca, best, len(p.channels), n_sub, dm_loaded,
-                           not bool(np.all(gjo == gjo[0])))
+                           not bool(np.all(gjo == gjo[0])), de_loaded)
     kern = r["kernels"]
     dom = max(kern, key=kern.get)
     achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9

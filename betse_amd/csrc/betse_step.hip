@@ -112,6 +112,9 @@ struct KCluster {
     // its membranes (NaN otherwise): the membrane kernel then skips the loads
     double dm_u[BT_MAX_IONS];
     double gjo_u;
+    // likewise D_env per ion over the cluster's squares: k_env_field then
+    // stages the constant instead of loading the array
+    double denv_u[BT_MAX_IONS];
 };
 
 // x clamped into [0, n - 1]
@@ -732,11 +735,18 @@ k_env_field(FieldArgs A, KParams P) {
             const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
             tc[it] = cenv[clampi(r, ny) * nx + clampi(cc, nx)];
         }
+        // the cluster's D_env of this ion when it is one value (no loads), else the array
+        const double du = A.kcl[et[0]].denv_u[i];
+        if (du == du) {
 #pragma unroll
-        for (int it = 0; it < ND; ++it) {
-            const int e = min(tid + it * TY * TX, ER * EC - 1);
-            const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
-            td[it] = D[clampi(r, ny) * nx + clampi(cc, nx)];
+            for (int it = 0; it < ND; ++it) td[it] = du;
+        } else {
+#pragma unroll
+            for (int it = 0; it < ND; ++it) {
+                const int e = min(tid + it * TY * TX, ER * EC - 1);
+                const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
+                td[it] = D[clampi(r, ny) * nx + clampi(cc, nx)];
+            }
         }
 #pragma unroll
         for (int it = 0; it < NC; ++it) {
@@ -1405,7 +1415,7 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
     k.kee = (b.ko_env * p.eo) * p.er;
     k.screen = (2.0 / (b.ko_env * b.delta)) * (p.cell_radius / p.true_cell_size);
     k.nak_coef = -3.0 * p.NaK_block * b.alpha_NaK;
-    for (int i = 0; i < BT_MAX_IONS; ++i) k.dm_u[i] = NAN;
+    for (int i = 0; i < BT_MAX_IONS; ++i) k.dm_u[i] = k.denv_u[i] = NAN;
     k.gjo_u = NAN;
     return k;
 }
@@ -1668,6 +1678,22 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
             bool same = true;
             for (int m = lo[q]; m < hi[q] && same; ++m) same = (gj[m] == gj[lo[q]]);
             c->kcl_host[q].gjo_u = same ? gj[lo[q]] : NAN;
+        }
+        // D_env likewise over each cluster's grid (read once, like Dm)
+        std::vector<double> de((size_t)p->n_ions * p->n_squares);
+        e = hipMemcpy(de.data(), b->D_env, de.size() * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            delete c;
+            return set_err(nullptr, BT_EHIP, std::string("reading D_env: ") + hipGetErrorString(e));
+        }
+        for (int q = 0; q < n_clusters; ++q) {
+            const size_t k0 = (size_t)cl[q].sq_off, k1 = k0 + (size_t)cl[q].grid_ny * cl[q].grid_nx;
+            for (int i = 0; i < p->n_ions; ++i) {
+                const double* d = de.data() + (size_t)i * p->n_squares;
+                bool same = true;
+                for (size_t k = k0; k < k1 && same; ++k) same = (d[k] == d[k0]);
+                c->kcl_host[q].denv_u[i] = same ? d[k0] : NAN;
+            }
         }
     }
     (void)hipGetDevice(&c->device);

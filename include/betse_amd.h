@@ -179,11 +179,11 @@ typedef struct {
  * buffers, post-diffusion env buffer, per-membrane exchange fluxes, per-cell
  * gating rates, surface-charge field, the list of squares with membranes,
  * status words) on the current device. It reads tile_ptr, mem_ptr,
- * tile_cluster, sq_ptr, sq_has_mem, Dm and gjopen with blocking copies, so
- * their contents must be complete when it is called. Dm and gjopen are read
- * once here (a cluster whose values are uniform gets them as constants); they
- * do not change within a phase on this path, and a caller that changes them
- * creates a new context. Between calls the caller's state buffers hold the
+ * tile_cluster, sq_ptr, sq_has_mem, Dm, gjopen and D_env with blocking
+ * copies, so their contents must be complete when it is called. Dm, gjopen
+ * and D_env are read once here (a cluster whose values are uniform gets them
+ * as constants); they do not change within a phase on this path, and a caller
+ * that changes them creates a new context. Between calls the caller's state buffers hold the
  * state (the workspace copies are internal).
  * `stream` is a hipStream_t (may be NULL for the default stream; graphs need a
  * non-default stream). Replaces the state set-up of init_core/init_dynamics

@@ -47,6 +47,9 @@ def test_algorithmic_bytes_counts_each_array_once():
     bc = bench.algorithmic_bytes(cells, ni, ca=False, flux_mode="fp64", n_ch=2)
     assert bc["mem_cell"] - b["mem_cell"] == M * 2 * 40 + C * 2 * 40 + n_in * 8 * 2
     assert bc["env_field"] - b["env_field"] == C * (8 + 32 * 2)
+    # a D_env that is one value per ion is not loaded by the env kernel
+    bd = bench.algorithmic_bytes(cells, ni, ca=False, flux_mode="fp64", denv_loaded=0)
+    assert b["env_field"] - bd["env_field"] == G2 * 8 * ni
     # per-cluster uniform Dm and gjopen are not loaded per membrane
     bu = bench.algorithmic_bytes(cells, ni, ca=False, flux_mode="fp64", dm_loaded=0, gj_loaded=False)
     assert b["mem_cell"] - bu["mem_cell"] == M * 8 * (ni + 1)

@@ -455,3 +455,21 @@ def test_coarse_grid_many_membranes_per_square():
     sim = gpu_run(p, cells, 300)
     o = oracle_run(p, cells, 300)
     check(sim, o, "coarse grid")
+
+
+def test_nonuniform_env_diffusivity():
+    """Adherens and tight junctions lower D_env inside the cluster and at its
+    boundary (initDenv, sim.py:2347-2400), so D_env is not one value per ion and
+    the env kernel loads it per square. FP64 and the FP32-flux path against the
+    oracle over 200 steps."""
+    p = Parameters.preset("tiny")
+    p.D_adh = 0.5
+    p.D_tj = 0.3
+    cells = Cells.make_world(p)
+    say("case: non-uniform D_env (D_adh 0.5, D_tj 0.3), 200 steps")
+    o = oracle_run(p, cells, 200)
+    assert len(np.unique(o.D_env[0])) > 1
+    check(gpu_run(p, cells, 200), o, "non-uniform D_env fp64")
+    dv, dc, de, lag, eex = errors(gpu_run(p, cells, 200, flux_mode="fp32"), o)
+    say(f"  fp32: |dVm|={dv:.3e} rel denv={de:.3e}")
+    assert dv < 1e-6 and de < 1e-6
