@@ -410,11 +410,14 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         // slot by slot in the same sequence of additions
         double vmut = with_nonces(v, W * NS + 1);
         double vnmut = with_nonces(vn, W * NS + 1);
+        // software pipeline: the gathers of slot s + 1 are issued before slot
+        // s's arithmetic, so one slot's latency hides behind the other's work
+        if (W * NS < NI) gather_ion(0);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int i = W * NS + s;
             if (i >= NI) break;
-            gather_ion(s);
+            if (s + 1 < NS && i + 1 < NI) gather_ion(s + 1);
             if (s > 0) {
                 vmut = with_nonces(vmut, 1);
                 vnmut = with_nonces(vnmut, 1);
