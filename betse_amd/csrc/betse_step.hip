@@ -67,7 +67,9 @@ constexpr int TY = 16, TX = 16;          // k_env_field tile
 #ifndef ENV_IONS_PER_BLOCK
 #define ENV_IONS_PER_BLOCK 2               // ions per k_env_field block when it splits
 #endif
-constexpr int EX_SPLIT_SQUARES = 1 << 19; // below this many squares k_env_exchange splits by ion
+#ifndef EX_SPLIT_SQUARES
+#define EX_SPLIT_SQUARES (1 << 19)  // below this many squares with membranes k_env_exchange splits by ion
+#endif
 constexpr double FLOAT_NONCE = 1.0e-25;  // sim_toolbox.py:50
 
 // Constants derived on the host, each in the reference's evaluation order.
@@ -1498,7 +1500,6 @@ static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, co
 // which: -1 all, 0 membrane/cell, 1 env field+diffusion, 2 exchange, 3 field only
 static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     const bt_buffers& B = c->buf;
-    const int nsq = c->kp.n_sq;
     double* ci = parity ? c->c_cell_b : B.c_cell;
     double* li = parity ? c->c_lag_b : B.c_lag;
     double* vi = parity ? c->vm_b : B.vm_cell;
@@ -1524,7 +1525,7 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
         ExArgs x{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u, c->flags,
                  c->step_ctr, which < 0 ? 1 : 0, c->sq_live, c->n_live};
         const int nl = c->n_live > 0 ? c->n_live : 1;    // block 0 also counts the step
-        if (nsq < EX_SPLIT_SQUARES)
+        if (c->n_live < EX_SPLIT_SQUARES)
             hipLaunchKernelGGL(k_env_exchange, dim3((nl + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0,
                                c->stream, x, c->kp);
         else
