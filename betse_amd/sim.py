@@ -503,44 +503,55 @@ class Simulator(HostState):
                 h.copy_(t, non_blocking=True)
 
     def _fetch_start_async(self, tensors: dict) -> None:
-        """Sampled steps: snapshot the tensors on the simulation stream (device
-        copies), then copy the snapshots to pinned host memory on a side stream
-        (default priority), so the host transfer overlaps the next steps.
-        ``_fetch_finish`` waits for it. The snapshots and pinned buffers are
-        reused: a caller finishes one fetch before it starts the next."""
+        """Sampled steps: snapshot the (FP64) tensors into one flat device buffer
+        on the simulation stream (a single multi-tensor copy), then copy that
+        buffer to pinned host memory on a side stream (default priority), so the
+        host transfer overlaps the next steps. ``_fetch_finish_async`` waits for
+        it. The buffers are reused: a caller finishes one fetch before it starts
+        the next."""
         torch = self.torch
-        snaps = self.__dict__.setdefault("_snaps", {})
-        pinned = self.__dict__.setdefault("_pinned", {})
+        layout = [(k, tuple(t.shape)) for k, t in tensors.items()]
+        if any(t.dtype != torch.float64 for t in tensors.values()):
+            raise BetseSimException("sampled-step fetch expects FP64 tensors")
+        if self.__dict__.get("_flat_layout") != layout:
+            total = sum(int(np.prod(shape)) for _, shape in layout)
+            self._flat_dev = torch.empty(total, dtype=torch.float64, device=self.device)
+            self._flat_host = torch.empty(total, dtype=torch.float64, pin_memory=True)
+            self._flat_at, o = {}, 0
+            for k, shape in layout:
+                n = int(np.prod(shape))
+                self._flat_at[k] = (o, n, shape)
+                o += n
+            self._flat_layout = layout
         if self.__dict__.get("_copy_stream") is None:
             self._copy_stream = torch.cuda.Stream(device=self.device)     # default priority
         with torch.cuda.device(self.device):
             with torch.cuda.stream(self.stream):
-                for k, t in tensors.items():
-                    sn = snaps.get(k)
-                    if sn is None or sn.shape != t.shape or sn.dtype != t.dtype:
-                        sn = torch.empty_like(t)
-                        snaps[k] = sn
-                    sn.copy_(t, non_blocking=True)
+                dst = [self._flat_dev[o:o + n].view(shape) for o, n, shape in self._flat_at.values()]
+                torch._foreach_copy_(dst, [tensors[k] for k in self._flat_at])
                 taken = torch.cuda.Event()
                 taken.record(self.stream)
             with torch.cuda.stream(self._copy_stream):
                 self._copy_stream.wait_event(taken)
-                for k in tensors:
-                    sn, h = snaps[k], pinned.get(k)
-                    if h is None or h.shape != sn.shape or h.dtype != sn.dtype:
-                        h = torch.empty(sn.shape, dtype=sn.dtype, pin_memory=True)
-                        pinned[k] = h
-                    h.copy_(sn, non_blocking=True)
+                self._flat_host.copy_(self._flat_dev, non_blocking=True)
                 self._copied = torch.cuda.Event()
                 self._copied.record(self._copy_stream)
+
+    def _fetch_finish_async(self, keys) -> dict:
+        """Host arrays of the last ``_fetch_start_async`` (NumPy copies out of
+        the pinned buffer, which is reused)."""
+        self.__dict__.pop("_copied").synchronize()
+        flat = self._flat_host.numpy()
+        out = {}
+        for k in keys:
+            o, n, shape = self._flat_at[k]
+            out[k] = flat[o:o + n].reshape(shape).copy()
+        return out
 
     def _fetch_finish(self, keys) -> dict:
         """Host arrays of pinned copies that have completed (NumPy copies out of
         the pinned buffers, which are reused; torch's CPU clone of a pinned
         tensor was measured ~14x slower on the largest cluster)."""
-        copied = self.__dict__.pop("_copied", None)
-        if copied is not None:
-            copied.synchronize()
         pinned = self._pinned
         return {k: pinned[k].numpy().copy() for k in keys}
 
@@ -735,7 +746,7 @@ class Simulator(HostState):
 
         def finish(t_sample):
             # host side of a sampled step: runs while the device steps on
-            h = self._fetch_finish(state_keys + list(self._OBS_KEEP))
+            h = self._fetch_finish_async(state_keys + list(self._OBS_KEEP))
             self._set_state(h, False)
             self._set_observables(h)
             phase.callbacks.progressed_next()
