@@ -1109,61 +1109,90 @@ k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cel
 struct ObsMemArgs {
     const int32_t* mem_ptr; const int32_t* nn_mem; const double* mem_sa; const double* mem_nx;
     const double* mem_ny; const double* cell_sa; const double* vm_cell; const double* vm_gj;
-    const int32_t* mem_cell;
+    const int32_t* mem_cell; const int32_t* tile_ptr;
     const double* fm; const double* fgj; const double* fch;
     double* Jn; double* I_mem; double* E_gj_x; double* E_gj_y;
     double* J_cell_x; double* J_cell_y; double* vm_ave;
     double smooth_cells;
 };
 
-// get_current cell/membrane part (ion_current.py:18-47), E_gj (sim.py:2166-2172,
-// from the Vmem the step started with, vm_gj), vm_ave (sim.py:2038); one thread
-// per cell, membranes in order
-__global__ void __launch_bounds__(BLOCK)
-k_obs_mem(ObsMemArgs A, KParams P) {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c >= P.n_cells) return;
-    const int a = A.mem_ptr[c], b = A.mem_ptr[c + 1];
+// raw normal current of membrane m: -sum(zF*f_mem) + extra_J_mem (channels) +
+// sum(zF*f_gj), in the reference's order (ion_current.py:18-30)
+__device__ __forceinline__ double obs_jn_raw(const ObsMemArgs& A, const KParams& P, int m) {
     const int M = P.n_mems;
-    auto jn_raw = [&](int m) -> double {
-        double extra = 0.0;                       // extra_J_mem: channel currents
-        for (int ch = 0; ch < P.n_chan; ++ch) {
-            const int ion = P.chan_ion[ch];
-            extra = extra + ((-A.fch[(size_t)ch * M + m]) * P.F) * P.zq[ion];
-        }
-        double d = P.zF[0] * A.fm[m];
-        double g = P.zF[0] * A.fgj[m];
-        for (int i = 1; i < P.n_ions; ++i) {
-            d = d + P.zF[i] * A.fm[(size_t)i * M + m];
-            g = g + P.zF[i] * A.fgj[(size_t)i * M + m];
-        }
-        return (-d + extra) + g;
-    };
-    double s = 0.0;
-    for (int m = a; m < b; ++m) s = (m == a) ? jn_raw(m) * A.mem_sa[m] : s + jn_raw(m) * A.mem_sa[m];
-    const double jn_ave = s / A.cell_sa[c];
-    const double nm = A.smooth_cells * (double)(b - a);
-    const double w_mem = (nm - 1) / nm, w_o = 1 / nm;
-    double sx = 0.0, sy = 0.0, svm = 0.0;
-    const double v = A.vm_cell[c];
-    const double vg = A.vm_gj[c];
-    for (int m = a; m < b; ++m) {
-        const double jn = w_mem * jn_raw(m) + jn_ave * w_o;
+    double extra = 0.0;                       // extra_J_mem: channel currents
+    for (int ch = 0; ch < P.n_chan; ++ch) {
+        const int ion = P.chan_ion[ch];
+        extra = extra + ((-A.fch[(size_t)ch * M + m]) * P.F) * P.zq[ion];
+    }
+    double d = P.zF[0] * A.fm[m];
+    double g = P.zF[0] * A.fgj[m];
+    for (int i = 1; i < P.n_ions; ++i) {
+        d = d + P.zF[i] * A.fm[(size_t)i * M + m];
+        g = g + P.zF[i] * A.fgj[(size_t)i * M + m];
+    }
+    return (-d + extra) + g;
+}
+
+// get_current cell/membrane part (ion_current.py:18-47), E_gj (sim.py:2166-2172,
+// from the Vmem the step started with, vm_gj), vm_ave (sim.py:2038): one
+// wavefront per tile of whole cells, as k_mem_cell. Membrane lanes evaluate,
+// cell lanes take the sums over their membranes in membrane order from LDS.
+__global__ void __launch_bounds__(TMB)
+k_obs_mem(ObsMemArgs A, KParams P) {
+    __shared__ double s_j[TMB], s_sa[TMB], s_jx[TMB], s_jy[TMB];
+    __shared__ double s_ave[TMB], s_wm[TMB], s_wo[TMB];      // per cell of the tile
+    __shared__ int s_lc[TMB];                                 // cell of each membrane, in the tile
+    const int tile = blockIdx.x;
+    const int c0 = A.tile_ptr[tile], c1 = A.tile_ptr[tile + 1];
+    const int m0 = A.mem_ptr[c0], m1 = A.mem_ptr[c1];
+    const int j = threadIdx.x;
+    const int m = m0 + j, c = c0 + j;
+    int ca = 0, cb = 0;
+    if (c < c1) {
+        ca = A.mem_ptr[c] - m0;
+        cb = A.mem_ptr[c + 1] - m0;
+    }
+    if (m < m1) {
+        s_j[j] = obs_jn_raw(A, P, m);
+        s_sa[j] = A.mem_sa[m];
+        s_lc[j] = A.mem_cell[m] - c0;
+    }
+    __syncthreads();
+    if (c < c1) {
+        double s = 0.0;
+        for (int q = ca; q < cb; ++q) s = (q == ca) ? s_j[q] * s_sa[q] : s + s_j[q] * s_sa[q];
+        s_ave[j] = s / A.cell_sa[c];
+        const double nm = A.smooth_cells * (double)(cb - ca);
+        s_wm[j] = (nm - 1) / nm;
+        s_wo[j] = 1 / nm;
+    }
+    __syncthreads();
+    if (m < m1) {
+        const int lc = s_lc[j];
+        const double jn = s_wm[lc] * s_j[j] + s_ave[lc] * s_wo[lc];
         A.Jn[m] = jn;
-        A.I_mem[m] = -jn * A.mem_sa[m];
-        const double jx = (jn * A.mem_nx[m]) * A.mem_sa[m];
-        const double jy = (jn * A.mem_ny[m]) * A.mem_sa[m];
-        sx = (m == a) ? jx : sx + jx;
-        sy = (m == a) ? jy : sy + jy;
-        svm = (m == a) ? v : svm + v;
-        const double vgj = A.vm_gj[A.mem_cell[A.nn_mem[m]]] - vg;
+        A.I_mem[m] = -jn * s_sa[j];
+        s_jx[j] = (jn * A.mem_nx[m]) * s_sa[j];
+        s_jy[j] = (jn * A.mem_ny[m]) * s_sa[j];
+        const double vgj = A.vm_gj[A.mem_cell[A.nn_mem[m]]] - A.vm_gj[c0 + lc];
         const double egj = -vgj / P.gj_len;
         A.E_gj_x[m] = egj * A.mem_nx[m];
         A.E_gj_y[m] = egj * A.mem_ny[m];
     }
-    A.J_cell_x[c] = sx / A.cell_sa[c];
-    A.J_cell_y[c] = sy / A.cell_sa[c];
-    A.vm_ave[c] = svm / (double)(b - a);
+    __syncthreads();
+    if (c < c1) {
+        const double v = A.vm_cell[c];
+        double sx = s_jx[ca], sy = s_jy[ca], svm = v;
+        for (int q = ca + 1; q < cb; ++q) {
+            sx = sx + s_jx[q];
+            sy = sy + s_jy[q];
+            svm = svm + v;
+        }
+        A.J_cell_x[c] = sx / A.cell_sa[c];
+        A.J_cell_y[c] = sy / A.cell_sa[c];
+        A.vm_ave[c] = svm / (double)(cb - ca);
+    }
 }
 
 // env current J_env_o = F*sum(z*fluxes_env) (ion_current.py:53-58) and the two
@@ -2019,10 +2048,10 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
     if (rc != BT_OK) return rc;
     const bt_buffers& B = c->buf;
     ObsMemArgs a{B.mem_ptr, B.nn_mem, B.mem_sa, B.mem_nx, B.mem_ny, B.cell_sa, B.vm_cell,
-                 c->vm_prev_ok ? c->vm_prev : B.vm_cell, B.mem_cell,
+                 c->vm_prev_ok ? c->vm_prev : B.vm_cell, B.mem_cell, B.tile_ptr,
                  o->fluxes_mem, o->fluxes_gj, o->chan_flux, o->Jn, o->I_mem, o->E_gj_x, o->E_gj_y,
                  o->J_cell_x, o->J_cell_y, o->vm_ave, c->prm.smooth_cells};
-    hipLaunchKernelGGL(k_obs_mem, dim3(grid_blocks(c->kp.n_cells)), dim3(BLOCK), 0, c->stream, a, c->kp);
+    hipLaunchKernelGGL(k_obs_mem, dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
     hipLaunchKernelGGL(k_obs_env_rhs, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
                        (const double*)o->fluxes_env_x, (const double*)o->fluxes_env_y, c->jxo, c->jyo, c->kp);
     const size_t pb = (size_t)c->n1p * c->n2p * 8;
