@@ -607,7 +607,9 @@ struct FieldArgs {
 template <bool DIFFUSE, bool RATES, bool RECIP>
 __global__ void __launch_bounds__(TY * TX)
 k_env_field(FieldArgs A, KParams P) {
-    if (halted(A.flags)) return;
+    // the halt flag is read first and tested before the first store, so its
+    // latency overlaps the block's first loads
+    const bool stop = halted(A.flags);
     constexpr int UH = 6, UR = TY + 2 * UH, UC = TX + 2 * UH;    // u on tile +- 6
     constexpr int TR = TY + 4;                                  // row-blurred rows tile +- 2
     constexpr int VR = TY + 4, VC = TX + 4;                     // v on tile +- 2
@@ -640,18 +642,109 @@ k_env_field(FieldArgs A, KParams P) {
     auto per_dl = [&](double x) { return RECIP ? x * rdl : x / dl; };
     auto per_d2 = [&](double x) { return RECIP ? x * rd2 : x / d2; };
 
-    // u with zero padding outside the grid (gaussian_filter mode='constant')
-    // (each thread issues all of its loads, from clamped in-grid addresses,
-    // before the first LDS write, so they overlap)
-    {
-        constexpr int NIT = (UR * UC + TY * TX - 1) / (TY * TX);
-        double t[NIT];
+    // block row ion_id takes the ions [ion_id * ipb, ion_id * ipb + ipb), with
+    // ipb = ceil(n_ions / gridDim.y), and rebuilds its tile's field for them
+    const int ipb = (P.n_ions + gridDim.y - 1) / gridDim.y;
+    const int i_lo = ion_id * ipb;
+    const int i_hi = min(i_lo + ipb, P.n_ions);
+
+    // Every load the block needs first is issued here before any of them is
+    // used: the gating-rate Vmem, the cluster's uniform D_env per ion, the
+    // live flag, the first ion's concentrations (and D_env), then u, so their
+    // latencies overlap. Later ions are loaded one ion ahead, in flight
+    // through the previous ion's flux stages (barriers wait for LDS only).
+    const int nc = P.n_cells;
+    const int items = RATES ? P.n_chan * nc : 0;
+    const int nthr = gridDim.x * gridDim.y * (TY * TX);
+    const int t_first = (blockIdx.y * gridDim.x + blockIdx.x) * (TY * TX) + tid;
+    double vm_first = 0.0;
+    if (RATES && t_first < items) vm_first = A.vm[t_first % nc];
+    __shared__ double s_du[BT_MAX_IONS];
+    double du_mine = 0.0;
+    if (DIFFUSE && tid < P.n_ions) du_mine = A.kcl[et[0]].denv_u[tid];
+
+    // whether this thread's square goes to the exchange (c_mid) or is final (c_env_out)
+    // (an empty asm at the store keeps the compiler from testing it, and so
+    // waiting for every load issued before it, right after the load)
+    unsigned live_b = 0;
+    if (DIFFUSE) {
+        const int r = r0 + tid / TX, cc = cc0 + tid % TX;
+        if (r < ny && cc < nx) live_b = A.live[so + r * nx + cc];
+    }
+    // one ion's concentrations on tile +- 2 and D_env on tile +- 1, from
+    // clamped in-grid addresses (du: the cluster's D_env of the ion when it is
+    // one value, no loads; NaN when it varies)
+    constexpr int NC = (VR * VC + TY * TX - 1) / (TY * TX);
+    constexpr int ND = (ER * EC + TY * TX - 1) / (TY * TX);
+    double tc[NC], td[ND];
+    auto load_ion = [&](int i, double du) {
+        const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
+        const double* D = A.D_env + (size_t)i * P.n_sq + so;
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int e = min(tid + it * TY * TX, UR * UC - 1);
-            const int rr = r0 - UH + e / UC, cc = cc0 - UH + e % UC;
-            t[it] = A.u[so + clampi(rr, ny) * nx + clampi(cc, nx)];
+        for (int it = 0; it < NC; ++it) {
+            const int e = min(tid + it * TY * TX, VR * VC - 1);
+            const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
+            tc[it] = cenv[clampi(r, ny) * nx + clampi(cc, nx)];
         }
+        if (du == du) {
+#pragma unroll
+            for (int it = 0; it < ND; ++it) td[it] = du;
+        } else {
+#pragma unroll
+            for (int it = 0; it < ND; ++it) {
+                const int e = min(tid + it * TY * TX, ER * EC - 1);
+                const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
+                td[it] = D[clampi(r, ny) * nx + clampi(cc, nx)];
+            }
+        }
+    };
+    if (DIFFUSE && i_lo < i_hi) load_ion(i_lo, A.kcl[et[0]].denv_u[i_lo]);
+    // u last: its loads are the only unconditional ones, so the waits for
+    // them count the same on every path
+    constexpr int NIT = (UR * UC + TY * TX - 1) / (TY * TX);
+    double t[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int e = min(tid + it * TY * TX, UR * UC - 1);
+        const int rr = r0 - UH + e / UC, cc = cc0 - UH + e % UC;
+        t[it] = A.u[so + clampi(rr, ny) * nx + clampi(cc, nx)];
+    }
+
+    // the gating rates of every (channel, cell) for this step's k_mem_cell.
+    // They depend on the cell's Vmem alone, so evaluating them here, one
+    // (channel, cell) per thread, replaces one FP64 evaluation per membrane.
+    // Run after the ion loop (the Vmem of a thread's first item was loaded
+    // with the block's first loads); run first, their registers would stay
+    // live beside the prefetched loads and cost occupancy.
+    auto gating_rates = [&]() {
+        for (int t_it = t_first; t_it < items; t_it += nthr) {
+            const int ch = t_it / nc, cell = t_it - ch * nc;
+            // Vmem carrying the n_ions + ch nonces of the membrane stage
+            const double vmc = (t_it == t_first) ? vm_first : A.vm[cell];
+            const double V = with_nonces(vmc, P.n_ions + ch) * 1000.0 + 0.0;
+            double mInf, mTau, hInf, hTau;
+            gate_rates(P.chan_kind[ch], V, mInf, mTau, hInf, hTau);
+            double* R = A.rates + (size_t)(4 * ch) * nc + cell;
+            if (A.gate_coef) {
+                // update_mh as x' = a x + b: a = tau/(tau+dt), b = dt*inf/(tau+dt)
+                // (FP64, an ulp from the quotient form; the FP32-flux path only)
+                const double dtm = P.chan_dt;
+                R[0] = mTau / (mTau + dtm);
+                R[nc] = (dtm * mInf) / (mTau + dtm);
+                R[2 * nc] = hTau / (hTau + dtm);
+                R[3 * nc] = (dtm * hInf) / (hTau + dtm);
+            } else {
+                R[0] = mInf;
+                R[nc] = mTau;
+                R[2 * nc] = hInf;
+                R[3 * nc] = hTau;
+            }
+        }
+    };
+    if (DIFFUSE && tid < P.n_ions) s_du[tid] = du_mine;
+
+    // u with zero padding outside the grid (gaussian_filter mode='constant')
+    {
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int e = tid + it * TY * TX;
@@ -701,6 +794,8 @@ k_env_field(FieldArgs A, KParams P) {
         sEy[er][ec] = ey;
     }
     __syncthreads();
+    // a halted run writes nothing (every thread read the same flag)
+    if (stop) return;
     if (!DIFFUSE) {
         const int r = r0 + tid / TX, cc = cc0 + tid % TX;
         if (r < ny && cc < nx) {
@@ -711,48 +806,13 @@ k_env_field(FieldArgs A, KParams P) {
         return;
     }
 
-    // block row ion_id takes the ions [ion_id * ipb, ion_id * ipb + ipb), with
-    // ipb = ceil(n_ions / gridDim.y), and rebuilds its tile's field for them
-    const int ipb = (P.n_ions + gridDim.y - 1) / gridDim.y;
-    const int i_lo = ion_id * ipb;
-    const int i_hi = min(i_lo + ipb, P.n_ions);
-    // whether this thread's square goes to the exchange (c_mid) or is final (c_env_out)
-    bool out_live = false;
-    {
-        const int r = r0 + tid / TX, cc = cc0 + tid % TX;
-        if (r < ny && cc < nx) out_live = A.live[so + r * nx + cc] != 0;
-    }
     for (int i = i_lo; i < i_hi; ++i) {
-        const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
-        const double* D = A.D_env + (size_t)i * P.n_sq + so;
         const double cb = P.c_bound[i];
         const double zi = P.zq[i];
         const double zq_kt = (zi * P.q) / P.kbT;        // RECIP: alpha = D * zq_kt
         __syncthreads();
-        // concentrations with the world boundary reset to c_env_bound (Q3)
-        // (all loads first, from clamped in-grid addresses, then the LDS writes)
-        constexpr int NC = (VR * VC + TY * TX - 1) / (TY * TX);
-        constexpr int ND = (ER * EC + TY * TX - 1) / (TY * TX);
-        double tc[NC], td[ND];
-#pragma unroll
-        for (int it = 0; it < NC; ++it) {
-            const int e = min(tid + it * TY * TX, VR * VC - 1);
-            const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
-            tc[it] = cenv[clampi(r, ny) * nx + clampi(cc, nx)];
-        }
-        // the cluster's D_env of this ion when it is one value (no loads), else the array
-        const double du = A.kcl[et[0]].denv_u[i];
-        if (du == du) {
-#pragma unroll
-            for (int it = 0; it < ND; ++it) td[it] = du;
-        } else {
-#pragma unroll
-            for (int it = 0; it < ND; ++it) {
-                const int e = min(tid + it * TY * TX, ER * EC - 1);
-                const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
-                td[it] = D[clampi(r, ny) * nx + clampi(cc, nx)];
-            }
-        }
+        // concentrations of ion i (loaded one ion ahead) with the world
+        // boundary reset to c_env_bound (Q3)
 #pragma unroll
         for (int it = 0; it < NC; ++it) {
             const int e = tid + it * TY * TX;
@@ -772,6 +832,8 @@ k_env_field(FieldArgs A, KParams P) {
                 sD[e / EC][e % EC] = (r >= 0 && r < ny && cc >= 0 && cc < nx) ? td[it] : 0.0;
             }
         }
+        // the next ion's loads, in flight through this ion's flux stages
+        if (i + 1 < i_hi) load_ion(i + 1, s_du[i + 1]);
         __syncthreads();
         // Nernst-Planck fluxes (sim_toolbox.py:409-411, gv = -E, u = 0, mu = 0)
         for (int e = tid; e < TY * (TX + 2); e += TY * TX) {
@@ -822,7 +884,9 @@ k_env_field(FieldArgs A, KParams P) {
                 const double div = gx + gy;
                 const size_t e = (size_t)i * P.n_sq + so + r * nx + cc;
                 const double cn = sC[lr + 2][lc + 2] + div * P.dt;
-                if (out_live) A.c_mid[e] = cn;
+                unsigned lv = live_b;
+                asm volatile("" : "+v"(lv));
+                if (lv != 0) A.c_mid[e] = cn;
                 else A.c_env_out[e] = cn;
                 if (A.obs_fx) {
                     A.obs_fx[(size_t)i * P.n_sq + so + r * nx + cc] = sFx[lr][lc + 1];
@@ -831,36 +895,7 @@ k_env_field(FieldArgs A, KParams P) {
             }
         }
     }
-    // The gating rates of every (channel, cell) for this step's k_mem_cell.
-    // They depend on the cell's Vmem alone, so evaluating them here, one
-    // (channel, cell) per thread, replaces one FP64 evaluation per membrane.
-    if (RATES) {
-        const int nc = P.n_cells;
-        const int items = P.n_chan * nc;
-        const int nthr = gridDim.x * gridDim.y * (TY * TX);
-        for (int t = (blockIdx.y * gridDim.x + blockIdx.x) * (TY * TX) + tid; t < items; t += nthr) {
-            const int ch = t / nc, cell = t - ch * nc;
-            // Vmem carrying the n_ions + ch nonces of the membrane stage
-            const double V = with_nonces(A.vm[cell], P.n_ions + ch) * 1000.0 + 0.0;
-            double mInf, mTau, hInf, hTau;
-            gate_rates(P.chan_kind[ch], V, mInf, mTau, hInf, hTau);
-            double* R = A.rates + (size_t)(4 * ch) * nc + cell;
-            if (A.gate_coef) {
-                // update_mh as x' = a x + b: a = tau/(tau+dt), b = dt*inf/(tau+dt)
-                // (FP64, an ulp from the quotient form; the FP32-flux path only)
-                const double dtm = P.chan_dt;
-                R[0] = mTau / (mTau + dtm);
-                R[nc] = (dtm * mInf) / (mTau + dtm);
-                R[2 * nc] = hTau / (hTau + dtm);
-                R[3 * nc] = (dtm * hInf) / (hTau + dtm);
-            } else {
-                R[0] = mInf;
-                R[nc] = mTau;
-                R[2 * nc] = hInf;
-                R[3 * nc] = hTau;
-            }
-        }
-    }
+    if (RATES) gating_rates();
 }
 
 // ---------------------------------------------------------------------------
