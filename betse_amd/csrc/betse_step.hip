@@ -140,10 +140,47 @@ __device__ __forceinline__ bool halted(const unsigned* flags) {
 // ---------------------------------------------------------------------------
 // flux laws
 // ---------------------------------------------------------------------------
-// IEEE division by computed values in both paths (a reciprocal-based FP32
-// quotient of those was measured: no gain, larger FP32-path error)
+// Quotient of computed values: the FP64 path divides (IEEE). The FP32-flux
+// path takes the hardware reciprocal (1 ulp) and one Newton step on the
+// quotient, q + (a - b q) / b: within an ulp of the IEEE quotient for
+// operands in the normal range, four instructions instead of the ~10 of the
+// scaled IEEE sequence.
 __device__ __forceinline__ double dv(double a, double b) { return a / b; }
-__device__ __forceinline__ float dv(float a, float b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float q = a * r;
+    return fmaf(fmaf(-b, q, a), r, q);
+}
+
+// exp(x) on the FP32-flux path, for |x| < 80 (the laws' exponents stay far
+// inside): the hardware exp2 (1 ulp) of t = fl(x log2 e), times 1 + r ln 2
+// with r = x log2 e - t carried in FP32 (log2 e as a sum of two floats):
+// within about 2 ulps of exp(x), six instructions instead of the ~11 of the
+// range-reduced library expf. The FP64 path calls exp.
+__device__ __forceinline__ float exp32(float x) {
+    constexpr float L_HI = 1.44269502162933349609375f;     // log2(e) rounded to FP32
+    constexpr float L_LO = 1.925963033500011e-08f;          // log2(e) - L_HI
+    const float t = x * L_HI;
+    const float r = fmaf(x, L_LO, fmaf(x, L_HI, -t));
+    const float e2 = __builtin_amdgcn_exp2f(t);
+    return fmaf(e2, r * 0.693147182f, e2);
+}
+// expm1(x) on the FP32-flux path for |x| <= 1/2: its Taylor series to x^9
+// in Horner form, x + x^2 (1/2! + x (1/3! + ... + x / 9!)) (truncation below
+// 1.1e-8 relative, about 1 ulp of rounding), ten instructions.
+__device__ __forceinline__ float expm1_32(float x) {
+    float p = 2.75573192e-6f;                 // 1/9!
+    p = fmaf(p, x, 2.48015873e-5f);           // 1/8!
+    p = fmaf(p, x, 1.98412698e-4f);           // 1/7!
+    p = fmaf(p, x, 1.38888889e-3f);           // 1/6!
+    p = fmaf(p, x, 8.33333333e-3f);           // 1/5!
+    p = fmaf(p, x, 4.16666667e-2f);           // 1/4!
+    p = fmaf(p, x, 1.66666667e-1f);           // 1/3!
+    p = fmaf(p, x, 0.5f);                     // 1/2!
+    return fmaf(p * x, x, x);
+}
+__device__ __forceinline__ float exp_path(float x) { return exp32(x); }
+__device__ __forceinline__ double exp_path(double x) { return exp(x); }
 
 // electroflux (sim_toolbox.py:18-69); vBA already carries its +1e-25.
 template <typename FT>
@@ -164,10 +201,10 @@ __device__ __forceinline__ float ghk32(float cA, float cB, float Dc, float zc, f
     const float alpha = (zc * vBA) * f_rt;
     float exp_alpha, deno;
     if (fabsf(alpha) > 0.5f) {
-        exp_alpha = exp(-alpha);
+        exp_alpha = exp32(-alpha);
         deno = 1.0f - exp_alpha;
     } else {
-        const float em1 = expm1(-alpha);
+        const float em1 = expm1_32(-alpha);
         exp_alpha = 1.0f + em1;
         deno = -em1;
     }
@@ -208,7 +245,7 @@ __device__ __forceinline__ FT pump_nak(FT cNai, FT cNao, FT cKi, FT cKo, FT Vm, 
     FT Qdenomo = (FT)P.qd_scal * cube(c) * (d * d);
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-15;
     FT Q = dv(Qnumo, Qdenomo);
-    FT Keq = exp(-((FT)P.dG_RT - dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
+    FT Keq = exp_path(-((FT)P.dG_RT - dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
     FT na = cube(dvc<FT>(cNai, P.km_na, P.inv_kmna32));
     FT kk = dvc<FT>(cKo, P.km_k, P.inv_kmk32);
     FT k2 = kk * kk;
@@ -225,7 +262,7 @@ __device__ __forceinline__ FT pump_ca(FT cCai, FT cCao, FT Vm, const KParams& P)
     FT Qdenomo = (FT)P.cATP * cCai;
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-16;
     FT Q = dv(Qnumo, Qdenomo);
-    FT Keq = exp(-((FT)P.dG_RT - (FT)2 * dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
+    FT Keq = exp_path(-((FT)P.dG_RT - (FT)2 * dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
     FT numo_E = dvc<FT>(cCai, P.km_ca, P.inv_kmca32) * (FT)P.atp_kmca;
     FT denomo_E = ((FT)1 + dvc<FT>(cCai, P.km_ca, P.inv_kmca32)) * (FT)P.one_p_atp_kmca;
     FT frwd = dv(numo_E, denomo_E);
