@@ -1649,6 +1649,15 @@ static int copy_back(bt_ctx* c) {
     return BT_OK;
 }
 
+// the FP32-flux path's elementary functions over n floats (test hook)
+__global__ void k_fp32_math(const float* x, const float* y, float* out, int n, int fn) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (fn == 0) out[t] = exp32(x[t]);
+    else if (fn == 1) out[t] = expm1_32(x[t]);
+    else out[t] = dv(x[t], y[t]);
+}
+
 extern "C" {
 
 int bt_version(void) { return 200; }
@@ -2044,6 +2053,13 @@ int bt_gemm_f64(bt_ctx* c, const double* A, const double* B, double* C, int32_t 
     if (!c || !A || !B || !C || m < 16 || n < 16 || k < 16 || (m | n | k) % 16)
         return set_err(c, BT_EINVAL, "gemm sizes must be positive multiples of 16");
     return launch_gemm(c, A, B, C, m, n, k);
+}
+
+int bt_fp32_math(const float* x, const float* y, float* out, int32_t n, int32_t fn, void* stream) {
+    if (!x || !out || n < 0 || fn < 0 || fn > 2 || (fn == 2 && !y)) return BT_EINVAL;
+    if (n == 0) return BT_OK;
+    hipLaunchKernelGGL(k_fp32_math, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, y, out, n, fn);
+    return hipGetLastError() == hipSuccess ? BT_OK : BT_EHIP;
 }
 
 // Dirichlet Poisson solve on the interior by DST-I: X = S1 ((S1 B S2) .* invlam) S2

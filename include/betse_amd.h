@@ -254,6 +254,11 @@ int bt_observables(bt_ctx* ctx, const bt_obs* obs);
 int bt_gemm_f64(bt_ctx* ctx, const double* A, const double* B, double* C,
                 int32_t m, int32_t n, int32_t k);
 
+/* The FP32-flux path's elementary functions on n device floats, asynchronous
+ * on `stream` (exposed for tests; no counterpart in the reference, whose laws
+ * are FP64): fn 0 out = exp(x), 1 out = expm1(x) for |x| <= 1/2, 2 out = x / y. */
+int bt_fp32_math(const float* x, const float* y, float* out, int32_t n, int32_t fn, void* stream);
+
 /* ------------------------------------------------------------------------
  * Warm-start graph network (no counterpart in the reference: it predicts the
  * state the init phase reaches, from which the plain stepper then runs to its
