@@ -5,7 +5,7 @@
 // three kernels on one stream:
 //
 //   k_env_field     one block per (16x16 tile of the environment grid, pair
-//                   of ions) (per tile over all ions on large batches), staged
+//                   of ions) below 1024 tiles, else per tile over all ions, staged
 //                   through LDS: the surface-charge field u of the previous
 //                   update_V -> SciPy Gaussian (sigma 1, constant 0) -> v_env ->
 //                   E = -grad(screen*v_env) (ion_current.py:93-109), then each
@@ -62,7 +62,9 @@ constexpr int TMB = BT_TILE_MEMBRANES;   // membranes per k_mem_cell tile (one w
 constexpr int BLOCK = 256;               // threads per per-square kernel block
 constexpr int TY = 16, TX = 16;          // k_env_field tile
 #ifndef ENV_SPLIT_TILES
-#define ENV_SPLIT_TILES 2048               // below this many env tiles k_env_field splits blocks by ion
+// below this many env tiles k_env_field splits blocks by ion (measured: pairs
+// of ions win by 35% at 256 tiles, one block per tile by 10% at 1024)
+#define ENV_SPLIT_TILES 1024
 #endif
 #ifndef ENV_IONS_PER_BLOCK
 #define ENV_IONS_PER_BLOCK 2               // ions per k_env_field block when it splits

@@ -245,6 +245,28 @@ def test_many_tiles_single_wavefront_path():
         assert dv <= vm_tol and dc <= rel and de <= rel
 
 
+def test_env_one_block_per_tile_path():
+    """A grid of 1024 or more 16x16 env tiles runs the env kernel with one block
+    per tile looping over all ions, each next ion's loads issued during the
+    previous ion's stages (smaller grids split the ions into pairs): the large
+    cluster on a 512x512 grid, FP64 and FP32 flux paths against the oracle
+    over 16 steps (one replay of the eight-pair step graph)."""
+    from betse_amd.channels import draw_channel_state
+    p = Parameters.preset("large")
+    p.grid_size = 512
+    cells = Cells.make_world(p)
+    ny, nx = cells.X.shape
+    assert ((ny + 15) // 16) * ((nx + 15) // 16) >= 1024
+    draw_channel_state(p, cells.n_cells, cells.n_mems)
+    say(f"case: {cells.n_cells} cells on a {ny}x{nx} grid, one env block per tile, 16 steps")
+    o = oracle_run(p, cells, 16)
+    for mode, vm_tol, rel in (("fp64", VM_TOL, REL_TOL), ("fp32", 1e-6, 1e-6)):
+        sim = gpu_run(p, cells, 16, flux_mode=mode)
+        dv, dc, de, lag, eex = errors(sim, o)
+        say(f"  {mode}: |dVm|={dv:.3e} rel dc={dc:.3e} rel denv={de:.3e}")
+        assert dv <= vm_tol and dc <= rel and de <= rel
+
+
 def test_channels_fp32_flux_parity():
     """The mixed-precision path with Nav1p2 + Kv1p1 (FP32 flux laws, FP64
     gating and state): |dVm| < 1 uV and gates within 1e-5 of the FP64 oracle
