@@ -1359,38 +1359,56 @@ k_obs_hh_out(const double* AA, const double* BB, double* Jx, double* Jy, double*
 }
 
 // C = A*B (row-major, all sizes multiples of 16) with v_mfma_f64_16x16x4f64:
-// one wavefront per 16x16 tile of C. Lane l holds A[l%16][k0+l/16] and
+// one block of four wavefronts per 16x16 tile of C, wavefront w taking the
+// quarter [w K/4, (w+1) K/4) of the inner dimension (a multiple of 4), its
+// operands loaded sixteen k-steps at a time before their MFMAs; the four
+// partial tiles are then added in wavefront order through LDS (a fixed order:
+// the result does not vary between runs). Lane l holds A[l%16][k0+l/16] and
 // B[k0+l/16][l%16]; result element r of lane l is C[l/16+4*r][l%16].
 typedef double v4d __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256)
 k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
+    __shared__ double part[3][64][4];            // partial tiles of wavefronts 1..3
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int tiles_n = n / 16;
-    const int tile = blockIdx.x * 4 + wave;
+    const int tile = blockIdx.x;                 // block-uniform
     if (tile >= (m / 16) * tiles_n) return;
     const int tm = tile / tiles_n, tn = tile % tiles_n;
     const int row = tm * 16 + (lane % 16), col = tn * 16 + (lane % 16), kk = lane / 16;
+    const int kq = kd / 4;
+    const int ke = (wave + 1) * kq;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
-    // eight k-steps of operands are loaded before their eight MFMAs, so the
-    // loads overlap instead of each step waiting on its own (same k order)
-    int k0 = 0;
-    for (; k0 + 32 <= kd; k0 += 32) {
-        double a[8], b[8];
+    int k0 = wave * kq;
+    for (; k0 + 64 <= ke; k0 += 64) {
+        double a[16], b[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 16; ++u) {
             a[u] = A[(size_t)row * kd + k0 + 4 * u + kk];
             b[u] = B[(size_t)(k0 + 4 * u + kk) * n + col];
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+        for (int u = 0; u < 16; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
     }
-    for (; k0 < kd; k0 += 4) {
+    for (; k0 < ke; k0 += 4) {
         const double a = A[(size_t)row * kd + k0 + kk];
         const double b = B[(size_t)(k0 + kk) * n + col];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
+    if (wave > 0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) C[(size_t)(tm * 16 + kk + 4 * r) * n + col] = acc[r];
+        for (int r = 0; r < 4; ++r) part[wave - 1][lane][r] = acc[r];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            double sum = acc[r];
+            sum = sum + part[0][lane][r];
+            sum = sum + part[1][lane][r];
+            sum = sum + part[2][lane][r];
+            C[(size_t)(tm * 16 + kk + 4 * r) * n + col] = sum;
+        }
+    }
 }
 
 // per-cluster max over cells and ions of |c - c_prev| / |c| (init residual)
@@ -2046,7 +2064,7 @@ int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs
 
 static int launch_gemm(bt_ctx* c, const double* A, const double* B, double* C, int m, int n, int k) {
     const int tiles = (m / 16) * (n / 16);
-    hipLaunchKernelGGL(k_gemm_f64, dim3((tiles + 3) / 4), dim3(256), 0, c->stream, A, B, C, m, n, k);
+    hipLaunchKernelGGL(k_gemm_f64, dim3(tiles), dim3(256), 0, c->stream, A, B, C, m, n, k);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }

@@ -381,20 +381,23 @@ def test_mfma_gemm_f64():
     ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
     sim.init_core(ph)
     sim.prepare(ph)
-    say("case: mfma gemm 32x48x64")
     rng = np.random.RandomState(2)
-    a = rng.randn(32, 48)
-    b = rng.randn(48, 64)
-    with torch.cuda.stream(sim.stream):
-        A = torch.tensor(a, device="cuda:0")
-        B = torch.tensor(b, device="cuda:0")
-        Cm = torch.full((32, 64), float("nan"), dtype=torch.float64, device="cuda:0")
-    sim.stream.synchronize()
-    rc = sim._lib.bt_gemm_f64(sim._ctx, ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(B.data_ptr()),
-                              ctypes.c_void_p(Cm.data_ptr()), 32, 64, 48)
-    assert rc == _native.BT_OK
-    sim.stream.synchronize()
-    np.testing.assert_allclose(Cm.cpu().numpy(), a @ b, rtol=1e-12, atol=1e-12)
+    # inner dimensions whose quarters take the tail loop only (48), one block of
+    # sixteen k-steps (256) and two blocks (512)
+    for m, k, n in ((32, 48, 64), (256, 256, 256), (64, 512, 32)):
+        say(f"case: mfma gemm {m}x{k}x{n}")
+        a = rng.randn(m, k)
+        b = rng.randn(k, n)
+        with torch.cuda.stream(sim.stream):
+            A = torch.tensor(a, device="cuda:0")
+            B = torch.tensor(b, device="cuda:0")
+            Cm = torch.full((m, n), float("nan"), dtype=torch.float64, device="cuda:0")
+        sim.stream.synchronize()
+        rc = sim._lib.bt_gemm_f64(sim._ctx, ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(B.data_ptr()),
+                                  ctypes.c_void_p(Cm.data_ptr()), m, n, k)
+        assert rc == _native.BT_OK
+        sim.stream.synchronize()
+        np.testing.assert_allclose(Cm.cpu().numpy(), a @ b, rtol=1e-12, atol=1e-11)
 
 
 @pytest.mark.parametrize("preset", ["tiny", "tiny_channels"])
