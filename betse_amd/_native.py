@@ -119,11 +119,30 @@ class BtObs(ctypes.Structure):
     _fields_ = [(n, _vp) for n in OBS_FIELDS]
 
 
+class BtTuning(ctypes.Structure):
+    _fields_ = [("mem_waves", ctypes.c_int32), ("env_ions_per_block", ctypes.c_int32),
+                ("exchange_form", ctypes.c_int32), ("graph_pairs", ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+# bt_autotune's trial order
+AUTOTUNE_TRIALS = ("mem_waves=1", "mem_waves=2", "env_ions_per_block=1", "env_ions_per_block=2",
+                   "env_ions_per_block=3", "env_ions_per_block=n_ions", "exchange_form=0", "exchange_form=1")
+
+
 # symbol -> (restype, argtypes); every function include/betse_amd.h declares
 SIGNATURES = {
     "bt_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(BtParams), ctypes.POINTER(BtCluster),
                                   ctypes.c_int32, ctypes.POINTER(BtBuffers), _vp]),
     "bt_destroy": (ctypes.c_int, [_vp]),
+    "bt_validate": (ctypes.c_int, [ctypes.POINTER(BtParams), ctypes.POINTER(BtCluster), ctypes.c_int32,
+                                    ctypes.POINTER(BtBuffers)]),
+    "bt_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(BtTuning)]),
+    "bt_set_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(BtTuning)]),
+    "bt_autotune": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(BtTuning), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.c_int32]),
     "bt_update_v": (ctypes.c_int, [_vp]),
     "bt_step": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32]),
     "bt_sync": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),

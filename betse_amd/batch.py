@@ -21,17 +21,20 @@ import numpy as np
 
 from . import _native
 from .cells import Cells
-from .channels import CHANNEL_KINDS, draw_channel_state
+from .channels import draw_channel_state
 from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableException
 from .parameters import Parameters
-from .sim import HostState, SimPhase, SimPhaseKind, gaussian_taps, make_tiles
+from .assemble import (BUFFER_FIELDS, batch_geometry, batch_state, check_compatible, make_clusters,
+                       make_params)
+from .sim import HostState, SimPhase, SimPhaseKind
 
 
 class BatchSimulator:
     """A batch of independent clusters on one device."""
 
     def __init__(self, worlds: Sequence[Tuple[Parameters, Cells]], *, device: Optional[str] = None,
-                 flux_mode: str = "fp32", use_graph: bool = True):
+                 flux_mode: str = "fp32", use_graph: bool = True, autotune: bool = False,
+                 tuning: Optional[dict] = None):
         import torch
         if not torch.cuda.is_available():
             raise BetseSimException("no GPU visible: the GPU simulator has no CPU fallback")
@@ -47,6 +50,11 @@ class BatchSimulator:
         self.worlds = [(p, c) for p, c in worlds]
         self.states: List[HostState] = []
         self._dev = {}
+        # launch variants: forced (tuning), measured on the batch (autotune), or
+        # bt_create's choice by problem size
+        self.autotune = autotune
+        self.tuning = tuning
+        self.autotune_report = None
 
     # ------------------------------------------------------------------
     def init(self, kind: str = "init") -> None:
@@ -76,7 +84,7 @@ class BatchSimulator:
         state (pulled from this batch); this batch is left as it is."""
         self.pull()
         sub = BatchSimulator([self.worlds[i] for i in keep], device=str(self.device), flux_mode=self.flux_mode,
-                             use_graph=self.use_graph)
+                             use_graph=self.use_graph, autotune=self.autotune, tuning=self.tuning)
         sub.kind = self.kind
         sub.states = [self.states[i] for i in keep]
         sub._check_compatible()
@@ -89,140 +97,34 @@ class BatchSimulator:
         self._build()
 
     def _check_compatible(self) -> None:
-        s0, p0 = self.states[0], self.worlds[0][0]
-        for st, (p, _) in zip(self.states, self.worlds):
-            if st.names != s0.names:
-                raise BetseSimException("clusters of a batch need the same ions")
-            if p.dt != p0.dt or p.T != p0.T or p.alpha_Ca != p0.alpha_Ca:
-                raise BetseSimException("clusters of a batch need the same dt, T and Ca pump")
-            if tuple(st.channel_specs) != tuple(s0.channel_specs):
-                raise BetseSimException("clusters of a batch need the same channels")
-            if st.c_env_bound != s0.c_env_bound:
-                raise BetseSimException("clusters of a batch need the same boundary concentrations")
-            if (st.network is None) != (s0.network is None) or (
-                    st.network is not None and bytes(st.network.pack(p.R, st.T)) != bytes(s0.network.pack(p0.R, s0.T))):
-                raise BetseSimException("clusters of a batch need the same reaction network")
-            if (p.alpha_NaK > 0.0) != (p0.alpha_NaK > 0.0):
-                raise BetseSimException("the NaK pump must be on in all clusters or in none")
+        check_compatible(self.states, [p for p, _ in self.worlds], [c for _, c in self.worlds])
 
     # ------------------------------------------------------------------
     def _build(self) -> None:
         torch = self.torch
-        i32, f64 = torch.int32, torch.float64
         self._destroy()
         sts = self.states
+        params = [p for p, _ in self.worlds]
         cells_l = [c for _, c in self.worlds]
-        C = [c.n_cells for c in cells_l]
-        M = [c.n_mems for c in cells_l]
-        G = [c.X.size for c in cells_l]
-        coff = np.concatenate([[0], np.cumsum(C)])
-        moff = np.concatenate([[0], np.cumsum(M)])
-        soff = np.concatenate([[0], np.cumsum(G)])
-        self.offsets = (coff, moff, soff)
-        nI = len(sts[0].names)
-        nch = len(sts[0].channel_specs)
-
-        def cat(f):
-            return np.concatenate([f(i, c) for i, c in enumerate(cells_l)])
-
-        mem_ptr = np.concatenate([cells_l[0].mem_ptr[:1]] + [c.mem_ptr[1:] + moff[i] for i, c in enumerate(cells_l)])
-        sq_ptr = np.concatenate([cells_l[0].sq_ptr[:1]] + [c.sq_ptr[1:] + moff[i] for i, c in enumerate(cells_l)])
-        sq_mems = cat(lambda i, c: c.sq_mems + moff[i])
-        pos = np.empty(int(moff[-1]), dtype=np.int64)
-        pos[sq_mems] = np.arange(int(moff[-1]))
-        tiles, tcl = [np.zeros(1, dtype=np.int64)], []
-        for i, c in enumerate(cells_l):
-            t = make_tiles(c.mem_ptr)
-            tiles.append(t[1:].astype(np.int64) + coff[i])
-            tcl.append(np.full(len(t) - 1, i, dtype=np.int32))
-        tile_ptr = np.concatenate(tiles)
-        tile_cluster = np.concatenate(tcl)
-        self.n_tiles = len(tile_cluster)
-        sq_cluster = cat(lambda i, c: np.full(c.X.size, i, dtype=np.int32))
-        counts = np.diff(sq_ptr)
-
-        host = {
-            "mem_ptr": (mem_ptr, i32), "mem_cell": (cat(lambda i, c: c.mem_to_cells + coff[i]), i32),
-            "nn_mem": (cat(lambda i, c: c.nn_i + moff[i]), i32),
-            "nn_cell": (cat(lambda i, c: c.mem_to_cells[c.nn_i] + coff[i]), i32),
-            "mem_sq": (cat(lambda i, c: c.map_mem2ecm + soff[i]), i32),
-            "sq_ptr": (sq_ptr, i32), "sq_mems": (sq_mems, i32), "mem_pos": (pos, i32),
-            "tile_ptr": (tile_ptr, i32), "tile_cluster": (tile_cluster, i32), "sq_cluster": (sq_cluster, i32),
-            "mem_sa": (cat(lambda i, c: c.mem_sa), f64), "cell_vol": (cat(lambda i, c: c.cell_vol), f64),
-            "diviterm": (cat(lambda i, c: c.diviterm), f64), "cell_sa": (cat(lambda i, c: c.cell_sa), f64),
-            "mem_nx": (cat(lambda i, c: c.mem_vects_flat[:, 2]), f64),
-            "mem_ny": (cat(lambda i, c: c.mem_vects_flat[:, 3]), f64),
-            "sq_has_mem": ((counts > 0).astype(np.uint8), torch.uint8),
-            "gjopen": (np.concatenate([st.gj_block * np.ones(st.mdl) * c.gj_default_weights
-                                       for st, c in zip(sts, cells_l)]), f64),
-            "Dm": (np.concatenate([(st.Dm_cells if self.kind is SimPhaseKind.INIT
-                                    else st.Dm_scheduled + st.Dm_base) for st in sts], axis=1), f64),
-            "D_env": (np.concatenate([st.D_env * st.TJ_modulator for st in sts], axis=1), f64),
-            "c_cell": (np.concatenate([st.cc_cells for st in sts], axis=1), f64),
-            "c_lag": (np.concatenate([st.c_lag for st in sts], axis=1), f64),
-            "vm_cell": (np.concatenate([st.vm_cell for st in sts]), f64),
-            "c_env": (np.concatenate([st.cc_env for st in sts], axis=1), f64),
-            "E_x": (np.concatenate([st.E_env_x.ravel() for st in sts]), f64),
-            "E_y": (np.concatenate([st.E_env_y.ravel() for st in sts]), f64),
-            "v_env": (np.concatenate([st.v_env for st in sts]), f64),
-            "gates": (np.concatenate([st.chan_gates for st in sts], axis=2) if nch else np.zeros((1, 2, 1)), f64),
-            "chan_mod": (np.concatenate([st.chan_mod for st in sts], axis=1) if nch else np.ones((1, 1)), f64),
-        }
-        host["cell_cluster"] = (np.concatenate([np.full(n, i, dtype=np.int32) for i, n in enumerate(C)]), i32)
+        geo, self.offsets = batch_geometry(cells_l)
+        soff = self.offsets[2]
+        self.n_tiles = len(geo["tile_ptr"]) - 1
+        host = dict(geo)
+        host.update(batch_state(sts, cells_l, self.kind is SimPhaseKind.SIM))
         net = sts[0].network
-        if net is not None:
-            host["c_net"] = (np.concatenate([st.c_net for st in sts], axis=1), f64)
         self._dev = {}
         d = self._dev
         with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
-            for k, (a, dt) in host.items():
-                d[k] = torch.as_tensor(np.ascontiguousarray(a), dtype=dt).to(self.device)
+            for k, a in host.items():
+                d[k] = torch.from_numpy(a).to(self.device)
             self.stream.synchronize()
 
-        s0, p0 = sts[0], self.worlds[0][0]
-        prm = _native.BtParams()
-        prm.n_cells, prm.n_mems, prm.n_squares = int(coff[-1]), int(moff[-1]), int(soff[-1])
-        prm.n_ions = nI
-        prm.i_na = s0.names.index("Na") if ("Na" in s0.names and "K" in s0.names and p0.alpha_NaK > 0.0) else -1
-        prm.i_k = s0.names.index("K") if prm.i_na >= 0 else -1
-        prm.i_ca = s0.names.index("Ca") if ("Ca" in s0.names and p0.alpha_Ca > 0.0) else -1
-        prm.flux_mode = _native.BT_FLUX_FP32 if self.flux_mode == "fp32" else _native.BT_FLUX_FP64
+        p0 = params[0]
         active = self.kind is SimPhaseKind.SIM or getattr(p0, "channels_init_active", True)
-        prm.n_chan = nch if active else 0
-        for k, spec in enumerate(s0.channel_specs):
-            prm.chan_kind[k] = _native.BT_CHAN_KINDS[spec.kind]
-            prm.chan_ion[k] = s0.names.index(CHANNEL_KINDS[spec.kind][0])
-            prm.chan_max_dm[k] = float(spec.max_dm)
-        prm.chan_dt = p0.dt * 1.0e3
-        for i in range(nI):
-            prm.z[i] = float(s0.zs[i])
-            prm.d_gj[i] = float(s0.D_gj[i][0])
-            prm.c_bound[i] = float(s0.c_env_bound[i])
-        prm.dt, prm.T, prm.T_gj = p0.dt, s0.T, p0.T
-        prm.F, prm.R, prm.q, prm.kb, prm.cm, prm.tm = p0.F, p0.R, p0.q, p0.kb, p0.cm, p0.tm
-        prm.gj_len, prm.gj_surface = cells_l[0].gj_len, p0.gj_surface
-        prm.NaK_block, prm.deltaGATP = float(s0.NaKATP_block), p0.deltaGATP
-        prm.cATP, prm.cADP, prm.cPi = p0.cATP, p0.cADP, p0.cPi
-        prm.KmNK_Na, prm.KmNK_K, prm.KmNK_ATP = p0.KmNK_Na, p0.KmNK_K, p0.KmNK_ATP
-        prm.alpha_Ca, prm.KmCa_Ca, prm.KmCa_ATP = p0.alpha_Ca, p0.KmCa_Ca, p0.KmCa_ATP
-        prm.cell_height, prm.eo, prm.er = p0.cell_height, p0.eo, p0.er
-        prm.cell_radius, prm.true_cell_size = p0.cell_radius, p0.true_cell_size
-        for j, w in enumerate(gaussian_taps(1.0)):
-            prm.gauss_w[j] = float(w)
-        prm.smooth_cells, prm.mu = p0.smooth_cells, p0.mu
-
-        cls = (_native.BtCluster * len(sts))()
-        for i, (st, (p, c)) in enumerate(zip(sts, self.worlds)):
-            if c.gj_len != cells_l[0].gj_len:
-                raise BetseSimException("clusters of a batch need the same gap-junction length")
-            cls[i].grid_ny, cls[i].grid_nx = c.X.shape
-            cls[i].sq_off = int(soff[i])
-            cls[i].delta = c.delta
-            cls[i].mean_sa = float(c.memSa_per_envSquare[c.map_mem2ecm].mean())
-            cls[i].ko_env = float(st.ko_env)
-            cls[i].alpha_NaK = p.alpha_NaK
+        prm = make_params(sts, params, cells_l, self.flux_mode, active)
+        cls = make_clusters(sts, params, cells_l, soff)
         buf = _native.BtBuffers()
-        for name, _ in _native.BtBuffers._fields_[:-2]:
+        for name in BUFFER_FIELDS:
             setattr(buf, name, d[name].data_ptr())
         buf.n_tiles = self.n_tiles
         buf.tile_m = _native.BT_TILE_MEMBRANES
@@ -233,10 +135,38 @@ class BatchSimulator:
         if rc != _native.BT_OK:
             raise BetseNativeError(f"bt_create failed ({rc}): {self._lib.bt_last_error(None).decode()}")
         if net is not None:
-            self._net = net.pack(p0.R, s0.T)
+            self._net = net.pack(p0.R, sts[0].T)
             self._check(self._lib.bt_set_network(self._ctx, ctypes.byref(self._net),
                                                  ctypes.c_void_p(d["c_net"].data_ptr())))
         self._check(self._lib.bt_update_v(self._ctx))
+        if self.tuning is not None:
+            self.set_tuning(self.tuning)
+        elif self.autotune:
+            self.run_autotune()
+
+    # ------------------------------------------------------------------
+    def get_tuning(self) -> dict:
+        t = _native.BtTuning()
+        self._check(self._lib.bt_get_tuning(self._ctx, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, tuning: dict) -> None:
+        """Force the step kernels' launch variants (``bt_set_tuning``); keys of
+        ``_native.BtTuning``, missing ones keep their current value."""
+        cur = self.get_tuning()
+        cur.update(tuning)
+        t = _native.BtTuning(**cur)
+        self._check(self._lib.bt_set_tuning(self._ctx, ctypes.byref(t)))
+
+    def run_autotune(self, n_launches: int = 20) -> dict:
+        """Time every launch variant on this batch and keep the fastest of each
+        kernel (``bt_autotune``); returns the choice and the trial times (ms)."""
+        t = _native.BtTuning()
+        ms = (ctypes.c_double * 8)()
+        self._check(self._lib.bt_autotune(self._ctx, int(n_launches), ctypes.byref(t), ms, 8))
+        self.autotune_report = {"chosen": t.as_dict(),
+                                "trial_ms": {k: (v if v >= 0 else None) for k, v in zip(_native.AUTOTUNE_TRIALS, ms)}}
+        return self.autotune_report
 
     # ------------------------------------------------------------------
     def _check(self, rc: int) -> None:

@@ -61,17 +61,13 @@ namespace {
 constexpr int TMB = BT_TILE_MEMBRANES;   // membranes per k_mem_cell tile (one wavefront)
 constexpr int BLOCK = 256;               // threads per per-square kernel block
 constexpr int TY = 16, TX = 16;          // k_env_field tile
-#ifndef ENV_SPLIT_TILES
-// below this many env tiles k_env_field splits blocks by ion (measured: pairs
-// of ions win by 35% at 256 tiles, one block per tile by 10% at 1024)
-#define ENV_SPLIT_TILES 1024
-#endif
-#ifndef ENV_IONS_PER_BLOCK
-#define ENV_IONS_PER_BLOCK 2               // ions per k_env_field block when it splits
-#endif
-#ifndef EX_SPLIT_SQUARES
-#define EX_SPLIT_SQUARES (1 << 19)  // below this many squares with membranes k_env_exchange splits by ion
-#endif
+// Default launch variants by problem size (bt_tuning; bt_autotune measures
+// them per problem): below ENV_SPLIT_TILES env tiles k_env_field splits its
+// blocks by pairs of ions (measured: pairs of ions win by 35% at 256 tiles,
+// one block per tile by 10% at 1024); below EX_SPLIT_SQUARES squares with
+// membranes k_env_exchange runs one wavefront per ion
+constexpr int ENV_SPLIT_TILES = 1024;
+constexpr int EX_SPLIT_SQUARES = 1 << 19;
 constexpr double FLOAT_NONCE = 1.0e-25;  // sim_toolbox.py:50
 
 // Constants derived on the host, each in the reference's evaluation order.
@@ -339,11 +335,6 @@ struct MemCellArgs {
     // observed step only (NULL otherwise): raw fluxes in membrane order
     double* obs_fm; double* obs_fgj; double* obs_fch; double* obs_rate;
 };
-
-// wavefronts per tile of k_mem_cell on problems of fewer than BT_MEM_WG_TILES tiles
-#ifndef BT_MEM_WG
-#define BT_MEM_WG 2
-#endif
 
 // LDS of one k_mem_cell block, shared by its wavefronts
 struct MemLds {
@@ -1072,18 +1063,25 @@ k_cell_vm(const double* c_cell, const double* diviterm, double* vm, KParams P) {
 // per-cell reaction network (MasterOfNetworks.run_loop, networks.py:2805-2982)
 // ---------------------------------------------------------------------------
 
-// (c/Km)**n with numpy's power: exponents 1 and 2 take the exact fast paths
+// (c/Km)**n: exponents 1 and 2 take numpy's exact fast paths; 4 is (x*x)*(x*x)
+// where numpy calls pow (the two differ by at most an ulp or two, and FP64 pow
+// costs an order of magnitude more); other exponents call pow
 __device__ __forceinline__ double hill_pow(double x, double n) {
     if (n == 1.0) return x;
     if (n == 2.0) return x * x;
+    if (n == 4.0) {
+        const double x2 = x * x;
+        return x2 * x2;
+    }
     return pow(x, n);
 }
 
 // activator term (((c/Km)**n)/(1 + (c/Km)**n)) and inhibitor term (1/(1 + (c/Km)**n))
-// (get_influencers, networks.py:5184-5200, 5396-5420)
+// (get_influencers, networks.py:5184-5200, 5396-5420); the reference evaluates
+// (c/Km)**n twice to the same value, here once
 __device__ __forceinline__ double hill_act(double c, double km, double n) {
     const double x = hill_pow(c / km, n);
-    return x / (1.0 + hill_pow(c / km, n));
+    return x / (1.0 + x);
 }
 __device__ __forceinline__ double hill_inh(double c, double km, double n) {
     return 1.0 / (1.0 + hill_pow(c / km, n));
@@ -1103,9 +1101,13 @@ constexpr int NET_WAVES = 4;
 constexpr int NET_BLOCK = NET_CELLS * NET_WAVES;
 
 __global__ void __launch_bounds__(NET_BLOCK)
-k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cells, double dt) {
+k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cells, double dt,
+          unsigned* flags, unsigned long long* bad_step, const long long* step_ctr, int step_ahead) {
+    if (halted(flags)) return;
     __shared__ double s_c[BT_NET_MAX_SPECIES][NET_CELLS];
     __shared__ double s_r[BT_NET_MAX_RATES][NET_CELLS];
+    __shared__ int s_bad;
+    if (threadIdx.x == 0) s_bad = 0;
     const int t = threadIdx.x % NET_CELLS;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / NET_CELLS));
     const int cell = blockIdx.x * NET_CELLS + t;
@@ -1163,17 +1165,27 @@ k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cel
         s_r[j][t] = rate;
     }
     __syncthreads();
-    if (!live) return;
-    for (int s = w; s < S; s += NET_WAVES) {
-        double d = 0.0;
-        for (int j = 0; j < R; ++j) {
-            const bt_rate& r = net->rates[j];
-            for (int q = 0; q < r.n_stoich; ++q)
-                if (r.stoich_species[q] == s) d = d + r.stoich_coeff[q] * s_r[j][t];
+    bool bad = false;
+    if (live) {
+        for (int s = w; s < S; s += NET_WAVES) {
+            double d = 0.0;
+            for (int j = 0; j < R; ++j) {
+                const bt_rate& r = net->rates[j];
+                for (int q = 0; q < r.n_stoich; ++q)
+                    if (r.stoich_species[q] == s) d = d + r.stoich_coeff[q] * s_r[j][t];
+            }
+            double v = s_c[s][t] + d * dt;
+            if (v < 0.0) v = 0.0;                    // stb.no_negs clamps negatives ...
+            if (isnan(v)) bad = true;                // ... and raises on NaN
+            cn[(size_t)s * n_cells + cell] = v;
         }
-        double v = s_c[s][t] + d * dt;
-        if (v < 0.0) v = 0.0;                    // stb.no_negs
-        cn[(size_t)s * n_cells + cell] = v;
+    }
+    if (bad) s_bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && s_bad) {
+        // the step's index: step_ctr counts the steps whose exchange has run
+        atomicOr(flags, 1u);
+        atomicMin(bad_step, (unsigned long long)(*step_ctr - step_ahead));
     }
 }
 
@@ -1455,7 +1467,9 @@ struct bt_ctx {
     int n_live = 0;
     unsigned* flags = nullptr; unsigned long long* bad_step = nullptr; long long* step_ctr = nullptr;
     hipGraph_t graph = nullptr; hipGraphExec_t gexec = nullptr;       // one pair of steps
-    hipGraph_t graph_n = nullptr; hipGraphExec_t gexec_n = nullptr;   // BT_GRAPH_PAIRS pairs
+    hipGraph_t graph_n = nullptr; hipGraphExec_t gexec_n = nullptr;   // tune.graph_pairs pairs
+    bt_tuning tune{};                  // launch variants of the step kernels
+    bool autotuned = false;
     long long steps_done = 0;
     const bt_obs* capture = nullptr;   // set while enqueueing an observed step
     // observables workspace (allocated on first use)
@@ -1546,14 +1560,12 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
     return k;
 }
 
-// Wavefronts per tile of k_mem_cell: two (ions split between them) while one
-// wavefront per tile would leave the SIMDs with few waves to switch between.
-#ifndef BT_GRAPH_PAIRS
-#define BT_GRAPH_PAIRS 8        // pairs of steps per replayed graph (fewer graph launches)
-#endif
-#ifndef BT_MEM_WG_TILES
-#define BT_MEM_WG_TILES 4096
-#endif
+// Default wavefronts per tile of k_mem_cell: two (ions split between them)
+// while one wavefront per tile would leave the SIMDs with few waves to switch
+// between; pairs of steps per replayed graph (fewer graph launches).
+constexpr int BT_MEM_WG_TILES = 4096;
+constexpr int BT_GRAPH_PAIRS = 8;
+constexpr int BT_MAX_GRAPH_PAIRS = 64;
 
 template <int NI, int WG, bool OBS>
 static void launch_mem_cell_o(bt_ctx* c, const MemCellArgs& a) {
@@ -1570,7 +1582,7 @@ static void launch_mem_cell_w(bt_ctx* c, const MemCellArgs& a) {
 }
 template <int NI>
 static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a) {
-    if (c->n_tiles < BT_MEM_WG_TILES) launch_mem_cell_w<NI, BT_MEM_WG>(c, a);
+    if (c->tune.mem_waves == 2) launch_mem_cell_w<NI, 2>(c, a);
     else launch_mem_cell_w<NI, 1>(c, a);
 }
 
@@ -1598,12 +1610,11 @@ static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, co
                 c->flags, ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
                 vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr,
                 c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0, c->live_flag, env_out};
-    // diffusion: one block per (tile, ENV_IONS_PER_BLOCK ions) while tiles alone
-    // would leave CUs idle, else one block per tile looping over the ions; field
-    // only: one block per tile
-    const bool split = diffuse && c->n_env_tiles < ENV_SPLIT_TILES;
-    const int ipb = ENV_IONS_PER_BLOCK;
-    dim3 grid(c->n_env_tiles, split ? (c->prm.n_ions + ipb - 1) / ipb : 1);
+    // diffusion: one block per (tile, env_ions_per_block ions): few ions per
+    // block while tiles alone would leave CUs idle, else one block per tile
+    // looping over the ions; field only: one block per tile
+    const int ipb = c->tune.env_ions_per_block;
+    dim3 grid(c->n_env_tiles, diffuse ? (c->prm.n_ions + ipb - 1) / ipb : 1);
     const bool recip = c->prm.flux_mode == BT_FLUX_FP32;
     if (diffuse && f.rates && recip)
         hipLaunchKernelGGL((k_env_field<true, true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
@@ -1646,7 +1657,7 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
         ExArgs x{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u, c->flags,
                  c->step_ctr, which < 0 ? 1 : 0, c->sq_live, c->n_live};
         const int nl = c->n_live > 0 ? c->n_live : 1;    // block 0 also counts the step
-        if (c->n_live < EX_SPLIT_SQUARES)
+        if (c->tune.exchange_form == 0)
             hipLaunchKernelGGL(k_env_exchange, dim3((nl + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0,
                                c->stream, x, c->kp);
         else
@@ -1655,7 +1666,40 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     if (which == 3) launch_field(c, false);
     if ((which < 0 || which == 4) && c->net && c->c_net)
         hipLaunchKernelGGL(k_network, dim3((c->prm.n_cells + NET_CELLS - 1) / NET_CELLS), dim3(NET_BLOCK), 0,
-                           c->stream, (const bt_network*)c->net, c->c_net, c->prm.n_cells, c->prm.dt);
+                           c->stream, (const bt_network*)c->net, c->c_net, c->prm.n_cells, c->prm.dt, c->flags,
+                           c->bad_step, (const long long*)c->step_ctr, which < 0 ? 1 : 0);
+}
+
+static void drop_graphs(bt_ctx* c) {
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (c->gexec_n) { (void)hipGraphExecDestroy(c->gexec_n); c->gexec_n = nullptr; }
+    if (c->graph_n) { (void)hipGraphDestroy(c->graph_n); c->graph_n = nullptr; }
+}
+
+// capture and instantiate the step graphs (one pair, and tune.graph_pairs
+// pairs when that is more than one) if they do not exist yet
+static int capture_graphs(bt_ctx* c) {
+    auto capture = [&](int npairs, hipGraph_t* g, hipGraphExec_t* x) -> int {
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < npairs; ++q) {
+            enqueue_step(c, 0);
+            enqueue_step(c, 1);
+        }
+        hipError_t e = hipStreamEndCapture(c->stream, g);
+        if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+        HIP_TRY(c, hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
+        return BT_OK;
+    };
+    if (!c->gexec) {
+        int rc = capture(1, &c->graph, &c->gexec);
+        if (rc != BT_OK) return rc;
+    }
+    if (c->tune.graph_pairs > 1 && !c->gexec_n) {
+        int rc = capture(c->tune.graph_pairs, &c->graph_n, &c->gexec_n);
+        if (rc != BT_OK) return rc;
+    }
+    return BT_OK;
 }
 
 static int copy_back(bt_ctx* c) {
@@ -1669,27 +1713,8 @@ static int copy_back(bt_ctx* c) {
     return BT_OK;
 }
 
-// the FP32-flux path's elementary functions over n floats (test hook)
-__global__ void k_fp32_math(const float* x, const float* y, float* out, int n, int fn) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    if (fn == 0) out[t] = exp32(x[t]);
-    else if (fn == 1) out[t] = expm1_32(x[t]);
-    else out[t] = dv(x[t], y[t]);
-}
-
-extern "C" {
-
-int bt_version(void) { return 200; }
-
-const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
-
-int64_t bt_steps_done(const bt_ctx* ctx) { return ctx ? ctx->steps_done : -1; }
-
-int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_clusters,
-              const bt_buffers* b, void* stream) {
-    if (!out || !p || !b || !cl) return set_err(nullptr, BT_EINVAL, "null argument");
-    *out = nullptr;
+// scalar arguments of bt_create / bt_validate
+static int check_params(const bt_params* p, const bt_cluster* cl, int32_t n_clusters, const bt_buffers* b) {
     if (p->n_ions < 1 || p->n_ions > BT_MAX_IONS) return set_err(nullptr, BT_EINVAL, "n_ions out of range");
     if (p->n_cells < 1 || p->n_mems < 1) return set_err(nullptr, BT_EINVAL, "empty cluster");
     if (n_clusters < 1) return set_err(nullptr, BT_EINVAL, "n_clusters < 1");
@@ -1723,48 +1748,135 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     }
     if (p->n_chan > 0 && (!b->gates || !b->chan_mod))
         return set_err(nullptr, BT_EINVAL, "channel buffers missing");
+    return BT_OK;
+}
+
+// the index arrays of a batch, from host copies (bt_validate's contract)
+static int check_geometry(const bt_params* p, const bt_cluster* cl, int32_t n_clusters, const bt_buffers* h) {
+    const int C = p->n_cells, M = p->n_mems, S = p->n_squares, T = h->n_tiles;
+    const int32_t* tp = h->tile_ptr; const int32_t* mp = h->mem_ptr; const int32_t* tcl = h->tile_cluster;
+    auto bad = [&](const std::string& msg) { return set_err(nullptr, BT_EINVAL, msg); };
+    // tiles and the cell CSR (k_mem_cell indexes LDS by a tile's membranes and cells)
+    if (tp[0] != 0 || tp[T] != C || mp[0] != 0 || mp[C] != M)
+        return bad("tile_ptr/mem_ptr do not cover the cells and membranes");
+    for (int q = 0; q < C; ++q)
+        if (mp[q + 1] <= mp[q]) return bad("a cell has no membranes (mem_ptr must increase)");
+    std::vector<int32_t> cell_cl((size_t)C);
+    for (int t = 0; t < T; ++t) {
+        const int a0 = tp[t], a1 = tp[t + 1];
+        if (a1 <= a0 || a1 - a0 > BT_TILE_CELLS || a1 > C || mp[a1] - mp[a0] > BT_TILE_MEMBRANES)
+            return bad("tile " + std::to_string(t) + " exceeds the tile limits");
+        if (tcl[t] < 0 || tcl[t] >= n_clusters) return bad("tile_cluster out of range");
+        for (int q = a0; q < a1; ++q) cell_cl[q] = tcl[t];
+    }
+    // squares: cluster of each square from the cluster grids
+    for (int q = 0; q < n_clusters; ++q) {
+        const long long k0 = cl[q].sq_off, k1 = k0 + (long long)cl[q].grid_ny * cl[q].grid_nx;
+        for (long long k = k0; k < k1; ++k)
+            if (h->sq_cluster[k] != q) return bad("sq_cluster does not match the cluster grids");
+    }
+    // membranes: owner cell, then gap-junction partner and square within the membrane's cluster
+    for (int c = 0; c < C; ++c)
+        for (int m = mp[c]; m < mp[c + 1]; ++m)
+            if (h->mem_cell[m] != c) return bad("mem_cell is not the CSR owner of membrane " + std::to_string(m));
+    for (int c = 0; c < C; ++c)
+        for (int m = mp[c]; m < mp[c + 1]; ++m) {
+            const int nm = h->nn_mem[m];
+            if (nm < 0 || nm >= M) return bad("nn_mem out of range at membrane " + std::to_string(m));
+            if (h->nn_cell[m] != h->mem_cell[nm]) return bad("nn_cell is not the cell of nn_mem at membrane " + std::to_string(m));
+            if (cell_cl[h->nn_cell[m]] != cell_cl[c]) return bad("a gap-junction partner lies in another cluster");
+            const int k = h->mem_sq[m];
+            if (k < 0 || k >= S) return bad("mem_sq out of range at membrane " + std::to_string(m));
+            if (h->sq_cluster[k] != cell_cl[c]) return bad("mem_sq maps a membrane to another cluster's grid");
+        }
+    // grid-side CSR: every membrane once, under its own square; mem_pos its inverse
+    const int32_t* sp = h->sq_ptr;
+    if (sp[0] != 0 || sp[S] != M) return bad("sq_ptr does not cover the membranes");
+    std::vector<uint8_t> seen((size_t)M, 0);
+    for (int k = 0; k < S; ++k) {
+        if (sp[k + 1] < sp[k]) return bad("sq_ptr decreases");
+        for (int q = sp[k]; q < sp[k + 1]; ++q) {
+            const int m = h->sq_mems[q];
+            if (m < 0 || m >= M || seen[m]) return bad("sq_mems is not a permutation of the membranes");
+            seen[m] = 1;
+            if (h->mem_sq[m] != k) return bad("sq_mems lists a membrane under another square");
+            if (h->mem_pos[m] != q) return bad("mem_pos is not the inverse of sq_mems");
+        }
+    }
+    return BT_OK;
+}
+
+// the FP32-flux path's elementary functions over n floats (test hook)
+__global__ void k_fp32_math(const float* x, const float* y, float* out, int n, int fn) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (fn == 0) out[t] = exp32(x[t]);
+    else if (fn == 1) out[t] = expm1_32(x[t]);
+    else out[t] = dv(x[t], y[t]);
+}
+
+extern "C" {
+
+int bt_version(void) { return 200; }
+
+const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int64_t bt_steps_done(const bt_ctx* ctx) { return ctx ? ctx->steps_done : -1; }
+
+int bt_validate(const bt_params* p, const bt_cluster* cl, int32_t n_clusters, const bt_buffers* h) {
+    if (!p || !cl || !h) return set_err(nullptr, BT_EINVAL, "null argument");
+    int rc = check_params(p, cl, n_clusters, h);
+    if (rc != BT_OK) return rc;
+    const void* req[] = {h->mem_ptr, h->mem_cell, h->nn_mem, h->nn_cell, h->mem_sq, h->sq_ptr, h->sq_mems,
+                         h->mem_pos, h->tile_ptr, h->tile_cluster, h->sq_cluster};
+    for (const void* q : req)
+        if (!q) return set_err(nullptr, BT_EINVAL, "null geometry array");
+    return check_geometry(p, cl, n_clusters, h);
+}
+
+int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_clusters,
+              const bt_buffers* b, void* stream) {
+    if (!out || !p || !b || !cl) return set_err(nullptr, BT_EINVAL, "null argument");
+    *out = nullptr;
+    int rc0 = check_params(p, cl, n_clusters, b);
+    if (rc0 != BT_OK) return rc0;
     const void* req[] = {b->mem_ptr, b->mem_cell, b->nn_mem, b->nn_cell, b->mem_sq, b->mem_pos, b->sq_ptr,
                          b->tile_ptr, b->tile_cluster, b->sq_cluster, b->mem_sa, b->gjopen, b->cell_vol, b->diviterm, b->Dm, b->D_env,
                          b->sq_has_mem, b->c_cell, b->c_lag, b->vm_cell, b->c_env, b->E_x, b->E_y, b->v_env};
     for (const void* q : req)
         if (!q) return set_err(nullptr, BT_EINVAL, "null device buffer");
-    std::vector<int32_t> tp((size_t)b->n_tiles + 1), mp((size_t)p->n_cells + 1), tcl((size_t)b->n_tiles);
+    // every index array the kernels gather or scatter through, checked on host copies
+    const size_t NM = (size_t)p->n_mems, NC = (size_t)p->n_cells, NS = (size_t)p->n_squares;
+    std::vector<int32_t> tp((size_t)b->n_tiles + 1), mp(NC + 1), tcl((size_t)b->n_tiles), sp(NS + 1),
+        mcell(NM), nnm(NM), nnc(NM), msq(NM), sqm(NM), mpos(NM), sqc(NS);
+    std::vector<uint8_t> hm(NS);
     {
-        // k_mem_cell indexes LDS by a tile's membranes and cells: check every tile
-        hipError_t e = hipMemcpy(tp.data(), b->tile_ptr, tp.size() * 4, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(mp.data(), b->mem_ptr, mp.size() * 4, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(tcl.data(), b->tile_cluster, tcl.size() * 4, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return set_err(nullptr, BT_EHIP, std::string("reading tiles: ") + hipGetErrorString(e));
-        if (tp[0] != 0 || tp[b->n_tiles] != p->n_cells || mp[0] != 0 || mp[p->n_cells] != p->n_mems)
-            return set_err(nullptr, BT_EINVAL, "tile_ptr/mem_ptr do not cover the cells and membranes");
-        for (int t = 0; t < b->n_tiles; ++t) {
-            const int a0 = tp[t], a1 = tp[t + 1];
-            if (a1 <= a0 || a1 - a0 > BT_TILE_CELLS || mp[a1] - mp[a0] > BT_TILE_MEMBRANES)
-                return set_err(nullptr, BT_EINVAL, "tile " + std::to_string(t) + " exceeds the tile limits");
+        struct { void* dst; const void* src; size_t bytes; } cp[] = {
+            {tp.data(), b->tile_ptr, tp.size() * 4}, {mp.data(), b->mem_ptr, mp.size() * 4},
+            {tcl.data(), b->tile_cluster, tcl.size() * 4}, {sp.data(), b->sq_ptr, sp.size() * 4},
+            {mcell.data(), b->mem_cell, NM * 4}, {nnm.data(), b->nn_mem, NM * 4}, {nnc.data(), b->nn_cell, NM * 4},
+            {msq.data(), b->mem_sq, NM * 4}, {sqm.data(), b->sq_mems, NM * 4}, {mpos.data(), b->mem_pos, NM * 4},
+            {sqc.data(), b->sq_cluster, NS * 4}, {hm.data(), b->sq_has_mem, NS}};
+        for (auto& q : cp) {
+            hipError_t e = hipMemcpy(q.dst, q.src, q.bytes, hipMemcpyDeviceToHost);
+            if (e != hipSuccess)
+                return set_err(nullptr, BT_EHIP, std::string("reading the geometry: ") + hipGetErrorString(e));
         }
-        for (int q = 0; q < p->n_cells; ++q)
-            if (mp[q + 1] <= mp[q]) return set_err(nullptr, BT_EINVAL, "a cell has no membranes");
-        for (int t = 0; t < b->n_tiles; ++t)
-            if (tcl[t] < 0 || tcl[t] >= n_clusters) return set_err(nullptr, BT_EINVAL, "tile_cluster out of range");
+        bt_buffers h = *b;
+        h.tile_ptr = tp.data(); h.mem_ptr = mp.data(); h.tile_cluster = tcl.data(); h.sq_ptr = sp.data();
+        h.mem_cell = mcell.data(); h.nn_mem = nnm.data(); h.nn_cell = nnc.data(); h.mem_sq = msq.data();
+        h.sq_mems = sqm.data(); h.mem_pos = mpos.data(); h.sq_cluster = sqc.data();
+        int rc = check_geometry(p, cl, n_clusters, &h);
+        if (rc != BT_OK) return rc;
     }
     // the squares k_env_exchange updates: those with membranes or a surface
     // charge (the others' exchange is the identity and their u stays 0)
     std::vector<int32_t> sq_live;
-    std::vector<uint8_t> live_flag((size_t)p->n_squares, 0);
-    {
-        std::vector<int32_t> sp((size_t)p->n_squares + 1);
-        std::vector<uint8_t> hm((size_t)p->n_squares);
-        hipError_t e = hipMemcpy(sp.data(), b->sq_ptr, sp.size() * 4, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(hm.data(), b->sq_has_mem, hm.size(), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return set_err(nullptr, BT_EHIP, std::string("reading squares: ") + hipGetErrorString(e));
-        if (sp[0] != 0 || sp[p->n_squares] != p->n_mems)
-            return set_err(nullptr, BT_EINVAL, "sq_ptr does not cover the membranes");
-        for (int k = 0; k < p->n_squares; ++k) {
-            if (sp[k + 1] < sp[k]) return set_err(nullptr, BT_EINVAL, "sq_ptr decreases");
-            if (sp[k + 1] > sp[k] || hm[k]) {
-                sq_live.push_back(k);
-                live_flag[k] = 1;
-            }
+    std::vector<uint8_t> live_flag(NS, 0);
+    for (size_t k = 0; k < NS; ++k) {
+        if (sp[k + 1] > sp[k] || hm[k]) {
+            sq_live.push_back((int32_t)k);
+            live_flag[k] = 1;
         }
     }
 
@@ -1853,6 +1965,10 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     alloc((void**)&c->sq_live, sizeof(int32_t) * (sq_live.empty() ? 1 : sq_live.size()));
     alloc((void**)&c->live_flag, ns);
     c->n_live = (int)sq_live.size();
+    c->tune.mem_waves = (c->n_tiles < BT_MEM_WG_TILES) ? 2 : 1;
+    c->tune.env_ions_per_block = (c->n_env_tiles < ENV_SPLIT_TILES) ? std::min(2, p->n_ions) : p->n_ions;
+    c->tune.exchange_form = (c->n_live < EX_SPLIT_SQUARES) ? 0 : 1;
+    c->tune.graph_pairs = BT_GRAPH_PAIRS;
     if (e == hipSuccess && !sq_live.empty())
         e = hipMemcpy(c->sq_live, sq_live.data(), sizeof(int32_t) * sq_live.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->live_flag, live_flag.data(), ns, hipMemcpyHostToDevice);
@@ -1878,10 +1994,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
 int bt_destroy(bt_ctx* c) {
     if (!c) return BT_OK;
     (void)hipStreamSynchronize(c->stream);
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    if (c->gexec_n) (void)hipGraphExecDestroy(c->gexec_n);
-    if (c->graph_n) (void)hipGraphDestroy(c->graph_n);
+    drop_graphs(c);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
                     c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
                     c->B1, c->B2, c->T1, c->T2, c->AA, c->BB, c->vm_prev, c->net};
@@ -1910,32 +2023,15 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     if (n_steps == 0) return BT_OK;
     int pairs = n_steps / 2;
     if (use_graph && pairs > 0) {
-        // a graph of BT_GRAPH_PAIRS pairs of steps for the bulk, one of a single
-        // pair for the rest (each captured on first use)
-        auto capture = [&](int npairs, hipGraph_t* g, hipGraphExec_t* x) -> int {
-            HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            for (int q = 0; q < npairs; ++q) {
-                enqueue_step(c, 0);
-                enqueue_step(c, 1);
-            }
-            hipError_t e = hipStreamEndCapture(c->stream, g);
-            if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-            HIP_TRY(c, hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
-            return BT_OK;
-        };
+        // a graph of tune.graph_pairs pairs of steps for the bulk, one of a
+        // single pair for the rest; both are captured and instantiated on the
+        // first call that replays graphs, so later calls only launch them
+        int rc = capture_graphs(c);
+        if (rc != BT_OK) return rc;
+        const int gp = c->tune.graph_pairs;
         int done = 0;
-        if (BT_GRAPH_PAIRS > 1 && pairs >= BT_GRAPH_PAIRS) {
-            if (!c->gexec_n) {
-                int rc = capture(BT_GRAPH_PAIRS, &c->graph_n, &c->gexec_n);
-                if (rc != BT_OK) return rc;
-            }
-            for (; done + BT_GRAPH_PAIRS <= pairs; done += BT_GRAPH_PAIRS)
-                HIP_TRY(c, hipGraphLaunch(c->gexec_n, c->stream));
-        }
-        if (done < pairs && !c->gexec) {
-            int rc = capture(1, &c->graph, &c->gexec);
-            if (rc != BT_OK) return rc;
-        }
+        if (c->gexec_n)
+            for (; done + gp <= pairs; done += gp) HIP_TRY(c, hipGraphLaunch(c->gexec_n, c->stream));
         for (; done < pairs; ++done) HIP_TRY(c, hipGraphLaunch(c->gexec, c->stream));
     } else {
         for (int s = 0; s < pairs; ++s) {
@@ -1955,6 +2051,117 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     return BT_OK;
 }
 
+int bt_get_tuning(const bt_ctx* c, bt_tuning* out) {
+    if (!c || !out) return BT_EINVAL;
+    *out = c->tune;
+    return BT_OK;
+}
+
+int bt_set_tuning(bt_ctx* c, const bt_tuning* t) {
+    if (!c || !t) return BT_EINVAL;
+    if (t->mem_waves != 1 && t->mem_waves != 2) return set_err(c, BT_EINVAL, "mem_waves must be 1 or 2");
+    if (t->env_ions_per_block < 1 || t->env_ions_per_block > c->prm.n_ions)
+        return set_err(c, BT_EINVAL, "env_ions_per_block must be in 1..n_ions");
+    if (t->exchange_form != 0 && t->exchange_form != 1) return set_err(c, BT_EINVAL, "exchange_form must be 0 or 1");
+    if (t->graph_pairs < 1 || t->graph_pairs > BT_MAX_GRAPH_PAIRS)
+        return set_err(c, BT_EINVAL, "graph_pairs must be in 1..64");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->tune = *t;
+    drop_graphs(c);
+    return BT_OK;
+}
+
+// average ms of one launch of a step kernel (bt_time_kernel's timing)
+static int time_launches(bt_ctx* c, int which, int n, double* ms_out) {
+    hipEvent_t e0, e1;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    enqueue_step(c, 0, which);   // warm-up launch
+    HIP_TRY(c, hipEventRecord(e0, c->stream));
+    for (int s = 0; s < n; ++s) enqueue_step(c, 0, which);
+    HIP_TRY(c, hipEventRecord(e1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_TRY(c, hipGetLastError());
+    *ms_out = (double)ms / n;
+    return BT_OK;
+}
+
+int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_ms, int32_t n_trial_ms) {
+    if (!c || n_launches < 1 || (trial_ms && n_trial_ms < 0)) return BT_EINVAL;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    unsigned flags = 0;
+    HIP_TRY(c, hipMemcpy(&flags, c->flags, sizeof(flags), hipMemcpyDeviceToHost));
+    if (flags) return set_err(c, BT_EINVAL, "autotune needs a stable state (a NaN was flagged)");
+    // the timed launches read the caller's state and write the workspace, except
+    // the gates (in place), the surface charge u and the substances: keep copies
+    const size_t M = (size_t)c->prm.n_mems, NS = (size_t)c->kp.n_sq;
+    const size_t gbytes = c->prm.n_chan > 0 ? (size_t)c->prm.n_chan * 2 * M * 8 : 0;
+    double* save = nullptr;
+    const size_t sbytes = gbytes + NS * 8;
+    HIP_TRY(c, hipMalloc((void**)&save, sbytes));
+    auto keep = [&](bool back) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (gbytes)
+            e = back ? hipMemcpyAsync(c->buf.gates, save, gbytes, hipMemcpyDeviceToDevice, c->stream)
+                     : hipMemcpyAsync(save, c->buf.gates, gbytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess)
+            e = back ? hipMemcpyAsync(c->u, save + gbytes / 8, NS * 8, hipMemcpyDeviceToDevice, c->stream)
+                     : hipMemcpyAsync(save + gbytes / 8, c->u, NS * 8, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        return e;
+    };
+    // (the network kernel is not among the timed variants: the substances are never touched)
+    hipError_t e = keep(false);
+    const bt_tuning prev = c->tune;
+    bt_tuning best = prev;
+    double t[8];
+    for (int q = 0; q < 8; ++q) t[q] = -1.0;
+    int rc = BT_OK;
+    if (e == hipSuccess) {
+        // membrane kernel: one or two wavefronts per tile
+        double bm = 1e30;
+        for (int w = 1; w <= 2 && rc == BT_OK; ++w) {
+            c->tune.mem_waves = w;
+            rc = time_launches(c, 0, n_launches, &t[w - 1]);
+            if (rc == BT_OK && t[w - 1] < bm) { bm = t[w - 1]; best.mem_waves = w; }
+        }
+        c->tune = prev;
+        // env kernel: 1, 2, 3 or all ions per block
+        const int cand[4] = {1, 2, 3, c->prm.n_ions};
+        double be = 1e30;
+        for (int q = 0; q < 4 && rc == BT_OK; ++q) {
+            if (q < 3 && cand[q] >= c->prm.n_ions) continue;     // the same as all ions
+            c->tune.env_ions_per_block = cand[q];
+            rc = time_launches(c, 1, n_launches, &t[2 + q]);
+            if (rc == BT_OK && t[2 + q] < be) { be = t[2 + q]; best.env_ions_per_block = cand[q]; }
+        }
+        c->tune = prev;
+        // exchange: per ion or per square
+        double bx = 1e30;
+        for (int f = 0; f <= 1 && rc == BT_OK; ++f) {
+            c->tune.exchange_form = f;
+            rc = time_launches(c, 2, n_launches, &t[6 + f]);
+            if (rc == BT_OK && t[6 + f] < bx) { bx = t[6 + f]; best.exchange_form = f; }
+        }
+        c->tune = prev;
+        e = keep(true);
+    }
+    (void)hipFree(save);
+    if (rc != BT_OK) return rc;
+    if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("autotune state copy: ") + hipGetErrorString(e));
+    c->tune = best;
+    c->autotuned = true;
+    drop_graphs(c);
+    if (chosen) *chosen = best;
+    if (trial_ms)
+        for (int q = 0; q < n_trial_ms && q < 8; ++q) trial_ms[q] = t[q];
+    return BT_OK;
+}
+
 int bt_sync(bt_ctx* c, int64_t* bad_step) {
     if (!c) return BT_EINVAL;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1970,20 +2177,7 @@ int bt_sync(bt_ctx* c, int64_t* bad_step) {
 int bt_time_kernel(bt_ctx* c, int32_t kernel, int32_t n_launches, double* ms_out) {
     if (!c || !ms_out || n_launches < 1 || kernel < 0 || kernel > 4) return BT_EINVAL;
     if (kernel == 4 && !c->c_net) return set_err(c, BT_EINVAL, "no reaction network attached");
-    hipEvent_t e0, e1;
-    HIP_TRY(c, hipEventCreate(&e0));
-    HIP_TRY(c, hipEventCreate(&e1));
-    enqueue_step(c, 0, kernel);   // warm-up launch
-    HIP_TRY(c, hipEventRecord(e0, c->stream));
-    for (int s = 0; s < n_launches; ++s) enqueue_step(c, 0, kernel);
-    HIP_TRY(c, hipEventRecord(e1, c->stream));
-    HIP_TRY(c, hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *ms_out = (double)ms / n_launches;
-    return BT_OK;
+    return time_launches(c, kernel, n_launches, ms_out);
 }
 
 int bt_set_network(bt_ctx* c, const bt_network* net, double* c_net) {
@@ -2017,10 +2211,7 @@ int bt_set_network(bt_ctx* c, const bt_network* net, double* c_net) {
         c->c_net = nullptr;
     }
     // captured steps do not match the new step body any more
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    if (c->gexec_n) { (void)hipGraphExecDestroy(c->gexec_n); c->gexec_n = nullptr; }
-    if (c->graph_n) { (void)hipGraphDestroy(c->graph_n); c->graph_n = nullptr; }
+    drop_graphs(c);
     return BT_OK;
 }
 

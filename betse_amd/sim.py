@@ -32,8 +32,7 @@ from .cells import Cells
 from .channels import CHANNEL_KINDS, draw_channel_state, init_gates
 from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableException
 from .parameters import Parameters
-
-TILE_M = _native.BT_TILE_MEMBRANES   # membranes per cell tile of k_mem_cell
+from .assemble import BUFFER_FIELDS, batch_geometry, batch_state, make_clusters, make_params, make_tiles  # noqa: F401
 
 
 class SimPhaseKind(enum.Enum):
@@ -111,26 +110,6 @@ def phase_time_steps(n_steps: int, dt: float, t_resample: float):
         i = int(i)
         sampled.add(time_steps[i])
     return time_steps, sampled
-
-
-def make_tiles(mem_ptr: np.ndarray, tile_m: int = TILE_M, tile_c: int = _native.BT_TILE_CELLS) -> np.ndarray:
-    """Greedy packing of consecutive whole cells into tiles of <= tile_m
-    membranes and <= tile_c cells."""
-    n = len(mem_ptr) - 1
-    counts = np.diff(mem_ptr)
-    if counts.max(initial=0) > tile_m:
-        raise BetseSimException(f"a cell has more than {tile_m} membranes")
-    if counts.min(initial=1) < 1:
-        raise BetseSimException("a cell has no membranes")
-    tiles = [0]
-    acc = 0
-    for c in range(n):
-        if acc + counts[c] > tile_m or c - tiles[-1] == tile_c:
-            tiles.append(c)
-            acc = 0
-        acc += counts[c]
-    tiles.append(n)
-    return np.asarray(tiles, dtype=np.int32)
 
 
 class HostState:
@@ -326,7 +305,7 @@ class Simulator(HostState):
     """
 
     def __init__(self, p: Optional[Parameters] = None, *, device: Optional[str] = None,
-                 flux_mode: str = "fp64", use_graph: bool = True):
+                 flux_mode: str = "fp64", use_graph: bool = True, tuning: Optional[dict] = None):
         import torch
         if not torch.cuda.is_available():
             raise BetseSimException("no GPU visible: the GPU simulator has no CPU fallback")
@@ -343,6 +322,8 @@ class Simulator(HostState):
         self._geom_key = None
         self._dev: Dict[str, "torch.Tensor"] = {}
         self.exception_instability = None
+        self.tuning = tuning            # forced launch variants (bt_set_tuning) or None
+        self.autotune_report = None
 
     # ------------------------------------------------------------------
     # device set-up
@@ -355,118 +336,34 @@ class Simulator(HostState):
         if self._geom_key is cells:
             return
         torch = self.torch
-        i32 = torch.int32
-        f64 = torch.float64
-        G2 = cells.X.size
-        counts = np.diff(cells.sq_ptr)
-        d = self._dev
-        d["mem_ptr"] = self._t(cells.mem_ptr, i32)
-        d["mem_cell"] = self._t(cells.mem_to_cells, i32)
-        d["nn_mem"] = self._t(cells.nn_i, i32)
-        d["nn_cell"] = self._t(cells.mem_to_cells[cells.nn_i], i32)
-        d["mem_sq"] = self._t(cells.map_mem2ecm, i32)
-        d["sq_ptr"] = self._t(cells.sq_ptr, i32)
-        d["sq_mems"] = self._t(cells.sq_mems, i32)
-        pos = np.empty(cells.n_mems, dtype=np.int64)
-        pos[cells.sq_mems] = np.arange(cells.n_mems)
-        d["mem_pos"] = self._t(pos, i32)
-        tiles = make_tiles(cells.mem_ptr)
-        d["tile_ptr"] = self._t(tiles, i32)
-        self.n_tiles = len(tiles) - 1
-        d["tile_cluster"] = self._t(np.zeros(self.n_tiles, dtype=np.int32), i32)
-        d["sq_cluster"] = self._t(np.zeros(G2, dtype=np.int32), i32)
-        d["mem_sa"] = self._t(cells.mem_sa, f64)
-        d["cell_vol"] = self._t(cells.cell_vol, f64)
-        d["diviterm"] = self._t(cells.diviterm, f64)
-        d["sq_has_mem"] = self._t((counts > 0).astype(np.uint8), torch.uint8)
-        d["mem_nx"] = self._t(cells.mem_vects_flat[:, 2], f64)
-        d["mem_ny"] = self._t(cells.mem_vects_flat[:, 3], f64)
-        d["cell_sa"] = self._t(cells.cell_sa, f64)
-        if G2 != len(counts):
-            raise BetseSimException("grid-side CSR does not match the grid")
+        geo, _ = batch_geometry([cells])
+        self.n_tiles = len(geo["tile_ptr"]) - 1
+        for k, a in geo.items():
+            self._dev[k] = torch.from_numpy(a).to(self.device)
         self._geom_key = cells
 
     def _device_setup(self, phase: SimPhase) -> None:
         p, cells = phase.p, phase.cells
         torch = self.torch
-        f64 = torch.float64
         self._destroy_ctx()
         with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
             self._upload_geometry(cells)
             d = self._dev
-            Dm = self.Dm_cells if phase.kind is SimPhaseKind.INIT else (self.Dm_scheduled + self.Dm_base)
-            d["Dm"] = self._t(Dm, f64)
-            d["D_env"] = self._t(self.D_env * self.TJ_modulator, f64)
-            d["gjopen"] = self._t(self.gj_block * np.ones(self.mdl) * cells.gj_default_weights, f64)
-            d["c_cell"] = self._t(self.cc_cells, f64)
-            d["c_lag"] = self._t(self.c_lag, f64)
-            d["vm_cell"] = self._t(self.vm_cell, f64)
-            d["c_env"] = self._t(self.cc_env, f64)
-            d["E_x"] = self._t(self.E_env_x.ravel(), f64)
-            d["E_y"] = self._t(self.E_env_y.ravel(), f64)
-            d["v_env"] = self._t(self.v_env, f64)
-            nch = len(self.channel_specs)
-            d["gates"] = self._t(self.chan_gates if nch else np.zeros((1, 2, 1)), f64)
-            d["chan_mod"] = self._t(self.chan_mod if nch else np.ones((1, 1)), f64)
-            if self.network is not None:
-                d["c_net"] = self._t(self.c_net, f64)
+            for k, a in batch_state([self], [cells], phase.kind is SimPhaseKind.SIM).items():
+                d[k] = torch.from_numpy(a).to(self.device)
             self.stream.synchronize()
 
-        prm = _native.BtParams()
-        prm.n_cells, prm.n_mems = self.cdl, self.mdl
-        prm.n_squares = cells.X.size
-        cl = _native.BtCluster()
-        cl.grid_ny, cl.grid_nx = cells.X.shape
-        cl.sq_off = 0
-        cl.delta = cells.delta
-        cl.mean_sa = float(cells.memSa_per_envSquare[cells.map_mem2ecm].mean())
-        cl.ko_env = float(self.ko_env)
-        cl.alpha_NaK = p.alpha_NaK
-        prm.n_ions = len(self.names)
-        prm.i_na = self.names.index("Na") if ("Na" in self.names and "K" in self.names and p.alpha_NaK > 0.0) else -1
-        prm.i_k = self.names.index("K") if prm.i_na >= 0 else -1
-        prm.i_ca = self.names.index("Ca") if ("Ca" in self.names and p.alpha_Ca > 0.0) else -1
-        prm.flux_mode = _native.BT_FLUX_FP32 if self.flux_mode == "fp32" else _native.BT_FLUX_FP64
         active = phase.kind is SimPhaseKind.SIM or getattr(p, "channels_init_active", True)
-        prm.n_chan = len(self.channel_specs) if active else 0
-        for k, spec in enumerate(self.channel_specs):
-            prm.chan_kind[k] = _native.BT_CHAN_KINDS[spec.kind]
-            prm.chan_ion[k] = self.names.index(CHANNEL_KINDS[spec.kind][0])
-            prm.chan_max_dm[k] = float(spec.max_dm)
-        prm.chan_dt = p.dt * 1.0e3        # time_unit of the supported channel models (ms)
-        for i in range(prm.n_ions):
-            prm.z[i] = float(self.zs[i])
-            if not np.all(self.D_gj[i] == self.D_gj[i][0]):
-                raise BetseSimException("the GPU step needs a uniform D_gj per ion")
-            prm.d_gj[i] = float(self.D_gj[i][0])
-            prm.c_bound[i] = float(self.c_env_bound[i])
-        prm.dt, prm.T, prm.T_gj = p.dt, self.T, p.T
-        prm.F, prm.R, prm.q, prm.kb, prm.cm, prm.tm = p.F, p.R, p.q, p.kb, p.cm, p.tm
-        prm.gj_len, prm.gj_surface = cells.gj_len, p.gj_surface
-        prm.NaK_block, prm.deltaGATP = float(self.NaKATP_block), p.deltaGATP
-        prm.cATP, prm.cADP, prm.cPi = p.cATP, p.cADP, p.cPi
-        prm.KmNK_Na, prm.KmNK_K, prm.KmNK_ATP = p.KmNK_Na, p.KmNK_K, p.KmNK_ATP
-        prm.alpha_Ca, prm.KmCa_Ca, prm.KmCa_ATP = p.alpha_Ca, p.KmCa_Ca, p.KmCa_ATP
-        prm.cell_height = p.cell_height
-        prm.eo, prm.er = p.eo, p.er
-        prm.cell_radius, prm.true_cell_size = p.cell_radius, p.true_cell_size
-        for j, w in enumerate(gaussian_taps(1.0)):
-            prm.gauss_w[j] = float(w)
-        prm.smooth_cells = p.smooth_cells
-        prm.mu = p.mu
-
+        prm = make_params([self], [p], [cells], self.flux_mode, active)
+        cl = make_clusters([self], [p], [cells], np.zeros(1, dtype=np.int64))
         buf = _native.BtBuffers()
-        for name in ("mem_ptr", "mem_cell", "nn_mem", "nn_cell", "mem_sq", "sq_ptr", "sq_mems", "mem_pos", "tile_ptr",
-                     "tile_cluster", "sq_cluster",
-                     "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
-                     "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "mem_nx", "mem_ny",
-                     "cell_sa", "gates", "chan_mod"):
+        for name in BUFFER_FIELDS:
             setattr(buf, name, d[name].data_ptr())
         buf.n_tiles = self.n_tiles
         buf.tile_m = _native.BT_TILE_MEMBRANES
         self._prm, self._buf, self._cl = prm, buf, cl
         with torch.cuda.device(self.device):
-            rc = self._lib.bt_create(ctypes.byref(self._ctx), ctypes.byref(prm), ctypes.byref(cl), 1,
+            rc = self._lib.bt_create(ctypes.byref(self._ctx), ctypes.byref(prm), cl, 1,
                                      ctypes.byref(buf), ctypes.c_void_p(self.stream.cuda_stream))
         if rc != _native.BT_OK:
             raise BetseNativeError(f"bt_create failed ({rc}): {self._lib.bt_last_error(None).decode()}")
@@ -474,6 +371,34 @@ class Simulator(HostState):
             self._net = self.network.pack(p.R, self.T)
             self._check(self._lib.bt_set_network(self._ctx, ctypes.byref(self._net),
                                                  ctypes.c_void_p(d["c_net"].data_ptr())))
+        if self.tuning is not None:
+            self.set_tuning(self.tuning)
+
+    # ------------------------------------------------------------------
+    # launch variants of the step kernels
+    # ------------------------------------------------------------------
+    def get_tuning(self) -> dict:
+        t = _native.BtTuning()
+        self._check(self._lib.bt_get_tuning(self._ctx, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, tuning: dict) -> None:
+        """Force the step kernels' launch variants (``bt_set_tuning``); keys of
+        ``_native.BtTuning``, missing ones keep their current value."""
+        cur = self.get_tuning()
+        cur.update(tuning)
+        t = _native.BtTuning(**cur)
+        self._check(self._lib.bt_set_tuning(self._ctx, ctypes.byref(t)))
+
+    def run_autotune(self, n_launches: int = 20) -> dict:
+        """Time every launch variant on this cluster and keep the fastest of
+        each kernel (``bt_autotune``; the state is left as it was)."""
+        t = _native.BtTuning()
+        ms = (ctypes.c_double * 8)()
+        self._check(self._lib.bt_autotune(self._ctx, int(n_launches), ctypes.byref(t), ms, 8))
+        self.autotune_report = {"chosen": t.as_dict(),
+                                "trial_ms": {k: (v if v >= 0 else None) for k, v in zip(_native.AUTOTUNE_TRIALS, ms)}}
+        return self.autotune_report
 
     def _destroy_ctx(self) -> None:
         if self._ctx:

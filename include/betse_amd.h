@@ -191,8 +191,51 @@ typedef struct {
 int bt_create(bt_ctx** out, const bt_params* params, const bt_cluster* clusters, int32_t n_clusters,
               const bt_buffers* buffers, void* stream);
 
+/* Check the geometry arrays of a batch on the host before any kernel gathers
+ * through them (no counterpart in the reference, whose NumPy indexing raises
+ * IndexError on an out-of-range index). `host` holds HOST pointers to
+ * mem_ptr, mem_cell, nn_mem, nn_cell, mem_sq, sq_ptr, sq_mems, mem_pos,
+ * tile_ptr, tile_cluster and sq_cluster (the other fields are not read) and
+ * n_tiles, tile_m. Checks: the cell CSR and the tiles cover every cell and
+ * membrane within the tile limits; mem_cell is the CSR owner of each
+ * membrane; nn_mem is a membrane of the same cluster and nn_cell its cell;
+ * mem_sq is a square of the membrane's cluster; sq_cluster matches the
+ * cluster grids; sq_ptr/sq_mems list every membrane once under its square and
+ * mem_pos is the inverse permutation. Returns BT_OK or BT_EINVAL (message in
+ * bt_last_error(NULL)). bt_create runs the same checks on copies of the
+ * device arrays. Touches no device. */
+int bt_validate(const bt_params* params, const bt_cluster* clusters, int32_t n_clusters,
+                const bt_buffers* host);
+
 /* Release the workspace, graphs and the context. */
 int bt_destroy(bt_ctx* ctx);
+
+/* Launch variants of the step kernels (no counterpart in the reference: the
+ * NumPy loop has no launches). bt_create picks them from the problem size;
+ * bt_set_tuning forces them (captured step graphs are rebuilt) and
+ * bt_autotune times every variant on the context's own problem and keeps the
+ * fastest of each kernel. The step's arithmetic is the same in every variant:
+ * only the division of the work between wavefronts and blocks changes. */
+typedef struct {
+    int32_t mem_waves;          /* k_mem_cell wavefronts per cell tile: 1, or 2 (each a contiguous block of ions) */
+    int32_t env_ions_per_block; /* ions per k_env_field block, 1..n_ions (n_ions: one block per 16x16 tile) */
+    int32_t exchange_form;      /* k_env_exchange: 0 one wavefront per ion over 64 squares, 1 one thread per square */
+    int32_t graph_pairs;        /* pairs of steps per replayed step graph, 1..64 */
+} bt_tuning;
+
+int bt_get_tuning(const bt_ctx* ctx, bt_tuning* out);
+int bt_set_tuning(bt_ctx* ctx, const bt_tuning* tuning);
+
+/* Time each kernel variant over n_launches launches (HIP events on the
+ * context stream) and keep the fastest per kernel; the chosen variants go to
+ * *chosen (may be NULL) and the average ms of each trial to trial_ms (may be
+ * NULL; n_trial_ms entries, unused ones -1) in the order: mem_waves 1, 2;
+ * env_ions_per_block 1, 2, 3, n_ions; exchange_form 0, 1. The state the
+ * timed launches modify in place (channel gates, surface charge, reaction
+ * network substances) is saved before and restored after, so a phase
+ * continues from where it was; the state must be stable (no NaN flagged).
+ * Allocates a temporary copy of that state. Synchronous. */
+int bt_autotune(bt_ctx* ctx, int32_t n_launches, bt_tuning* chosen, double* trial_ms, int32_t n_trial_ms);
 
 /* update_V's voltage part for the current state (sim.py:2007-2083 with
  * ion_current.py:13-114): rho_cells -> vm_cell, rho_env -> Gaussian -> v_env
@@ -209,14 +252,19 @@ int bt_step(bt_ctx* ctx, int32_t n_steps, int32_t use_graph);
 /* Attach a reaction network to the step (NULL net detaches it). c_net is the
  * caller's device buffer [n_species][n_cells] of substance concentrations
  * (mM), updated in place by every following step after the channels and
- * before the ion concentration update, as in sim.py:1286-1300. The network
- * is copied; captured step graphs are rebuilt. */
+ * before the ion concentration update, as in sim.py:1286-1300. A substance
+ * that becomes NaN halts the run like a NaN ion concentration (no_negs on
+ * the substances, networks.py:2939 -> sim_toolbox.py:475-503): bt_sync then
+ * returns BT_UNSTABLE with that step's index. The network is copied;
+ * captured step graphs are rebuilt. */
 int bt_set_network(bt_ctx* ctx, const bt_network* net, double* c_net);
 
 /* Observables of one step (get_current, ion_current.py:13-109; update_gj
  * E_gj, sim.py:2166-2172; update_V vm_ave, sim.py:2038). The reference
  * evaluates them every step; here they are produced for the last step of a
- * bt_step_observed call. All pointers are caller-owned device buffers. */
+ * bt_step_observed call. All pointers are caller-owned device buffers; in a
+ * batch the per-square fields are concatenated like c_env (each cluster's
+ * grid at its sq_off). */
 typedef struct {
     /* raw fluxes of the observed step, written by bt_step_observed */
     double* fluxes_mem;    /* [n_ions][n_mems] fluxes_mem (pumps + GHK)          */
@@ -245,8 +293,9 @@ int bt_step_observed(bt_ctx* ctx, int32_t n_steps, int32_t use_graph, const bt_o
 
 /* Derive the observables from the fluxes captured in obs. The two Poisson
  * solves of HH_Decomp (sim_toolbox.py:1236-1290, dense lapENVinv in the
- * reference) run as DST-I transforms written as FP64 MFMA matrix products.
- * Contexts of one cluster only. */
+ * reference) run as DST-I transforms written as FP64 MFMA matrix products,
+ * one pair per cluster of a batch (each on its own grid). The DST bases and
+ * eigenvalue tables are allocated and built on the first call. */
 int bt_observables(bt_ctx* ctx, const bt_obs* obs);
 
 /* C = A*B for row-major FP64 matrices whose sizes are multiples of 16, on the

@@ -34,7 +34,7 @@ Pinned by ``tests/test_network_oracle.py`` through closed forms: production
 and decay alone follow the explicit Euler recursion of ``dc/dt = p - d c``
 (and reach ``p/d``); an A -> B reaction conserves A + B; a reversible
 reaction at ``Q = Keq`` has zero rate; a Hill activator at ``c = Km`` is one
-half; ``no_negs`` clamps. The eval-string order itself is restated from the
+half; ``no_negs`` clamps and raises on NaN. The eval-string order itself is restated from the
 source, not pinned by a reference run (the reference has no network fixture).
 """
 
@@ -44,8 +44,10 @@ import numpy as np
 
 
 def no_negs(x):
-    x[x < 0] = 0
-    return x
+    """``stb.no_negs`` on a substance (``networks.py:2939`` ->
+    ``sim_toolbox.py:475-503``): negatives clamp to 0, a NaN raises."""
+    from oracle.betse_oracle import no_negs as _no_negs
+    return _no_negs(x)
 
 
 def _act(c, km, n):

@@ -42,7 +42,8 @@ def test_load_fails_loudly_when_missing(tmp_path):
 def test_struct_layout_matches_c(tmp_path):
     structs = {"p": ("bt_params", _native.BtParams), "b": ("bt_buffers", _native.BtBuffers),
                "c": ("bt_cluster", _native.BtCluster), "o": ("bt_obs", _native.BtObs),
-               "r": ("bt_rate", _native.BtRate), "n": ("bt_network", _native.BtNetwork)}
+               "r": ("bt_rate", _native.BtRate), "n": ("bt_network", _native.BtNetwork),
+               "t": ("bt_tuning", _native.BtTuning)}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "betse_amd.h"', 'int main(void){']
     for tag, (cname, cls) in structs.items():
         src.append(f'printf("s {tag} %zu\\n", sizeof({cname}));')

@@ -1,0 +1,215 @@
+"""The measured workloads themselves against the FP64 oracle, and every launch
+variant of the step kernels.
+
+* The launch variants (``bt_tuning``: one or two wavefronts per cell tile,
+  1..n_ions ions per env block, the per-ion or per-square exchange) divide the
+  same arithmetic differently between lanes, so they must give bit-identical
+  states; the default variant matches the oracle at the FP64 bars.
+* The batched workload (256 clusters of 100-5000 cells, ~500k cells) for 16
+  steps, a sample of its clusters (the largest and smallest among them)
+  against per-cluster oracle runs.
+* The full workload (1.7 mm disc, ~100k cells, 512x512 grid, Nav/Kv/Cav and
+  the 8-substance network) for 16 steps against the oracle, then over its
+  whole 20,000-step horizon: FP32-flux against FP64 within 1 uV, and the
+  state stays physical (finite, non-negative, gates in [0, 1]).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from betse_amd.batch import batched_worlds
+from oracle.betse_oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+VM_TOL = 1e-9
+REL_TOL = 1e-10
+
+VARIANTS = [{"mem_waves": 1}, {"mem_waves": 2}, {"env_ions_per_block": 1}, {"env_ions_per_block": 2},
+            {"env_ions_per_block": 3}, {"env_ions_per_block": 5}, {"exchange_form": 0}, {"exchange_form": 1},
+            {"graph_pairs": 1}, {"graph_pairs": 3}]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def batch(worlds, flux_mode, tuning=None, kind="init"):
+    from betse_amd.batch import BatchSimulator
+    b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=flux_mode, tuning=tuning)
+    b.init(kind)
+    return b
+
+
+def state(b):
+    d = b._dev
+    b.sync()
+    return {k: d[k].cpu().numpy().copy() for k in ("c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y")}
+
+
+def oracle_steps(p, cells, n):
+    o = OracleSim(cells, p.copy(), mode="sparse")
+    o.init_core()
+    o.run_sim_core("init", n_steps=n)
+    return o
+
+
+def compare(st, o, tag, vm_tol=VM_TOL, rel_tol=REL_TOL):
+    dv = float(np.abs(st.vm - o.vm).max())
+    dc = float((np.abs(st.cc_cells - o.cc_cells) / np.abs(o.cc_cells).clip(1e-300)).max())
+    de = float((np.abs(st.cc_env - o.cc_env) / np.abs(o.cc_env).clip(1e-300)).max())
+    emax = max(np.abs(o.E_env_x).max(), np.abs(o.E_env_y).max(), 1e-6)
+    dE = float(max(np.abs(st.E_env_x - o.E_env_x).max(), np.abs(st.E_env_y - o.E_env_y).max()) / emax)
+    say(f"{tag}: |dVm|={dv:.3e} V rel dc={dc:.3e} rel denv={de:.3e} rel dE={dE:.3e}")
+    assert dv <= vm_tol and dc <= rel_tol and de <= rel_tol and dE <= 1e-8 * (vm_tol / VM_TOL)
+
+
+@pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
+def test_launch_variants_are_bit_identical(flux_mode):
+    worlds = batched_worlds(n_clusters=5, cells_range=(30, 400), grid_range=(16, 40), seed=3)
+    n = 33                                   # graph pairs and an odd eager step
+    ref = batch(worlds, flux_mode)
+    say(f"case: launch variants, {flux_mode}, default {ref.get_tuning()}")
+    ref.step_n(n)
+    s_ref = state(ref)
+    ref.pull()
+    if flux_mode == "fp64":
+        for k, ((p, cells), st) in enumerate(zip(worlds, ref.states)):
+            compare(st, oracle_steps(p, cells, n), f"  cluster {k}")
+    del ref
+    for v in VARIANTS:
+        say(f"  variant {v}")
+        b = batch(worlds, flux_mode, tuning=v)
+        assert all(b.get_tuning()[k] == x for k, x in v.items())
+        b.step_n(n)
+        s = state(b)
+        for k in s_ref:
+            assert np.array_equal(s[k], s_ref[k]), (v, k)
+        del b
+
+
+def test_autotune_leaves_the_trajectory_unchanged():
+    """bt_autotune in the middle of a phase times every variant and restores the
+    state those launches touch: the run continues bit-identically."""
+    worlds = batched_worlds(n_clusters=3, cells_range=(30, 200), grid_range=(16, 24), seed=5,
+                            base="tiny_channels")
+    from betse_amd.channels import draw_channel_state
+    for p, c in worlds:
+        draw_channel_state(p, c.n_cells, c.n_mems)
+    say("case: autotune mid-run, 3 clusters with channels")
+    a = batch(worlds, "fp64")
+    a.step_n(20)
+    rep = a.run_autotune(5)
+    say(f"  autotune: {rep}")
+    assert set(rep["chosen"]) == {"mem_waves", "env_ions_per_block", "exchange_form", "graph_pairs"}
+    a.step_n(20)
+    b = batch(worlds, "fp64", tuning=rep["chosen"])
+    b.step_n(40)
+    sa, sb = state(a), state(b)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    ga = a._dev["gates"].cpu().numpy()
+    gb = b._dev["gates"].cpu().numpy()
+    assert np.array_equal(ga, gb)
+
+
+@pytest.fixture(scope="module")
+def c3_worlds():
+    return batched_worlds(256)
+
+
+@pytest.mark.parametrize("flux_mode,vm_tol,rel_tol", [("fp64", VM_TOL, REL_TOL), ("fp32", 1e-6, 1e-6)])
+def test_batched_workload_sample_matches_oracle(c3_worlds, flux_mode, vm_tol, rel_tol):
+    """The real batched workload, with the launch variants its size selects
+    (one wavefront per tile, one exchange thread per square) and the other
+    ones forced, against per-cluster oracle runs of a sample of clusters."""
+    worlds = c3_worlds
+    n = 16
+    sizes = np.array([c.n_cells for _, c in worlds])
+    pick = sorted({int(sizes.argmax()), int(sizes.argmin()), 17, 130})
+    b = batch(worlds, flux_mode)
+    tune = b.get_tuning()
+    say(f"case: batched workload {flux_mode}, {len(worlds)} clusters, {sizes.sum()} cells, {n} steps, "
+        f"default variants {tune}; sample {pick} ({sizes[pick].tolist()} cells)")
+    assert tune["mem_waves"] == 1 and tune["exchange_form"] == 1
+    b.step_n(n)
+    s_def = state(b)
+    b.pull()
+    for k in pick:
+        p, cells = worlds[k]
+        compare(b.states[k], oracle_steps(p, cells, n), f"  cluster {k} ({cells.n_cells} cells)", vm_tol, rel_tol)
+    del b
+    other = {"mem_waves": 2, "exchange_form": 0, "env_ions_per_block": 2}
+    say(f"  forced variants {other}")
+    b2 = batch(worlds, flux_mode, tuning=other)
+    b2.step_n(n)
+    s2 = state(b2)
+    for k in s_def:
+        assert np.array_equal(s2[k], s_def[k]), k
+
+
+@pytest.fixture(scope="module")
+def c5_world():
+    from betse_amd.cells import Cells
+    from betse_amd.channels import draw_channel_state
+    from betse_amd.parameters import Parameters
+    p = Parameters.preset("full")
+    cells = Cells.make_world(p)
+    draw_channel_state(p, cells.n_cells, cells.n_mems)
+    return p, cells
+
+
+def test_full_workload_matches_oracle(c5_world):
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    p, cells = c5_world
+    n = 16
+    say(f"case: full workload fp64, {cells.n_cells} cells, grid {cells.X.shape}, {n} steps")
+    o = oracle_steps(p, cells, n)
+    sim = Simulator(p.copy(), device="cuda:0")
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    sim.prepare(ph)
+    sim.step_n(n)
+    sim.pull()
+    compare(sim, o, "  full fp64")
+    for k, ch in enumerate(o.channels):
+        np.testing.assert_allclose(sim.chan_gates[k, 0], ch.m, rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(sim.chan_gates[k, 1], ch.h, rtol=1e-10, atol=1e-14)
+    rel = np.abs(sim.c_net - o.c_net) / np.maximum(np.abs(o.c_net), 1e-30)
+    say(f"  substances: max rel err {rel.max():.3e}")
+    assert rel.max() < 1e-10
+
+
+def test_full_workload_whole_horizon_properties(c5_world):
+    """The 20,000-step sim horizon of the full workload, FP64 and FP32-flux:
+    the mixed path stays within 1 uV of the FP64 path, and the state stays
+    finite and physical."""
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    p, cells = c5_world
+    n = 20_000
+    runs = {}
+    for mode in ("fp64", "fp32"):
+        say(f"case: full workload {mode}, {n} steps")
+        sim = Simulator(p.copy(), device="cuda:0", flux_mode=mode)
+        ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+        sim.init_core(ph)
+        sim.prepare(ph)
+        for _ in range(n // 5000):
+            sim.step_n(5000)
+            say(f"  {mode}: {sim._lib.bt_steps_done(sim._ctx)} steps")
+        sim.pull()
+        runs[mode] = sim
+        assert np.all(np.isfinite(sim.cc_cells)) and np.all(sim.cc_cells >= 0)
+        assert np.all(np.isfinite(sim.cc_env)) and np.all(sim.cc_env > 0)
+        assert np.all(np.abs(sim.vm_cell) < 0.2)
+        assert np.all((sim.chan_gates >= 0) & (sim.chan_gates <= 1))
+        assert np.all(np.isfinite(sim.c_net)) and np.all(sim.c_net >= 0)
+    a, b = runs["fp64"], runs["fp32"]
+    dv = float(np.abs(a.vm_cell - b.vm_cell).max())
+    dc = float((np.abs(a.cc_cells - b.cc_cells) / np.abs(a.cc_cells).clip(1e-300)).max())
+    say(f"  fp32 vs fp64 after {n} steps: |dVm|={dv:.3e} V rel dc={dc:.3e}; "
+        f"Vm range {a.vm_cell.min():.4f}..{a.vm_cell.max():.4f} V")
+    assert dv < 1e-6

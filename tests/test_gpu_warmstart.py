@@ -34,3 +34,25 @@ def test_warm_start_gains_over_default_start():
     assert ev["warm_converged"] >= ev["cold_converged"]
     assert ev["warm_total_steps"] < 0.8 * ev["cold_total_steps"]
     assert ev["warm_seconds"] < ev["cold_seconds"]
+
+
+def test_warm_start_gains_at_spec_cluster_sizes():
+    """The learned warm start with the workload's cluster sizes (500-2000
+    cells, grids 48-88, Dm and NaK rate over two decades): 64 training clusters
+    and 16 held-out ones (the full run uses 4096 and 256)."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from betse_amd import warmstart as ws
+    say("case: warm start at spec sizes, 64 training clusters of 500-2000 cells")
+    data, stats, base = ws.make_dataset(64, seed=1000, batch=64, horizon=20_000, flux_mode="fp32", log=say)
+    say(f"labels: {stats}")
+    assert sum(d["ok"] for d in data) >= 32
+    starter, hist, _ = ws.train(data, epochs=60, clusters_per_batch=16, log=say)
+    assert hist[-1] < 0.2 * hist[0]
+    say("case: warm start evaluation at spec sizes, 16 held-out clusters")
+    ev = ws.evaluate(starter, 16, seed=999_000, max_steps=60_000, window=100, flux_mode="fp32")
+    say(f"evaluation: {ev}")
+    assert ev["warm_injected"] >= 12 and ev["restarts"] == 0
+    assert ev["warm_converged"] >= ev["cold_converged"]
+    assert ev["warm_total_steps"] < 0.8 * ev["cold_total_steps"]

@@ -74,6 +74,22 @@ def test_reaction_matrix_and_no_negs():
     assert out[0, 0] == 0.0 and out[2, 0] > 0
 
 
+def test_nan_substance_raises_like_no_negs():
+    """A reversible reaction with reactant and product both at zero has
+    Q = 0/0: the substance turns NaN and ``no_negs`` raises
+    (``sim_toolbox.py:475-503``, called on every substance by run_loop)."""
+    import warnings
+    from oracle.betse_oracle import OracleUnstable
+    net = _net([Growth(0), Growth(1)], [Reaction([(0, 1.0, 1.0)], [(1, 1.0, 1.0)], 1.0, delta_Go=-1.0e3)], [1, 1])
+    C = np.array([[0.0, 1.0], [0.0, 1.0]])          # cell 0: 0/0; cell 1 finite
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        with pytest.raises(OracleUnstable):
+            no.step(net, C, 1e-3, R, T)
+        out = no.step(net, C[:, 1:], 1e-3, R, T)   # the finite cell alone steps
+    assert np.all(np.isfinite(out))
+
+
 def test_drawn_network_is_well_formed_and_packs():
     net = draw_network(seed=5)
     assert net.n_species == 8 and len(net.reactions) == 8

@@ -90,3 +90,43 @@ def test_seeded_channel_state_required():
     cells = Cells.make_world(p)
     with pytest.raises(ValueError):
         O.OracleSim(cells, p.copy(), "sparse").init_core()
+
+
+# ---------------------------------------------------------------------------
+# against the reference's own methods (tests/golden/make_golden_vectors.py:
+# Nav1p2/Kv1p1/Cav1p2._calculate_state and ChannelsABC.update_mh run on seeded
+# inputs, including the 0/0 points of the rate laws where the reference gives NaN)
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def chan_gold():
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_channels.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("tag,fn", [("nav", O.nav1p2_rates), ("kv", O.kv1p1_rates), ("cav", O.cav1p2_rates)])
+def test_rate_laws_match_reference(chan_gold, tag, fn):
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        got = fn(chan_gold["V"].copy())
+    for k, g in zip(("_mInf", "_mTau", "_hInf", "_hTau"), got):
+        ref = chan_gold[tag + k]
+        assert np.array_equal(np.isnan(g), np.isnan(ref)), (tag, k)
+        ok = ~np.isnan(ref)
+        np.testing.assert_array_equal(np.asarray(g)[ok], ref[ok], err_msg=f"{tag}{k}")
+
+
+def test_gate_update_matches_reference(chan_gold):
+    g = chan_gold
+    ch = O.OracleChannel("Kv1p1", 1e-17)
+    rates = (g["upd_in_mInf"], g["upd_in_mTau"], g["upd_in_hInf"], g["upd_in_hTau"])
+    ch._rates = lambda V: rates
+    ch.m, ch.h = g["upd_inm"].copy(), g["upd_inh"].copy()
+
+    class P:
+        dt = float(g["upd_dt"])
+    ch.run(np.zeros(len(ch.m)), P)
+    np.testing.assert_array_equal(ch.m, g["upd_m"])
+    np.testing.assert_array_equal(ch.h, g["upd_h"])
