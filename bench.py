@@ -1,22 +1,33 @@
 """Benchmark of the electrodiffusion step on one MI355X.
 
-Workload (default): the single large cluster -- jittered hex lattice (10 um
-spacing, +-10% jitter, numpy seed 0) clipped to an 800 um disc (23,222 cells,
-139,332 membranes), 256x256 environment grid, ions Na/K/Cl/Ca/P/M, dt 1e-4 s.
-A step is one iteration of the reference loop over that cluster; the timed
-region is ``--steps`` steps with all state resident in HBM.
+Default workload: the batched configuration -- 256 independent clusters of
+100-5000 cells (jittered hex lattices of 10 um spacing, one numpy seed per
+cluster; ~500k cells, ~3.0M membranes), environment grids 32x32 to 128x128
+scaled to each cluster's radius, ions Na/K/Cl/P/M, per-cluster Dm(Na/K/Cl)
+and NaK-ATPase rate log-uniform over one decade, dt 1e-3 s. It is the largest
+configuration that fits one GPU. A step is one iteration of
+``_run_sim_core_loop`` over every cluster; the timed region is ``--steps``
+steps of that loop (``BatchSimulator.run_loop``: the phase's sampled steps, if
+any fall inside, run the observables and ``write2storage``) with the state
+resident in HBM. The headline path is FP64 throughout; the FP32-flux path
+(FP32 flux laws, FP64 state, charge and voltage) is reported beside it.
 
-Printed: one JSON line with cell-steps/s, the roofline of the dominant kernel
-(algorithmic bytes per launch / HIP-event launch time), and the CPU baseline
-(the FP64 NumPy oracle, timed on a bounded prefix of the same workload on this
-host), plus |dVmem| of the GPU path against that oracle on the same prefix.
+Printed: one JSON line with cell-steps/s, the roofline of the step (the
+compulsory bytes of one ideal fused pass per step, per unit as the
+hot-path scope table of the survey defines them, over the measured step time),
+the measured HBM traffic per step from the committed rocprofv3 summaries, the
+launch variants the autotuner chose, and the CPU baseline: the FP64 NumPy
+oracle on the host's cores (clusters in parallel processes) and on one
+thread, plus the reference's own dense-operator form on the small cluster.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
+import multiprocessing
 import os
+import platform
 import sys
 import time
 
@@ -27,163 +38,289 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+MAX_CPU_WORKERS = 16          # the GPU box's CPU share for one GPU
 
 
-def _squares_with_membranes(cells) -> int:
-    if hasattr(cells, "n_live"):
-        return int(cells.n_live)
-    return int(np.count_nonzero(np.diff(cells.sq_ptr)))
+def log(msg: str) -> None:
+    print("# " + msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(cells, n_ions: int, ca: bool, flux_mode: str, n_ch: int = 0, n_sub: int = 0,
-                      dm_loaded: int = None, gj_loaded: bool = True, denv_loaded: int = None) -> dict:
-    """Compulsory bytes per launch of each step kernel (each array element the
-    kernel must touch, counted once at its storage width). ``dm_loaded`` ions
-    have per-membrane Dm (the others are one value per cluster and come with
-    the cluster constants); ``gj_loaded`` likewise for gjopen, ``denv_loaded``
-    for D_env over the grid."""
-    C, M = cells.n_cells, cells.n_mems
-    G2 = cells.X.size
-    n_in = len(cells.envInds_inClust)
-    ni = n_ions
-    ndm = ni if dm_loaded is None else dm_loaded
-    b = {}
-    b["mem_cell"] = (M * (5 * 4 + 8 + (8 if gj_loaded else 0) + 8 * ndm)   # indices, mem_sa, gjopen, Dm
-                     + M * 8 * ni                           # fsa write
-                     + M * n_ch * (16 + 16 + 8)              # gates r/w, channel flux write
-                     + C * (8 * ni * 2 + 8 * ni + 8 * ni     # c_cell r, c_lag r, c_cell w, c_lag w
-                            + 16 + 16 + 4 + 8 * n_ch         # vm r/w, cell_vol, diviterm, mem_ptr, modulators
-                            + 32 * n_ch)                     # gating rates r
-                     + n_in * 8 * (ni + (1 if ca else 0) + n_ch))  # env gathers (pre- and post-diffusion)
-    nde = ni if denv_loaded is None else denv_loaded
-    b["env_field"] = (G2 * (8 + 8 * ni + 8 * nde + 8 * ni)  # u r, c_env r, D_env r, c_mid / next c_env w
-                      + (C * (8 + 32 * n_ch) if n_ch else 0))  # vm r, gating rates w
-    # the exchange runs over the squares with membranes (the others take their
-    # new state from k_env_field): c_mid r, c_env w, u w, has_mem, sq_cluster, sq_ptr, list
-    n_live = _squares_with_membranes(cells)
-    b["env_exchange"] = n_live * (8 * ni + 8 * ni + 8 + 1 + 4 + 4 + 4) + M * 8 * (ni + n_ch)
-    if n_sub:
-        b["network"] = C * n_sub * 16                          # substances read and written
-    b["step"] = sum(b.values())
-    return b
+# ---------------------------------------------------------------------------
+# algorithmic bytes of one step (one ideal fused pass)
+# ---------------------------------------------------------------------------
+def b_step(n_cells: int, n_mems: int, n_squares: int, n_ions: int, n_gates: int = 0, n_mult: int = 0,
+           n_species: int = 0, p: int = 8) -> dict:
+    """Compulsory HBM bytes of one step: every array element once per step at
+    its storage width, reads and writes separately, gathers of counted arrays
+    and the sampled-only observables excluded. Per membrane: two int32 indices
+    (nn_i, map_mem2ecm), mem_sa, the GJ conductance and Dm per ion, each gate
+    read and written; per cell: c_cell and c_lag read and written per ion,
+    Vmem read and written, cell_vol and diviterm, the CSR row pointer, the
+    channel multipliers and each substance read and written; per square:
+    c_env read and written per ion, D_env per ion, E_env (x, y) read and
+    written, v_env read and written, the charge scale and mask. ``p`` is the
+    storage width of flux parameters, gates and E (8 here: this build stores
+    them as FP64 on both flux paths)."""
+    b_M = 8 + p * (2 + n_ions) + 2 * p * n_gates
+    b_C = 2 * 2 * 8 * n_ions + 16 + 16 + 4 + p * n_mult + 32 * n_species
+    b_G = 2 * 8 * n_ions + p * n_ions + 2 * 2 * p + 16 + 8
+    total = n_mems * b_M + n_cells * b_C + n_squares * b_G
+    return {"b_M": b_M, "b_C": b_C, "b_G": b_G, "bytes": total, "per_cell": total / max(n_cells, 1)}
 
 
-class _Tot:
-    """Totals of a batch of clusters, shaped like Cells for algorithmic_bytes."""
-
-    def __init__(self, worlds):
-        self.n_cells = sum(c.n_cells for _, c in worlds)
-        self.n_mems = sum(c.n_mems for _, c in worlds)
-        self.envInds_inClust = range(sum(len(c.envInds_inClust) for _, c in worlds))
-        self.n_live = sum(int(np.count_nonzero(np.diff(c.sq_ptr))) for _, c in worlds)
-
-        class _X:
-            size = sum(c.X.size for _, c in worlds)
-        self.X = _X()
-
-
-def bench_batched(a):
-    """Batched workload: 256 independent clusters of 100-5000 cells."""
-    import torch
-    from betse_amd.batch import BatchSimulator, batched_worlds
-    from oracle.betse_oracle import OracleSim
-    t0 = time.time()
-    worlds = batched_worlds(a.clusters)
-    tot = _Tot(worlds)
-    C = tot.n_cells
-    p0 = worlds[0][0]
+def workload_b_step(worlds) -> dict:
+    p0, _ = worlds[0]
     n_ions = len(p0.ion_names())
-    print(f"# batch: {len(worlds)} clusters, {C} cells, {tot.n_mems} membranes, {tot.X.size} squares, "
-          f"built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    modes = ["fp32", "fp64"] if a.flux == "both" else [a.flux]
-    res = {}
-    for mode in modes:
-        b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=mode)
-        b.init("init")
-        uniform = all(np.all(st.Dm_cells == st.Dm_cells[:, :1]) for st in b.states)
-        de_uniform = all(np.all((st.D_env * st.TJ_modulator) == (st.D_env * st.TJ_modulator)[:, :1])
-                         for st in b.states)
-        b.step_n(a.warmup)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(b.stream)
-        b.step_n(a.steps, sync=False)
-        e1.record(b.stream)
-        b.sync()
-        ev_ms = e0.elapsed_time(e1)
-        kern = {k: b.time_kernel(k, 20) for k in ("mem_cell", "env_field", "env_exchange")}
-        res[mode] = dict(ev_ms=ev_ms, kernels=kern)
-        print(f"# {mode}: {ev_ms / a.steps * 1e3:.1f} us/step, kernels(us): "
-              + ", ".join(f"{k}={v * 1e3:.1f}" for k, v in kern.items()), file=sys.stderr, flush=True)
-        del b
-    cpu, parity = None, {}
-    if not a.no_cpu:
-        sub = worlds[:a.cpu_clusters]
-        # time one step per cluster to size the sample
-        n_cpu = 2
+    n_ch = len(p0.channels)
+    net = getattr(p0, "network", None)
+    return b_step(sum(c.n_cells for _, c in worlds), sum(c.n_mems for _, c in worlds),
+                  sum(c.X.size for _, c in worlds), n_ions, n_gates=2 * n_ch, n_mult=n_ch,
+                  n_species=net.n_species if net is not None else 0)
+
+
+def measured_traffic(workload: str, flux_mode: str):
+    """HBM bytes per step of the step kernels from the committed rocprofv3 PMC
+    summary (profiles/hbm_traffic_<workload>.txt: FETCH_SIZE and WRITE_SIZE per
+    dispatch, scaled by the plain-copy calibration), or None when a kernel of
+    the step has no figure for this flux path."""
+    path = os.path.join(PROFILES, f"hbm_traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    per = {}
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 6 and t[0] == "kernel" and t[3] == flux_mode:
+                per[t[1]] = float(t[5])
+    need = {"mem_cell", "env_field", "env_exchange"}
+    if not need <= set(per):
+        return None
+    return {"bytes": sum(per.values()), "kernels": per}
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def make_worlds(config: str, clusters: int):
+    """[(Parameters, Cells)] of a configuration and its description."""
+    from betse_amd.batch import batched_worlds
+    t0 = time.time()
+    if config == "batched":
+        worlds = batched_worlds(clusters)
+        p0 = worlds[0][0]
+        desc = (f"batched: {len(worlds)} independent clusters of 100-5000 cells on 10 um jittered hex lattices, "
+                f"grids 32-128 scaled to radius, ions {'/'.join(p0.ion_names())}, per-cluster Dm(Na/K/Cl) and "
+                f"NaK rate log-uniform over one decade, dt {p0.dt:g} s")
+    else:
+        p, cells, _ = make_problem(config)
+        worlds = [(p, cells)]
+        desc = (f"single cluster ({config}): {cells.n_cells} cells, {cells.n_mems} membranes, "
+                f"{cells.X.shape[0]}x{cells.X.shape[1]} grid, ions {'/'.join(p.ion_names())}, dt {p.dt:g} s; "
+                f"channels {'/'.join(c.kind for c in p.channels) or 'none'}"
+                + ("; 8-substance reaction network" if getattr(p, "network", None) is not None else ""))
+    log(f"{desc}: {sum(c.n_cells for _, c in worlds)} cells, {sum(c.n_mems for _, c in worlds)} membranes, "
+        f"{sum(c.X.size for _, c in worlds)} squares, built in {time.time() - t0:.1f}s")
+    return worlds, desc
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the FP64 oracle on the host (run before the GPU is touched:
+# the workers are forked)
+# ---------------------------------------------------------------------------
+def _cpu_job(args):
+    """Oracle steps of the given clusters in one process, one BLAS thread."""
+    from threadpoolctl import threadpool_limits
+    from oracle.betse_oracle import OracleSim
+    worlds, n_steps, mode = args
+    with threadpool_limits(1):
         t0 = time.perf_counter()
-        oracles = []
-        for p, c in sub:
-            o = OracleSim(c, p.copy(), mode="sparse")
+        out = []
+        for p, cells in worlds:
+            o = OracleSim(cells, p.copy(), mode=mode)
             o.init_core()
             o.init_dynamics("init")
             o.p.set_phase_time("init")
-            o.update_V(observables=False)
-            o.step("init", observables=False)
-            oracles.append(o)
-        one = time.perf_counter() - t0
-        n_cpu = max(2, int(a.cpu_seconds / max(one, 1e-6)))
-        t0 = time.perf_counter()
-        for o in oracles:
-            for _ in range(n_cpu - 1):
-                o.step("init", observables=False)
-        el = one + time.perf_counter() - t0
-        csub = sum(c.n_cells for _, c in sub)
-        cpu = {"value": csub * n_cpu / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 NumPy oracle (sparse operators), the first {len(sub)} clusters "
-                         f"({csub} cells) for {n_cpu} steps each, {el:.1f}s on this host, single thread"}
-        for mode in modes:
-            b = BatchSimulator([(p.copy(), c) for p, c in sub], device="cuda:0", flux_mode=mode)
-            b.init("init")
-            b.step_n(n_cpu)
-            b.pull()
-            parity[mode] = {"steps": n_cpu, "clusters": len(sub),
-                            "max_abs_dVm_V": float(max(np.abs(st.vm - o.vm).max() for st, o in zip(b.states, oracles))),
-                            "max_rel_dc_cell": float(max((np.abs(st.cc_cells - o.cc_cells) / np.abs(o.cc_cells)).max()
-                                                         for st, o in zip(b.states, oracles)))}
-            del b
-        print(f"# cpu oracle sample: {cpu['sample']}; parity {parity}", file=sys.stderr, flush=True)
-    ok = [m for m in modes if (not parity) or parity[m]["max_abs_dVm_V"] < 1e-6]
-    best = min(ok or modes, key=lambda m: res[m]["ev_ms"])
-    r = res[best]
-    ms_step = r["ev_ms"] / a.steps
-    # Dm (per-cluster factors) and gjopen are one value per cluster in this workload
-    ab = algorithmic_bytes(tot, n_ions, "Ca" in p0.ion_names(), best, len(p0.channels), 0,
-                           0 if uniform else n_ions, not uniform, 0 if de_uniform else n_ions)
-    kern = r["kernels"]
-    dom = max(kern, key=kern.get)
-    achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
-    line = {
-        "metric": "cell-steps/s at |dVmem|<1uV vs FP64 oracle", "value": C * a.steps / (r["ev_ms"] / 1e3),
-        "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "f32-flux/f64-state", "data": "synthetic",
-        "config": {"workload": f"batched: {len(worlds)} independent clusters, {C} cells, {tot.n_mems} membranes, "
-                               f"grids 32-128 ({tot.X.size} squares), ions {'/'.join(p0.ion_names())}, "
-                               f"per-cluster Dm and NaK rate over one decade, dt {p0.dt:g} s",
-                   "clusters": len(worlds), "cells": C, "flux_mode": best},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batched", dom, best),
-                     "launch_us": kern[dom] * 1e3,
-                     "algorithmic_bytes": ab[dom], "step_algorithmic_bytes": ab["step"],
-                     "step_achieved_GBs": ab["step"] / (ms_step * 1e-3) / 1e9},
-        "cpu_baseline": cpu, "parity_prefix": parity,
-        "paths": {m: {"cell_steps_per_s": C * a.steps / (res[m]["ev_ms"] / 1e3),
-                      "us_per_step": res[m]["ev_ms"] / a.steps * 1e3,
-                      "kernels_us": {k: v * 1e3 for k, v in res[m]["kernels"].items()}} for m in modes},
+            o.update_V(observables=(mode == "dense"))
+            t1 = time.perf_counter()
+            for _ in range(n_steps):
+                o.step("init", observables=(mode == "dense"))
+            out.append({"cells": cells.n_cells, "seconds": time.perf_counter() - t1, "vm": o.vm.copy(),
+                        "cc_cells": o.cc_cells.copy(), "cc_env": o.cc_env.copy()})
+        return out
+
+
+def host_info() -> dict:
+    model = platform.processor()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    blas = None
+    try:
+        from threadpoolctl import threadpool_info
+        info = [i for i in threadpool_info() if i.get("user_api") == "blas"]
+        if info:
+            blas = f"{info[0].get('internal_api')} {info[0].get('version')} ({info[0].get('num_threads')} threads)"
+    except Exception:
+        pass
+    return {"cpu_model": model, "cores_available": len(os.sched_getaffinity(0)), "blas": blas}
+
+
+def cpu_baseline(worlds, target_s: float, sample_clusters: int):
+    """The sparse FP64 oracle on a sample of the workload's clusters: on all
+    allotted cores (one process per core, clusters dealt out largest first)
+    and on one thread. Returns (baseline dict, per-cluster final states, the
+    sample's indices, steps)."""
+    n_all = len(worlds)
+    order = np.argsort([-c.n_cells for _, c in worlds])
+    if n_all > sample_clusters:
+        # every k-th cluster by size: the sample spans the size range
+        pick = sorted(int(order[i]) for i in np.linspace(0, n_all - 1, sample_clusters).astype(int))
+    else:
+        pick = list(range(n_all))
+    sample = [worlds[i] for i in pick]
+    cores = max(1, min(len(os.sched_getaffinity(0)), MAX_CPU_WORKERS, len(sample)))
+    # size the run: one step of the whole sample on one thread
+    t_one = _cpu_job((sample, 1, "sparse"))
+    one = sum(r["seconds"] for r in t_one)
+    n_steps = max(2, int(target_s * min(cores, len(sample)) / max(one, 1e-6) / 2))
+    # all cores: each process gets whole clusters (LPT on cells)
+    loads = [[] for _ in range(cores)]
+    tot = np.zeros(cores)
+    for i in sorted(range(len(sample)), key=lambda j: -sample[j][1].n_cells):
+        k = int(np.argmin(tot))
+        loads[k].append(i)
+        tot[k] += sample[i][1].n_cells
+    ctx = multiprocessing.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_job, [([sample[i] for i in ld], n_steps, "sparse") for ld in loads])
+    wall = time.perf_counter() - t0
+    states = {}
+    for ld, r in zip(loads, res):
+        for i, st in zip(ld, r):
+            states[pick[i]] = st
+    cells_total = sum(c.n_cells for _, c in sample)
+    par = cells_total * n_steps / wall
+    # one thread: the same sample for fewer steps
+    n1 = max(2, int(target_s / 2 / max(one, 1e-6)))
+    r1 = _cpu_job((sample, n1, "sparse"))
+    single = cells_total * n1 / sum(r["seconds"] for r in r1)
+    info = host_info()
+    out = {
+        "value": par, "unit": "cell-steps/s", "cores": cores, "kind": "port",
+        "sample": (f"FP64 NumPy oracle (sparse operators: segment sums, scatter, sparse Poisson solve), "
+                   f"{len(sample)} of the workload's {n_all} clusters ({cells_total} cells) for {n_steps} steps, "
+                   f"one process per core with one BLAS thread, {wall:.1f}s wall"),
+        "one_thread": {"value": single, "steps": n1},
+        **info,
     }
-    print(json.dumps(line), flush=True)
+    return out, states, pick, n_steps
+
+
+def cpu_reference_form(target_s: float) -> dict:
+    """The reference's own operator form on the small cluster (362 cells, 50x50
+    grid): dense M_sum_mems, M_divmap_mem2ecm and the pinv Laplacian, every
+    observable each step, as sim.py runs it; BLAS on all allotted cores and on
+    one thread."""
+    from threadpoolctl import threadpool_limits
+    from betse_amd.cells import Cells
+    from betse_amd.parameters import Parameters
+    from oracle.betse_oracle import OracleSim
+    p = Parameters.preset("small")
+    cells = Cells.load(os.path.join(ROOT, "tests", "golden", "world_small.npz"))
+    cores = max(1, min(len(os.sched_getaffinity(0)), MAX_CPU_WORKERS))
+    out = {"workload": f"small cluster, {cells.n_cells} cells, {cells.n_mems} membranes, 50x50 grid, dt 1e-3 s",
+           "operators": "dense (reference form)"}
+    for tag, thr in (("all_cores", cores), ("one_thread", 1)):
+        with threadpool_limits(thr):
+            o = OracleSim(cells, p.copy(), mode="dense")
+            o.init_core()
+            o.init_dynamics("init")
+            o.p.set_phase_time("init")
+            o.update_V(observables=True)
+            t0 = time.perf_counter()
+            o.step("init", observables=True)
+            one = time.perf_counter() - t0
+            n = max(2, int(target_s / max(one, 1e-6)))
+            t0 = time.perf_counter()
+            for _ in range(n):
+                o.step("init", observables=True)
+            el = time.perf_counter() - t0
+        out[tag] = {"threads": thr, "steps": n, "ms_per_step": el / n * 1e3,
+                    "cell_steps_per_s": cells.n_cells * n / el}
+    return out
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def gpu_batch(worlds, mode, autotune=True):
+    from betse_amd.batch import BatchSimulator
+    b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=mode, autotune=autotune)
+    b.init("init")
+    return b
+
+
+def time_loop(worlds, mode, steps, warmup):
+    """K steps of the batch's run_loop after W warm-up steps (the graphs are
+    captured at set-up), bracketed by device synchronisations; HIP events on
+    the simulation stream give the device time."""
+    import torch
+    b = gpu_batch(worlds, mode)
+    b.step_n(warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(b.stream)
+    n_samp = b.run_loop(steps)
+    e1.record(b.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    names = ("mem_cell", "env_field", "env_exchange") + (("network",) if "c_net" in b._dev else ())
+    kernels = {k: b.time_kernel(k, 20) * 1e3 for k in names}
+    res = {"wall_s": wall, "events_ms": ev_ms, "sampled_steps": n_samp, "kernels_us": kernels,
+           "tuning": b.get_tuning(), "autotune": b.autotune_report}
+    del b
+    return res
+
+
+def time_sampled(worlds, mode, steps):
+    """The workload's own horizon through run_loop with its sampling (every
+    t_resample steps: observables of get_current, HH decomposition on FP64
+    MFMA, write2storage per cluster on the host)."""
+    import torch
+    b = gpu_batch(worlds, mode)
+    b.step_n(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_samp = b.run_loop(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    del b
+    return {"steps": steps, "sampled_steps": n_samp, "wall_s": wall}
+
+
+def gpu_parity(worlds, pick, states, n_steps, mode):
+    b = gpu_batch([worlds[i] for i in pick], mode, autotune=False)
+    b.step_n(n_steps)
+    b.pull()
+    dv, dc, de = 0.0, 0.0, 0.0
+    for st, i in zip(b.states, pick):
+        o = states[i]
+        dv = max(dv, float(np.abs(st.vm - o["vm"]).max()))
+        dc = max(dc, float((np.abs(st.cc_cells - o["cc_cells"]) / np.abs(o["cc_cells"]).clip(1e-300)).max()))
+        de = max(de, float((np.abs(st.cc_env - o["cc_env"]) / np.abs(o["cc_env"]).clip(1e-300)).max()))
+    del b
+    return {"steps": n_steps, "clusters": len(pick), "max_abs_dVm_V": dv, "max_rel_dc_cell": dc,
+            "max_rel_dc_env": de}
 
 
 def bench_warmstart(a):
@@ -308,114 +445,20 @@ def make_problem(name: str):
     return p, cells, time.time() - t0
 
 
-def gpu_session(p, cells, flux_mode):
-    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
-    sim = Simulator(p.copy(), device="cuda:0", flux_mode=flux_mode, use_graph=True)
-    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
-    sim.init_core(ph)
-    sim.prepare(ph)
-    return sim
-
-
-def time_gpu(p, cells, flux_mode, steps, warmup):
-    import torch
-    sim = gpu_session(p, cells, flux_mode)
-    sim.step_n(warmup)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(sim.stream)
-    sim.step_n(steps, sync=False)
-    e1.record(sim.stream)
-    sim.pull()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = e0.elapsed_time(e1)
-    names = ("mem_cell", "env_field", "env_exchange") + (("network",) if sim.network is not None else ())
-    kernels = {k: sim.time_kernel(k, 100) for k in names}
-    return sim, wall, ev_ms, kernels
-
-
-PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
-
-
-def measured_traffic(workload: str, kernel: str, flux_mode: str):
-    """HBM bytes per launch of a step kernel from the committed rocprofv3 PMC
-    summary (profiles/hbm_traffic_<workload>.txt: FETCH_SIZE and WRITE_SIZE per
-    dispatch, already scaled by the plain-copy calibration), or None."""
-    path = os.path.join(PROFILES, f"hbm_traffic_{workload}.txt")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        for line in f:
-            t = line.split()
-            if len(t) >= 6 and t[0] == "kernel" and t[1] == kernel and t[3] == flux_mode:
-                return float(t[5])
-    return None
-
-
-def time_sampled_loop(p, cells, flux_mode, steps):
-    """The init phase's step loop (_run_sim_core_loop) with the preset's
-    sampling: sampled steps run bt_step_observed, derive the get_current
-    observables (HH decomposition by DST on FP64 MFMA) and go through
-    write2storage on the host. Set-up (run_sim_core's init_dynamics, upload,
-    first update_V, observables workspace) is outside the timed loop."""
-    import torch
-    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator, phase_time_steps
-    q = p.copy()
-    q.init_time_total = steps * q.init_time_step
-    sim = Simulator(q, device="cuda:0", flux_mode=flux_mode, use_graph=True)
-    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
-    sim.init_core(ph)
-    sim.prepare(ph)
-    sim.step_observed(1)                 # allocates the observables workspace
-    # an untimed pass over two sampled steps allocates the pinned and snapshot
-    # buffers of the sampled-step copies
-    warm, warm_sampled = phase_time_steps(int(3 * q.t_resample) + 1, q.dt, q.t_resample)
-    sim._run_sim_core_loop(ph, warm, warm_sampled)
-    time_steps, sampled = phase_time_steps(q.init_tsteps, q.dt, q.t_resample)
-    walls = []
-    for _ in range(3):                   # median of three passes: the host side varies
-        sim.clear_storage()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        sim._run_sim_core_loop(ph, time_steps, sampled)
-        torch.cuda.synchronize()
-        walls.append(time.perf_counter() - t0)
-    return float(np.median(walls)), len(sim.time)
-
-
-def cpu_baseline(p, cells, target_s: float):
-    """FP64 oracle (sparse operators) on this host: a bounded prefix of the run."""
-    from oracle.betse_oracle import OracleSim
-    o = OracleSim(cells, p.copy(), mode="sparse")
-    o.init_core()
-    o.init_dynamics("init")
-    o.p.set_phase_time("init")
-    o.update_V(observables=False)
-    t0 = time.perf_counter()
-    o.step("init", observables=False)
-    one = time.perf_counter() - t0
-    n = max(2, int(target_s / max(one, 1e-6)))
-    t0 = time.perf_counter()
-    for _ in range(n - 1):
-        o.step("init", observables=False)
-    el = one + (time.perf_counter() - t0)
-    return o, n, el
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--config", default="large")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="batched",
+                    choices=["batched", "large", "full", "small", "warmstart", "full_pipeline"])
     ap.add_argument("--flux", default="both", choices=["fp64", "fp32", "both"])
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sampled", action="store_true", help="skip the run over the workload's own horizon")
+    ap.add_argument("--horizon", type=int, default=2000, help="steps of the sampled run (the workload's own)")
     ap.add_argument("--clusters", type=int, default=256, help="batched config: number of clusters")
-    ap.add_argument("--cpu-clusters", type=int, default=8, help="batched config: clusters in the CPU sample")
+    ap.add_argument("--cpu-clusters", type=int, default=32, help="batched config: clusters in the CPU sample")
     ap.add_argument("--train-clusters", type=int, default=4096, help="warmstart config: training clusters")
     ap.add_argument("--eval-clusters", type=int, default=256, help="warmstart config: held-out clusters")
     ap.add_argument("--max-init-steps", type=int, default=100_000, help="warmstart config: init step cap")
@@ -429,6 +472,10 @@ def main(argv=None):
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(each GPU box has one GPU; the sharding design is in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
     if a.copy_calib_mib:
         import torch
         n = a.copy_calib_mib * (1 << 20) // 8
@@ -439,90 +486,54 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8, "launches": 5}))
         return
-    if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(each GPU box has one GPU; the sharding design is in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
-    if a.config == "batched":
-        return bench_batched(a)
     if a.config == "warmstart":
         return bench_warmstart(a)
     if a.config == "full_pipeline":
         return bench_full_pipeline(a)
-    p, cells, t_world = make_problem(a.config)
-    n_ions = len(p.ion_names())
-    has_ca = "Ca" in p.ion_names()
-    C = cells.n_cells
-    print(f"# world: {C} cells, {cells.n_mems} membranes, grid {cells.X.shape}, ions {p.ion_names()}, "
-          f"built in {t_world:.1f}s", file=sys.stderr, flush=True)
+
+    worlds, desc = make_worlds(a.config, a.clusters)
+    C = sum(c.n_cells for _, c in worlds)
+    # CPU first: its worker processes are forked before anything touches the GPU
+    cpu, states, pick, n_cpu, ref_form = None, {}, [], 0, None
+    if not a.no_cpu:
+        cpu, states, pick, n_cpu = cpu_baseline(worlds, a.cpu_seconds, a.cpu_clusters)
+        log(f"cpu oracle: {cpu['value']:.3g} cell-steps/s on {cpu['cores']} cores "
+            f"({cpu['one_thread']['value']:.3g} on one thread); {cpu['sample']}")
+        ref_form = cpu_reference_form(a.cpu_seconds / 3)
+        log(f"reference form (dense operators), small cluster: {ref_form}")
 
     modes = ["fp64", "fp32"] if a.flux == "both" else [a.flux]
     res = {}
     for mode in modes:
-        sim, wall, ev_ms, kern = time_gpu(p, cells, mode, a.steps, a.warmup)
-        res[mode] = dict(sim=sim, wall=wall, ev_ms=ev_ms, kernels=kern)
-        print(f"# {mode}: {a.steps} steps in {wall:.3f}s wall ({ev_ms/a.steps*1e3:.1f} us/step by events), "
-              f"kernels(us): " + ", ".join(f"{k}={v*1e3:.2f}" for k, v in kern.items()),
-              file=sys.stderr, flush=True)
-
-    # the same steps through run_sim_core with the preset's sampling (observables + storage)
+        res[mode] = time_loop(worlds, mode, a.steps, a.warmup)
+        r = res[mode]
+        log(f"{mode}: {a.steps} steps in {r['wall_s'] * 1e3:.2f} ms wall ({r['events_ms'] / a.steps * 1e3:.1f} us/step "
+            f"by events, {r['sampled_steps']} sampled), kernels(us): "
+            + ", ".join(f"{k}={v:.1f}" for k, v in r["kernels_us"].items()) + f"; variants {r['tuning']}")
     sampled = {}
-    for mode in modes:
-        wall_s, n_samp = time_sampled_loop(p, cells, mode, a.steps)
-        sampled[mode] = {"cell_steps_per_s": C * a.steps / wall_s, "wall_s": wall_s, "sampled_steps": n_samp}
-        print(f"# {mode} run_sim_core with {n_samp} sampled steps: {wall_s / a.steps * 1e6:.1f} us/step",
-              file=sys.stderr, flush=True)
-
-    # CPU baseline + parity on the same prefix
-    cpu = None
+    if not a.no_sampled:
+        mode = modes[0]
+        s = time_sampled(worlds, mode, a.horizon)
+        s["cell_steps_per_s"] = C * s["steps"] / s["wall_s"]
+        sampled[mode] = s
+        log(f"{mode} over the workload's {a.horizon}-step horizon with {s['sampled_steps']} sampled steps: "
+            f"{s['wall_s'] / a.horizon * 1e6:.1f} us/step")
     parity = {}
-    if not a.no_cpu:
-        o, n_cpu, el = cpu_baseline(p, cells, a.cpu_seconds)
-        cpu = {"value": C * n_cpu / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 NumPy oracle (sparse operators), first {n_cpu} steps of the same "
-                         f"workload, {el:.1f}s on this host, single thread"}
+    if states:
         for mode in modes:
-            sim = gpu_session(p, cells, mode)
-            sim.step_n(n_cpu)
-            sim.pull()
-            parity[mode] = {
-                "steps": n_cpu,
-                "max_abs_dVm_V": float(np.abs(sim.vm - o.vm).max()),
-                "max_rel_dc_cell": float((np.abs(sim.cc_cells - o.cc_cells) / np.abs(o.cc_cells)).max()),
-                "max_rel_dc_env": float((np.abs(sim.cc_env - o.cc_env) / np.abs(o.cc_env)).max()),
-                "per_ion_rel_dc_cell": {n: float((np.abs(sim.cc_cells[i] - o.cc_cells[i]) / np.abs(o.cc_cells[i])).max())
-                                        for i, n in enumerate(sim.names)},
-                "per_ion_rel_dc_env": {n: float((np.abs(sim.cc_env[i] - o.cc_env[i]) / np.abs(o.cc_env[i])).max())
-                                       for i, n in enumerate(sim.names)},
-                "max_abs_dE_env_V_per_m": float(max(np.abs(sim.E_env_x - o.E_env_x).max(),
-                                                    np.abs(sim.E_env_y - o.E_env_y).max())),
-            }
-            if len(p.channels):
-                parity[mode]["max_abs_dgate"] = {ch.kind: float(max(np.abs(sim.chan_gates[k, 0] - ch.m).max(),
-                                                                    np.abs(sim.chan_gates[k, 1] - ch.h).max()))
-                                                 for k, ch in enumerate(o.channels)}
-            if getattr(p, "network", None) is not None:
-                parity[mode]["max_rel_dc_substance"] = float(
-                    (np.abs(sim.c_net - o.c_net) / np.maximum(np.abs(o.c_net), 1e-300)).max())
-        print(f"# cpu oracle: {n_cpu} steps in {el:.2f}s; parity {parity}", file=sys.stderr, flush=True)
+            parity[mode] = gpu_parity(worlds, pick, states, n_cpu, mode)
+        log(f"parity against the cpu sample: {parity}")
 
-    # headline: fastest path that holds |dVm| < 1 uV against the oracle prefix
-    ok = [m for m in modes if (not parity) or parity[m]["max_abs_dVm_V"] < 1e-6]
-    best = min(ok or modes, key=lambda m: res[m]["ev_ms"])
-    r = res[best]
-    ms_step = r["ev_ms"] / a.steps
-    value = C * a.steps / (r["ev_ms"] / 1e3)
-    n_sub = p.network.n_species if getattr(p, "network", None) is not None else 0
-    Dm, gjo = r["sim"].Dm_cells, r["sim"].gjopen
-    De = r["sim"].D_env * r["sim"].TJ_modulator
-    dm_loaded = int(sum(not np.all(Dm[i] == Dm[i][0]) for i in range(n_ions)))
-    de_loaded = int(sum(not np.all(De[i] == De[i][0]) for i in range(n_ions)))
-    ab = algorithmic_bytes(cells, n_ions, has_ca, best, len(p.channels), n_sub, dm_loaded,
-                           not bool(np.all(gjo == gjo[0])), de_loaded)
-    kern = r["kernels"]
+    head = "fp64" if "fp64" in modes else modes[0]
+    r = res[head]
+    ms_step = r["wall_s"] * 1e3 / a.steps
+    value = C * a.steps / r["wall_s"]
+    bs = workload_b_step(worlds)
+    ev_step_us = r["events_ms"] / a.steps * 1e3
+    achieved = bs["bytes"] / (ev_step_us * 1e-6) / 1e9
+    traffic = measured_traffic(a.config, head)
+    kern = r["kernels_us"]
     dom = max(kern, key=kern.get)
-    achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
     line = {
         "metric": "cell-steps/s at |dVmem|<1uV vs FP64 oracle",
         "value": value,
@@ -534,25 +545,28 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "f32-flux/f64-state",
+        "dtype": "f64" if head == "fp64" else "f32-flux/f64-state",
         "data": "synthetic",
-        "config": {"workload": f"single large cluster ({a.config}): {C} cells, {cells.n_mems} membranes, "
-                               f"{cells.X.shape[0]}x{cells.X.shape[1]} grid, ions {'/'.join(p.ion_names())}, "
-                               f"dt {p.dt:g} s; channels {'/'.join(c.kind for c in p.channels) or 'none'}",
-                   "cells": C, "membranes": cells.n_mems, "grid": list(cells.X.shape), "n_ions": n_ions,
-                   "flux_mode": best},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(a.config, dom, best),
-                     "launch_us": kern[dom] * 1e3, "algorithmic_bytes": ab[dom],
-                     "step_algorithmic_bytes": ab["step"],
-                     "step_achieved_GBs": ab["step"] / (ms_step * 1e-3) / 1e9},
+        "config": {"workload": desc, "clusters": len(worlds), "cells": C,
+                   "membranes": int(sum(c.n_mems for _, c in worlds)),
+                   "squares": int(sum(c.X.size for _, c in worlds)), "flux_mode": head},
+        "roofline": {"bound": "hbm", "kernel": "step (" + " + ".join(f"k_{k}" for k in kern) + ", graph-replayed)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "algorithmic_bytes": bs["bytes"], "bytes_per_unit": {k: bs[k] for k in ("b_M", "b_C", "b_G")},
+                     "step_us": ev_step_us, "kernels_us": kern,
+                     "dominant_kernel": {"name": dom, "launch_us": kern[dom]},
+                     "traffic_per_kernel": traffic["kernels"] if traffic else None},
         "cpu_baseline": cpu,
+        "cpu_reference_form": ref_form,
         "parity_prefix": parity,
         "with_sampling": sampled,
-        "paths": {m: {"cell_steps_per_s": C * a.steps / (res[m]["ev_ms"] / 1e3),
-                      "us_per_step": res[m]["ev_ms"] / a.steps * 1e3,
-                      "kernels_us": {k: v * 1e3 for k, v in res[m]["kernels"].items()}} for m in modes},
+        "paths": {m: {"cell_steps_per_s": C * a.steps / res[m]["wall_s"],
+                      "us_per_step": res[m]["wall_s"] / a.steps * 1e6,
+                      "events_us_per_step": res[m]["events_ms"] / a.steps * 1e3,
+                      "sampled_steps": res[m]["sampled_steps"],
+                      "kernels_us": res[m]["kernels_us"], "tuning": res[m]["tuning"],
+                      "autotune": res[m]["autotune"]} for m in modes},
     }
     print(json.dumps(line), flush=True)
 

@@ -26,10 +26,10 @@ from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableExc
 from .parameters import Parameters
 from .assemble import (BUFFER_FIELDS, batch_geometry, batch_state, check_compatible, make_clusters,
                        make_params)
-from .sim import HostState, SimPhase, SimPhaseKind
+from .sim import DeviceFetch, HostState, SimPhase, SimPhaseKind, phase_time_steps
 
 
-class BatchSimulator:
+class BatchSimulator(DeviceFetch):
     """A batch of independent clusters on one device."""
 
     def __init__(self, worlds: Sequence[Tuple[Parameters, Cells]], *, device: Optional[str] = None,
@@ -143,6 +143,8 @@ class BatchSimulator:
             self.set_tuning(self.tuning)
         elif self.autotune:
             self.run_autotune()
+        if self.use_graph:
+            self._check(self._lib.bt_capture_graphs(self._ctx))
 
     # ------------------------------------------------------------------
     def get_tuning(self) -> dict:
@@ -221,6 +223,91 @@ class BatchSimulator:
                 st.chan_gates = h["gates"][:, :, moff[i]:moff[i + 1]].copy()
             if "c_net" in h:
                 st.c_net = h["c_net"][:, a:b].copy()
+
+    # ------------------------------------------------------------------
+    # the loop with sampled steps (observables and write2storage per cluster)
+    # ------------------------------------------------------------------
+    def _obs_buffers(self):
+        if getattr(self, "_obs", None) is not None and self._obs_key is self._ctx.value:
+            return self._obs
+        torch = self.torch
+        coff, moff, soff = self.offsets
+        ni, C, M, G2 = len(self.states[0].names), int(coff[-1]), int(moff[-1]), int(soff[-1])
+        nch = max(len(self.states[0].channel_specs), 1)
+        shapes = {"fluxes_mem": (ni, M), "fluxes_gj": (ni, M), "fluxes_env_x": (ni, G2), "fluxes_env_y": (ni, G2),
+                  "chan_flux": (nch, M), "rate_NaKATP": (M,), "Jn": (M,), "I_mem": (M,), "E_gj_x": (M,),
+                  "E_gj_y": (M,), "J_cell_x": (C,), "J_cell_y": (C,), "vm_ave": (C,), "J_env_x": (G2,),
+                  "J_env_y": (G2,), "Jtx": (G2,), "Jty": (G2,), "B_field": (G2,)}
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            t = {k: torch.zeros(v, dtype=torch.float64, device=self.device) for k, v in shapes.items()}
+        obs = _native.BtObs()
+        for k in _native.OBS_FIELDS:
+            setattr(obs, k, t[k].data_ptr())
+        self._obs = (obs, t)
+        self._obs_key = self._ctx.value
+        return self._obs
+
+    # per-cluster slicing of the batch's fields: by cell, membrane or square
+    _CELL_FIELDS = ("c_cell", "vm_cell", "c_net", "J_cell_x", "J_cell_y", "vm_ave")
+    _MEM_FIELDS = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y")
+
+    def _split(self, h: dict, i: int) -> dict:
+        coff, moff, soff = self.offsets
+        out = {}
+        for k, a in h.items():
+            lo, hi = ((coff[i], coff[i + 1]) if k in self._CELL_FIELDS else
+                      (moff[i], moff[i + 1]) if k in self._MEM_FIELDS else (soff[i], soff[i + 1]))
+            out[k] = a[..., lo:hi]
+        return out
+
+    def run_loop(self, n_steps: int, t_resample: Optional[float] = None, callbacks=None) -> int:
+        """``_run_sim_core_loop`` (``sim.py:1132-1390``) of every cluster at once:
+        ``n_steps`` steps, and at the sampled steps of the phase's time vector
+        (``time_steps_sampled``, Q16) the observables of ``get_current`` for the
+        whole batch (``bt_observables``) and each cluster's ``write2storage``
+        (``sim.py:1789-1884``) into its own host state. The host side of a
+        sample runs while the device steps on. Returns the number of sampled
+        steps."""
+        p0 = self.worlds[0][0]
+        tr = p0.t_resample if t_resample is None else t_resample
+        time_steps, sampled = phase_time_steps(int(n_steps), p0.dt, tr)
+        sampled_idx = [i for i, t in enumerate(time_steps) if t in sampled]
+        for st in self.states:
+            st.clear_storage()
+        d = self._dev
+        obs, t_obs = self._obs_buffers() if sampled_idx else (None, None)
+        state_keys = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"] + (["c_net"] if "c_net" in d else [])
+        keep = list(HostState._OBS_KEEP)
+        use_graph = 1 if self.use_graph else 0
+        pending = None
+
+        def finish(t_sample):
+            h = self._fetch_finish_async(state_keys + keep)
+            for i, st in enumerate(self.states):
+                hi = self._split(h, i)
+                st._set_state(hi, False)
+                st._set_observables(hi)
+                st.write2storage(t_sample)
+            if callbacks is not None:
+                callbacks.progressed_next()
+
+        pos = 0
+        for s in sampled_idx:
+            self._check(self._lib.bt_step_observed(self._ctx, int(s + 1 - pos), use_graph, ctypes.byref(obs)))
+            pos = s + 1
+            if pending is not None:
+                finish(pending)
+                pending = None
+            self.sync()
+            self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
+            self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}})
+            pending = time_steps[s]
+        if pos < len(time_steps):
+            self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, use_graph))
+        if pending is not None:
+            finish(pending)
+        self.sync()
+        return len(sampled_idx)
 
     def snapshot(self):
         """(c_cell [n_ions][C], vm_cell [C]) of the whole batch, on the host."""

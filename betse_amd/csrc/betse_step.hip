@@ -1315,7 +1315,7 @@ __device__ __forceinline__ double div_at(const double* Fx, const double* Fy, int
 // rhs of the stream-function solve (-div(-Jy, Jx)) and of the potential solve
 // (div(Jx, Jy)) on the interior, written row-major into ld-wide padded matrices
 __global__ void __launch_bounds__(BLOCK)
-k_obs_rhs(const double* jxo, const double* jyo, double* b1, double* b2, int ld, KCluster K, KParams P) {
+k_obs_rhs(const double* jxo, const double* jyo, double* b1, double* b2, int ld, KCluster K) {
     const int n1 = K.ny - 2, n2 = K.nx - 2;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= n1 * n2) return;
@@ -1336,9 +1336,9 @@ k_scale(double* x, const double* s, int n) {
 // interior solution -> full grid (boundary rows of A are identity/delta^2 and
 // their right-hand sides are zero here)
 __global__ void __launch_bounds__(BLOCK)
-k_scatter_interior(const double* X, int ld, double* out, KCluster K, KParams P) {
+k_scatter_interior(const double* X, int ld, double* out, KCluster K) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= P.n_sq) return;
+    if (k >= K.ny * K.nx) return;
     const int r = k / K.nx, c = k % K.nx;
     const bool bnd = (r == 0 || c == 0 || r == K.ny - 1 || c == K.nx - 1);
     out[k] = bnd ? 0.0 : X[(r - 1) * ld + (c - 1)];
@@ -1347,9 +1347,9 @@ k_scatter_interior(const double* X, int ld, double* out, KCluster K, KParams P) 
 // gradients of the stream function AA and potential BB -> HH_Decomp outputs
 __global__ void __launch_bounds__(BLOCK)
 k_obs_hh_out(const double* AA, const double* BB, double* Jx, double* Jy, double* Jtx, double* Jty,
-             double* Bf, double mu, KCluster K, KParams P) {
+             double* Bf, double mu, KCluster K) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= P.n_sq) return;
+    if (k >= K.ny * K.nx) return;
     const int ny = K.ny, nx = K.nx, r = k / nx, c = k % nx;
     auto grad = [&](const double* F, double& gx, double& gy) {
         if (c == 0) gx = (F[r * nx + 1] - F[r * nx]) / K.delta;
@@ -1473,8 +1473,11 @@ struct bt_ctx {
     long long steps_done = 0;
     const bt_obs* capture = nullptr;   // set while enqueueing an observed step
     // observables workspace (allocated on first use)
-    int n1 = 0, n2 = 0, n1p = 0, n2p = 0;
-    double* S1 = nullptr; double* S2 = nullptr; double* invlam = nullptr;
+    // per cluster: interior size, padded size, its DST bases (shared by the
+    // clusters of one interior size) and its eigenvalue table
+    struct ObsGrid { int n1, n2, n1p, n2p; double* S1; double* S2; double* invlam; };
+    std::vector<ObsGrid> obs_grid;
+    std::vector<double*> obs_alloc;      // every observables buffer, freed in bt_destroy
     double* jxo = nullptr; double* jyo = nullptr; double* B1 = nullptr; double* B2 = nullptr;
     double* T1 = nullptr; double* T2 = nullptr; double* AA = nullptr; double* BB = nullptr;
     // Vmem per cell before the last observed step: update_gj's E_gj uses it (sim.py:2166-2172)
@@ -1996,10 +1999,11 @@ int bt_destroy(bt_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
-                    c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles, c->S1, c->S2, c->invlam, c->jxo, c->jyo,
-                    c->B1, c->B2, c->T1, c->T2, c->AA, c->BB, c->vm_prev, c->net};
+                    c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles,
+                    c->vm_prev, c->net};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    for (double* q : c->obs_alloc) (void)hipFree(q);
     delete c;
     return BT_OK;
 }
@@ -2049,6 +2053,11 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     HIP_TRY(c, hipGetLastError());
     c->steps_done += n_steps;
     return BT_OK;
+}
+
+int bt_capture_graphs(bt_ctx* c) {
+    if (!c) return BT_EINVAL;
+    return capture_graphs(c);
 }
 
 int bt_get_tuning(const bt_ctx* c, bt_tuning* out) {
@@ -2273,68 +2282,87 @@ int bt_fp32_math(const float* x, const float* y, float* out, int32_t n, int32_t 
     return hipGetLastError() == hipSuccess ? BT_OK : BT_EHIP;
 }
 
-// Dirichlet Poisson solve on the interior by DST-I: X = S1 ((S1 B S2) .* invlam) S2
-static int dst_solve(bt_ctx* c, double* Bm, double* out) {
+// Dirichlet Poisson solve of cluster q's interior by DST-I:
+// X = S1 ((S1 B S2) .* invlam) S2, scattered into out (the cluster's grid)
+static int dst_solve(bt_ctx* c, int q, double* Bm, double* out) {
+    const bt_ctx::ObsGrid& g = c->obs_grid[q];
     int rc;
-    if ((rc = launch_gemm(c, c->S1, Bm, c->T1, c->n1p, c->n2p, c->n1p))) return rc;
-    if ((rc = launch_gemm(c, c->T1, c->S2, c->T2, c->n1p, c->n2p, c->n2p))) return rc;
-    const int nn = c->n1p * c->n2p;
-    hipLaunchKernelGGL(k_scale, dim3(grid_blocks(nn)), dim3(BLOCK), 0, c->stream, c->T2, c->invlam, nn);
-    if ((rc = launch_gemm(c, c->S1, c->T2, c->T1, c->n1p, c->n2p, c->n1p))) return rc;
-    if ((rc = launch_gemm(c, c->T1, c->S2, c->T2, c->n1p, c->n2p, c->n2p))) return rc;
-    hipLaunchKernelGGL(k_scatter_interior, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
-                       (const double*)c->T2, c->n2p, out, c->kcl_host[0], c->kp);
+    if ((rc = launch_gemm(c, g.S1, Bm, c->T1, g.n1p, g.n2p, g.n1p))) return rc;
+    if ((rc = launch_gemm(c, c->T1, g.S2, c->T2, g.n1p, g.n2p, g.n2p))) return rc;
+    const int nn = g.n1p * g.n2p;
+    hipLaunchKernelGGL(k_scale, dim3(grid_blocks(nn)), dim3(BLOCK), 0, c->stream, c->T2, (const double*)g.invlam, nn);
+    if ((rc = launch_gemm(c, g.S1, c->T2, c->T1, g.n1p, g.n2p, g.n1p))) return rc;
+    if ((rc = launch_gemm(c, c->T1, g.S2, c->T2, g.n1p, g.n2p, g.n2p))) return rc;
+    const KCluster& K = c->kcl_host[q];
+    hipLaunchKernelGGL(k_scatter_interior, dim3(grid_blocks(K.ny * K.nx)), dim3(BLOCK), 0, c->stream,
+                       (const double*)c->T2, g.n2p, out, K);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }
 
+// the observables workspace: per-square buffers for the batch, per-cluster
+// eigenvalue tables, DST bases per interior size, and GEMM buffers of the
+// largest padded grid (the clusters are solved one after the other)
 static int obs_workspace(bt_ctx* c) {
-    if (c->S1) return BT_OK;
-    const int n1 = c->kcl_host[0].ny - 2, n2 = c->kcl_host[0].nx - 2;
-    const int n1p = (n1 + 15) / 16 * 16, n2p = (n2 + 15) / 16 * 16;
-    c->n1 = n1; c->n2 = n2; c->n1p = n1p; c->n2p = n2p;
+    if (!c->obs_grid.empty()) return BT_OK;
+    auto dalloc = [&](double** ptr, size_t n) -> int {
+        HIP_TRY(c, hipMalloc((void**)ptr, (n < 32 ? 32 : n) * 8));
+        c->obs_alloc.push_back(*ptr);
+        return BT_OK;
+    };
     const size_t ns = (size_t)c->kp.n_sq;
-    HIP_TRY(c, hipMalloc((void**)&c->S1, (size_t)n1p * n1p * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->S2, (size_t)n2p * n2p * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->invlam, (size_t)n1p * n2p * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->B1, (size_t)n1p * n2p * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->B2, (size_t)n1p * n2p * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->T1, (size_t)n1p * n2p * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->T2, (size_t)n1p * n2p * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->jxo, ns * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->jyo, ns * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->AA, ns * 8));
-    HIP_TRY(c, hipMalloc((void**)&c->BB, ns * 8));
-    // DST-I basis S[j][k] = sin(pi (j+1)(k+1) / (n+1)), zero in the padding;
-    // eigenvalues of the 1-D second difference 2 cos(pi (j+1)/(n+1)) - 2
-    std::string err;
-    double* h1 = new double[(size_t)n1p * n1p]();
-    double* h2 = new double[(size_t)n2p * n2p]();
-    double* hl = new double[(size_t)n1p * n2p]();
-    for (int j = 0; j < n1; ++j)
-        for (int k = 0; k < n1; ++k) h1[(size_t)j * n1p + k] = sin(M_PI * (j + 1) * (k + 1) / (n1 + 1));
-    for (int j = 0; j < n2; ++j)
-        for (int k = 0; k < n2; ++k) h2[(size_t)j * n2p + k] = sin(M_PI * (j + 1) * (k + 1) / (n2 + 1));
-    const double dl2 = c->kcl_host[0].delta * c->kcl_host[0].delta;
-    const double scale = 4.0 / ((double)(n1 + 1) * (double)(n2 + 1));
-    for (int j = 0; j < n1; ++j)
-        for (int k = 0; k < n2; ++k) {
-            const double lam = (2.0 * cos(M_PI * (j + 1) / (n1 + 1)) - 2.0) + (2.0 * cos(M_PI * (k + 1) / (n2 + 1)) - 2.0);
-            hl[(size_t)j * n2p + k] = dl2 * scale / lam;
-        }
-    hipError_t e = hipMemcpy(c->S1, h1, (size_t)n1p * n1p * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->S2, h2, (size_t)n2p * n2p * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(c->invlam, hl, (size_t)n1p * n2p * 8, hipMemcpyHostToDevice);
-    delete[] h1;
-    delete[] h2;
-    delete[] hl;
-    if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("observables set-up: ") + hipGetErrorString(e));
+    int rc;
+    if ((rc = dalloc(&c->jxo, ns)) || (rc = dalloc(&c->jyo, ns)) || (rc = dalloc(&c->AA, ns)) ||
+        (rc = dalloc(&c->BB, ns)))
+        return rc;
+    // DST-I basis S[j][k] = sin(pi (j+1)(k+1) / (n+1)), zero in the padding
+    std::vector<std::pair<int, double*>> bases;
+    auto basis = [&](int n, double** out) -> int {
+        for (auto& b : bases)
+            if (b.first == n) { *out = b.second; return BT_OK; }
+        const int np = (n + 15) / 16 * 16;
+        std::vector<double> h((size_t)np * np, 0.0);
+        for (int j = 0; j < n; ++j)
+            for (int k = 0; k < n; ++k) h[(size_t)j * np + k] = sin(M_PI * (j + 1) * (k + 1) / (n + 1));
+        int r = dalloc(out, h.size());
+        if (r != BT_OK) return r;
+        HIP_TRY(c, hipMemcpy(*out, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        bases.push_back({n, *out});
+        return BT_OK;
+    };
+    size_t maxpp = 0;
+    std::vector<bt_ctx::ObsGrid> grids;
+    for (size_t q = 0; q < c->kcl_host.size(); ++q) {
+        const KCluster& K = c->kcl_host[q];
+        bt_ctx::ObsGrid g{};
+        g.n1 = K.ny - 2; g.n2 = K.nx - 2;
+        g.n1p = (g.n1 + 15) / 16 * 16; g.n2p = (g.n2 + 15) / 16 * 16;
+        if ((rc = basis(g.n1, &g.S1)) || (rc = basis(g.n2, &g.S2))) return rc;
+        // eigenvalues of the 1-D second difference 2 cos(pi (j+1)/(n+1)) - 2,
+        // with the transform's normalisation and delta^2
+        std::vector<double> hl((size_t)g.n1p * g.n2p, 0.0);
+        const double dl2 = K.delta * K.delta;
+        const double scale = 4.0 / ((double)(g.n1 + 1) * (double)(g.n2 + 1));
+        for (int j = 0; j < g.n1; ++j)
+            for (int k = 0; k < g.n2; ++k) {
+                const double lam = (2.0 * cos(M_PI * (j + 1) / (g.n1 + 1)) - 2.0) +
+                                   (2.0 * cos(M_PI * (k + 1) / (g.n2 + 1)) - 2.0);
+                hl[(size_t)j * g.n2p + k] = dl2 * scale / lam;
+            }
+        if ((rc = dalloc(&g.invlam, hl.size()))) return rc;
+        HIP_TRY(c, hipMemcpy(g.invlam, hl.data(), hl.size() * 8, hipMemcpyHostToDevice));
+        maxpp = std::max(maxpp, (size_t)g.n1p * g.n2p);
+        grids.push_back(g);
+    }
+    if ((rc = dalloc(&c->B1, maxpp)) || (rc = dalloc(&c->B2, maxpp)) || (rc = dalloc(&c->T1, maxpp)) ||
+        (rc = dalloc(&c->T2, maxpp)))
+        return rc;
+    c->obs_grid = grids;
     return BT_OK;
 }
 
 int bt_observables(bt_ctx* c, const bt_obs* o) {
     if (!c || !o) return BT_EINVAL;
-    if (c->clusters.size() != 1) return set_err(c, BT_EINVAL, "observables are computed for one cluster");
     const void* need[] = {o->fluxes_mem, o->fluxes_gj, o->fluxes_env_x, o->fluxes_env_y, o->Jn, o->I_mem,
                           o->E_gj_x, o->E_gj_y, o->J_cell_x, o->J_cell_y, o->vm_ave, o->J_env_x, o->J_env_y,
                           o->Jtx, o->Jty, o->B_field};
@@ -2353,16 +2381,22 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
     hipLaunchKernelGGL(k_obs_mem, dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
     hipLaunchKernelGGL(k_obs_env_rhs, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
                        (const double*)o->fluxes_env_x, (const double*)o->fluxes_env_y, c->jxo, c->jyo, c->kp);
-    const size_t pb = (size_t)c->n1p * c->n2p * 8;
-    HIP_TRY(c, hipMemsetAsync(c->B1, 0, pb, c->stream));
-    HIP_TRY(c, hipMemsetAsync(c->B2, 0, pb, c->stream));
-    hipLaunchKernelGGL(k_obs_rhs, dim3(grid_blocks(c->n1 * c->n2)), dim3(BLOCK), 0, c->stream,
-                       (const double*)c->jxo, (const double*)c->jyo, c->B1, c->B2, c->n2p, c->kcl_host[0], c->kp);
-    if ((rc = dst_solve(c, c->B1, c->AA))) return rc;
-    if ((rc = dst_solve(c, c->B2, c->BB))) return rc;
-    hipLaunchKernelGGL(k_obs_hh_out, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
-                       (const double*)c->AA, (const double*)c->BB, o->J_env_x, o->J_env_y, o->Jtx, o->Jty,
-                       o->B_field, c->prm.mu, c->kcl_host[0], c->kp);
+    // the Helmholtz-Hodge decomposition of each cluster's env current on its own grid
+    for (size_t q = 0; q < c->obs_grid.size(); ++q) {
+        const bt_ctx::ObsGrid& g = c->obs_grid[q];
+        const KCluster& K = c->kcl_host[q];
+        const int so = K.sq_off, nsq = K.ny * K.nx;
+        const size_t pb = (size_t)g.n1p * g.n2p * 8;
+        HIP_TRY(c, hipMemsetAsync(c->B1, 0, pb, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->B2, 0, pb, c->stream));
+        hipLaunchKernelGGL(k_obs_rhs, dim3(grid_blocks(g.n1 * g.n2)), dim3(BLOCK), 0, c->stream,
+                           (const double*)(c->jxo + so), (const double*)(c->jyo + so), c->B1, c->B2, g.n2p, K);
+        if ((rc = dst_solve(c, (int)q, c->B1, c->AA + so))) return rc;
+        if ((rc = dst_solve(c, (int)q, c->B2, c->BB + so))) return rc;
+        hipLaunchKernelGGL(k_obs_hh_out, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream,
+                           (const double*)(c->AA + so), (const double*)(c->BB + so), o->J_env_x + so,
+                           o->J_env_y + so, o->Jtx + so, o->Jty + so, o->B_field + so, c->prm.mu, K);
+    }
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }

@@ -275,6 +275,71 @@ class HostState:
         self.Jn = None
 
     # ------------------------------------------------------------------
+    # host copies of the device state and the sampled-step storage
+    # ------------------------------------------------------------------
+    def _set_state(self, h: dict, full: bool) -> None:
+        shp = self.cells.X.shape
+        self.cc_cells = h["c_cell"]
+        self.vm_cell = h["vm_cell"]
+        self.cc_env = h["c_env"]
+        self.E_env_x = h["E_x"].reshape(shp)
+        self.E_env_y = h["E_y"].reshape(shp)
+        self.v_env = h["v_env"]
+        if full:
+            self.c_lag = h["c_lag"]
+            if len(self.channel_specs):
+                self.chan_gates = h["gates"]
+        if self.network is not None:
+            self.c_net = h["c_net"]
+
+    # the observables write2storage keeps; the raw fluxes stay on the device (pull_raw_fluxes)
+    _OBS_KEEP = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "J_env_x",
+                 "J_env_y", "Jtx", "Jty", "B_field")
+
+    def _set_observables(self, h: dict) -> None:
+        shp = self.cells.X.shape
+        self.rate_NaKATP = h["rate_NaKATP"]
+        self.Jn, self.I_mem = h["Jn"], h["I_mem"]
+        self.E_gj_x, self.E_gj_y = h["E_gj_x"], h["E_gj_y"]
+        self.J_cell_x, self.J_cell_y = h["J_cell_x"], h["J_cell_y"]
+        self.vm_ave_obs = h["vm_ave"]
+        self.J_env_x, self.J_env_y = h["J_env_x"].reshape(shp), h["J_env_y"].reshape(shp)
+        self.Jtx, self.Jty = h["Jtx"].reshape(shp), h["Jty"].reshape(shp)
+        self.B_field = h["B_field"].reshape(shp)
+
+    def write2storage(self, t, cells=None, p=None) -> None:
+        """``sim.py:1789-1884`` for the fields of this path. Every sampled step
+        pulls fresh host arrays from the device (nothing on the host modifies
+        them afterwards), so they are stored without a further copy; Dm and
+        gjopen do not change within a phase on this path and are stored as one
+        snapshot per phase."""
+        self.time.append(t)
+        if getattr(self, "Jn", None) is not None:
+            self.efield_gj_x_time.append(self.E_gj_x)
+            self.efield_gj_y_time.append(self.E_gj_y)
+            self.I_cell_x_time.append(self.J_cell_x)
+            self.I_cell_y_time.append(self.J_cell_y)
+            self.I_mem_time.append(self.I_mem)
+            self.rate_NaKATP_time.append(self.rate_NaKATP)
+            self.I_tot_x_time.append(self.J_env_x)
+            self.I_tot_y_time.append(self.J_env_y)
+        self.cc_time.append(self.cc_cells)
+        self.cc_env_time.append(self.cc_env)
+        snap = self.__dict__.get("_phase_snapshot")
+        if snap is None or len(self.time) == 1:
+            snap = (np.copy(self.Dm_cells), self.gjopen * 1)
+            self._phase_snapshot = snap
+        self.dd_time.append(snap[0])
+        vm = self.vm
+        self.vm_time.append(vm)
+        self.vm_ave_time.append(np.add.reduceat(vm, self.cells.mem_ptr[:-1]) / self.cells.num_mems)
+        self.rho_cells_time.append(self.rho_cells)
+        self.venv_time.append(self.v_env)
+        self.efield_ecm_x_time.append(self.E_env_x)
+        self.efield_ecm_y_time.append(self.E_env_y)
+        self.gjopen_time.append(snap[1])
+
+    # ------------------------------------------------------------------
     # host views of the device state
     # ------------------------------------------------------------------
     @property
@@ -296,7 +361,86 @@ class HostState:
         return np.dot(self.zs * self.p.F, self.cc_cells)
 
 
-class Simulator(HostState):
+class DeviceFetch:
+    """Device -> host copies through cached pinned buffers, for objects with
+    ``torch``, ``device`` and ``stream`` attributes (``Simulator``,
+    ``BatchSimulator``)."""
+
+    def _fetch_start(self, tensors: dict) -> None:
+        """Queue device -> pinned host copies on the simulation stream."""
+        torch = self.torch
+        pinned = self.__dict__.setdefault("_pinned", {})
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            for k, t in tensors.items():
+                h = pinned.get(k)
+                if h is None or h.shape != t.shape or h.dtype != t.dtype:
+                    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+                    pinned[k] = h
+                h.copy_(t, non_blocking=True)
+
+    def _fetch_start_async(self, tensors: dict) -> None:
+        """Sampled steps: snapshot the (FP64) tensors into one flat device buffer
+        on the simulation stream (a single multi-tensor copy), then copy that
+        buffer to pinned host memory on a side stream (default priority), so the
+        host transfer overlaps the next steps. ``_fetch_finish_async`` waits for
+        it. The buffers are reused: a caller finishes one fetch before it starts
+        the next."""
+        torch = self.torch
+        layout = [(k, tuple(t.shape)) for k, t in tensors.items()]
+        if any(t.dtype != torch.float64 for t in tensors.values()):
+            raise BetseSimException("sampled-step fetch expects FP64 tensors")
+        if self.__dict__.get("_flat_layout") != layout:
+            total = sum(int(np.prod(shape)) for _, shape in layout)
+            self._flat_dev = torch.empty(total, dtype=torch.float64, device=self.device)
+            self._flat_host = torch.empty(total, dtype=torch.float64, pin_memory=True)
+            self._flat_at, o = {}, 0
+            for k, shape in layout:
+                n = int(np.prod(shape))
+                self._flat_at[k] = (o, n, shape)
+                o += n
+            self._flat_layout = layout
+        if self.__dict__.get("_copy_stream") is None:
+            self._copy_stream = torch.cuda.Stream(device=self.device)     # default priority
+        with torch.cuda.device(self.device):
+            with torch.cuda.stream(self.stream):
+                dst = [self._flat_dev[o:o + n].view(shape) for o, n, shape in self._flat_at.values()]
+                torch._foreach_copy_(dst, [tensors[k] for k in self._flat_at])
+                taken = torch.cuda.Event()
+                taken.record(self.stream)
+            with torch.cuda.stream(self._copy_stream):
+                self._copy_stream.wait_event(taken)
+                self._flat_host.copy_(self._flat_dev, non_blocking=True)
+                self._copied = torch.cuda.Event()
+                self._copied.record(self._copy_stream)
+
+    def _fetch_finish_async(self, keys) -> dict:
+        """Host arrays of the last ``_fetch_start_async``: views into one fresh
+        NumPy copy of the pinned buffer (which is reused), so each sample's
+        arrays are its own."""
+        self.__dict__.pop("_copied").synchronize()
+        flat = self._flat_host.numpy().copy()
+        out = {}
+        for k in keys:
+            o, n, shape = self._flat_at[k]
+            out[k] = flat[o:o + n].reshape(shape)
+        return out
+
+    def _fetch_finish(self, keys) -> dict:
+        """Host arrays of pinned copies that have completed (NumPy copies out of
+        the pinned buffers, which are reused; torch's CPU clone of a pinned
+        tensor was measured ~14x slower on the largest cluster)."""
+        pinned = self._pinned
+        return {k: pinned[k].numpy().copy() for k in keys}
+
+    def _fetch(self, tensors: dict) -> dict:
+        """Device tensors -> host arrays through cached pinned buffers: all copies
+        are queued on the simulation stream, then one synchronisation."""
+        self._fetch_start(tensors)
+        self.stream.synchronize()
+        return self._fetch_finish(tensors.keys())
+
+
+class Simulator(HostState, DeviceFetch):
     """GPU-backed BETSE ``Simulator`` for the full electrodiffusion solver.
 
     ``flux_mode`` is ``"fp64"`` (every operation in FP64, the parity path) or
@@ -373,6 +517,8 @@ class Simulator(HostState):
                                                  ctypes.c_void_p(d["c_net"].data_ptr())))
         if self.tuning is not None:
             self.set_tuning(self.tuning)
+        if self.use_graph:
+            self._check(self._lib.bt_capture_graphs(self._ctx))
 
     # ------------------------------------------------------------------
     # launch variants of the step kernels
@@ -415,78 +561,6 @@ class Simulator(HostState):
         if rc != _native.BT_OK:
             raise BetseNativeError(f"libbetse_amd error {rc}: {self._lib.bt_last_error(self._ctx).decode()}")
 
-    def _fetch_start(self, tensors: dict) -> None:
-        """Queue device -> pinned host copies on the simulation stream."""
-        torch = self.torch
-        pinned = self.__dict__.setdefault("_pinned", {})
-        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
-            for k, t in tensors.items():
-                h = pinned.get(k)
-                if h is None or h.shape != t.shape or h.dtype != t.dtype:
-                    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-                    pinned[k] = h
-                h.copy_(t, non_blocking=True)
-
-    def _fetch_start_async(self, tensors: dict) -> None:
-        """Sampled steps: snapshot the (FP64) tensors into one flat device buffer
-        on the simulation stream (a single multi-tensor copy), then copy that
-        buffer to pinned host memory on a side stream (default priority), so the
-        host transfer overlaps the next steps. ``_fetch_finish_async`` waits for
-        it. The buffers are reused: a caller finishes one fetch before it starts
-        the next."""
-        torch = self.torch
-        layout = [(k, tuple(t.shape)) for k, t in tensors.items()]
-        if any(t.dtype != torch.float64 for t in tensors.values()):
-            raise BetseSimException("sampled-step fetch expects FP64 tensors")
-        if self.__dict__.get("_flat_layout") != layout:
-            total = sum(int(np.prod(shape)) for _, shape in layout)
-            self._flat_dev = torch.empty(total, dtype=torch.float64, device=self.device)
-            self._flat_host = torch.empty(total, dtype=torch.float64, pin_memory=True)
-            self._flat_at, o = {}, 0
-            for k, shape in layout:
-                n = int(np.prod(shape))
-                self._flat_at[k] = (o, n, shape)
-                o += n
-            self._flat_layout = layout
-        if self.__dict__.get("_copy_stream") is None:
-            self._copy_stream = torch.cuda.Stream(device=self.device)     # default priority
-        with torch.cuda.device(self.device):
-            with torch.cuda.stream(self.stream):
-                dst = [self._flat_dev[o:o + n].view(shape) for o, n, shape in self._flat_at.values()]
-                torch._foreach_copy_(dst, [tensors[k] for k in self._flat_at])
-                taken = torch.cuda.Event()
-                taken.record(self.stream)
-            with torch.cuda.stream(self._copy_stream):
-                self._copy_stream.wait_event(taken)
-                self._flat_host.copy_(self._flat_dev, non_blocking=True)
-                self._copied = torch.cuda.Event()
-                self._copied.record(self._copy_stream)
-
-    def _fetch_finish_async(self, keys) -> dict:
-        """Host arrays of the last ``_fetch_start_async`` (NumPy copies out of
-        the pinned buffer, which is reused)."""
-        self.__dict__.pop("_copied").synchronize()
-        flat = self._flat_host.numpy()
-        out = {}
-        for k in keys:
-            o, n, shape = self._flat_at[k]
-            out[k] = flat[o:o + n].reshape(shape).copy()
-        return out
-
-    def _fetch_finish(self, keys) -> dict:
-        """Host arrays of pinned copies that have completed (NumPy copies out of
-        the pinned buffers, which are reused; torch's CPU clone of a pinned
-        tensor was measured ~14x slower on the largest cluster)."""
-        pinned = self._pinned
-        return {k: pinned[k].numpy().copy() for k in keys}
-
-    def _fetch(self, tensors: dict) -> dict:
-        """Device tensors -> host arrays through cached pinned buffers: all copies
-        are queued on the simulation stream, then one synchronisation."""
-        self._fetch_start(tensors)
-        self.stream.synchronize()
-        return self._fetch_finish(tensors.keys())
-
     def _state_names(self, full: bool):
         names = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"]
         if full:
@@ -503,21 +577,6 @@ class Simulator(HostState):
         the channel gates."""
         d = self._dev
         self._set_state(self._fetch({k: d[k] for k in self._state_names(full)}), full)
-
-    def _set_state(self, h: dict, full: bool) -> None:
-        shp = self.cells.X.shape
-        self.cc_cells = h["c_cell"]
-        self.vm_cell = h["vm_cell"]
-        self.cc_env = h["c_env"]
-        self.E_env_x = h["E_x"].reshape(shp)
-        self.E_env_y = h["E_y"].reshape(shp)
-        self.v_env = h["v_env"]
-        if full:
-            self.c_lag = h["c_lag"]
-            if len(self.channel_specs):
-                self.chan_gates = h["gates"]
-        if self.network is not None:
-            self.c_net = h["c_net"]
 
     def _sync(self, phase_step0: int = 0) -> None:
         bad = ctypes.c_int64(-1)
@@ -563,21 +622,6 @@ class Simulator(HostState):
         self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
         self._set_observables(self._fetch({k: t[k] for k in self._OBS_KEEP}))
 
-    # the observables write2storage keeps; the raw fluxes stay on the device (pull_raw_fluxes)
-    _OBS_KEEP = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "J_env_x",
-                 "J_env_y", "Jtx", "Jty", "B_field")
-
-    def _set_observables(self, h: dict) -> None:
-        shp = self.cells.X.shape
-        self.rate_NaKATP = h["rate_NaKATP"]
-        self.Jn, self.I_mem = h["Jn"], h["I_mem"]
-        self.E_gj_x, self.E_gj_y = h["E_gj_x"], h["E_gj_y"]
-        self.J_cell_x, self.J_cell_y = h["J_cell_x"], h["J_cell_y"]
-        self.vm_ave_obs = h["vm_ave"]
-        self.J_env_x, self.J_env_y = h["J_env_x"].reshape(shp), h["J_env_y"].reshape(shp)
-        self.Jtx, self.Jty = h["Jtx"].reshape(shp), h["Jty"].reshape(shp)
-        self.B_field = h["B_field"].reshape(shp)
-
     def pull_raw_fluxes(self) -> None:
         """The last observed step's raw fluxes (fluxes_mem, fluxes_gj,
         fluxes_env_x/y, channel fluxes) to the host."""
@@ -600,38 +644,6 @@ class Simulator(HostState):
     def update_V(self, cells=None, p=None) -> None:
         """Voltage part of ``update_V`` (``sim.py:2007-2038``) on the device."""
         self._check(self._lib.bt_update_v(self._ctx))
-
-    def write2storage(self, t, cells=None, p=None) -> None:
-        """``sim.py:1789-1884`` for the fields of this path. Every sampled step
-        pulls fresh host arrays from the device (nothing on the host modifies
-        them afterwards), so they are stored without a further copy; Dm and
-        gjopen do not change within a phase on this path and are stored as one
-        snapshot per phase."""
-        self.time.append(t)
-        if getattr(self, "Jn", None) is not None:
-            self.efield_gj_x_time.append(self.E_gj_x)
-            self.efield_gj_y_time.append(self.E_gj_y)
-            self.I_cell_x_time.append(self.J_cell_x)
-            self.I_cell_y_time.append(self.J_cell_y)
-            self.I_mem_time.append(self.I_mem)
-            self.rate_NaKATP_time.append(self.rate_NaKATP)
-            self.I_tot_x_time.append(self.J_env_x)
-            self.I_tot_y_time.append(self.J_env_y)
-        self.cc_time.append(self.cc_cells)
-        self.cc_env_time.append(self.cc_env)
-        snap = self.__dict__.get("_phase_snapshot")
-        if snap is None or len(self.time) == 1:
-            snap = (np.copy(self.Dm_cells), self.gjopen * 1)
-            self._phase_snapshot = snap
-        self.dd_time.append(snap[0])
-        vm = self.vm
-        self.vm_time.append(vm)
-        self.vm_ave_time.append(np.add.reduceat(vm, self.cells.mem_ptr[:-1]) / self.cells.num_mems)
-        self.rho_cells_time.append(self.rho_cells)
-        self.venv_time.append(self.v_env)
-        self.efield_ecm_x_time.append(self.E_env_x)
-        self.efield_ecm_y_time.append(self.E_env_y)
-        self.gjopen_time.append(snap[1])
 
     def run_sim_core(self, phase: SimPhase) -> None:
         """``sim.py:1016-1128``."""

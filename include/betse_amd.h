@@ -223,6 +223,10 @@ typedef struct {
     int32_t graph_pairs;        /* pairs of steps per replayed step graph, 1..64 */
 } bt_tuning;
 
+/* Capture and instantiate the step graphs now (bt_step otherwise does it on
+ * its first call that replays graphs), so that no later bt_step pays for it. */
+int bt_capture_graphs(bt_ctx* ctx);
+
 int bt_get_tuning(const bt_ctx* ctx, bt_tuning* out);
 int bt_set_tuning(bt_ctx* ctx, const bt_tuning* tuning);
 

@@ -22,35 +22,36 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in r.stdout + r.stderr
 
 
-def test_algorithmic_bytes_counts_each_array_once():
-    """The model of the tiny world, term by term: every array element the step
-    kernels must touch, at its storage width, reads and writes separately."""
-    p = Parameters.preset("tiny")
+def test_step_bytes_match_the_scope_tables_figures():
+    """Per-unit bytes of the step against the figures the hot-path scope table
+    of the survey lists for its configurations (C1/C3: 64/196/176 bytes per
+    membrane/cell/square; C2 with Nav/Kv m,h gates and two per-cell
+    multipliers: 136/244/200; C5 with three channels and eight substances:
+    168/508/200) and its B_step totals (C1 0.643 MB, C2 37.8 MB)."""
+    c1 = bench.b_step(350, 2100, 2500, 5)
+    assert (c1["b_M"], c1["b_C"], c1["b_G"]) == (64, 196, 176)
+    assert abs(c1["bytes"] / 1e6 - 0.643) < 0.001
+    c2 = bench.b_step(23000, 140000, 65536, 6, n_gates=4, n_mult=2)
+    assert (c2["b_M"], c2["b_C"], c2["b_G"]) == (136, 244, 200)
+    assert abs(c2["bytes"] / 1e6 - 37.8) < 0.05
+    c5 = bench.b_step(100000, 600000, 262144, 6, n_gates=6, n_mult=3, n_species=8)
+    assert (c5["b_M"], c5["b_C"], c5["b_G"]) == (168, 508, 200)
+    assert abs(c5["bytes"] / 1e6 - 204) < 1
+
+
+def test_workload_bytes_follow_the_physics():
+    """The workload's B_step takes its ions, channels and substances from the
+    parameters: the large preset has Nav/Kv (four gates, two multipliers)."""
+    p = Parameters.preset("tiny_channels")
     cells = Cells.make_world(p)
-    C, M, G2 = cells.n_cells, cells.n_mems, cells.X.size
-    n_in = len(cells.envInds_inClust)
-    ni = 5
-    b = bench.algorithmic_bytes(cells, ni, ca=False, flux_mode="fp64")
-    # membranes: 5 int32 index arrays, mem_sa, gjopen, Dm per ion; flux writes
-    mem = M * (5 * 4 + 8 + 8 + 8 * ni) + M * 8 * ni
-    # cells: c_cell and c_lag read and written per ion, Vmem read and written,
-    # cell_vol, diviterm, mem_ptr; env gathers of the squares under the cluster
-    mem += C * (8 * ni * 4 + 16 + 16 + 4) + n_in * 8 * ni
-    assert b["mem_cell"] == mem
-    assert b["env_field"] == G2 * (8 + 24 * ni)
-    n_live = int(np.count_nonzero(np.diff(cells.sq_ptr)))        # squares with membranes
-    assert 0 < n_live < G2
-    assert b["env_exchange"] == n_live * (16 * ni + 21) + M * 8 * ni
-    assert b["step"] == b["mem_cell"] + b["env_field"] + b["env_exchange"]
-    # channels add the gates (read and write), the channel fluxes, the per-cell
-    # multipliers and rates, and the rates the env kernel writes from Vmem
-    bc = bench.algorithmic_bytes(cells, ni, ca=False, flux_mode="fp64", n_ch=2)
-    assert bc["mem_cell"] - b["mem_cell"] == M * 2 * 40 + C * 2 * 40 + n_in * 8 * 2
-    assert bc["env_field"] - b["env_field"] == C * (8 + 32 * 2)
-    # a D_env that is one value per ion is not loaded by the env kernel
-    bd = bench.algorithmic_bytes(cells, ni, ca=False, flux_mode="fp64", denv_loaded=0)
-    assert b["env_field"] - bd["env_field"] == G2 * 8 * ni
-    # per-cluster uniform Dm and gjopen are not loaded per membrane
-    bu = bench.algorithmic_bytes(cells, ni, ca=False, flux_mode="fp64", dm_loaded=0, gj_loaded=False)
-    assert b["mem_cell"] - bu["mem_cell"] == M * 8 * (ni + 1)
-    assert np.isfinite(float(b["step"]))
+    b = bench.workload_b_step([(p, cells)])
+    assert (b["b_M"], b["b_C"], b["b_G"]) == (136, 244, 200)
+    assert b["bytes"] == cells.n_mems * 136 + cells.n_cells * 244 + cells.X.size * 200
+    assert np.isfinite(b["per_cell"])
+
+
+def test_measured_traffic_reads_the_profile_summary():
+    t = bench.measured_traffic("large", "fp64")
+    assert t is not None and set(t["kernels"]) >= {"mem_cell", "env_field", "env_exchange"}
+    assert t["bytes"] == sum(t["kernels"].values())
+    assert bench.measured_traffic("no_such_workload", "fp64") is None

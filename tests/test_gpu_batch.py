@@ -82,3 +82,49 @@ def test_batch_with_full_physics():
         for ch in range(len(o.channels)):
             np.testing.assert_allclose(st.chan_gates[ch, 0], o.channels[ch].m, rtol=1e-10, atol=1e-14)
         assert dn < 1e-10
+
+
+def test_batch_sampled_observables_match_oracle():
+    """The batch's loop with sampled steps: get_current's observables for every
+    cluster (each Helmholtz-Hodge decomposition on its own grid, the
+    cell/membrane ones over the whole batch) and each cluster's
+    write2storage, against per-cluster oracle runs with the same sampling."""
+    from betse_amd.batch import BatchSimulator
+    worlds = batched_worlds(n_clusters=3, cells_range=(30, 200), grid_range=(16, 30), seed=5,
+                            base="tiny_channels")
+    n = 40
+    for p, c in worlds:
+        draw_channel_state(p, c.n_cells, c.n_mems)
+        p.init_time_sampling = 10 * p.init_time_step
+        p.set_phase_time("init")
+    say(f"case: batch sampled observables, 3 clusters with Nav/Kv, grids {[c.X.shape[0] for _, c in worlds]}")
+    b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode="fp64")
+    b.init("init")
+    n_samp = b.run_loop(n)
+    assert n_samp == 3
+    pairs = {"I_mem_time": "I_mem", "I_cell_x_time": "J_cell_x", "I_cell_y_time": "J_cell_y",
+             "efield_gj_x_time": "E_gj_x", "rate_NaKATP_time": "rate_NaKATP", "I_tot_x_time": "J_env_x",
+             "I_tot_y_time": "J_env_y", "vm_time": "vm", "cc_time": "cc_cells"}
+    for k, ((p, cells), st) in enumerate(zip(worlds, b.states)):
+        o = OracleSim(cells, p.copy(), mode="sparse")
+        o.init_core()
+        o.run_sim_core("init", n_steps=n)
+        assert len(o.storage) == len(st.I_mem_time) == n_samp
+        for rec_i, rec in enumerate(o.storage):
+            jn_scale = np.abs(np.asarray(rec["I_mem"]) / cells.mem_sa).max()
+            for mine, theirs in pairs.items():
+                a = np.asarray(getattr(st, mine)[rec_i]).ravel()
+                r = np.asarray(rec[theirs]).ravel()
+                scale = max(np.abs(r).max(), 1e-300)
+                if mine.startswith("I_cell"):
+                    scale = max(scale, jn_scale)
+                err = np.abs(a - r).max()
+                if mine.startswith("efield_gj"):
+                    assert err <= max(1e-8 * scale, 2e-9 / cells.gj_len), (k, mine, rec_i, err, scale)
+                elif mine == "vm_time":
+                    assert err <= 1e-9, (k, mine, rec_i, err)
+                else:
+                    assert err / scale < 1e-8, (k, mine, rec_i, err / scale)
+        np.testing.assert_allclose(st.B_field.ravel(), o.B_field.ravel(), rtol=0,
+                                   atol=1e-8 * np.abs(o.B_field).max())
+        say(f"cluster {k}: {n_samp} sampled steps match")
