@@ -121,7 +121,7 @@ class BtObs(ctypes.Structure):
 
 class BtTuning(ctypes.Structure):
     _fields_ = [("mem_waves", ctypes.c_int32), ("env_ions_per_block", ctypes.c_int32),
-                ("exchange_form", ctypes.c_int32), ("graph_pairs", ctypes.c_int32)]
+                ("exchange_form", ctypes.c_int32), ("graph_pairs", ctypes.c_int32), ("persistent", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -129,7 +129,8 @@ class BtTuning(ctypes.Structure):
 
 # bt_autotune's trial order
 AUTOTUNE_TRIALS = ("mem_waves=1", "mem_waves=2", "env_ions_per_block=1", "env_ions_per_block=2",
-                   "env_ions_per_block=3", "env_ions_per_block=n_ions", "exchange_form=0", "exchange_form=1")
+                   "env_ions_per_block=3", "env_ions_per_block=n_ions", "exchange_form=0", "exchange_form=1",
+                   "persistent=0 (ms per step)", "persistent=1 (ms per step)")
 
 
 # symbol -> (restype, argtypes); every function include/betse_amd.h declares

@@ -164,8 +164,8 @@ class BatchSimulator(DeviceFetch):
         """Time every launch variant on this batch and keep the fastest of each
         kernel (``bt_autotune``); returns the choice and the trial times (ms)."""
         t = _native.BtTuning()
-        ms = (ctypes.c_double * 8)()
-        self._check(self._lib.bt_autotune(self._ctx, int(n_launches), ctypes.byref(t), ms, 8))
+        ms = (ctypes.c_double * len(_native.AUTOTUNE_TRIALS))()
+        self._check(self._lib.bt_autotune(self._ctx, int(n_launches), ctypes.byref(t), ms, len(ms)))
         self.autotune_report = {"chosen": t.as_dict(),
                                 "trial_ms": {k: (v if v >= 0 else None) for k, v in zip(_native.AUTOTUNE_TRIALS, ms)}}
         return self.autotune_report

@@ -95,8 +95,9 @@ struct KParams {
     double qca_num;                 // cADP*cPi
     double cATP, km_ca, atp_kmca, one_p_atp_kmca;
     double gw[5];
-    // FP32-flux path: F/RT, F/RT_gj, 1/tm, 1/gj_len, 1/RT and the pumps'
-    // 1/Km rounded once to FP32
+    // F/RT, F/RT_gj, 1/tm, 1/gj_len, 1/RT and the pumps' 1/Km rounded once,
+    // to FP64 for the FP64 path and to FP32 for the FP32-flux path
+    double f_rt, f_rtgj, inv_tm, inv_gjlen, inv_rt, inv_kmna, inv_kmk, inv_kmca;
     float f_rt32, f_rtgj32, inv_tm32, inv_gjlen32, inv_rt32, inv_kmna32, inv_kmk32, inv_kmca32;
 };
 
@@ -180,42 +181,39 @@ __device__ __forceinline__ float expm1_32(float x) {
 __device__ __forceinline__ float exp_path(float x) { return exp32(x); }
 __device__ __forceinline__ double exp_path(double x) { return exp(x); }
 
-// electroflux (sim_toolbox.py:18-69); vBA already carries its +1e-25.
-template <typename FT>
-__device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT d, FT zc, FT vBA, FT F, FT RT) {
-    FT alpha = dv(zc * vBA * F, RT);
-    FT exp_alpha = exp(-alpha);
-    FT deno = -expm1(-alpha);
-    return -dv(Dc * alpha, d) * dv(cB - cA * exp_alpha, deno);
-}
+__device__ __forceinline__ float expm1_path(float x) { return expm1_32(x); }
+__device__ __forceinline__ double expm1_path(double x) { return expm1(x); }
 
-// The same law on the FP32-flux path with the two constant quotients as
-// products by FP32 reciprocals (F/RT and 1/d): one division instead of three.
-// One transcendental per evaluation: exp(-alpha) where |alpha| > 1/2 (then
-// |1 - exp(-alpha)| >= 0.39 loses nothing to cancellation), else expm1(-alpha)
-// and exp(-alpha) = 1 + expm1(-alpha) (then in [0.6, 1.7], so the sum is
-// within an ulp or two).
-__device__ __forceinline__ float ghk32(float cA, float cB, float Dc, float zc, float vBA, float f_rt, float inv_d) {
-    const float alpha = (zc * vBA) * f_rt;
-    float exp_alpha, deno;
-    if (fabsf(alpha) > 0.5f) {
-        exp_alpha = exp32(-alpha);
-        deno = 1.0f - exp_alpha;
+// electroflux (sim_toolbox.py:18-69), flux = -(Dc alpha / d) (cB - cA e^-alpha)
+// / (-expm1(-alpha)) with alpha = zc vBA F / (R T); zc and vBA already carry
+// their +1e-25 (Q5). The law's constant quotients are products by reciprocals
+// rounded once on the host (F/RT and 1/d: an ulp each of the path's precision)
+// and it evaluates one transcendental instead of two: exp(-alpha) where
+// |alpha| > 1/2 (then |1 - exp(-alpha)| >= 0.39 loses nothing to
+// cancellation), else expm1(-alpha) and exp(-alpha) = 1 + expm1(-alpha) (then
+// in [0.6, 1.7], so the sum is within an ulp or two). FT is double on the FP64
+// path, float on the FP32-flux path.
+template <typename FT>
+__device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT zc, FT vBA, FT f_rt, FT inv_d) {
+    const FT alpha = (zc * vBA) * f_rt;
+    FT exp_alpha, deno;
+    if (fabs(alpha) > (FT)0.5) {
+        exp_alpha = exp_path(-alpha);
+        deno = (FT)1 - exp_alpha;
     } else {
-        const float em1 = expm1_32(-alpha);
-        exp_alpha = 1.0f + em1;
+        const FT em1 = expm1_path(-alpha);
+        exp_alpha = (FT)1 + em1;
         deno = -em1;
     }
     return -((Dc * alpha) * inv_d) * dv(cB - cA * exp_alpha, deno);
 }
 
-// membrane / gap-junction / channel GHK of either path: d and RT as the
-// reference divides by them (FP64), or their FP32 reciprocals
+// membrane / gap-junction / channel GHK with the path's reciprocal constants
 template <typename FT>
-__device__ __forceinline__ FT ghk_path(FT cA, FT cB, FT Dc, double d, float inv_d32, FT zc, FT vBA, double F,
-                                       double RT, float f_rt32) {
-    if constexpr (sizeof(FT) == 4) return ghk32(cA, cB, Dc, zc, vBA, f_rt32, inv_d32);
-    else return ghk<FT>(cA, cB, Dc, (FT)d, zc, vBA, (FT)F, (FT)RT);
+__device__ __forceinline__ FT ghk_path(FT cA, FT cB, FT Dc, FT zc, FT vBA, double f_rt, double inv_d, float f_rt32,
+                                       float inv_d32) {
+    if constexpr (sizeof(FT) == 4) return ghk<float>(cA, cB, Dc, zc, vBA, f_rt32, inv_d32);
+    else return ghk<double>(cA, cB, Dc, zc, vBA, f_rt, inv_d);
 }
 
 // x**3: two multiplications (NumPy calls pow; the results differ by at most an
@@ -223,12 +221,12 @@ __device__ __forceinline__ FT ghk_path(FT cA, FT cB, FT Dc, double d, float inv_
 __device__ __forceinline__ double cube(double x) { return (x * x) * x; }
 __device__ __forceinline__ float cube(float x) { return (x * x) * x; }
 
-// x / c with c a constant of the law: the quotient itself on the FP64 path,
-// the product by the FP32 reciprocal inv32 on the FP32-flux path
+// x / c with c a constant of the law: the product by the reciprocal of c
+// rounded once on the host (inv64 on the FP64 path, inv32 on the FP32-flux path)
 template <typename FT>
-__device__ __forceinline__ FT dvc(FT x, double c, float inv32) {
+__device__ __forceinline__ FT dvc(FT x, double inv64, float inv32) {
     if constexpr (sizeof(FT) == 4) return x * inv32;
-    else return dv(x, (FT)c);
+    else return x * inv64;
 }
 
 // pumpNaKATP (sim_toolbox.py:71-122): returns f_Na; f_K = -(2/3)*f_Na.
@@ -243,9 +241,9 @@ __device__ __forceinline__ FT pump_nak(FT cNai, FT cNao, FT cKi, FT cKo, FT Vm, 
     FT Qdenomo = (FT)P.qd_scal * cube(c) * (d * d);
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-15;
     FT Q = dv(Qnumo, Qdenomo);
-    FT Keq = exp_path(-((FT)P.dG_RT - dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
-    FT na = cube(dvc<FT>(cNai, P.km_na, P.inv_kmna32));
-    FT kk = dvc<FT>(cKo, P.km_k, P.inv_kmk32);
+    FT Keq = exp_path(-((FT)P.dG_RT - dvc<FT>((FT)P.F * Vm, P.inv_rt, P.inv_rt32)));
+    FT na = cube(dvc<FT>(cNai, P.inv_kmna, P.inv_kmna32));
+    FT kk = dvc<FT>(cKo, P.inv_kmk, P.inv_kmk32);
     FT k2 = kk * kk;
     FT numo_E = na * k2 * (FT)P.atp_km;
     FT denomo_E = ((FT)1 + na) * ((FT)1 + k2) * (FT)P.one_p_atp_km;
@@ -260,9 +258,9 @@ __device__ __forceinline__ FT pump_ca(FT cCai, FT cCao, FT Vm, const KParams& P)
     FT Qdenomo = (FT)P.cATP * cCai;
     if (Qdenomo == (FT)0) Qdenomo = (FT)1.0e-16;
     FT Q = dv(Qnumo, Qdenomo);
-    FT Keq = exp_path(-((FT)P.dG_RT - (FT)2 * dvc<FT>((FT)P.F * Vm, P.RT, P.inv_rt32)));
-    FT numo_E = dvc<FT>(cCai, P.km_ca, P.inv_kmca32) * (FT)P.atp_kmca;
-    FT denomo_E = ((FT)1 + dvc<FT>(cCai, P.km_ca, P.inv_kmca32)) * (FT)P.one_p_atp_kmca;
+    FT Keq = exp_path(-((FT)P.dG_RT - (FT)2 * dvc<FT>((FT)P.F * Vm, P.inv_rt, P.inv_rt32)));
+    FT numo_E = dvc<FT>(cCai, P.inv_kmca, P.inv_kmca32) * (FT)P.atp_kmca;
+    FT denomo_E = ((FT)1 + dvc<FT>(cCai, P.inv_kmca, P.inv_kmca32)) * (FT)P.one_p_atp_kmca;
     FT frwd = dv(numo_E, denomo_E);
     return (FT)P.ca_coef * frwd * ((FT)1 - dv(Q, Keq));
 }
@@ -345,10 +343,26 @@ struct MemLds {
     int* bad;
 };
 
+// Synchronisation of the lanes that share a tile's LDS: the block (WSYNC
+// false: the tile's one or two wavefronts are the block) or the wavefront
+// alone (WSYNC true: in the persistent step kernel every wavefront of a block
+// works on a tile of its own; the lanes of a wavefront run in step and its LDS
+// accesses complete in order, so a compiler fence is all it needs).
+template <bool WSYNC>
+__device__ __forceinline__ void tile_sync() {
+    if constexpr (WSYNC) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // The work of wavefront W of a k_mem_cell block. W is a template argument so
 // that every ion index below is a compile-time constant (the per-ion kernel
 // parameters then load once, up front).
-template <int NI, typename FT, int WG, int W, bool OBS>
+template <int NI, typename FT, int WG, int W, bool OBS, bool WSYNC = false>
 __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParams& P, int tile, const MemLds& L) {
     constexpr int NS = (NI + WG - 1) / WG;             // ion slots per wavefront
     constexpr int NQ = (BT_MAX_CHAN + WG - 1) / WG;    // channel slots per wavefront
@@ -362,7 +376,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
     const KCluster* KC = A.kcl + A.tile_cluster[tile];
     const double nak_coef = KC->nak_coef;
     const double gjo_u = KC->gjo_u;
-    if (threadIdx.x == 0) *L.bad = 0;
+    if (W == 0 && j == 0) *L.bad = 0;
 
     // per-cell inputs of the second stage, fetched up front
     const int c = c0 + j;
@@ -403,7 +417,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             g_dm[s] = *dmp;
         };
         double h_m[NQ], h_h[NQ], h_mod[NQ], h_out[NQ], h_in[NQ];
-        // mInf, mTau, hInf, hTau (FP64 path) or a_m, b_m, a_h, b_h (FP32-flux path)
+        // the update coefficients a_m, b_m, a_h, b_h of each gate (per cell, from k_env_field)
         double r_mi[NQ], r_mt[NQ], r_hi[NQ], r_ht[NQ];
         auto gather_chan = [&](int q) {
             const int ch = W + q * WG;
@@ -459,14 +473,14 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             if (i == P.i_k) f = f + (double)((FT)P.k_per_na * fNa);
             // GHK across the membrane and across the gap junction (sim.py:1253-1269, 2162-2206)
             const double clag = g_lag[s];
-            const FT fED = ghk_path<FT>((FT)g_env[s], (FT)clag, (FT)g_dm[s], P.tm, P.inv_tm32,
-                                        (FT)P.zplus[i], (FT)vmut, P.F, P.RT, P.f_rt32);
+            const FT fED = ghk_path<FT>((FT)g_env[s], (FT)clag, (FT)g_dm[s], (FT)P.zplus[i], (FT)vmut, P.f_rt,
+                                        P.inv_tm, P.f_rt32, P.inv_tm32);
             f = f + (double)fED;
             double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
             if (!bnd) {
                 const double vBA = (vnmut - vmut) + FLOAT_NONCE;
-                fgj = (double)ghk_path<FT>((FT)clag, (FT)g_lagn[s], (FT)(P.dgj_surf[i] * gjo), P.gj_len,
-                                           P.inv_gjlen32, (FT)P.zplus[i], (FT)vBA, P.F, P.RT_gj, P.f_rtgj32);
+                fgj = (double)ghk_path<FT>((FT)clag, (FT)g_lagn[s], (FT)(P.dgj_surf[i] * gjo), (FT)P.zplus[i],
+                                           (FT)vBA, P.f_rtgj, P.inv_gjlen, P.f_rtgj32, P.inv_gjlen32);
             }
             // Ca-ATPase after the ion loop: refreshed cell Ca, post-diffusion env Ca,
             // Vmem carrying all n_ions nonces (sim.py:2126-2159)
@@ -499,41 +513,34 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             const int i = P.chan_ion[ch];
             const double vg = with_nonces(v, NI + ch);
             const int kind = P.chan_kind[ch];
-            double mm, hh;
-            if constexpr (sizeof(FT) == 4) {
-                // the FP32-flux path reads the update coefficients a, b of each gate
-                mm = r_mi[q] * h_m[q] + r_mt[q];
-                hh = r_hi[q] * h_h[q] + r_ht[q];
-            } else {
-                const double dtm = P.chan_dt;
-                mm = (r_mt[q] * h_m[q] + dtm * r_mi[q]) / (r_mt[q] + dtm);
-                hh = (r_ht[q] * h_h[q] + dtm * r_hi[q]) / (r_ht[q] + dtm);
-            }
+            // update_mh as x' = a x + b with the per-cell coefficients a, b of each gate
+            const double mm = r_mi[q] * h_m[q] + r_mt[q];
+            const double hh = r_hi[q] * h_h[q] + r_ht[q];
             A.gates[(size_t)(2 * ch) * M + m] = mm;
             A.gates[(size_t)(2 * ch + 1) * M + m] = hh;
             // m**p * h**q: p = 2 squares as NumPy does, p = 3 is cube()
             const double Pch = (kind == BT_CHAN_NAV1P2) ? cube(mm) * hh
                              : (kind == BT_CHAN_KV1P1) ? mm * (hh * hh) : (mm * mm) * hh;
             const FT Dch = (FT)(Pch * P.chan_max_dm[ch] * h_mod[q]);
-            const FT fc = ghk_path<FT>((FT)h_out[q], (FT)h_in[q], Dch, P.tm, P.inv_tm32,
-                                       (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE), P.F, P.RT, P.f_rt32);
+            const FT fc = ghk_path<FT>((FT)h_out[q], (FT)h_in[q], Dch, (FT)P.zplus[i], (FT)(vg + FLOAT_NONCE),
+                                       P.f_rt, P.inv_tm, P.f_rt32, P.inv_tm32);
             const double x = (double)fc * sa;
             L.ch[ch][j] = x;
             A.fch[(size_t)ch * M + pos] = x;
             if (OBS) A.obs_fch[(size_t)ch * M + m] = (double)fc;
         }
     }
-    __syncthreads();
+    tile_sync<WSYNC>();
 
     // per-cell segment sums in membrane order (np.dot(M_sum_mems, .)), then
     // update_Co, the GJ update, no_negs, and rho_cells -> Vmem (ion order)
     if (own_cell) {
         bool bad = false;
         double rho = 0.0;
-        // x / vol: the quotient on the FP64 path; on the FP32-flux path the
-        // product by 1/vol (an FP64 ulp, far below that path's FP32 fluxes)
+        // x / vol as the product by 1/vol (an FP64 ulp; one division per cell
+        // instead of three per ion)
         const double rvol = 1.0 / vol;
-        auto per_vol = [&](double x) { return (sizeof(FT) == 4) ? x * rvol : x / vol; };
+        auto per_vol = [&](double x) { return x * rvol; };
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             const int i = W * NS + s;
@@ -568,7 +575,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         if (bad) *L.bad = 1;
     }
     if (WG > 1) {
-        __syncthreads();
+        tile_sync<WSYNC>();
         if (W == 0 && own_cell) {
             double rho = P.zF[0] * L.c2[0][j];
 #pragma unroll
@@ -578,8 +585,8 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             A.vm_out[c] = vm;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && *L.bad) {
+    tile_sync<WSYNC>();
+    if (W == 0 && j == 0 && *L.bad) {
         atomicOr(A.flags, 1u);
         atomicMin(A.bad_step, (unsigned long long)(*A.step_ctr));
     }
@@ -623,7 +630,6 @@ struct FieldArgs {
     double* obs_fx; double* obs_fy;          // fluxes_env_x/y of the observed step, or NULL
     const double* vm;         // Vmem the step's k_mem_cell reads (diffuse mode)
     double* rates;            // [n_chan][4][n_cells] gating rates out (diffuse mode), or NULL
-    int gate_coef;            // 1: store the gate-update coefficients instead (FP32-flux path)
     // diffuse mode: a square without membranes has no exchange, so its
     // post-diffusion value is already the step's new env state and goes to
     // c_env_out; squares with membranes go to c_mid for k_env_exchange
@@ -631,27 +637,59 @@ struct FieldArgs {
     double* c_env_out;        // the step's env state buffer (never the one read here)
 };
 
-// RATES: also the gating rates (diffuse mode with channels). RECIP (the
-// FP32-flux path): divisions by the grid spacing and by kT as products by
-// their FP64 reciprocals (an FP64 ulp each; the FP64 path divides).
-template <bool DIFFUSE, bool RATES, bool RECIP>
-__global__ void __launch_bounds__(TY * TX)
-k_env_field(FieldArgs A, KParams P) {
+// RATES: also the gating rates (diffuse mode with channels). Divisions by the
+// grid spacing and by kT are products by their FP64 reciprocals (an FP64 ulp
+// each).
+// the gate-update coefficients of item t_it = ch * n_cells + cell from the
+// cell's Vmem vmc: the channel's rates at Vmem carrying the n_ions + ch
+// nonces of the membrane stage, and update_mh (channelsabc.py:54-58),
+// x' = (tau x + dt inf)/(tau + dt), as x' = a x + b with a = tau/(tau+dt),
+// b = dt*inf/(tau+dt) (FP64, an ulp from the quotient form)
+__device__ __forceinline__ void gate_coef_item(const FieldArgs& A, const KParams& P, int t_it, double vmc) {
+    const int nc = P.n_cells;
+    const int ch = t_it / nc, cell = t_it - ch * nc;
+    const double V = with_nonces(vmc, P.n_ions + ch) * 1000.0 + 0.0;
+    double mInf, mTau, hInf, hTau;
+    gate_rates(P.chan_kind[ch], V, mInf, mTau, hInf, hTau);
+    double* R = A.rates + (size_t)(4 * ch) * nc + cell;
+    const double dtm = P.chan_dt;
+    R[0] = mTau / (mTau + dtm);
+    R[nc] = (dtm * mInf) / (mTau + dtm);
+    R[2 * nc] = hTau / (hTau + dtm);
+    R[3 * nc] = (dtm * hInf) / (hTau + dtm);
+}
+
+
+// LDS of one k_env_field tile, in doubles
+constexpr int ENV_UH = 6;                                        // u on tile +- 6
+constexpr int ENV_A1 = (TY + 2 * ENV_UH) * (TX + 2 * ENV_UH) + (TY + 4) * (TX + 2 * ENV_UH);
+constexpr int ENV_A2 = (TY + 4) * (TX + 4) + (TY + 2) * (TX + 2) + TY * (TX + 2) + (TY + 2) * TX;
+constexpr int ENV_POOL = ENV_A1 > ENV_A2 ? ENV_A1 : ENV_A2;
+constexpr int ENV_LDS = ENV_POOL + (TY + 4) * (TX + 4) + 2 * (TY + 2) * (TX + 2) + BT_MAX_IONS;
+
+// One env tile's work for the ion group ion_id of n_groups (the ions
+// [ion_id * ipb, ion_id * ipb + ipb), ipb = ceil(n_ions / n_groups)), with
+// the block's LDS at lds (ENV_LDS doubles). With RATES the block's threads also
+// evaluate the gating-rate items rate_first, rate_first + rate_stride, ...
+template <bool DIFFUSE, bool RATES>
+__device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams& P, int tile_id, int ion_id,
+                                               int n_groups, double* lds, int rate_first, int rate_stride,
+                                               bool check_halt = true) {
     // the halt flag is read first and tested before the first store, so its
-    // latency overlaps the block's first loads
-    const bool stop = halted(A.flags);
-    constexpr int UH = 6, UR = TY + 2 * UH, UC = TX + 2 * UH;    // u on tile +- 6
+    // latency overlaps the block's first loads (a caller that has already
+    // decided for the whole block passes check_halt = false)
+    const bool stop = check_halt && halted(A.flags);
+    constexpr int UH = ENV_UH, UR = TY + 2 * UH, UC = TX + 2 * UH;   // u on tile +- 6
     constexpr int TR = TY + 4;                                  // row-blurred rows tile +- 2
     constexpr int VR = TY + 4, VC = TX + 4;                     // v on tile +- 2
     constexpr int ER = TY + 2, EC = TX + 2;                     // E on tile +- 1
     // the Gaussian staging (sU, sT) is dead once v_env is built, so the
     // diffusion staging (sC, sD, sFx, sFy) reuses its LDS
-    constexpr int A1 = UR * UC + TR * UC;
-    constexpr int A2 = VR * VC + ER * EC + TY * (TX + 2) + (TY + 2) * TX;
-    __shared__ double s_pool[A1 > A2 ? A1 : A2];
-    __shared__ double sV[VR][VC];
-    __shared__ double sEx[ER][EC];
-    __shared__ double sEy[ER][EC];
+    double* s_pool = lds;
+    double (&sV)[VR][VC] = *reinterpret_cast<double (*)[VR][VC]>(lds + ENV_POOL);
+    double (&sEx)[ER][EC] = *reinterpret_cast<double (*)[ER][EC]>(lds + ENV_POOL + VR * VC);
+    double (&sEy)[ER][EC] = *reinterpret_cast<double (*)[ER][EC]>(lds + ENV_POOL + VR * VC + ER * EC);
+    double* s_du = lds + ENV_POOL + VR * VC + 2 * ER * EC;
     double (&sU)[UR][UC] = *reinterpret_cast<double (*)[UR][UC]>(s_pool);
     double (&sT)[TR][UC] = *reinterpret_cast<double (*)[TR][UC]>(s_pool + UR * UC);
     double (&sC)[VR][VC] = *reinterpret_cast<double (*)[VR][VC]>(s_pool);
@@ -659,9 +697,6 @@ k_env_field(FieldArgs A, KParams P) {
     double (&sFx)[TY][TX + 2] = *reinterpret_cast<double (*)[TY][TX + 2]>(s_pool + VR * VC + ER * EC);
     double (&sFy)[TY + 2][TX] =
         *reinterpret_cast<double (*)[TY + 2][TX]>(s_pool + VR * VC + ER * EC + TY * (TX + 2));
-    // (tile, ion group) from the XCD-chunked linear block index, tiles fastest
-    const int lin = xcd_chunk(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int tile_id = lin % gridDim.x, ion_id = lin / gridDim.x;
     const int* et = A.env_tiles + 3 * tile_id;
     const KCluster K = A.kcl[et[0]];
     const int ny = K.ny, nx = K.nx, so = K.sq_off;
@@ -669,12 +704,12 @@ k_env_field(FieldArgs A, KParams P) {
     const int tid = threadIdx.x;
     const double dl = K.delta, d2 = K.two_delta;
     const double rdl = 1.0 / dl, rd2 = 1.0 / d2;
-    auto per_dl = [&](double x) { return RECIP ? x * rdl : x / dl; };
-    auto per_d2 = [&](double x) { return RECIP ? x * rd2 : x / d2; };
+    auto per_dl = [&](double x) { return x * rdl; };
+    auto per_d2 = [&](double x) { return x * rd2; };
 
-    // block row ion_id takes the ions [ion_id * ipb, ion_id * ipb + ipb), with
-    // ipb = ceil(n_ions / gridDim.y), and rebuilds its tile's field for them
-    const int ipb = (P.n_ions + gridDim.y - 1) / gridDim.y;
+    // ion group ion_id takes the ions [ion_id * ipb, ion_id * ipb + ipb), with
+    // ipb = ceil(n_ions / n_groups), and rebuilds its tile's field for them
+    const int ipb = (P.n_ions + n_groups - 1) / n_groups;
     const int i_lo = ion_id * ipb;
     const int i_hi = min(i_lo + ipb, P.n_ions);
 
@@ -685,11 +720,10 @@ k_env_field(FieldArgs A, KParams P) {
     // through the previous ion's flux stages (barriers wait for LDS only).
     const int nc = P.n_cells;
     const int items = RATES ? P.n_chan * nc : 0;
-    const int nthr = gridDim.x * gridDim.y * (TY * TX);
-    const int t_first = (blockIdx.y * gridDim.x + blockIdx.x) * (TY * TX) + tid;
+    const int nthr = rate_stride;
+    const int t_first = rate_first;
     double vm_first = 0.0;
-    if (RATES && t_first < items) vm_first = A.vm[t_first % nc];
-    __shared__ double s_du[BT_MAX_IONS];
+    if (RATES && t_first >= 0 && t_first < items) vm_first = A.vm[t_first % nc];
     double du_mine = 0.0;
     if (DIFFUSE && tid < P.n_ions) du_mine = A.kcl[et[0]].denv_u[tid];
 
@@ -747,29 +781,8 @@ k_env_field(FieldArgs A, KParams P) {
     // with the block's first loads); run first, their registers would stay
     // live beside the prefetched loads and cost occupancy.
     auto gating_rates = [&]() {
-        for (int t_it = t_first; t_it < items; t_it += nthr) {
-            const int ch = t_it / nc, cell = t_it - ch * nc;
-            // Vmem carrying the n_ions + ch nonces of the membrane stage
-            const double vmc = (t_it == t_first) ? vm_first : A.vm[cell];
-            const double V = with_nonces(vmc, P.n_ions + ch) * 1000.0 + 0.0;
-            double mInf, mTau, hInf, hTau;
-            gate_rates(P.chan_kind[ch], V, mInf, mTau, hInf, hTau);
-            double* R = A.rates + (size_t)(4 * ch) * nc + cell;
-            if (A.gate_coef) {
-                // update_mh as x' = a x + b: a = tau/(tau+dt), b = dt*inf/(tau+dt)
-                // (FP64, an ulp from the quotient form; the FP32-flux path only)
-                const double dtm = P.chan_dt;
-                R[0] = mTau / (mTau + dtm);
-                R[nc] = (dtm * mInf) / (mTau + dtm);
-                R[2 * nc] = hTau / (hTau + dtm);
-                R[3 * nc] = (dtm * hInf) / (hTau + dtm);
-            } else {
-                R[0] = mInf;
-                R[nc] = mTau;
-                R[2 * nc] = hInf;
-                R[3 * nc] = hTau;
-            }
-        }
+        for (int t_it = t_first; t_it >= 0 && t_it < items; t_it += nthr)
+            gate_coef_item(A, P, t_it, (t_it == t_first) ? vm_first : A.vm[t_it % nc]);
     };
     if (DIFFUSE && tid < P.n_ions) s_du[tid] = du_mine;
 
@@ -839,7 +852,7 @@ k_env_field(FieldArgs A, KParams P) {
     for (int i = i_lo; i < i_hi; ++i) {
         const double cb = P.c_bound[i];
         const double zi = P.zq[i];
-        const double zq_kt = (zi * P.q) / P.kbT;        // RECIP: alpha = D * zq_kt
+        const double zq_kt = (zi * P.q) / P.kbT;        // alpha = D * zq_kt
         __syncthreads();
         // concentrations of ion i (loaded one ion ahead) with the world
         // boundary reset to c_env_bound (Q3)
@@ -877,7 +890,7 @@ k_env_field(FieldArgs A, KParams P) {
                 else if (cc == nx - 1) g = per_dl((sC[sr][sc] - sC[sr][sc - 1]));
                 else g = per_d2(-(sC[sr][sc - 1] - sC[sr][sc + 1]));
                 const double Dv = sD[lr + 1][lc];
-                const double alpha = RECIP ? Dv * zq_kt : (Dv * zi * P.q) / P.kbT;
+                const double alpha = Dv * zq_kt;
                 fx = (-Dv) * g - (alpha * (-sEx[lr + 1][lc])) * sC[sr][sc];
             }
             sFx[lr][lc] = fx;
@@ -893,7 +906,7 @@ k_env_field(FieldArgs A, KParams P) {
                 else if (r == ny - 1) g = per_dl((sC[sr][sc] - sC[sr - 1][sc]));
                 else g = per_d2(-(sC[sr - 1][sc] - sC[sr + 1][sc]));
                 const double Dv = sD[lr][lc + 1];
-                const double alpha = RECIP ? Dv * zq_kt : (Dv * zi * P.q) / P.kbT;
+                const double alpha = Dv * zq_kt;
                 fy = (-Dv) * g - (alpha * (-sEy[lr][lc + 1])) * sC[sr][sc];
             }
             sFy[lr][lc] = fy;
@@ -926,6 +939,18 @@ k_env_field(FieldArgs A, KParams P) {
         }
     }
     if (RATES) gating_rates();
+}
+
+template <bool DIFFUSE, bool RATES>
+__global__ void __launch_bounds__(TY * TX)
+k_env_field(FieldArgs A, KParams P) {
+    __shared__ double lds[ENV_LDS];
+    // (tile, ion group) from the XCD-chunked linear block index, tiles fastest
+    const int lin = xcd_chunk(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int tile_id = lin % gridDim.x, ion_id = lin / gridDim.x;
+    env_field_tile<DIFFUSE, RATES>(A, P, tile_id, ion_id, gridDim.y, lds,
+                                   (blockIdx.y * gridDim.x + blockIdx.x) * (TY * TX) + threadIdx.x,
+                                   gridDim.x * gridDim.y * (TY * TX));
 }
 
 // ---------------------------------------------------------------------------
@@ -1003,13 +1028,8 @@ k_env_exchange(ExArgs A, KParams P) {
     }
 }
 
-// one thread per square over all ions (large batches: enough squares to fill the GPU)
-__global__ void __launch_bounds__(BLOCK)
-k_env_exchange_sq(ExArgs A, KParams P) {
-    if (halted(A.flags)) return;
-    const int q = blockIdx.x * BLOCK + threadIdx.x;
-    if (A.count_step && q == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
-    if (q >= A.n_live) return;
+// the exchange of the q-th square with membranes, over all ions
+__device__ __forceinline__ void exchange_square(const ExArgs& A, const KParams& P, int q) {
     const int k = A.sq_live[q];
     const int a = A.sq_ptr[k], b = A.sq_ptr[k + 1];
     const KCluster& K = A.kcl[A.sq_cluster[k]];
@@ -1029,6 +1049,16 @@ k_env_exchange_sq(ExArgs A, KParams P) {
     double uu = 0.0;
     if (A.has_mem[k]) uu = ((rho * K.surf_scale) / K.mean_sa) / K.kee;
     A.u[k] = uu;
+}
+
+// one thread per square over all ions (large batches: enough squares to fill the GPU)
+__global__ void __launch_bounds__(BLOCK)
+k_env_exchange_sq(ExArgs A, KParams P) {
+    if (halted(A.flags)) return;
+    const int q = blockIdx.x * BLOCK + threadIdx.x;
+    if (A.count_step && q == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
+    if (q >= A.n_live) return;
+    exchange_square(A, P, q);
 }
 
 // rho_env -> u for the current env state (before a phase's loop)
@@ -1063,6 +1093,27 @@ k_cell_vm(const double* c_cell, const double* diviterm, double* vm, KParams P) {
 // per-cell reaction network (MasterOfNetworks.run_loop, networks.py:2805-2982)
 // ---------------------------------------------------------------------------
 
+// The network as the kernel reads it, built on the host from bt_network:
+// quotients by constants as products by reciprocals (1/Km, 1/gad_scale,
+// 1/Keq, each an ulp from the quotient), and per substance the list of its
+// (rate, coefficient) terms, the nonzero entries of its row of the reaction
+// matrix in ascending rate order (the order of np.dot(reaction_matrix, rates)).
+struct KNetRate {
+    int kind, species, reversible, n_act, n_inh, n_reac, n_prod, pad;
+    int act[BT_NET_MAX_HILL], inh[BT_NET_MAX_HILL], reac[BT_NET_MAX_TERMS], prod[BT_NET_MAX_TERMS];
+    double act_rkm[BT_NET_MAX_HILL], act_n[BT_NET_MAX_HILL], inh_rkm[BT_NET_MAX_HILL], inh_n[BT_NET_MAX_HILL];
+    double reac_coeff[BT_NET_MAX_TERMS], reac_rkm[BT_NET_MAX_TERMS], prod_coeff[BT_NET_MAX_TERMS];
+    double r_prod, r_decay, decay_max, inv_gad, vmax, inv_keq;
+};
+constexpr int NET_MAX_TERMS_ALL = BT_NET_MAX_RATES * BT_NET_MAX_STOICH;
+struct KNet {
+    int n_species, n_rates;
+    int sub_ptr[BT_NET_MAX_SPECIES + 1];
+    int term_rate[NET_MAX_TERMS_ALL];
+    double term_coeff[NET_MAX_TERMS_ALL];
+    KNetRate rates[BT_NET_MAX_RATES];
+};
+
 // (c/Km)**n: exponents 1 and 2 take numpy's exact fast paths; 4 is (x*x)*(x*x)
 // where numpy calls pow (the two differ by at most an ulp or two, and FP64 pow
 // costs an order of magnitude more); other exponents call pow
@@ -1078,13 +1129,13 @@ __device__ __forceinline__ double hill_pow(double x, double n) {
 
 // activator term (((c/Km)**n)/(1 + (c/Km)**n)) and inhibitor term (1/(1 + (c/Km)**n))
 // (get_influencers, networks.py:5184-5200, 5396-5420); the reference evaluates
-// (c/Km)**n twice to the same value, here once
-__device__ __forceinline__ double hill_act(double c, double km, double n) {
-    const double x = hill_pow(c / km, n);
+// (c/Km)**n twice to the same value, here once; rkm = 1/Km
+__device__ __forceinline__ double hill_act(double c, double rkm, double n) {
+    const double x = hill_pow(c * rkm, n);
     return x / (1.0 + x);
 }
-__device__ __forceinline__ double hill_inh(double c, double km, double n) {
-    return 1.0 / (1.0 + hill_pow(c / km, n));
+__device__ __forceinline__ double hill_inh(double c, double rkm, double n) {
+    return 1.0 / (1.0 + hill_pow(c * rkm, n));
 }
 
 // A block holds NET_CELLS cells and NET_WAVES wavefronts: lane l of every
@@ -1092,16 +1143,16 @@ __device__ __forceinline__ double hill_inh(double c, double km, double n) {
 // NET_WAVES, ... from the pre-step concentrations (rate-uniform per wavefront,
 // so the network description is read through scalar loads and no lane
 // diverges from its wavefront). After a barrier, wavefront w updates the
-// substances w, w + NET_WAVES, ...: delta[s] sums coeff * rate over the rates
-// in ascending order (np.dot(reaction_matrix, rates)) in the reference's
-// order (growth/decay terms of every substance, then reactions), then
-// c + delta*dt and no_negs.
+// substances w, w + NET_WAVES, ...: delta[s] sums coeff * rate over the
+// substance's terms in ascending rate order (np.dot(reaction_matrix, rates)),
+// in the reference's order (growth/decay terms of every substance, then
+// reactions), then c + delta*dt and no_negs.
 constexpr int NET_CELLS = 64;
 constexpr int NET_WAVES = 4;
 constexpr int NET_BLOCK = NET_CELLS * NET_WAVES;
 
 __global__ void __launch_bounds__(NET_BLOCK)
-k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cells, double dt,
+k_network(const KNet* __restrict__ net, double* __restrict__ cn, int n_cells, double dt,
           unsigned* flags, unsigned long long* bad_step, const long long* step_ctr, int step_ahead) {
     if (halted(flags)) return;
     __shared__ double s_c[BT_NET_MAX_SPECIES][NET_CELLS];
@@ -1118,22 +1169,22 @@ k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cel
     __syncthreads();
     const int R = net->n_rates;
     for (int j = w; j < R; j += NET_WAVES) {
-        const bt_rate& r = net->rates[j];
+        const KNetRate& r = net->rates[j];
         // alpha = (product of activator terms) * (product of inhibitor terms), each
         // product taken left to right; an empty product is np.ones
         double act = 1.0, inh = 1.0;
         for (int q = 0; q < r.n_act; ++q) {
-            const double h = hill_act(s_c[r.act[q]][t], r.act_km[q], r.act_n[q]);
+            const double h = hill_act(s_c[r.act[q]][t], r.act_rkm[q], r.act_n[q]);
             act = (q == 0) ? h : act * h;
         }
         for (int q = 0; q < r.n_inh; ++q) {
-            const double h = hill_inh(s_c[r.inh[q]][t], r.inh_km[q], r.inh_n[q]);
+            const double h = hill_inh(s_c[r.inh[q]][t], r.inh_rkm[q], r.inh_n[q]);
             inh = (q == 0) ? h : inh * h;
         }
         double rate;
         if (r.kind == BT_RATE_GROWTH) {
             // mod*r_prod*alpha - r_decay*cc - decay_max*1*cc, cc = c/gad_scale (networks.py:1233-1249)
-            const double cc = s_c[r.species][t] / r.gad_scale;
+            const double cc = s_c[r.species][t] * r.inv_gad;
             rate = ((1.0 * r.r_prod) * act) * inh;
             rate = rate - r.r_decay * cc;
             rate = rate - ((r.decay_max * 1.0) * 1.0) * cc;
@@ -1150,12 +1201,12 @@ k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cel
                     const double f = 1.0e-3 * hill_pow(s_c[r.reac[q]][t], r.reac_coeff[q]);
                     den = (q == 0) ? f : den * f;
                 }
-                x = 1.0 - ((num / den) / r.keq);
+                x = 1.0 - ((num / den) * r.inv_keq);
             } else {
                 // forward coefficient: product over reactants of ((c/Km)**k)/(1 + (c/Km)**k)
                 double fwd = 1.0;
                 for (int q = 0; q < r.n_reac; ++q) {
-                    const double f = hill_act(s_c[r.reac[q]][t], r.reac_km[q], r.reac_coeff[q]);
+                    const double f = hill_act(s_c[r.reac[q]][t], r.reac_rkm[q], r.reac_coeff[q]);
                     fwd = (q == 0) ? f : fwd * f;
                 }
                 x = fwd - (0.0 * 0.0);
@@ -1169,11 +1220,7 @@ k_network(const bt_network* __restrict__ net, double* __restrict__ cn, int n_cel
     if (live) {
         for (int s = w; s < S; s += NET_WAVES) {
             double d = 0.0;
-            for (int j = 0; j < R; ++j) {
-                const bt_rate& r = net->rates[j];
-                for (int q = 0; q < r.n_stoich; ++q)
-                    if (r.stoich_species[q] == s) d = d + r.stoich_coeff[q] * s_r[j][t];
-            }
+            for (int q = net->sub_ptr[s]; q < net->sub_ptr[s + 1]; ++q) d = d + net->term_coeff[q] * s_r[net->term_rate[q]][t];
             double v = s_c[s][t] + d * dt;
             if (v < 0.0) v = 0.0;                    // stb.no_negs clamps negatives ...
             if (isnan(v)) bad = true;                // ... and raises on NaN
@@ -1440,6 +1487,143 @@ k_residual(const double* c, const double* c_prev, const int32_t* cell_cluster, u
     atomicMax(&rmax[cell_cluster[cidx]], (unsigned long long)__double_as_longlong(r));
 }
 
+// ---------------------------------------------------------------------------
+// k_step_persistent: many steps in one cooperative launch
+// ---------------------------------------------------------------------------
+// Each step's three dependent phases (env field + diffusion, membranes and
+// cells, exchange) run in one launch whose blocks all stay resident
+// (hipLaunchCooperativeKernel) and meet at a grid barrier between phases: the
+// per-launch drain and ramp of the three-kernel step are gone, and a launch
+// runs any even number of steps. Work is dealt out in grid-stride order: env
+// (tile, ion group) items per block, cell tiles per wavefront (each wavefront
+// owns one tile at a time, with the membrane kernel's single-wavefront form),
+// squares with membranes per thread.
+constexpr int PB = 256;                      // threads per persistent block
+constexpr int PW = PB / TMB;                 // wavefronts per persistent block
+#ifndef PERSIST_WAVES_PER_SIMD
+#define PERSIST_WAVES_PER_SIMD 4             // occupancy the register budget is held to
+#endif
+constexpr unsigned BT_FLAG_BARRIER = 2u;     // flags bit: a grid barrier gave up waiting
+constexpr unsigned BARRIER_SPINS = 1u << 24; // about a second of s_sleep before it gives up
+
+struct PersistArgs {
+    FieldArgs field[2];                      // by step parity
+    MemCellArgs mem[2];
+    ExArgs ex[2];
+    unsigned* bar;                           // [0] arrivals, [1] generation
+    int n_env_tiles, n_groups, n_tiles, n_steps;
+};
+
+// LDS of one membrane-phase wavefront, in doubles: flux*sa and GJ flux per ion,
+// channel flux per channel, and the tile's NaN flag
+template <int NI>
+constexpr int mem_wave_lds() { return (2 * NI + BT_MAX_CHAN) * TMB + 1; }
+
+// Grid barrier of a cooperative launch. Thread 0 of each block reads the
+// generation, arrives, and the last arrival resets the count and opens the next
+// generation; the others wait for it. The agent-scope fences make every block's
+// global writes before the barrier visible to every block after it (across the
+// XCDs' L2 caches). A wait that exceeds BARRIER_SPINS sets BT_FLAG_BARRIER and
+// the block goes on, so every wave reaches the end of the kernel whatever
+// happens. Returns false once the flag is set.
+__device__ __forceinline__ bool grid_sync(unsigned* bar, unsigned* flags) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __threadfence();
+        const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (atomicAdd(&bar[0], 1u) == gridDim.x - 1) {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            atomicAdd(&bar[1], 1u);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > BARRIER_SPINS) {
+                    atomicOr(flags, BT_FLAG_BARRIER);
+                    break;
+                }
+            }
+        }
+        __threadfence();
+        if (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & BT_FLAG_BARRIER) ok = 0;
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+template <int NI, typename FT, bool RATES>
+__global__ void __launch_bounds__(PB, PERSIST_WAVES_PER_SIMD)
+k_step_persistent(PersistArgs PA, KParams P) {
+    constexpr int MW = mem_wave_lds<NI>();
+    constexpr int LDS = ENV_LDS > PW * MW ? ENV_LDS : PW * MW;
+    __shared__ double lds[LDS];
+    __shared__ int s_stop;
+    const int nb = gridDim.x;
+    const int wave = threadIdx.x / TMB;
+    const int gtid = blockIdx.x * PB + threadIdx.x;
+    const int nthr = nb * PB;
+    const int n_env_items = PA.n_env_tiles * PA.n_groups;
+    // this wavefront's slice of the LDS in the membrane phase
+    double* wl = lds + wave * MW;
+    const MemLds L{reinterpret_cast<double (*)[TMB]>(wl), reinterpret_cast<double (*)[TMB]>(wl + NI * TMB),
+                   reinterpret_cast<double (*)[TMB]>(wl + 2 * NI * TMB), reinterpret_cast<double (*)[TMB]>(wl),
+                   reinterpret_cast<int*>(wl + (2 * NI + BT_MAX_CHAN) * TMB)};
+    // one step of parity PAR (a compile-time index into the argument sets);
+    // false once a grid barrier gave up
+    auto step = [&](auto par_c) -> bool {
+        constexpr int par = decltype(par_c)::value;
+        const FieldArgs& FA = PA.field[par];
+        // phase A: env field and diffusion (and the gating coefficients). The
+        // flags change only in the membrane phase, so one read decides for the block.
+        if (threadIdx.x == 0) s_stop = halted(FA.flags) ? 1 : 0;
+        __syncthreads();
+        if (!s_stop) {
+            for (int it = blockIdx.x; it < n_env_items; it += nb) {
+                const int item = (nb % N_XCD == 0) ? xcd_chunk(it, n_env_items) : it;
+                env_field_tile<true, false>(FA, P, item % PA.n_env_tiles, item / PA.n_env_tiles, PA.n_groups, lds,
+                                            -1, 0, false);
+                __syncthreads();
+            }
+            if (RATES) {
+                const int items = P.n_chan * P.n_cells;
+                for (int t_it = gtid; t_it < items; t_it += nthr) gate_coef_item(FA, P, t_it, FA.vm[t_it % P.n_cells]);
+            }
+        }
+        if (!grid_sync(PA.bar, PA.mem[par].flags)) return false;
+        // phase B: membranes and cells, one tile per wavefront at a time; the
+        // halt check is uniform over the wavefront
+        {
+            const MemCellArgs& MA = PA.mem[par];
+            const bool stop = __builtin_amdgcn_readfirstlane(halted(MA.flags) ? 1 : 0) != 0;
+            if (!stop) {
+                const int nw = nb * PW;
+                for (int w = blockIdx.x * PW + wave; w < PA.n_tiles; w += nw) {
+                    mem_cell_wave<NI, FT, 1, 0, false, true>(MA, P, w, L);
+                    tile_sync<true>();
+                }
+            }
+        }
+        if (!grid_sync(PA.bar, PA.mem[par].flags)) return false;
+        // phase C: the exchange of the squares with membranes; the step counter
+        {
+            const ExArgs& XA = PA.ex[par];
+            if (!halted(XA.flags)) {
+                if (gtid == 0) XA.step_ctr[0] = XA.step_ctr[0] + 1;
+                for (int q = gtid; q < XA.n_live; q += nthr) exchange_square(XA, P, q);
+            }
+        }
+        return grid_sync(PA.bar, PA.mem[par].flags);
+    };
+    for (int s = 0; s < PA.n_steps; s += 2) {
+        if (!step(std::integral_constant<int, 0>{})) break;
+        if (!step(std::integral_constant<int, 1>{})) break;
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1470,6 +1654,8 @@ struct bt_ctx {
     hipGraph_t graph_n = nullptr; hipGraphExec_t gexec_n = nullptr;   // tune.graph_pairs pairs
     bt_tuning tune{};                  // launch variants of the step kernels
     bool autotuned = false;
+    unsigned* bar = nullptr;           // grid barrier of the persistent step kernel
+    int pers_grid = -1;                // its cooperative grid (-1 not yet known, 0 unavailable)
     long long steps_done = 0;
     const bt_obs* capture = nullptr;   // set while enqueueing an observed step
     // observables workspace (allocated on first use)
@@ -1483,7 +1669,7 @@ struct bt_ctx {
     // Vmem per cell before the last observed step: update_gj's E_gj uses it (sim.py:2166-2172)
     double* vm_prev = nullptr; bool vm_prev_ok = false;
     // reaction network (device copy of the description) and its state buffer
-    bt_network* net = nullptr; double* c_net = nullptr;
+    KNet* net = nullptr; double* c_net = nullptr; int net_species = 0;
     std::string err;
 };
 
@@ -1524,6 +1710,14 @@ static void derive_params(const bt_params& p, KParams& k) {
     k.dt = p.dt; k.tm = p.tm; k.gj_len = p.gj_len; k.F = p.F;
     k.RT = p.R * p.T; k.RT_gj = p.R * p.T_gj; k.q = p.q; k.kbT = p.kb * p.T;
     k.cm_inv = 1.0 / p.cm;
+    k.f_rt = p.F / (p.R * p.T);
+    k.f_rtgj = p.F / (p.R * p.T_gj);
+    k.inv_tm = 1.0 / p.tm;
+    k.inv_gjlen = 1.0 / p.gj_len;
+    k.inv_rt = 1.0 / (p.R * p.T);
+    k.inv_kmna = 1.0 / p.KmNK_Na;
+    k.inv_kmk = 1.0 / p.KmNK_K;
+    k.inv_kmca = 1.0 / p.KmCa_Ca;
     k.f_rt32 = (float)(p.F / (p.R * p.T));
     k.f_rtgj32 = (float)(p.F / (p.R * p.T_gj));
     k.inv_tm32 = (float)(1.0 / p.tm);
@@ -1569,6 +1763,8 @@ static KCluster derive_cluster(const bt_params& p, const bt_cluster& b) {
 constexpr int BT_MEM_WG_TILES = 4096;
 constexpr int BT_GRAPH_PAIRS = 8;
 constexpr int BT_MAX_GRAPH_PAIRS = 64;
+constexpr int BT_PERSIST_STEPS = 512;   // steps per cooperative launch of the persistent kernel (even)
+constexpr int BT_AUTOTUNE_TRIALS = 10;
 
 template <int NI, int WG, bool OBS>
 static void launch_mem_cell_o(bt_ctx* c, const MemCellArgs& a) {
@@ -1606,34 +1802,39 @@ static inline int grid_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 // diffuse: the step's env field and diffusion, and the gating rates from vm;
 // else the field alone (v_env, E) from u
+static FieldArgs field_args(bt_ctx* c, bool diffuse, const bt_obs* ob, const double* vm, const double* env_in,
+                            double* env_out) {
+    const bt_buffers& B = c->buf;
+    return FieldArgs{c->env_tiles, c->kcl, c->u, env_in ? env_in : B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y,
+                     c->flags, ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
+                     vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr, c->live_flag, env_out};
+}
+
 static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, const double* vm = nullptr,
                          const double* env_in = nullptr, double* env_out = nullptr) {
-    const bt_buffers& B = c->buf;
-    FieldArgs f{c->env_tiles, c->kcl, c->u, env_in ? env_in : B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y,
-                c->flags, ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
-                vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr,
-                c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0, c->live_flag, env_out};
+    const FieldArgs f = field_args(c, diffuse, ob, vm, env_in, env_out);
     // diffusion: one block per (tile, env_ions_per_block ions): few ions per
     // block while tiles alone would leave CUs idle, else one block per tile
     // looping over the ions; field only: one block per tile
     const int ipb = c->tune.env_ions_per_block;
     dim3 grid(c->n_env_tiles, diffuse ? (c->prm.n_ions + ipb - 1) / ipb : 1);
-    const bool recip = c->prm.flux_mode == BT_FLUX_FP32;
-    if (diffuse && f.rates && recip)
-        hipLaunchKernelGGL((k_env_field<true, true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
-    else if (diffuse && f.rates)
-        hipLaunchKernelGGL((k_env_field<true, true, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
-    else if (diffuse && recip)
-        hipLaunchKernelGGL((k_env_field<true, false, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+    if (diffuse && f.rates)
+        hipLaunchKernelGGL((k_env_field<true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else if (diffuse)
-        hipLaunchKernelGGL((k_env_field<true, false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<true, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
     else
-        hipLaunchKernelGGL((k_env_field<false, false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
 }
 
-// One step. parity 0: cell state caller -> workspace; parity 1: workspace -> caller.
-// which: -1 all, 0 membrane/cell, 1 env field+diffusion, 2 exchange, 3 field only
-static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
+// The kernel arguments of one step. parity 0: cell state caller -> workspace;
+// parity 1: workspace -> caller (the env state ping-pongs likewise).
+struct StepArgs {
+    FieldArgs field;
+    MemCellArgs mem;
+    ExArgs ex;
+};
+
+static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_step) {
     const bt_buffers& B = c->buf;
     double* ci = parity ? c->c_cell_b : B.c_cell;
     double* li = parity ? c->c_lag_b : B.c_lag;
@@ -1641,36 +1842,103 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     double* co = parity ? B.c_cell : c->c_cell_b;
     double* lo = parity ? B.c_lag : c->c_lag_b;
     double* vo = parity ? B.vm_cell : c->vm_b;
-    double* ei = parity ? c->c_env_b : B.c_env;     // env state in and out, ping-pong like the cells
+    double* ei = parity ? c->c_env_b : B.c_env;
     double* eo = parity ? B.c_env : c->c_env_b;
+    StepArgs a{field_args(c, true, ob, vi, ei, eo),
+               MemCellArgs{B.mem_ptr, B.mem_cell, B.nn_mem, B.nn_cell, B.mem_sq, B.mem_pos, B.tile_ptr,
+                           B.tile_cluster, c->kcl,
+                           B.mem_sa, B.gjopen, B.cell_vol, B.diviterm, B.Dm, ei, c->c_mid,
+                           ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->rates, c->fch,
+                           c->flags, c->bad_step, c->step_ctr,
+                           ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
+                           ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr},
+               ExArgs{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u, c->flags,
+                      c->step_ctr, count_step, c->sq_live, c->n_live}};
+    return a;
+}
 
+// One step on the stream.
+// which: -1 all, 0 membrane/cell, 1 env field+diffusion, 2 exchange, 3 field only
+static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
     const bt_obs* ob = c->capture;
-    if (which < 0 || which == 1) launch_field(c, true, ob, vi, ei, eo);
-    if (which < 0 || which == 0) {
-        MemCellArgs a{B.mem_ptr, B.mem_cell, B.nn_mem, B.nn_cell, B.mem_sq, B.mem_pos, B.tile_ptr,
-                      B.tile_cluster, c->kcl,
-                      B.mem_sa, B.gjopen, B.cell_vol, B.diviterm, B.Dm, ei, c->c_mid,
-                      ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->rates, c->fch,
-                      c->flags, c->bad_step, c->step_ctr,
-                      ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
-                      ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr};
-        launch_mem_cell(c, a);
-    }
+    const StepArgs a = step_args(c, parity, ob, which < 0 ? 1 : 0);
+    if (which < 0 || which == 1) launch_field(c, true, ob, a.field.vm, a.field.c_env, a.field.c_env_out);
+    if (which < 0 || which == 0) launch_mem_cell(c, a.mem);
     if (which < 0 || which == 2) {
-        ExArgs x{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u, c->flags,
-                 c->step_ctr, which < 0 ? 1 : 0, c->sq_live, c->n_live};
         const int nl = c->n_live > 0 ? c->n_live : 1;    // block 0 also counts the step
         if (c->tune.exchange_form == 0)
             hipLaunchKernelGGL(k_env_exchange, dim3((nl + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0,
-                               c->stream, x, c->kp);
+                               c->stream, a.ex, c->kp);
         else
-            hipLaunchKernelGGL(k_env_exchange_sq, dim3(grid_blocks(nl)), dim3(BLOCK), 0, c->stream, x, c->kp);
+            hipLaunchKernelGGL(k_env_exchange_sq, dim3(grid_blocks(nl)), dim3(BLOCK), 0, c->stream, a.ex, c->kp);
     }
     if (which == 3) launch_field(c, false);
     if ((which < 0 || which == 4) && c->net && c->c_net)
         hipLaunchKernelGGL(k_network, dim3((c->prm.n_cells + NET_CELLS - 1) / NET_CELLS), dim3(NET_BLOCK), 0,
-                           c->stream, (const bt_network*)c->net, c->c_net, c->prm.n_cells, c->prm.dt, c->flags,
+                           c->stream, (const KNet*)c->net, c->c_net, c->prm.n_cells, c->prm.dt, c->flags,
                            c->bad_step, (const long long*)c->step_ctr, which < 0 ? 1 : 0);
+}
+
+// The persistent step kernel of this problem, or nullptr where it has no
+// instantiation: five or six ions, no reaction network (the network keeps its
+// own kernel).
+static const void* persistent_kernel(bt_ctx* c) {
+    if (c->net && c->c_net) return nullptr;
+    const bool f32 = c->prm.flux_mode == BT_FLUX_FP32;
+    const bool rates = c->prm.n_chan > 0;
+    switch (c->prm.n_ions) {
+        case 5:
+            if (f32) return rates ? (const void*)&k_step_persistent<5, float, true> : (const void*)&k_step_persistent<5, float, false>;
+            return rates ? (const void*)&k_step_persistent<5, double, true> : (const void*)&k_step_persistent<5, double, false>;
+        case 6:
+            if (f32) return rates ? (const void*)&k_step_persistent<6, float, true> : (const void*)&k_step_persistent<6, float, false>;
+            return rates ? (const void*)&k_step_persistent<6, double, true> : (const void*)&k_step_persistent<6, double, false>;
+        default:
+            return nullptr;
+    }
+}
+
+// blocks of a cooperative launch of the persistent kernel: as many as stay
+// resident together (0 where the kernel or cooperative launches are unavailable)
+static int persistent_grid(bt_ctx* c) {
+    if (c->pers_grid >= 0) return c->pers_grid;
+    c->pers_grid = 0;
+    const void* k = persistent_kernel(c);
+    int coop = 0, cus = 0, per_cu = 0;
+    if (!k || hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) != hipSuccess || !coop)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus < 1)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PB, 0) != hipSuccess || per_cu < 1) return 0;
+    c->pers_grid = per_cu * cus;
+    return c->pers_grid;
+}
+
+// n_steps (even) steps in cooperative launches of the persistent kernel
+static int launch_persistent(bt_ctx* c, int n_steps) {
+    const void* k = persistent_kernel(c);
+    const int grid = persistent_grid(c);
+    if (!k || grid < 1) return set_err(c, BT_EINVAL, "no persistent step kernel for this problem");
+    PersistArgs pa;
+    for (int par = 0; par < 2; ++par) {
+        const StepArgs a = step_args(c, par, nullptr, 1);
+        pa.field[par] = a.field;
+        pa.mem[par] = a.mem;
+        pa.ex[par] = a.ex;
+    }
+    pa.bar = c->bar;
+    pa.n_env_tiles = c->n_env_tiles;
+    pa.n_groups = (c->prm.n_ions + c->tune.env_ions_per_block - 1) / c->tune.env_ions_per_block;
+    pa.n_tiles = c->n_tiles;
+    for (int done = 0; done < n_steps;) {
+        // at most BT_PERSIST_STEPS steps per launch
+        pa.n_steps = std::min(n_steps - done, BT_PERSIST_STEPS);
+        HIP_TRY(c, hipMemsetAsync(c->bar, 0, 2 * sizeof(unsigned), c->stream));
+        void* args[] = {(void*)&pa, (void*)&c->kp};
+        HIP_TRY(c, hipLaunchCooperativeKernel(k, dim3(grid), dim3(PB), args, 0, c->stream));
+        done += pa.n_steps;
+    }
+    return BT_OK;
 }
 
 static void drop_graphs(bt_ctx* c) {
@@ -1962,6 +2230,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     alloc((void**)&c->flags, 256);
     alloc((void**)&c->bad_step, 256);
     alloc((void**)&c->step_ctr, 256);
+    alloc((void**)&c->bar, 256);
     alloc((void**)&c->kcl, sizeof(KCluster) * (size_t)n_clusters);
     alloc((void**)&c->env_tiles, sizeof(int) * tiles.size());
     alloc((void**)&c->c_env_b, ni * ns * 8);
@@ -1972,6 +2241,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     c->tune.env_ions_per_block = (c->n_env_tiles < ENV_SPLIT_TILES) ? std::min(2, p->n_ions) : p->n_ions;
     c->tune.exchange_form = (c->n_live < EX_SPLIT_SQUARES) ? 0 : 1;
     c->tune.graph_pairs = BT_GRAPH_PAIRS;
+    c->tune.persistent = 0;
     if (e == hipSuccess && !sq_live.empty())
         e = hipMemcpy(c->sq_live, sq_live.data(), sizeof(int32_t) * sq_live.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->live_flag, live_flag.data(), ns, hipMemcpyHostToDevice);
@@ -1981,6 +2251,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->bad_step, 0xff, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->step_ctr, 0, 256, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->bar, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->u, 0, ns * 8, c->stream);
     if (e == hipSuccess)
         e = hipMemsetAsync(c->rates, 0, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * 4 * nc * 8, c->stream);
@@ -2000,7 +2271,7 @@ int bt_destroy(bt_ctx* c) {
     drop_graphs(c);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
                     c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles,
-                    c->vm_prev, c->net};
+                    c->vm_prev, c->net, c->bar};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (double* q : c->obs_alloc) (void)hipFree(q);
@@ -2026,7 +2297,11 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     if (n_steps < 0) return set_err(c, BT_EINVAL, "n_steps < 0");
     if (n_steps == 0) return BT_OK;
     int pairs = n_steps / 2;
-    if (use_graph && pairs > 0) {
+    if (c->tune.persistent && pairs > 0) {
+        // every pair of steps in cooperative launches of the persistent kernel
+        int rc = launch_persistent(c, 2 * pairs);
+        if (rc != BT_OK) return rc;
+    } else if (use_graph && pairs > 0) {
         // a graph of tune.graph_pairs pairs of steps for the bulk, one of a
         // single pair for the rest; both are captured and instantiated on the
         // first call that replays graphs, so later calls only launch them
@@ -2074,6 +2349,10 @@ int bt_set_tuning(bt_ctx* c, const bt_tuning* t) {
     if (t->exchange_form != 0 && t->exchange_form != 1) return set_err(c, BT_EINVAL, "exchange_form must be 0 or 1");
     if (t->graph_pairs < 1 || t->graph_pairs > BT_MAX_GRAPH_PAIRS)
         return set_err(c, BT_EINVAL, "graph_pairs must be in 1..64");
+    if (t->persistent != 0 && t->persistent != 1) return set_err(c, BT_EINVAL, "persistent must be 0 or 1");
+    if (t->persistent && persistent_grid(c) < 1)
+        return set_err(c, BT_EINVAL, "no persistent step kernel for this problem (it needs 5 or 6 ions, no "
+                                     "reaction network, and cooperative launches)");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->tune = *t;
     drop_graphs(c);
@@ -2099,43 +2378,63 @@ static int time_launches(bt_ctx* c, int which, int n, double* ms_out) {
     return BT_OK;
 }
 
+// The state a step modifies in place, as (buffer, bytes) regions: the
+// caller's cell and env state, fields and gates, the surface charge u, the
+// substances, and the status words.
+static std::vector<std::pair<void*, size_t>> state_regions(bt_ctx* c) {
+    const size_t C = (size_t)c->prm.n_cells, M = (size_t)c->prm.n_mems, NS = (size_t)c->kp.n_sq;
+    const size_t ni = (size_t)c->prm.n_ions;
+    std::vector<std::pair<void*, size_t>> r = {
+        {c->buf.c_cell, ni * C * 8}, {c->buf.c_lag, ni * C * 8}, {c->buf.vm_cell, C * 8},
+        {c->buf.c_env, ni * NS * 8}, {c->buf.E_x, NS * 8}, {c->buf.E_y, NS * 8}, {c->buf.v_env, NS * 8},
+        {c->u, NS * 8}, {c->flags, 256}, {c->bad_step, 256}, {c->step_ctr, 256}};
+    if (c->prm.n_chan > 0) r.push_back({c->buf.gates, (size_t)c->prm.n_chan * 2 * M * 8});
+    if (c->net && c->c_net) r.push_back({c->c_net, (size_t)c->net_species * C * 8});
+    return r;
+}
+
+// copy the regions into save (back = false) or back out of it, synchronously
+static hipError_t state_copy(bt_ctx* c, const std::vector<std::pair<void*, size_t>>& regions, char* save,
+                             bool back) {
+    size_t o = 0;
+    for (const auto& r : regions) {
+        hipError_t e = back ? hipMemcpyAsync(r.first, save + o, r.second, hipMemcpyDeviceToDevice, c->stream)
+                            : hipMemcpyAsync(save + o, r.first, r.second, hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) return e;
+        o += (r.second + 255) / 256 * 256;
+    }
+    return hipStreamSynchronize(c->stream);
+}
+
 int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_ms, int32_t n_trial_ms) {
     if (!c || n_launches < 1 || (trial_ms && n_trial_ms < 0)) return BT_EINVAL;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     unsigned flags = 0;
     HIP_TRY(c, hipMemcpy(&flags, c->flags, sizeof(flags), hipMemcpyDeviceToHost));
     if (flags) return set_err(c, BT_EINVAL, "autotune needs a stable state (a NaN was flagged)");
-    // the timed launches read the caller's state and write the workspace, except
-    // the gates (in place), the surface charge u and the substances: keep copies
-    const size_t M = (size_t)c->prm.n_mems, NS = (size_t)c->kp.n_sq;
-    const size_t gbytes = c->prm.n_chan > 0 ? (size_t)c->prm.n_chan * 2 * M * 8 : 0;
-    double* save = nullptr;
-    const size_t sbytes = gbytes + NS * 8;
+    // every timed launch starts from the state as it is now: keep a copy of all
+    // that a step modifies in place and put it back after each trial
+    const auto regions = state_regions(c);
+    size_t sbytes = 0;
+    for (const auto& r : regions) sbytes += (r.second + 255) / 256 * 256;
+    char* save = nullptr;
     HIP_TRY(c, hipMalloc((void**)&save, sbytes));
-    auto keep = [&](bool back) -> hipError_t {
-        hipError_t e = hipSuccess;
-        if (gbytes)
-            e = back ? hipMemcpyAsync(c->buf.gates, save, gbytes, hipMemcpyDeviceToDevice, c->stream)
-                     : hipMemcpyAsync(save, c->buf.gates, gbytes, hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess)
-            e = back ? hipMemcpyAsync(c->u, save + gbytes / 8, NS * 8, hipMemcpyDeviceToDevice, c->stream)
-                     : hipMemcpyAsync(save + gbytes / 8, c->u, NS * 8, hipMemcpyDeviceToDevice, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        return e;
-    };
-    // (the network kernel is not among the timed variants: the substances are never touched)
-    hipError_t e = keep(false);
+    hipError_t e = state_copy(c, regions, save, false);
     const bt_tuning prev = c->tune;
     bt_tuning best = prev;
-    double t[8];
-    for (int q = 0; q < 8; ++q) t[q] = -1.0;
+    double t[BT_AUTOTUNE_TRIALS];
+    for (int q = 0; q < BT_AUTOTUNE_TRIALS; ++q) t[q] = -1.0;
     int rc = BT_OK;
+    auto restore = [&]() {
+        if (e == hipSuccess) e = state_copy(c, regions, save, true);
+    };
     if (e == hipSuccess) {
         // membrane kernel: one or two wavefronts per tile
         double bm = 1e30;
         for (int w = 1; w <= 2 && rc == BT_OK; ++w) {
             c->tune.mem_waves = w;
             rc = time_launches(c, 0, n_launches, &t[w - 1]);
+            restore();
             if (rc == BT_OK && t[w - 1] < bm) { bm = t[w - 1]; best.mem_waves = w; }
         }
         c->tune = prev;
@@ -2146,6 +2445,7 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
             if (q < 3 && cand[q] >= c->prm.n_ions) continue;     // the same as all ions
             c->tune.env_ions_per_block = cand[q];
             rc = time_launches(c, 1, n_launches, &t[2 + q]);
+            restore();
             if (rc == BT_OK && t[2 + q] < be) { be = t[2 + q]; best.env_ions_per_block = cand[q]; }
         }
         c->tune = prev;
@@ -2154,10 +2454,43 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
         for (int f = 0; f <= 1 && rc == BT_OK; ++f) {
             c->tune.exchange_form = f;
             rc = time_launches(c, 2, n_launches, &t[6 + f]);
+            restore();
             if (rc == BT_OK && t[6 + f] < bx) { bx = t[6 + f]; best.exchange_form = f; }
         }
+        // whole steps: replayed graphs or the persistent kernel, with the
+        // variants chosen above (the persistent kernel deals out its own work)
+        if (rc == BT_OK && e == hipSuccess && persistent_grid(c) > 0) {
+            const int n_steps = 2 * std::max(n_launches, 4);
+            double bs = 1e30;
+            for (int pm = 0; pm <= 1 && rc == BT_OK && e == hipSuccess; ++pm) {
+                c->tune = best;
+                c->tune.persistent = pm;
+                drop_graphs(c);
+                if (!pm) rc = capture_graphs(c);
+                hipEvent_t e0, e1;
+                if (rc == BT_OK) {
+                    HIP_TRY(c, hipEventCreate(&e0));
+                    HIP_TRY(c, hipEventCreate(&e1));
+                    rc = bt_step(c, 2, 1);                        // warm-up
+                    HIP_TRY(c, hipEventRecord(e0, c->stream));
+                    if (rc == BT_OK) rc = bt_step(c, n_steps, 1);
+                    HIP_TRY(c, hipEventRecord(e1, c->stream));
+                    HIP_TRY(c, hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    unsigned fl = 0;
+                    HIP_TRY(c, hipMemcpy(&fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost));
+                    if (fl & BT_FLAG_BARRIER) rc = set_err(c, BT_EHIP, "autotune: a grid barrier gave up waiting");
+                    t[8 + pm] = (double)ms / n_steps;
+                    c->steps_done -= n_steps + 2;
+                }
+                restore();
+                if (rc == BT_OK && t[8 + pm] < bs) { bs = t[8 + pm]; best.persistent = pm; }
+            }
+        }
         c->tune = prev;
-        e = keep(true);
     }
     (void)hipFree(save);
     if (rc != BT_OK) return rc;
@@ -2167,7 +2500,7 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
     drop_graphs(c);
     if (chosen) *chosen = best;
     if (trial_ms)
-        for (int q = 0; q < n_trial_ms && q < 8; ++q) trial_ms[q] = t[q];
+        for (int q = 0; q < n_trial_ms && q < BT_AUTOTUNE_TRIALS; ++q) trial_ms[q] = t[q];
     return BT_OK;
 }
 
@@ -2179,6 +2512,8 @@ int bt_sync(bt_ctx* c, int64_t* bad_step) {
     HIP_TRY(c, hipMemcpy(&flags, c->flags, sizeof(flags), hipMemcpyDeviceToHost));
     HIP_TRY(c, hipMemcpy(&bs, c->bad_step, sizeof(bs), hipMemcpyDeviceToHost));
     if (bad_step) *bad_step = (bs == ~0ull) ? -1 : (int64_t)bs;
+    if (flags & BT_FLAG_BARRIER)
+        return set_err(c, BT_EHIP, "a grid barrier of the persistent step kernel gave up waiting");
     if (flags) return set_err(c, BT_UNSTABLE, "NaN in cell concentrations or Vmem: simulation unstable");
     return BT_OK;
 }
@@ -2212,15 +2547,57 @@ int bt_set_network(bt_ctx* c, const bt_network* net, double* c_net) {
             for (int q = 0; ok && q < r.n_stoich; ++q) ok = r.stoich_species[q] >= 0 && r.stoich_species[q] < S;
             if (!ok) return set_err(c, BT_EINVAL, "reaction network rate " + std::to_string(j) + " is invalid");
         }
-        if (!c->net) HIP_TRY(c, hipMalloc((void**)&c->net, sizeof(bt_network)));
-        HIP_TRY(c, hipMemcpyAsync(c->net, net, sizeof(bt_network), hipMemcpyHostToDevice, c->stream));
+        // the kernel's form of the network: reciprocal constants, per-substance terms
+        KNet k;
+        memset(&k, 0, sizeof(k));
+        k.n_species = net->n_species;
+        k.n_rates = net->n_rates;
+        for (int j = 0; j < net->n_rates; ++j) {
+            const bt_rate& r = net->rates[j];
+            KNetRate& q = k.rates[j];
+            q.kind = r.kind; q.species = r.species; q.reversible = r.reversible;
+            q.n_act = r.n_act; q.n_inh = r.n_inh; q.n_reac = r.n_reac; q.n_prod = r.n_prod;
+            for (int h = 0; h < BT_NET_MAX_HILL; ++h) {
+                q.act[h] = r.act[h]; q.inh[h] = r.inh[h];
+                q.act_rkm[h] = (h < r.n_act) ? 1.0 / r.act_km[h] : 0.0;
+                q.inh_rkm[h] = (h < r.n_inh) ? 1.0 / r.inh_km[h] : 0.0;
+                q.act_n[h] = r.act_n[h]; q.inh_n[h] = r.inh_n[h];
+            }
+            for (int h = 0; h < BT_NET_MAX_TERMS; ++h) {
+                q.reac[h] = r.reac[h]; q.prod[h] = r.prod[h];
+                q.reac_coeff[h] = r.reac_coeff[h]; q.prod_coeff[h] = r.prod_coeff[h];
+                q.reac_rkm[h] = (h < r.n_reac && !r.reversible) ? 1.0 / r.reac_km[h] : 0.0;
+            }
+            q.r_prod = r.r_prod; q.r_decay = r.r_decay; q.decay_max = r.decay_max;
+            q.inv_gad = (r.kind == BT_RATE_GROWTH) ? 1.0 / r.gad_scale : 0.0;
+            q.vmax = r.vmax;
+            q.inv_keq = (r.kind == BT_RATE_REACTION && r.reversible) ? 1.0 / r.keq : 0.0;
+        }
+        int nt = 0;
+        for (int sp = 0; sp < net->n_species; ++sp) {
+            k.sub_ptr[sp] = nt;
+            for (int j = 0; j < net->n_rates; ++j)
+                for (int h = 0; h < net->rates[j].n_stoich; ++h)
+                    if (net->rates[j].stoich_species[h] == sp) {
+                        k.term_rate[nt] = j;
+                        k.term_coeff[nt] = net->rates[j].stoich_coeff[h];
+                        ++nt;
+                    }
+        }
+        k.sub_ptr[net->n_species] = nt;
+        if (!c->net) HIP_TRY(c, hipMalloc((void**)&c->net, sizeof(KNet)));
+        HIP_TRY(c, hipMemcpyAsync(c->net, &k, sizeof(KNet), hipMemcpyHostToDevice, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         c->c_net = c_net;
+        c->net_species = net->n_species;
     } else {
         c->c_net = nullptr;
     }
-    // captured steps do not match the new step body any more
+    // captured steps do not match the new step body any more, and the
+    // persistent kernel has no network phase
     drop_graphs(c);
+    c->pers_grid = -1;
+    if (c->tune.persistent && persistent_grid(c) < 1) c->tune.persistent = 0;
     return BT_OK;
 }
 

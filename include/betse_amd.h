@@ -221,6 +221,9 @@ typedef struct {
     int32_t env_ions_per_block; /* ions per k_env_field block, 1..n_ions (n_ions: one block per 16x16 tile) */
     int32_t exchange_form;      /* k_env_exchange: 0 one wavefront per ion over 64 squares, 1 one thread per square */
     int32_t graph_pairs;        /* pairs of steps per replayed step graph, 1..64 */
+    int32_t persistent;         /* 1: bt_step runs its pairs of steps in cooperative launches of one persistent
+                                   kernel (grid barriers between a step's phases; 5 or 6 ions, no reaction
+                                   network); 0: replayed step graphs */
 } bt_tuning;
 
 /* Capture and instantiate the step graphs now (bt_step otherwise does it on
@@ -231,14 +234,16 @@ int bt_get_tuning(const bt_ctx* ctx, bt_tuning* out);
 int bt_set_tuning(bt_ctx* ctx, const bt_tuning* tuning);
 
 /* Time each kernel variant over n_launches launches (HIP events on the
- * context stream) and keep the fastest per kernel; the chosen variants go to
- * *chosen (may be NULL) and the average ms of each trial to trial_ms (may be
- * NULL; n_trial_ms entries, unused ones -1) in the order: mem_waves 1, 2;
- * env_ions_per_block 1, 2, 3, n_ions; exchange_form 0, 1. The state the
- * timed launches modify in place (channel gates, surface charge, reaction
- * network substances) is saved before and restored after, so a phase
- * continues from where it was; the state must be stable (no NaN flagged).
- * Allocates a temporary copy of that state. Synchronous. */
+ * context stream) and keep the fastest per kernel, then time whole steps
+ * (2 * max(n_launches, 4)) replayed from graphs and in the persistent kernel
+ * where that exists; the chosen variants go to *chosen (may be NULL) and the
+ * average ms of each trial to trial_ms (may be NULL; n_trial_ms entries,
+ * unused ones -1) in the order: mem_waves 1, 2; env_ions_per_block 1, 2, 3,
+ * n_ions; exchange_form 0, 1 (ms per launch); persistent 0, 1 (ms per step).
+ * All the state a step modifies in place is saved before and restored after
+ * each trial, so a phase continues from where it was; the state must be
+ * stable (no NaN flagged). Allocates a temporary copy of that state.
+ * Synchronous. */
 int bt_autotune(bt_ctx* ctx, int32_t n_launches, bt_tuning* chosen, double* trial_ms, int32_t n_trial_ms);
 
 /* update_V's voltage part for the current state (sim.py:2007-2083 with

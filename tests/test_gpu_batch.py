@@ -9,6 +9,7 @@ import pytest
 from betse_amd.batch import batched_worlds
 from betse_amd.channels import draw_channel_state
 from oracle.betse_oracle import OracleSim
+from test_gpu_parity import jenv_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -121,6 +122,11 @@ def test_batch_sampled_observables_match_oracle():
                 err = np.abs(a - r).max()
                 if mine.startswith("efield_gj"):
                     assert err <= max(1e-8 * scale, 2e-9 / cells.gj_len), (k, mine, rec_i, err, scale)
+                elif mine.startswith("I_tot"):
+                    bound = jenv_bound(rec, p, scale)
+                    if rec_i == 0:
+                        say(f"  cluster {k} {mine}: err {err:.3e}, |J| {scale:.3e}, bar {bound:.3e}")
+                    assert err <= bound, (k, mine, rec_i, err, scale, bound)
                 elif mine == "vm_time":
                     assert err <= 1e-9, (k, mine, rec_i, err)
                 else:

@@ -30,6 +30,8 @@ REL_TOL = 1e-10
 VARIANTS = [{"mem_waves": 1}, {"mem_waves": 2}, {"env_ions_per_block": 1}, {"env_ions_per_block": 2},
             {"env_ions_per_block": 3}, {"env_ions_per_block": 5}, {"exchange_form": 0}, {"exchange_form": 1},
             {"graph_pairs": 1}, {"graph_pairs": 3}]
+PERSISTENT_VARIANTS = [{"persistent": 1}, {"persistent": 1, "env_ions_per_block": 2},
+                       {"persistent": 1, "env_ions_per_block": 1}]
 
 
 def say(msg):
@@ -67,12 +69,11 @@ def compare(st, o, tag, vm_tol=VM_TOL, rel_tol=REL_TOL):
     assert dv <= vm_tol and dc <= rel_tol and de <= rel_tol and dE <= 1e-8 * (vm_tol / VM_TOL)
 
 
-@pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
-def test_launch_variants_are_bit_identical(flux_mode):
+def check_variants(flux_mode, variants, tag):
     worlds = batched_worlds(n_clusters=5, cells_range=(30, 400), grid_range=(16, 40), seed=3)
     n = 33                                   # graph pairs and an odd eager step
     ref = batch(worlds, flux_mode)
-    say(f"case: launch variants, {flux_mode}, default {ref.get_tuning()}")
+    say(f"case: {tag}, {flux_mode}, default {ref.get_tuning()}")
     ref.step_n(n)
     s_ref = state(ref)
     ref.pull()
@@ -80,7 +81,7 @@ def test_launch_variants_are_bit_identical(flux_mode):
         for k, ((p, cells), st) in enumerate(zip(worlds, ref.states)):
             compare(st, oracle_steps(p, cells, n), f"  cluster {k}")
     del ref
-    for v in VARIANTS:
+    for v in variants:
         say(f"  variant {v}")
         b = batch(worlds, flux_mode, tuning=v)
         assert all(b.get_tuning()[k] == x for k, x in v.items())
@@ -89,6 +90,39 @@ def test_launch_variants_are_bit_identical(flux_mode):
         for k in s_ref:
             assert np.array_equal(s[k], s_ref[k]), (v, k)
         del b
+
+
+@pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
+def test_launch_variants_are_bit_identical(flux_mode):
+    check_variants(flux_mode, VARIANTS, "launch variants")
+
+
+@pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
+def test_persistent_kernel_is_bit_identical(flux_mode):
+    """The persistent step kernel (grid barriers between a step's phases) runs
+    the same arithmetic as the replayed three-kernel steps."""
+    check_variants(flux_mode, PERSISTENT_VARIANTS, "persistent kernel")
+
+
+@pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
+def test_persistent_kernel_with_channels_is_bit_identical(flux_mode):
+    """The persistent step kernel with voltage-gated channels (six ions: the
+    gating coefficients are a phase of their own) against the replayed graphs."""
+    worlds = batched_worlds(n_clusters=3, cells_range=(30, 200), grid_range=(16, 24), seed=5,
+                            base="tiny_channels")
+    from betse_amd.channels import draw_channel_state
+    for p, c in worlds:
+        draw_channel_state(p, c.n_cells, c.n_mems)
+    say(f"case: persistent kernel with channels, {flux_mode}")
+    runs = []
+    for tuning in (None, {"persistent": 1}):
+        b = batch(worlds, flux_mode, tuning=tuning)
+        b.step_n(31)
+        runs.append((state(b), b._dev["gates"].cpu().numpy()))
+        del b
+    for k in runs[0][0]:
+        assert np.array_equal(runs[0][0][k], runs[1][0][k]), k
+    assert np.array_equal(runs[0][1], runs[1][1])
 
 
 def test_autotune_leaves_the_trajectory_unchanged():
@@ -104,7 +138,8 @@ def test_autotune_leaves_the_trajectory_unchanged():
     a.step_n(20)
     rep = a.run_autotune(5)
     say(f"  autotune: {rep}")
-    assert set(rep["chosen"]) == {"mem_waves", "env_ions_per_block", "exchange_form", "graph_pairs"}
+    assert set(rep["chosen"]) == {"mem_waves", "env_ions_per_block", "exchange_form", "graph_pairs", "persistent"}
+    assert rep["trial_ms"]["persistent=1 (ms per step)"] is not None
     a.step_n(20)
     b = batch(worlds, "fp64", tuning=rep["chosen"])
     b.step_n(40)
@@ -148,7 +183,22 @@ def test_batched_workload_sample_matches_oracle(c3_worlds, flux_mode, vm_tol, re
     b2.step_n(n)
     s2 = state(b2)
     for k in s_def:
-        assert np.array_equal(s2[k], s_def[k]), k
+        assert np.array_equal(s2[k], s_def[k]), (other, k)
+
+
+@pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
+def test_batched_workload_persistent_kernel(c3_worlds, flux_mode):
+    """The batched workload in the persistent step kernel against the replayed
+    graphs, bit for bit, over 16 steps."""
+    runs = []
+    for tuning in (None, {"persistent": 1}):
+        say(f"case: batched workload {flux_mode}, tuning {tuning}")
+        b = batch(c3_worlds, flux_mode, tuning=tuning)
+        b.step_n(16)
+        runs.append(state(b))
+        del b
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
 
 
 @pytest.fixture(scope="module")

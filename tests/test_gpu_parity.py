@@ -167,14 +167,16 @@ def test_run_sim_core_sampling(tiny):
     check(sim, o, "run_sim_core")
 
 
-def test_nan_raises_unstable(tiny):
+@pytest.mark.parametrize("tuning", [None, {"persistent": 1}], ids=["graphs", "persistent"])
+def test_nan_raises_unstable(tiny, tuning):
     """A NaN written into one device cell concentration stops the run with
-    BetseSimUnstableException at that step."""
+    BetseSimUnstableException at that step (replayed graphs and the
+    persistent step kernel)."""
     p, cells = tiny
     from betse_amd.exceptions import BetseSimUnstableException
     from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
-    say("case: NaN -> unstable")
-    sim = Simulator(p.copy(), device="cuda:0")
+    say(f"case: NaN -> unstable, tuning {tuning}")
+    sim = Simulator(p.copy(), device="cuda:0", tuning=tuning)
     ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
     sim.init_core(ph)
     sim.prepare(ph)
@@ -376,44 +378,56 @@ def test_network_kernel_alone_matches_oracle():
 
 
 def test_network_nan_halts_like_reference():
-    """A substance that turns NaN (a reversible reaction at Q = 0/0) raises in
-    the reference's no_negs (sim_toolbox.py:475-503, networks.py:2939): the
-    GPU run halts with BetseSimUnstableException at the step where the oracle
-    raises, and the state stays frozen there."""
+    """A substance that turns NaN raises in the reference's no_negs
+    (sim_toolbox.py:475-503, networks.py:2939): here a reversible reaction
+    whose reactant and product are both exactly zero in one cell (Q = 0/0)
+    after three ordinary steps. The GPU run halts with
+    BetseSimUnstableException at the step where the oracle raises, and the
+    state stays frozen there."""
+    import torch
     import warnings
     from betse_amd.exceptions import BetseSimUnstableException
     from betse_amd.network import draw_network
     from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
     from oracle.betse_oracle import OracleUnstable
     p = Parameters.preset("tiny")
-    p.network = draw_network(seed=11, n_species=8, n_reactions=20)
+    p.network = draw_network(seed=13, n_species=8, n_reactions=20)
+    rev = [r for r in p.network.reactions if r.delta_Go is not None]
+    assert rev, "the drawn network needs a reversible reaction"
+    a_sp, b_sp = rev[0].reactants[0][0], rev[0].products[0][0]
     cells = Cells.make_world(p)
-    rng = np.random.RandomState(3)
-    spread = rng.uniform(0.2, 5.0, (8, cells.n_cells))
+    cell = 7
     o = OracleSim(cells, p.copy(), mode="sparse")
     o.init_core()
-    o.c_net = o.c_net * spread
     o.init_dynamics("init")
     o.p.set_phase_time("init")
     o.update_V(observables=False)
+    for _ in range(3):
+        o.step("init", observables=False)
+    o.c_net[a_sp, cell] = 0.0
+    o.c_net[b_sp, cell] = 0.0
     bad_at = None
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
-        for s in range(50):
+        for s in range(3, 20):
             try:
                 o.step("init", observables=False)
             except OracleUnstable:
                 bad_at = s
                 break
-    assert bad_at is not None, "the drawn network was expected to hit Q = 0/0"
-    say(f"case: network NaN, oracle raises at step {bad_at}")
+    assert bad_at == 3
+    say(f"case: network NaN (Q = 0/0), oracle raises at step {bad_at}")
     sim = Simulator(p.copy(), device="cuda:0")
     ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
     sim.init_core(ph)
-    sim.c_net = sim.c_net * spread
     sim.prepare(ph)
+    sim.step_n(3)
+    with torch.cuda.stream(sim.stream):
+        sim._dev["c_net"][a_sp, cell] = 0.0
+        sim._dev["c_net"][b_sp, cell] = 0.0
+    sim.stream.synchronize()
     with pytest.raises(BetseSimUnstableException) as ei:
-        sim.step_n(40)
+        sim.step_n(10)
     assert ei.value.step == bad_at
     frozen = sim.cc_cells.copy()
     with pytest.raises(BetseSimUnstableException):
@@ -453,6 +467,19 @@ def test_mfma_gemm_f64():
         assert rc == _native.BT_OK
         sim.stream.synchronize()
         np.testing.assert_allclose(Cm.cpu().numpy(), a @ b, rtol=1e-12, atol=1e-11)
+
+
+def jenv_bound(rec, p, scale):
+    """Error bar of the env current J_env = F sum_i z_i fluxes_env_i (and of its
+    Helmholtz-Hodge parts): the ion currents cancel to a small fraction of
+    their terms in the near-neutral environment, so an ulp or two of each
+    ion's flux (the env kernel multiplies by 1/delta where the reference
+    divides) is a large part of J_env. The bar is 1e-8 of |J_env| plus four
+    ulps of the term scale F sum_i |z_i| max|fluxes_env_i|."""
+    fx, fy = np.asarray(rec["fluxes_env_x"]), np.asarray(rec["fluxes_env_y"])
+    z = np.array([p.ion_charge[n] for n in p.ion_names()], dtype=float)
+    terms = p.F * float(np.sum(np.abs(z) * np.maximum(np.abs(fx).max(axis=1), np.abs(fy).max(axis=1))))
+    return 1e-8 * scale + 4 * np.finfo(float).eps * terms
 
 
 @pytest.mark.parametrize("preset,n_steps", [("tiny", 40), ("tiny_channels", 40), ("large", 20)])
@@ -497,6 +524,11 @@ def test_sampled_observables(preset, n_steps):
             err = np.abs(a - b).max()
             if mine.startswith("efield_gj"):
                 assert err <= max(1e-8 * scale, 2e-9 / cells.gj_len), (mine, rec_i, err, scale)
+            elif mine.startswith("I_tot"):
+                bound = jenv_bound(rec, p, scale)
+                if rec_i == 0:
+                    say(f"  {mine}: err {err:.3e}, |J| {scale:.3e}, bar {bound:.3e}")
+                assert err <= bound, (mine, rec_i, err, scale, bound)
             else:
                 assert err / scale < 1e-8, (mine, rec_i, err / scale)
     np.testing.assert_allclose(sim.B_field.ravel(), o.B_field.ravel(), rtol=0,
