@@ -36,6 +36,7 @@ BT_NET_MAX_TERMS = 4
 BT_NET_MAX_STOICH = 8
 BT_RATE_GROWTH = 0
 BT_RATE_REACTION = 1
+BT_GNN_SCRATCH_FLOATS = 1 << 18
 
 _d8 = ctypes.c_double * BT_MAX_IONS
 _d5 = ctypes.c_double * 5
@@ -171,6 +172,7 @@ SIGNATURES = {
                                          ctypes.c_int32, ctypes.c_int32, _vp]),
     "bt_gnn_colsum": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _vp, _vp,
                                      ctypes.c_int32, _vp]),
+    "bt_gnn_colsum_noalloc": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     "bt_gnn_mse": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_float, _vp, _vp, ctypes.c_int32, _vp]),
     "bt_gnn_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_float, ctypes.c_float, ctypes.c_int32, _vp]),

@@ -8,6 +8,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "betse_amd.h"
 
 namespace {
@@ -186,6 +188,46 @@ inline unsigned blocks(long long n) { return (unsigned)((n + EB - 1) / EB); }
 
 }  // namespace
 
+// Scratch of bt_gnn_colsum_noalloc: the per-block partial sums live in this
+// file-scope device array instead of a caller workspace. Each GPU the library
+// runs on holds its own copy until the process ends; host code reaches it only
+// through the device address hipGetSymbolAddress returns for the current
+// device (the array's name on the host is not a device address).
+static __device__ __align__(256) float scratch[BT_GNN_SCRATCH_FLOATS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch[MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device, looked up once per device
+int scratch_address(float** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return BT_EHIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return BT_EHIP;
+        g_scratch[dev] = static_cast<float*>(ptr);
+    }
+    *out = g_scratch[dev];
+    return BT_OK;
+}
+
+// rows per partial-sum block of the column sums: 256, doubled until the
+// partials (one row of f per block) fit in ws_len floats
+int colsum_rows_per_block(int n, int f, long long ws_len) {
+    int rpb = 256;
+    int parts = (n + rpb - 1) / rpb;
+    while ((long long)parts * f > ws_len) {
+        rpb *= 2;
+        parts = (n + rpb - 1) / rpb;
+    }
+    return rpb;
+}
+
+}  // namespace
+
 extern "C" {
 
 int bt_gnn_gemm(const float* A, int64_t sam, int64_t sak, const float* B, int64_t sbk, int64_t sbn,
@@ -250,12 +292,25 @@ int bt_gnn_mean_agg_T(const float* G, int64_t ldg, const int32_t* ptr, const int
 int bt_gnn_colsum(const float* G, int64_t ldg, int32_t n, int32_t f, float* out, float* ws, int32_t ws_len,
                   void* stream) {
     if (!G || !out || !ws || n < 1 || f < 1 || f > EB) return BT_EINVAL;
-    int rpb = 256;
-    int parts = (n + rpb - 1) / rpb;
-    while ((long long)parts * f > ws_len) {
-        rpb *= 2;
-        parts = (n + rpb - 1) / rpb;
-    }
+    const int rpb = colsum_rows_per_block(n, f, ws_len);
+    const int parts = (n + rpb - 1) / rpb;
+    hipLaunchKernelGGL(k_colsum_part, dim3(parts), dim3(EB), 0, (hipStream_t)stream, G, (long long)ldg, n, f,
+                       rpb, ws);
+    hipLaunchKernelGGL(k_colsum_final, dim3((f + EB - 1) / EB), dim3(EB), 0, (hipStream_t)stream,
+                       (const float*)ws, parts, f, out);
+    return hipGetLastError() == hipSuccess ? BT_OK : BT_EHIP;
+}
+
+int bt_gnn_colsum_noalloc(const float* G, int64_t ldg, int32_t n, int32_t f, float* out, void* stream) {
+    if (!G || !out || n < 1 || f < 1) return BT_EINVAL;
+    if (f > EB) return BT_EINVAL;          // at most 256 columns (one block of the final sum)
+    float* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != BT_OK) return rc;
+    // the same launches as bt_gnn_colsum with ws = scratch: every partial is
+    // written before it is read, so nothing needs clearing between calls
+    const int rpb = colsum_rows_per_block(n, f, BT_GNN_SCRATCH_FLOATS);
+    const int parts = (n + rpb - 1) / rpb;
     hipLaunchKernelGGL(k_colsum_part, dim3(parts), dim3(EB), 0, (hipStream_t)stream, G, (long long)ldg, n, f,
                        rpb, ws);
     hipLaunchKernelGGL(k_colsum_final, dim3((f + EB - 1) / EB), dim3(EB), 0, (hipStream_t)stream,

@@ -346,6 +346,19 @@ int bt_gnn_mean_agg_T(const float* G, int64_t ldg, const int32_t* ptr, const int
 /* out_c = sum_r G[r][c], with per-block partials in ws (ws_len floats). */
 int bt_gnn_colsum(const float* G, int64_t ldg, int32_t n, int32_t f, float* out, float* ws, int32_t ws_len,
                   void* stream);
+/* bt_gnn_colsum without the workspace argument, allocating nothing: no
+ * counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight training or serving loops, harnesses
+ * that forbid hipMalloc). The per-block partial sums go to a file-scope device
+ * array of BT_GNN_SCRATCH_FLOATS floats (1 MiB) that each GPU holds until the
+ * process ends; its device address is looked up once per device, on the first
+ * call with that device current (the current device must be the stream's).
+ * The partials are every time written before they are read. Same result as
+ * bt_gnn_colsum with ws_len = BT_GNN_SCRATCH_FLOATS; f > 256 returns
+ * BT_EINVAL. Calls that overlap on one GPU (other streams or host threads)
+ * share the array and race: serialise them. */
+#define BT_GNN_SCRATCH_FLOATS (1 << 18)
+int bt_gnn_colsum_noalloc(const float* G, int64_t ldg, int32_t n, int32_t f, float* out, void* stream);
 /* dY = 2 (Y-T) mask inv_n; part[b] = partial sums of (Y-T)^2 mask inv_n per 256 elements. */
 int bt_gnn_mse(const float* Y, const float* T, const float* mask, int64_t count, float inv_n, float* dY,
                float* part, int32_t part_len, void* stream);
