@@ -260,9 +260,14 @@ def cpu_reference_form(target_s: float) -> dict:
 # ---------------------------------------------------------------------------
 # GPU
 # ---------------------------------------------------------------------------
+TUNING = None          # launch variants forced by --tuning
+AUTOTUNE = True        # else the autotuner's choice (or, with --no-autotune, bt_create's by size)
+
+
 def gpu_batch(worlds, mode, autotune=True):
     from betse_amd.batch import BatchSimulator
-    b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=mode, autotune=autotune)
+    b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode=mode,
+                       autotune=autotune and AUTOTUNE and TUNING is None, tuning=TUNING)
     b.init("init")
     return b
 
@@ -273,6 +278,7 @@ def time_loop(worlds, mode, steps, warmup):
     the simulation stream give the device time."""
     import torch
     b = gpu_batch(worlds, mode)
+    b.prepare_sampling()              # set-up of the sampled steps, outside the timed region
     b.step_n(warmup)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -298,6 +304,7 @@ def time_sampled(worlds, mode, steps):
     MFMA, write2storage per cluster on the host)."""
     import torch
     b = gpu_batch(worlds, mode)
+    b.prepare_sampling()
     b.step_n(2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -448,8 +455,8 @@ def make_problem(name: str):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="batched",
                     choices=["batched", "large", "full", "small", "warmstart", "full_pipeline"])
     ap.add_argument("--flux", default="both", choices=["fp64", "fp32", "both"])
@@ -469,6 +476,9 @@ def main(argv=None):
     ap.add_argument("--sim-steps", type=int, default=20_000, help="full_pipeline config: sim-phase steps")
     ap.add_argument("--ws-flux", default="fp32", choices=["fp64", "fp32"],
                     help="warmstart config: flux arithmetic of the init runs")
+    ap.add_argument("--tuning", default="", help="force the launch variants (JSON object of bt_tuning fields) "
+                                                 "instead of the autotuner's choice")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the variants bt_create picks by size")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args(argv)
@@ -486,6 +496,10 @@ def main(argv=None):
         torch.cuda.synchronize()
         print(json.dumps({"copy_calib_bytes_read": n * 8, "copy_calib_bytes_written": n * 8, "launches": 5}))
         return
+    global TUNING, AUTOTUNE
+    if a.tuning:
+        TUNING = json.loads(a.tuning)
+    AUTOTUNE = not a.no_autotune
     if a.config == "warmstart":
         return bench_warmstart(a)
     if a.config == "full_pipeline":

@@ -260,6 +260,24 @@ class BatchSimulator(DeviceFetch):
             out[k] = a[..., lo:hi]
         return out
 
+    def _sample_keys(self):
+        d = self._dev
+        state_keys = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"] + (["c_net"] if "c_net" in d else [])
+        return state_keys, list(HostState._OBS_KEEP)
+
+    def prepare_sampling(self) -> None:
+        """Allocate what a sampled step uses -- the observation buffers, the
+        observables' workspace (DST bases and eigenvalue tables) and the pinned
+        snapshot buffers -- without stepping: the state is not touched (the
+        observables are derived once from the zeroed flux buffers)."""
+        obs, t_obs = self._obs_buffers()
+        self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
+        state_keys, keep = self._sample_keys()
+        d = self._dev
+        self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}})
+        self._fetch_finish_async(state_keys + keep)
+        self.sync()
+
     def run_loop(self, n_steps: int, t_resample: Optional[float] = None, callbacks=None) -> int:
         """``_run_sim_core_loop`` (``sim.py:1132-1390``) of every cluster at once:
         ``n_steps`` steps, and at the sampled steps of the phase's time vector
@@ -276,8 +294,7 @@ class BatchSimulator(DeviceFetch):
             st.clear_storage()
         d = self._dev
         obs, t_obs = self._obs_buffers() if sampled_idx else (None, None)
-        state_keys = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"] + (["c_net"] if "c_net" in d else [])
-        keep = list(HostState._OBS_KEEP)
+        state_keys, keep = self._sample_keys()
         use_graph = 1 if self.use_graph else 0
         pending = None
 

@@ -361,6 +361,9 @@ class HostState:
         return np.dot(self.zs * self.p.F, self.cc_cells)
 
 
+COPY_THREADS = max(1, min(8, len(os.sched_getaffinity(0))))   # host threads of a sampled step's copy-out
+
+
 class DeviceFetch:
     """Device -> host copies through cached pinned buffers, for objects with
     ``torch``, ``device`` and ``stream`` attributes (``Simulator``,
@@ -416,9 +419,21 @@ class DeviceFetch:
     def _fetch_finish_async(self, keys) -> dict:
         """Host arrays of the last ``_fetch_start_async``: views into one fresh
         NumPy copy of the pinned buffer (which is reused), so each sample's
-        arrays are its own."""
+        arrays are its own. The copy runs in chunks on a few host threads
+        (NumPy releases the GIL while it copies)."""
         self.__dict__.pop("_copied").synchronize()
-        flat = self._flat_host.numpy().copy()
+        src = self._flat_host.numpy()
+        flat = np.empty_like(src)
+        n = len(src)
+        if n < (1 << 22):
+            flat[:] = src
+        else:
+            pool = self.__dict__.get("_copy_pool")
+            if pool is None:
+                from concurrent.futures import ThreadPoolExecutor
+                pool = self._copy_pool = ThreadPoolExecutor(max_workers=COPY_THREADS)
+            step = (n + COPY_THREADS - 1) // COPY_THREADS
+            list(pool.map(lambda a: np.copyto(flat[a:a + step], src[a:a + step]), range(0, n, step)))
         out = {}
         for k in keys:
             o, n, shape = self._flat_at[k]
