@@ -290,13 +290,21 @@ class BatchSimulator(DeviceFetch):
         tr = p0.t_resample if t_resample is None else t_resample
         time_steps, sampled = phase_time_steps(int(n_steps), p0.dt, tr)
         sampled_idx = [i for i, t in enumerate(time_steps) if t in sampled]
-        for st in self.states:
-            st.clear_storage()
         d = self._dev
         obs, t_obs = self._obs_buffers() if sampled_idx else (None, None)
         state_keys, keep = self._sample_keys()
         use_graph = 1 if self.use_graph else 0
         pending = None
+        cleared = False
+
+        def clear():
+            # every cluster's series restart with the phase; cleared once the
+            # first steps are queued, so that this host work overlaps them
+            nonlocal cleared
+            if not cleared:
+                for st in self.states:
+                    st.clear_storage()
+                cleared = True
 
         def finish(t_sample):
             h = self._fetch_finish_async(state_keys + keep)
@@ -312,6 +320,7 @@ class BatchSimulator(DeviceFetch):
         for s in sampled_idx:
             self._check(self._lib.bt_step_observed(self._ctx, int(s + 1 - pos), use_graph, ctypes.byref(obs)))
             pos = s + 1
+            clear()
             if pending is not None:
                 finish(pending)
                 pending = None
@@ -321,6 +330,7 @@ class BatchSimulator(DeviceFetch):
             pending = time_steps[s]
         if pos < len(time_steps):
             self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, use_graph))
+        clear()
         if pending is not None:
             finish(pending)
         self.sync()

@@ -1359,52 +1359,82 @@ __device__ __forceinline__ double div_at(const double* Fx, const double* Fy, int
     return gx + gy;
 }
 
+// One cluster's share of the batched Helmholtz-Hodge solves: its interior
+// (n1 x n2 squares), padded to the GEMM tile (n1p x n2p), and the offset of
+// its padded matrices in the batch's workspace (in units of elements; each
+// workspace buffer holds, per cluster, the stream-function matrix and then the
+// potential matrix, pad = n1p * n2p elements each).
+struct ObsQ { int n1, n2, n1p, n2p; long long off; };
+
+// the cluster whose workspace range holds offset w: the last q with
+// off[q] <= w (binary search over the batch's ascending offsets)
+__device__ __forceinline__ int obs_cluster(const ObsQ* Q, int nq, long long w) {
+    int lo = 0, hi = nq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (Q[mid].off <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // rhs of the stream-function solve (-div(-Jy, Jx)) and of the potential solve
-// (div(Jx, Jy)) on the interior, written row-major into ld-wide padded matrices
+// (div(Jx, Jy)) of every cluster's interior, written row-major into its two
+// padded matrices with zeros in the padding: one thread per padded interior
+// element e of the batch, n_pad = sum of n1p * n2p over the clusters (the
+// matrices need no clearing beforehand)
 __global__ void __launch_bounds__(BLOCK)
-k_obs_rhs(const double* jxo, const double* jyo, double* b1, double* b2, int ld, KCluster K) {
-    const int n1 = K.ny - 2, n2 = K.nx - 2;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= n1 * n2) return;
-    const int r = e / n2 + 1, c = e % n2 + 1;
-    // Jxr = -Jy, Jyr = Jx
-    const double dr = div_at(jyo, jxo, r, c, K.ny, K.nx, K.delta, K.two_delta, -1.0, 1.0);
-    const double dd = div_at(jxo, jyo, r, c, K.ny, K.nx, K.delta, K.two_delta, 1.0, 1.0);
-    b1[(r - 1) * ld + (c - 1)] = -dr;
-    b2[(r - 1) * ld + (c - 1)] = dd;
+k_obs_rhs(const double* jxo, const double* jyo, double* Bw, const ObsQ* Q, int nq, long long n_pad,
+          const KCluster* kcl) {
+    const long long e = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= n_pad) return;
+    // a cluster's two matrices take 2 n1p n2p elements, so element e of the
+    // interiors lies in the cluster whose range holds offset 2 e
+    const int q = obs_cluster(Q, nq, 2 * e);
+    const ObsQ g = Q[q];
+    const int pad = g.n1p * g.n2p;
+    const int loc = (int)(e - g.off / 2);          // element of the cluster's padded interior
+    const int rr = loc / g.n2p, cc = loc % g.n2p;
+    double b1 = 0.0, b2 = 0.0;
+    if (rr < g.n1 && cc < g.n2) {
+        const KCluster& K = kcl[q];
+        const double* jx = jxo + K.sq_off;
+        const double* jy = jyo + K.sq_off;
+        // Jxr = -Jy, Jyr = Jx
+        const double dr = div_at(jy, jx, rr + 1, cc + 1, K.ny, K.nx, K.delta, K.two_delta, -1.0, 1.0);
+        const double dd = div_at(jx, jy, rr + 1, cc + 1, K.ny, K.nx, K.delta, K.two_delta, 1.0, 1.0);
+        b1 = -dr;
+        b2 = dd;
+    }
+    Bw[g.off + loc] = b1;
+    Bw[g.off + pad + loc] = b2;
 }
 
+// gradients of the stream function AA and potential BB -> HH_Decomp outputs,
+// over every square of the batch. AA and BB are the interior solutions
+// (padded, ld = n2p) with zero boundary rows and columns (their boundary
+// right-hand sides are zero without an applied voltage).
 __global__ void __launch_bounds__(BLOCK)
-k_scale(double* x, const double* s, int n) {
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e < n) x[e] = x[e] * s[e];
-}
-
-// interior solution -> full grid (boundary rows of A are identity/delta^2 and
-// their right-hand sides are zero here)
-__global__ void __launch_bounds__(BLOCK)
-k_scatter_interior(const double* X, int ld, double* out, KCluster K) {
+k_obs_hh_out(const double* Xw, const ObsQ* Q, const int32_t* sq_cluster, const KCluster* kcl, double* Jx,
+             double* Jy, double* Jtx, double* Jty, double* Bf, double mu, int n_sq) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= K.ny * K.nx) return;
-    const int r = k / K.nx, c = k % K.nx;
-    const bool bnd = (r == 0 || c == 0 || r == K.ny - 1 || c == K.nx - 1);
-    out[k] = bnd ? 0.0 : X[(r - 1) * ld + (c - 1)];
-}
-
-// gradients of the stream function AA and potential BB -> HH_Decomp outputs
-__global__ void __launch_bounds__(BLOCK)
-k_obs_hh_out(const double* AA, const double* BB, double* Jx, double* Jy, double* Jtx, double* Jty,
-             double* Bf, double mu, KCluster K) {
-    const int k = blockIdx.x * BLOCK + threadIdx.x;
-    if (k >= K.ny * K.nx) return;
-    const int ny = K.ny, nx = K.nx, r = k / nx, c = k % nx;
-    auto grad = [&](const double* F, double& gx, double& gy) {
-        if (c == 0) gx = (F[r * nx + 1] - F[r * nx]) / K.delta;
-        else if (c == nx - 1) gx = (F[r * nx + nx - 1] - F[r * nx + nx - 2]) / K.delta;
-        else gx = -(F[r * nx + c - 1] - F[r * nx + c + 1]) / K.two_delta;
-        if (r == 0) gy = (F[nx + c] - F[c]) / K.delta;
-        else if (r == ny - 1) gy = (F[(ny - 1) * nx + c] - F[(ny - 2) * nx + c]) / K.delta;
-        else gy = -(F[(r - 1) * nx + c] - F[(r + 1) * nx + c]) / K.two_delta;
+    if (k >= n_sq) return;
+    const int q = sq_cluster[k];
+    const KCluster& K = kcl[q];
+    const ObsQ g = Q[q];
+    const double* AA = Xw + g.off;
+    const double* BB = AA + g.n1p * g.n2p;
+    const int ny = K.ny, nx = K.nx, loc = k - K.sq_off, r = loc / nx, c = loc % nx;
+    auto F = [&](const double* X, int rr, int cc) {
+        return (rr == 0 || cc == 0 || rr == ny - 1 || cc == nx - 1) ? 0.0 : X[(rr - 1) * g.n2p + (cc - 1)];
+    };
+    auto grad = [&](const double* X, double& gx, double& gy) {
+        if (c == 0) gx = (F(X, r, 1) - F(X, r, 0)) / K.delta;
+        else if (c == nx - 1) gx = (F(X, r, nx - 1) - F(X, r, nx - 2)) / K.delta;
+        else gx = -(F(X, r, c - 1) - F(X, r, c + 1)) / K.two_delta;
+        if (r == 0) gy = (F(X, 1, c) - F(X, 0, c)) / K.delta;
+        else if (r == ny - 1) gy = (F(X, ny - 1, c) - F(X, ny - 2, c)) / K.delta;
+        else gy = -(F(X, r - 1, c) - F(X, r + 1, c)) / K.two_delta;
     };
     double gAx, gAy, Gx, Gy;
     grad(AA, gAx, gAy);
@@ -1414,24 +1444,22 @@ k_obs_hh_out(const double* AA, const double* BB, double* Jx, double* Jy, double*
     Jy[k] = Fy;
     Jtx[k] = Fx + Gx;
     Jty[k] = Fy + Gy;
-    Bf[k] = AA[k] * mu;
+    Bf[k] = F(AA, r, c) * mu;
 }
 
-// C = A*B (row-major, all sizes multiples of 16) with v_mfma_f64_16x16x4f64:
-// one block of four wavefronts per 16x16 tile of C, wavefront w taking the
-// quarter [w K/4, (w+1) K/4) of the inner dimension (a multiple of 4), its
-// operands loaded sixteen k-steps at a time before their MFMAs; the four
-// partial tiles are then added in wavefront order through LDS (a fixed order:
-// the result does not vary between runs). Lane l holds A[l%16][k0+l/16] and
-// B[k0+l/16][l%16]; result element r of lane l is C[l/16+4*r][l%16].
+// One 16x16 tile of C = A*B (row-major, all sizes multiples of 16), times S
+// element by element when S is given, with v_mfma_f64_16x16x4f64: a block of
+// four wavefronts, wavefront w taking the quarter [w K/4, (w+1) K/4) of the
+// inner dimension (a multiple of 4), its operands loaded sixteen k-steps at a
+// time before their MFMAs; the four partial tiles are then added in wavefront
+// order through LDS (a fixed order: the result does not vary between runs).
+// Lane l holds A[l%16][k0+l/16] and B[k0+l/16][l%16]; result element r of
+// lane l is C[l/16+4*r][l%16].
 typedef double v4d __attribute__((ext_vector_type(4)));
-__global__ void __launch_bounds__(256)
-k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
-    __shared__ double part[3][64][4];            // partial tiles of wavefronts 1..3
+__device__ __forceinline__ void gemm_tile(const double* A, const double* B, double* C, const double* S, int n,
+                                          int kd, int tile, double (*part)[64][4]) {
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int tiles_n = n / 16;
-    const int tile = blockIdx.x;                 // block-uniform
-    if (tile >= (m / 16) * tiles_n) return;
     const int tm = tile / tiles_n, tn = tile % tiles_n;
     const int row = tm * 16 + (lane % 16), col = tn * 16 + (lane % 16), kk = lane / 16;
     const int kq = kd / 4;
@@ -1465,9 +1493,29 @@ k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
             sum = sum + part[0][lane][r];
             sum = sum + part[1][lane][r];
             sum = sum + part[2][lane][r];
-            C[(size_t)(tm * 16 + kk + 4 * r) * n + col] = sum;
+            const size_t o = (size_t)(tm * 16 + kk + 4 * r) * n + col;
+            C[o] = S ? sum * S[o] : sum;
         }
     }
+}
+
+__global__ void __launch_bounds__(256)
+k_gemm_f64(const double* A, const double* B, double* C, int m, int n, int kd) {
+    __shared__ double part[3][64][4];            // partial tiles of wavefronts 1..3
+    const int tile = blockIdx.x;                 // block-uniform
+    if (tile >= (m / 16) * (n / 16)) return;
+    gemm_tile(A, B, C, nullptr, n, kd, tile, part);
+}
+
+// Many independent products in one launch (the clusters' DST stages): block b
+// computes tile b - tile0 of product block_prob[b].
+struct GemmProb { const double* A; const double* B; double* C; const double* S; int m, n, k, tile0; };
+
+__global__ void __launch_bounds__(256)
+k_gemm_grouped(const GemmProb* probs, const int32_t* block_prob) {
+    __shared__ double part[3][64][4];
+    const GemmProb P = probs[block_prob[blockIdx.x]];
+    gemm_tile(P.A, P.B, P.C, P.S, P.n, P.k, (int)blockIdx.x - P.tile0, part);
 }
 
 // per-cluster max over cells and ions of |c - c_prev| / |c| (init residual)
@@ -1661,11 +1709,15 @@ struct bt_ctx {
     // observables workspace (allocated on first use)
     // per cluster: interior size, padded size, its DST bases (shared by the
     // clusters of one interior size) and its eigenvalue table
-    struct ObsGrid { int n1, n2, n1p, n2p; double* S1; double* S2; double* invlam; };
-    std::vector<ObsGrid> obs_grid;
-    std::vector<double*> obs_alloc;      // every observables buffer, freed in bt_destroy
-    double* jxo = nullptr; double* jyo = nullptr; double* B1 = nullptr; double* B2 = nullptr;
-    double* T1 = nullptr; double* T2 = nullptr; double* AA = nullptr; double* BB = nullptr;
+    std::vector<void*> obs_alloc;        // every observables buffer, freed in bt_destroy
+    bool obs_ready = false;
+    double* jxo = nullptr; double* jyo = nullptr;
+    // the batched Poisson solves: per-cluster padded matrices (right-hand
+    // sides, two products' workspace), the cluster table, and the four DST
+    // stages as grouped products (n_prob products each, n_gemm_blocks tiles)
+    double* Bw = nullptr; double* T1w = nullptr; double* T2w = nullptr;
+    ObsQ* obs_q = nullptr; long long n_pad = 0;
+    GemmProb* gemm_probs = nullptr; int32_t* block_prob = nullptr; int n_prob = 0; int n_gemm_blocks = 0;
     // Vmem per cell before the last observed step: update_gj's E_gj uses it (sim.py:2166-2172)
     double* vm_prev = nullptr; bool vm_prev_ok = false;
     // reaction network (device copy of the description) and its state buffer
@@ -1960,6 +2012,7 @@ static int capture_graphs(bt_ctx* c) {
         hipError_t e = hipStreamEndCapture(c->stream, g);
         if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
         HIP_TRY(c, hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
+        HIP_TRY(c, hipGraphUpload(*x, c->stream));
         return BT_OK;
     };
     if (!c->gexec) {
@@ -2274,7 +2327,7 @@ int bt_destroy(bt_ctx* c) {
                     c->vm_prev, c->net, c->bar};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
-    for (double* q : c->obs_alloc) (void)hipFree(q);
+    for (void* q : c->obs_alloc) (void)hipFree(q);
     delete c;
     return BT_OK;
 }
@@ -2328,11 +2381,6 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     HIP_TRY(c, hipGetLastError());
     c->steps_done += n_steps;
     return BT_OK;
-}
-
-int bt_capture_graphs(bt_ctx* c) {
-    if (!c) return BT_EINVAL;
-    return capture_graphs(c);
 }
 
 int bt_get_tuning(const bt_ctx* c, bt_tuning* out) {
@@ -2406,6 +2454,29 @@ static hipError_t state_copy(bt_ctx* c, const std::vector<std::pair<void*, size_
     return hipStreamSynchronize(c->stream);
 }
 
+int bt_capture_graphs(bt_ctx* c) {
+    if (!c) return BT_EINVAL;
+    const bool fresh = !c->gexec || (c->tune.graph_pairs > 1 && !c->gexec_n);
+    int rc = capture_graphs(c);
+    if (rc != BT_OK || !fresh) return rc;
+    // replay each graph once from a saved copy of the state and put the state
+    // back: the first launch of an instantiated graph carries one-time costs
+    // (measured ~1.4 ms for eight pairs of steps on the 500k-cell batch) that
+    // would otherwise fall into the caller's first steps
+    const auto regions = state_regions(c);
+    size_t sbytes = 0;
+    for (const auto& r : regions) sbytes += (r.second + 255) / 256 * 256;
+    char* save = nullptr;
+    HIP_TRY(c, hipMalloc((void**)&save, sbytes));
+    hipError_t e = state_copy(c, regions, save, false);
+    if (e == hipSuccess && c->gexec_n) e = hipGraphLaunch(c->gexec_n, c->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(c->gexec, c->stream);
+    if (e == hipSuccess) e = state_copy(c, regions, save, true);
+    (void)hipFree(save);
+    if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("graph warm-up: ") + hipGetErrorString(e));
+    return BT_OK;
+}
+
 int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_ms, int32_t n_trial_ms) {
     if (!c || n_launches < 1 || (trial_ms && n_trial_ms < 0)) return BT_EINVAL;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -2471,7 +2542,8 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
                 if (rc == BT_OK) {
                     HIP_TRY(c, hipEventCreate(&e0));
                     HIP_TRY(c, hipEventCreate(&e1));
-                    rc = bt_step(c, 2, 1);                        // warm-up
+                    const int warm = 2 * std::max(1, (int)c->tune.graph_pairs);
+                    rc = bt_step(c, warm, 1);                     // warm-up: every graph launched once
                     HIP_TRY(c, hipEventRecord(e0, c->stream));
                     if (rc == BT_OK) rc = bt_step(c, n_steps, 1);
                     HIP_TRY(c, hipEventRecord(e1, c->stream));
@@ -2484,7 +2556,7 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
                     HIP_TRY(c, hipMemcpy(&fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost));
                     if (fl & BT_FLAG_BARRIER) rc = set_err(c, BT_EHIP, "autotune: a grid barrier gave up waiting");
                     t[8 + pm] = (double)ms / n_steps;
-                    c->steps_done -= n_steps + 2;
+                    c->steps_done -= n_steps + warm;
                 }
                 restore();
                 if (rc == BT_OK && t[8 + pm] < bs) { bs = t[8 + pm]; best.persistent = pm; }
@@ -2659,39 +2731,23 @@ int bt_fp32_math(const float* x, const float* y, float* out, int32_t n, int32_t 
     return hipGetLastError() == hipSuccess ? BT_OK : BT_EHIP;
 }
 
-// Dirichlet Poisson solve of cluster q's interior by DST-I:
-// X = S1 ((S1 B S2) .* invlam) S2, scattered into out (the cluster's grid)
-static int dst_solve(bt_ctx* c, int q, double* Bm, double* out) {
-    const bt_ctx::ObsGrid& g = c->obs_grid[q];
-    int rc;
-    if ((rc = launch_gemm(c, g.S1, Bm, c->T1, g.n1p, g.n2p, g.n1p))) return rc;
-    if ((rc = launch_gemm(c, c->T1, g.S2, c->T2, g.n1p, g.n2p, g.n2p))) return rc;
-    const int nn = g.n1p * g.n2p;
-    hipLaunchKernelGGL(k_scale, dim3(grid_blocks(nn)), dim3(BLOCK), 0, c->stream, c->T2, (const double*)g.invlam, nn);
-    if ((rc = launch_gemm(c, g.S1, c->T2, c->T1, g.n1p, g.n2p, g.n1p))) return rc;
-    if ((rc = launch_gemm(c, c->T1, g.S2, c->T2, g.n1p, g.n2p, g.n2p))) return rc;
-    const KCluster& K = c->kcl_host[q];
-    hipLaunchKernelGGL(k_scatter_interior, dim3(grid_blocks(K.ny * K.nx)), dim3(BLOCK), 0, c->stream,
-                       (const double*)c->T2, g.n2p, out, K);
-    HIP_TRY(c, hipGetLastError());
-    return BT_OK;
-}
-
-// the observables workspace: per-square buffers for the batch, per-cluster
-// eigenvalue tables, DST bases per interior size, and GEMM buffers of the
-// largest padded grid (the clusters are solved one after the other)
+// The observables workspace: per-square buffers for the batch, DST bases per
+// interior size, per-cluster eigenvalue tables and padded matrices, and the
+// product tables of the four DST stages. A cluster's Dirichlet Poisson solve
+// by DST-I is X = S1 ((S1 B S2) .* invlam) S2 (S the symmetric DST-I matrix
+// of the interior's size, invlam the inverse eigenvalues with the transform's
+// normalisation and delta^2); every cluster's two solves (stream function and
+// potential) are products of one grouped launch per stage.
 static int obs_workspace(bt_ctx* c) {
-    if (!c->obs_grid.empty()) return BT_OK;
-    auto dalloc = [&](double** ptr, size_t n) -> int {
-        HIP_TRY(c, hipMalloc((void**)ptr, (n < 32 ? 32 : n) * 8));
+    if (c->obs_ready) return BT_OK;
+    auto dalloc = [&](void** ptr, size_t bytes) -> int {
+        HIP_TRY(c, hipMalloc(ptr, bytes < 256 ? 256 : bytes));
         c->obs_alloc.push_back(*ptr);
         return BT_OK;
     };
     const size_t ns = (size_t)c->kp.n_sq;
     int rc;
-    if ((rc = dalloc(&c->jxo, ns)) || (rc = dalloc(&c->jyo, ns)) || (rc = dalloc(&c->AA, ns)) ||
-        (rc = dalloc(&c->BB, ns)))
-        return rc;
+    if ((rc = dalloc((void**)&c->jxo, ns * 8)) || (rc = dalloc((void**)&c->jyo, ns * 8))) return rc;
     // DST-I basis S[j][k] = sin(pi (j+1)(k+1) / (n+1)), zero in the padding
     std::vector<std::pair<int, double*>> bases;
     auto basis = [&](int n, double** out) -> int {
@@ -2701,22 +2757,35 @@ static int obs_workspace(bt_ctx* c) {
         std::vector<double> h((size_t)np * np, 0.0);
         for (int j = 0; j < n; ++j)
             for (int k = 0; k < n; ++k) h[(size_t)j * np + k] = sin(M_PI * (j + 1) * (k + 1) / (n + 1));
-        int r = dalloc(out, h.size());
+        int r = dalloc((void**)out, h.size() * 8);
         if (r != BT_OK) return r;
         HIP_TRY(c, hipMemcpy(*out, h.data(), h.size() * 8, hipMemcpyHostToDevice));
         bases.push_back({n, *out});
         return BT_OK;
     };
-    size_t maxpp = 0;
-    std::vector<bt_ctx::ObsGrid> grids;
-    for (size_t q = 0; q < c->kcl_host.size(); ++q) {
+    const size_t nq = c->kcl_host.size();
+    std::vector<ObsQ> qs(nq);
+    std::vector<double*> S1(nq), S2(nq);
+    long long off = 0;
+    for (size_t q = 0; q < nq; ++q) {
         const KCluster& K = c->kcl_host[q];
-        bt_ctx::ObsGrid g{};
+        ObsQ& g = qs[q];
         g.n1 = K.ny - 2; g.n2 = K.nx - 2;
         g.n1p = (g.n1 + 15) / 16 * 16; g.n2p = (g.n2 + 15) / 16 * 16;
-        if ((rc = basis(g.n1, &g.S1)) || (rc = basis(g.n2, &g.S2))) return rc;
-        // eigenvalues of the 1-D second difference 2 cos(pi (j+1)/(n+1)) - 2,
-        // with the transform's normalisation and delta^2
+        g.off = off;
+        off += 2LL * g.n1p * g.n2p;
+        if ((rc = basis(g.n1, &S1[q])) || (rc = basis(g.n2, &S2[q]))) return rc;
+    }
+    const size_t wbytes = (size_t)off * 8;
+    if ((rc = dalloc((void**)&c->Bw, wbytes)) || (rc = dalloc((void**)&c->T1w, wbytes)) ||
+        (rc = dalloc((void**)&c->T2w, wbytes)))
+        return rc;
+    // inverse eigenvalues of the 1-D second differences 2 cos(pi (j+1)/(n+1)) - 2,
+    // with the transform's normalisation and delta^2, per cluster (zero in the padding)
+    std::vector<double*> invlam(nq);
+    for (size_t q = 0; q < nq; ++q) {
+        const KCluster& K = c->kcl_host[q];
+        const ObsQ& g = qs[q];
         std::vector<double> hl((size_t)g.n1p * g.n2p, 0.0);
         const double dl2 = K.delta * K.delta;
         const double scale = 4.0 / ((double)(g.n1 + 1) * (double)(g.n2 + 1));
@@ -2726,15 +2795,37 @@ static int obs_workspace(bt_ctx* c) {
                                    (2.0 * cos(M_PI * (k + 1) / (g.n2 + 1)) - 2.0);
                 hl[(size_t)j * g.n2p + k] = dl2 * scale / lam;
             }
-        if ((rc = dalloc(&g.invlam, hl.size()))) return rc;
-        HIP_TRY(c, hipMemcpy(g.invlam, hl.data(), hl.size() * 8, hipMemcpyHostToDevice));
-        maxpp = std::max(maxpp, (size_t)g.n1p * g.n2p);
-        grids.push_back(g);
+        if ((rc = dalloc((void**)&invlam[q], hl.size() * 8))) return rc;
+        HIP_TRY(c, hipMemcpy(invlam[q], hl.data(), hl.size() * 8, hipMemcpyHostToDevice));
     }
-    if ((rc = dalloc(&c->B1, maxpp)) || (rc = dalloc(&c->B2, maxpp)) || (rc = dalloc(&c->T1, maxpp)) ||
-        (rc = dalloc(&c->T2, maxpp)))
+    // the four stages: T1 = S1 B; T2 = (T1 S2) .* invlam; T1 = S1 T2; T2 = T1 S2
+    const int np = (int)(2 * nq);
+    std::vector<GemmProb> probs((size_t)4 * np);
+    std::vector<int32_t> bmap;
+    for (size_t q = 0; q < nq; ++q)
+        for (int h = 0; h < 2; ++h) {
+            const ObsQ& g = qs[q];
+            const size_t o = (size_t)g.off + (size_t)h * g.n1p * g.n2p;
+            const int p = (int)(2 * q + h);
+            const int tile0 = (int)bmap.size();
+            const int tiles = (g.n1p / 16) * (g.n2p / 16);
+            for (int t = 0; t < tiles; ++t) bmap.push_back(p);
+            double* B = c->Bw + o; double* T1 = c->T1w + o; double* T2 = c->T2w + o;
+            probs[(size_t)0 * np + p] = GemmProb{S1[q], B, T1, nullptr, g.n1p, g.n2p, g.n1p, tile0};
+            probs[(size_t)1 * np + p] = GemmProb{T1, S2[q], T2, invlam[q], g.n1p, g.n2p, g.n2p, tile0};
+            probs[(size_t)2 * np + p] = GemmProb{S1[q], T2, T1, nullptr, g.n1p, g.n2p, g.n1p, tile0};
+            probs[(size_t)3 * np + p] = GemmProb{T1, S2[q], T2, nullptr, g.n1p, g.n2p, g.n2p, tile0};
+        }
+    if ((rc = dalloc((void**)&c->gemm_probs, probs.size() * sizeof(GemmProb))) ||
+        (rc = dalloc((void**)&c->block_prob, bmap.size() * 4)) || (rc = dalloc((void**)&c->obs_q, nq * sizeof(ObsQ))))
         return rc;
-    c->obs_grid = grids;
+    HIP_TRY(c, hipMemcpy(c->gemm_probs, probs.data(), probs.size() * sizeof(GemmProb), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->block_prob, bmap.data(), bmap.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->obs_q, qs.data(), nq * sizeof(ObsQ), hipMemcpyHostToDevice));
+    c->n_pad = off / 2;
+    c->n_prob = np;
+    c->n_gemm_blocks = (int)bmap.size();
+    c->obs_ready = true;
     return BT_OK;
 }
 
@@ -2758,22 +2849,19 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
     hipLaunchKernelGGL(k_obs_mem, dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
     hipLaunchKernelGGL(k_obs_env_rhs, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
                        (const double*)o->fluxes_env_x, (const double*)o->fluxes_env_y, c->jxo, c->jyo, c->kp);
-    // the Helmholtz-Hodge decomposition of each cluster's env current on its own grid
-    for (size_t q = 0; q < c->obs_grid.size(); ++q) {
-        const bt_ctx::ObsGrid& g = c->obs_grid[q];
-        const KCluster& K = c->kcl_host[q];
-        const int so = K.sq_off, nsq = K.ny * K.nx;
-        const size_t pb = (size_t)g.n1p * g.n2p * 8;
-        HIP_TRY(c, hipMemsetAsync(c->B1, 0, pb, c->stream));
-        HIP_TRY(c, hipMemsetAsync(c->B2, 0, pb, c->stream));
-        hipLaunchKernelGGL(k_obs_rhs, dim3(grid_blocks(g.n1 * g.n2)), dim3(BLOCK), 0, c->stream,
-                           (const double*)(c->jxo + so), (const double*)(c->jyo + so), c->B1, c->B2, g.n2p, K);
-        if ((rc = dst_solve(c, (int)q, c->B1, c->AA + so))) return rc;
-        if ((rc = dst_solve(c, (int)q, c->B2, c->BB + so))) return rc;
-        hipLaunchKernelGGL(k_obs_hh_out, dim3(grid_blocks(nsq)), dim3(BLOCK), 0, c->stream,
-                           (const double*)(c->AA + so), (const double*)(c->BB + so), o->J_env_x + so,
-                           o->J_env_y + so, o->Jtx + so, o->Jty + so, o->B_field + so, c->prm.mu, K);
-    }
+    // the Helmholtz-Hodge decomposition of each cluster's env current on its
+    // own grid: both right-hand sides, the four DST stages of both solves of
+    // every cluster (one grouped launch each), then the outputs
+    const int nq = (int)c->kcl_host.size();
+    hipLaunchKernelGGL(k_obs_rhs, dim3((unsigned)((c->n_pad + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, c->stream,
+                       (const double*)c->jxo, (const double*)c->jyo, c->Bw, (const ObsQ*)c->obs_q, nq, c->n_pad,
+                       (const KCluster*)c->kcl);
+    for (int st = 0; st < 4; ++st)
+        hipLaunchKernelGGL(k_gemm_grouped, dim3(c->n_gemm_blocks), dim3(256), 0, c->stream,
+                           (const GemmProb*)(c->gemm_probs + (size_t)st * c->n_prob), (const int32_t*)c->block_prob);
+    hipLaunchKernelGGL(k_obs_hh_out, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream, (const double*)c->T2w,
+                       (const ObsQ*)c->obs_q, (const int32_t*)B.sq_cluster, (const KCluster*)c->kcl, o->J_env_x,
+                       o->J_env_y, o->Jtx, o->Jty, o->B_field, c->prm.mu, c->kp.n_sq);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }

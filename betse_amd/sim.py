@@ -101,7 +101,11 @@ def gaussian_taps(sigma: float = 1.0, truncate: float = 4.0) -> np.ndarray:
 
 
 def phase_time_steps(n_steps: int, dt: float, t_resample: float):
-    """``time_steps`` and ``time_steps_sampled`` (``sim.py:2455-2462``, Q16)."""
+    """``time_steps`` and ``time_steps_sampled`` (``sim.py:2455-2462``, Q16).
+    ``t_resample`` is the sampling interval in steps; below one step the
+    reference's loop never advances, so it is refused here."""
+    if not t_resample >= 1:
+        raise ValueError(f"t_resample must be at least one step, got {t_resample!r}")
     time_steps = np.linspace(0, n_steps * dt, n_steps)
     sampled = set()
     i = 0

@@ -227,7 +227,10 @@ typedef struct {
 } bt_tuning;
 
 /* Capture and instantiate the step graphs now (bt_step otherwise does it on
- * its first call that replays graphs), so that no later bt_step pays for it. */
+ * its first call that replays graphs), so that no later bt_step pays for it.
+ * Newly captured graphs are also replayed once from a saved copy of the state,
+ * which is then put back (bit for bit), so that their first launch's one-time
+ * costs fall here too. Allocates the copy for the duration of the call. */
 int bt_capture_graphs(bt_ctx* ctx);
 
 int bt_get_tuning(const bt_ctx* ctx, bt_tuning* out);
