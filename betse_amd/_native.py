@@ -12,7 +12,8 @@ import os
 from typing import Optional
 
 LIB_NAME = "libbetse_amd.so"
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+# the in-tree build; BETSE_AMD_LIBRARY names another build of the same sources
+LIB_PATH = os.environ.get("BETSE_AMD_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 BT_OK = 0
 BT_UNSTABLE = 1
@@ -122,7 +123,8 @@ class BtObs(ctypes.Structure):
 
 class BtTuning(ctypes.Structure):
     _fields_ = [("mem_waves", ctypes.c_int32), ("env_ions_per_block", ctypes.c_int32),
-                ("exchange_form", ctypes.c_int32), ("graph_pairs", ctypes.c_int32), ("persistent", ctypes.c_int32)]
+                ("exchange_form", ctypes.c_int32), ("graph_pairs", ctypes.c_int32), ("persistent", ctypes.c_int32),
+                ("streams", ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -131,7 +133,10 @@ class BtTuning(ctypes.Structure):
 # bt_autotune's trial order
 AUTOTUNE_TRIALS = ("mem_waves=1", "mem_waves=2", "env_ions_per_block=1", "env_ions_per_block=2",
                    "env_ions_per_block=3", "env_ions_per_block=n_ions", "exchange_form=0", "exchange_form=1",
-                   "persistent=0 (ms per step)", "persistent=1 (ms per step)")
+                   "persistent=0 (ms per step)", "persistent=1 (ms per step)", "streams=2 (ms per step)",
+                   "streams=4 (ms per step)")
+
+BT_MAX_STREAMS = 4
 
 
 # symbol -> (restype, argtypes); every function include/betse_amd.h declares

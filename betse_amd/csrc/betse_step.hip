@@ -194,9 +194,7 @@ __device__ __forceinline__ double expm1_path(double x) { return expm1(x); }
 // in [0.6, 1.7], so the sum is within an ulp or two). FT is double on the FP64
 // path, float on the FP32-flux path.
 template <typename FT>
-__device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT zc, FT vBA, FT f_rt, FT inv_d) {
-    const FT alpha = (zc * vBA) * f_rt;
-    FT exp_alpha, deno;
+__device__ __forceinline__ void ghk_exp(FT alpha, FT& exp_alpha, FT& deno) {
     if (fabs(alpha) > (FT)0.5) {
         exp_alpha = exp_path(-alpha);
         deno = (FT)1 - exp_alpha;
@@ -205,7 +203,44 @@ __device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT zc, FT vBA, FT f_rt, F
         exp_alpha = (FT)1 + em1;
         deno = -em1;
     }
+}
+
+template <typename FT>
+__device__ __forceinline__ FT ghk(FT cA, FT cB, FT Dc, FT zc, FT vBA, FT f_rt, FT inv_d) {
+    const FT alpha = (zc * vBA) * f_rt;
+    FT exp_alpha, deno;
+    ghk_exp(alpha, exp_alpha, deno);
     return -((Dc * alpha) * inv_d) * dv(cB - cA * exp_alpha, deno);
+}
+
+// ghk for a membrane's ions in turn. The exponential of the last evaluated
+// exponent is kept and reused when the next ion's exponent is bit-identical
+// (ions of one valence: the +1e-25 nonces that tell their voltages apart are
+// lost in the rounding of any |Vmem| above ~1e-9 V), so the values are those
+// ghk computes. A zero permeability gives +0 without evaluating the law (ghk
+// gives a zero of either sign; the flux sums it enters start at +0, so the
+// concentrations are the same bit for bit).
+template <typename FT>
+struct GhkLast {
+    FT alpha, exp_alpha, deno;
+};
+
+template <typename FT>
+__device__ __forceinline__ FT ghk_reuse(FT cA, FT cB, FT Dc, FT zc, FT vBA, FT f_rt, FT inv_d, GhkLast<FT>& L) {
+    if (Dc == (FT)0) return (FT)0;
+    const FT alpha = (zc * vBA) * f_rt;
+    if (!(alpha == L.alpha)) {
+        ghk_exp(alpha, L.exp_alpha, L.deno);
+        L.alpha = alpha;
+    }
+    return -((Dc * alpha) * inv_d) * dv(cB - cA * L.exp_alpha, L.deno);
+}
+
+// a law constant on the path's precision: the FP64 value, or its FP32 rounding
+template <typename FT>
+__device__ __forceinline__ FT on_path(double c64, float c32) {
+    if constexpr (sizeof(FT) == 4) return c32;
+    else return c64;
 }
 
 // membrane / gap-junction / channel GHK with the path's reciprocal constants
@@ -378,22 +413,6 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
     const double gjo_u = KC->gjo_u;
     if (W == 0 && j == 0) *L.bad = 0;
 
-    // per-cell inputs of the second stage, fetched up front
-    const int c = c0 + j;
-    const bool own_cell = (c < c1);
-    int ca = 0, cb = 0;
-    double vol = 1.0, divi = 0.0;
-    double cin[NS];
-    if (own_cell) {
-        ca = A.mem_ptr[c] - m0;
-        cb = A.mem_ptr[c + 1] - m0;
-        vol = A.cell_vol[c];
-        divi = A.diviterm[c];
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-            if (W * NS + s < NI) cin[s] = A.c_cell_in[(W * NS + s) * C + c];
-    }
-
     if (m < m1) {
         const int mc = A.mem_cell[m];
         const int cn = A.nn_cell[m];
@@ -456,6 +475,9 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         // slot by slot in the same sequence of additions
         double vmut = with_nonces(v, W * NS + 1);
         double vnmut = with_nonces(vn, W * NS + 1);
+        // the last membrane and gap-junction exponentials (none yet: NaN
+        // compares unequal to every exponent)
+        GhkLast<FT> last_m{(FT)NAN, (FT)0, (FT)0}, last_g{(FT)NAN, (FT)0, (FT)0};
         // software pipeline: the gathers of slot s + 1 are issued before slot
         // s's arithmetic, so one slot's latency hides behind the other's work
         if (W * NS < NI) gather_ion(0);
@@ -473,14 +495,15 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             if (i == P.i_k) f = f + (double)((FT)P.k_per_na * fNa);
             // GHK across the membrane and across the gap junction (sim.py:1253-1269, 2162-2206)
             const double clag = g_lag[s];
-            const FT fED = ghk_path<FT>((FT)g_env[s], (FT)clag, (FT)g_dm[s], (FT)P.zplus[i], (FT)vmut, P.f_rt,
-                                        P.inv_tm, P.f_rt32, P.inv_tm32);
+            const FT fED = ghk_reuse<FT>((FT)g_env[s], (FT)clag, (FT)g_dm[s], (FT)P.zplus[i], (FT)vmut,
+                                         on_path<FT>(P.f_rt, P.f_rt32), on_path<FT>(P.inv_tm, P.inv_tm32), last_m);
             f = f + (double)fED;
             double fgj = 0.0;                                // fgj[bflags_mems] = 0 (sim.py:2203)
             if (!bnd) {
                 const double vBA = (vnmut - vmut) + FLOAT_NONCE;
-                fgj = (double)ghk_path<FT>((FT)clag, (FT)g_lagn[s], (FT)(P.dgj_surf[i] * gjo), (FT)P.zplus[i],
-                                           (FT)vBA, P.f_rtgj, P.inv_gjlen, P.f_rtgj32, P.inv_gjlen32);
+                fgj = (double)ghk_reuse<FT>((FT)clag, (FT)g_lagn[s], (FT)(P.dgj_surf[i] * gjo), (FT)P.zplus[i],
+                                            (FT)vBA, on_path<FT>(P.f_rtgj, P.f_rtgj32),
+                                            on_path<FT>(P.inv_gjlen, P.inv_gjlen32), last_g);
             }
             // Ca-ATPase after the ion loop: refreshed cell Ca, post-diffusion env Ca,
             // Vmem carrying all n_ions nonces (sim.py:2126-2159)
@@ -530,6 +553,24 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             if (OBS) A.obs_fch[(size_t)ch * M + m] = (double)fc;
         }
     }
+    // per-cell inputs of the second stage, loaded once the membrane stage is
+    // done: held from the start, their registers cost a wave per SIMD of
+    // occupancy (measured slower on the 500k-cell batch)
+    const int c = c0 + j;
+    const bool own_cell = (c < c1);
+    int ca = 0, cb = 0;
+    double vol = 1.0, divi = 0.0;
+    double cin[NS];
+    if (own_cell) {
+        ca = A.mem_ptr[c] - m0;
+        cb = A.mem_ptr[c + 1] - m0;
+        vol = A.cell_vol[c];
+        divi = A.diviterm[c];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            if (W * NS + s < NI) cin[s] = A.c_cell_in[(W * NS + s) * C + c];
+    }
+
     tile_sync<WSYNC>();
 
     // per-cell segment sums in membrane order (np.dot(M_sum_mems, .)), then
@@ -606,7 +647,10 @@ k_mem_cell(MemCellArgs A, KParams P) {
     // barriers.
     __shared__ double s_mem[NI][TMB];
     __shared__ double s_gj[NI][TMB];
-    __shared__ double s_ch[BT_MAX_CHAN][TMB];
+    // channel fluxes: n_chan rows, in dynamic LDS sized at launch (none
+    // without channels, which leaves room for more blocks per CU)
+    extern __shared__ double s_dyn[];
+    double (*s_ch)[TMB] = reinterpret_cast<double (*)[TMB]>(s_dyn);
     __shared__ double s_c2[WG > 1 ? NI : 1][TMB];
     __shared__ int s_bad;
     const MemLds L{s_mem, s_gj, s_ch, s_c2, &s_bad};
@@ -959,7 +1003,10 @@ k_env_field(FieldArgs A, KParams P) {
 struct ExArgs {
     const int32_t* sq_ptr; const uint8_t* has_mem; const int32_t* sq_cluster; const KCluster* kcl;
     const double* c_mid; const double* fsa; const double* fch; double* c_env; double* u;
-    const unsigned* flags; long long* step_ctr; int count_step;
+    // step counters: block 0 of a counted step advances step_ctr[g] for each
+    // bit g of ctr_mask (one counter per stream group; all of them when the
+    // launch covers the whole problem)
+    const unsigned* flags; long long* step_ctr; int count_step; unsigned ctr_mask;
     // the squares the exchange updates (membranes or a surface charge), ascending;
     // the others keep u = 0 and take their env state from k_env_field
     const int32_t* sq_live; int n_live;
@@ -983,6 +1030,12 @@ __device__ __forceinline__ double neg_seg_sum(const double* x, int a, int b) {
 
 constexpr int EXQ = 64;
 
+__device__ __forceinline__ void advance_step(const ExArgs& A) {
+#pragma unroll
+    for (int g = 0; g < BT_MAX_STREAMS; ++g)
+        if (A.ctr_mask & (1u << g)) A.step_ctr[g] = A.step_ctr[g] + 1;
+}
+
 // One wavefront per ion over EXQ squares; the charge of each square is then
 // summed over the ions in ion order from LDS.
 
@@ -992,7 +1045,7 @@ k_env_exchange(ExArgs A, KParams P) {
     __shared__ double s_cn[BT_MAX_IONS][EXQ];
     const int i = threadIdx.x / EXQ, j = threadIdx.x % EXQ;
     const int q = blockIdx.x * EXQ + j;
-    if (A.count_step && blockIdx.x == 0 && threadIdx.x == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
+    if (A.count_step && blockIdx.x == 0 && threadIdx.x == 0) advance_step(A);
     const bool live = (q < A.n_live);
     const int k = live ? A.sq_live[q] : 0;
     double surf_scale = 0.0, mean_sa = 1.0, kee = 1.0;
@@ -1056,7 +1109,7 @@ __global__ void __launch_bounds__(BLOCK)
 k_env_exchange_sq(ExArgs A, KParams P) {
     if (halted(A.flags)) return;
     const int q = blockIdx.x * BLOCK + threadIdx.x;
-    if (A.count_step && q == 0) A.step_ctr[0] = A.step_ctr[0] + 1;
+    if (A.count_step && q == 0) advance_step(A);
     if (q >= A.n_live) return;
     exchange_square(A, P, q);
 }
@@ -1660,7 +1713,7 @@ k_step_persistent(PersistArgs PA, KParams P) {
         {
             const ExArgs& XA = PA.ex[par];
             if (!halted(XA.flags)) {
-                if (gtid == 0) XA.step_ctr[0] = XA.step_ctr[0] + 1;
+                if (gtid == 0) advance_step(XA);
                 for (int q = gtid; q < XA.n_live; q += nthr) exchange_square(XA, P, q);
             }
         }
@@ -1698,8 +1751,18 @@ struct bt_ctx {
     uint8_t* live_flag = nullptr;        // per square: 1 if in sq_live
     int n_live = 0;
     unsigned* flags = nullptr; unsigned long long* bad_step = nullptr; long long* step_ctr = nullptr;
-    hipGraph_t graph = nullptr; hipGraphExec_t gexec = nullptr;       // one pair of steps
-    hipGraph_t graph_n = nullptr; hipGraphExec_t gexec_n = nullptr;   // tune.graph_pairs pairs
+    // step graphs per stream group g: one pair of steps, and tune.graph_pairs pairs
+    hipGraph_t graph[BT_MAX_STREAMS] = {}; hipGraphExec_t gexec[BT_MAX_STREAMS] = {};
+    hipGraph_t graph_n[BT_MAX_STREAMS] = {}; hipGraphExec_t gexec_n[BT_MAX_STREAMS] = {};
+    // stream groups (tune.streams > 1): per cluster q, the first of its cell
+    // tiles, env tiles and live squares and of its cells (n_clusters + 1
+    // entries each; empty where the clusters' work lists are not in cluster
+    // order); the groups' streams ([0] is the caller's) and fork/join events
+    std::vector<int> cl_tile0, cl_env0, cl_live0;
+    std::vector<long long> cl_cell0;
+    hipStream_t gstream[BT_MAX_STREAMS] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[BT_MAX_STREAMS] = {};
     bt_tuning tune{};                  // launch variants of the step kernels
     bool autotuned = false;
     unsigned* bar = nullptr;           // grid barrier of the persistent step kernel
@@ -1707,8 +1770,6 @@ struct bt_ctx {
     long long steps_done = 0;
     const bt_obs* capture = nullptr;   // set while enqueueing an observed step
     // observables workspace (allocated on first use)
-    // per cluster: interior size, padded size, its DST bases (shared by the
-    // clusters of one interior size) and its eigenvalue table
     std::vector<void*> obs_alloc;        // every observables buffer, freed in bt_destroy
     bool obs_ready = false;
     double* jxo = nullptr; double* jyo = nullptr;
@@ -1816,37 +1877,63 @@ constexpr int BT_MEM_WG_TILES = 4096;
 constexpr int BT_GRAPH_PAIRS = 8;
 constexpr int BT_MAX_GRAPH_PAIRS = 64;
 constexpr int BT_PERSIST_STEPS = 512;   // steps per cooperative launch of the persistent kernel (even)
-constexpr int BT_AUTOTUNE_TRIALS = 10;
+constexpr int BT_AUTOTUNE_TRIALS = 12;
+
+// A stream group's share of the step's work lists: its cell tiles [t0, t0 +
+// nt), env tiles [e0, e0 + ne) and live squares [l0, l0 + nl).
+struct Span { int t0, nt, e0, ne, l0, nl; };
+
+// The span of group g of G (G = 1: the whole problem). Group g takes the
+// clusters [q_g, q_{g+1}), q_g the first cluster whose cells start at or after
+// g/G of all cells, each group keeping at least one cluster.
+static Span group_span(const bt_ctx* c, int G, int g) {
+    if (G <= 1) return Span{0, c->n_tiles, 0, c->n_env_tiles, 0, c->n_live};
+    const int nq = (int)c->clusters.size();      // at least G (streams_allowed)
+    // b[gg]: the first cluster of group gg, strictly increasing, with room
+    // for one cluster in each later group
+    int b[BT_MAX_STREAMS + 1];
+    b[0] = 0;
+    for (int gg = 1; gg < G; ++gg) {
+        const long long target = c->cl_cell0[nq] * gg / G;
+        const int q = (int)(std::lower_bound(c->cl_cell0.begin(), c->cl_cell0.end(), target) - c->cl_cell0.begin());
+        b[gg] = std::max(std::min(q, nq - (G - gg)), b[gg - 1] + 1);
+    }
+    b[G] = nq;
+    const int qa = b[g], qz = b[g + 1];
+    return Span{c->cl_tile0[qa], c->cl_tile0[qz] - c->cl_tile0[qa], c->cl_env0[qa], c->cl_env0[qz] - c->cl_env0[qa],
+                c->cl_live0[qa], c->cl_live0[qz] - c->cl_live0[qa]};
+}
 
 template <int NI, int WG, bool OBS>
-static void launch_mem_cell_o(bt_ctx* c, const MemCellArgs& a) {
+static void launch_mem_cell_o(bt_ctx* c, const MemCellArgs& a, hipStream_t st, int nt) {
+    const size_t lds = (size_t)c->prm.n_chan * TMB * sizeof(double);     // s_ch
     if (c->prm.flux_mode == BT_FLUX_FP32)
-        hipLaunchKernelGGL((k_mem_cell<NI, float, WG, OBS>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a, c->kp);
+        hipLaunchKernelGGL((k_mem_cell<NI, float, WG, OBS>), dim3(nt), dim3(TMB * WG), lds, st, a, c->kp);
     else
-        hipLaunchKernelGGL((k_mem_cell<NI, double, WG, OBS>), dim3(c->n_tiles), dim3(TMB * WG), 0, c->stream, a,
-                           c->kp);
+        hipLaunchKernelGGL((k_mem_cell<NI, double, WG, OBS>), dim3(nt), dim3(TMB * WG), lds, st, a, c->kp);
 }
 template <int NI, int WG>
-static void launch_mem_cell_w(bt_ctx* c, const MemCellArgs& a) {
-    if (a.obs_fm) launch_mem_cell_o<NI, WG, true>(c, a);
-    else launch_mem_cell_o<NI, WG, false>(c, a);
+static void launch_mem_cell_w(bt_ctx* c, const MemCellArgs& a, hipStream_t st, int nt) {
+    if (a.obs_fm) launch_mem_cell_o<NI, WG, true>(c, a, st, nt);
+    else launch_mem_cell_o<NI, WG, false>(c, a, st, nt);
 }
 template <int NI>
-static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a) {
-    if (c->tune.mem_waves == 2) launch_mem_cell_w<NI, 2>(c, a);
-    else launch_mem_cell_w<NI, 1>(c, a);
+static void launch_mem_cell_t(bt_ctx* c, const MemCellArgs& a, hipStream_t st, int nt) {
+    if (c->tune.mem_waves == 2) launch_mem_cell_w<NI, 2>(c, a, st, nt);
+    else launch_mem_cell_w<NI, 1>(c, a, st, nt);
 }
 
-static void launch_mem_cell(bt_ctx* c, const MemCellArgs& a) {
+// k_mem_cell over nt cell tiles (a.tile_ptr and a.tile_cluster start at the first)
+static void launch_mem_cell(bt_ctx* c, const MemCellArgs& a, hipStream_t st, int nt) {
     switch (c->prm.n_ions) {
-        case 1: launch_mem_cell_t<1>(c, a); break;
-        case 2: launch_mem_cell_t<2>(c, a); break;
-        case 3: launch_mem_cell_t<3>(c, a); break;
-        case 4: launch_mem_cell_t<4>(c, a); break;
-        case 5: launch_mem_cell_t<5>(c, a); break;
-        case 6: launch_mem_cell_t<6>(c, a); break;
-        case 7: launch_mem_cell_t<7>(c, a); break;
-        default: launch_mem_cell_t<8>(c, a); break;
+        case 1: launch_mem_cell_t<1>(c, a, st, nt); break;
+        case 2: launch_mem_cell_t<2>(c, a, st, nt); break;
+        case 3: launch_mem_cell_t<3>(c, a, st, nt); break;
+        case 4: launch_mem_cell_t<4>(c, a, st, nt); break;
+        case 5: launch_mem_cell_t<5>(c, a, st, nt); break;
+        case 6: launch_mem_cell_t<6>(c, a, st, nt); break;
+        case 7: launch_mem_cell_t<7>(c, a, st, nt); break;
+        default: launch_mem_cell_t<8>(c, a, st, nt); break;
     }
 }
 
@@ -1862,20 +1949,24 @@ static FieldArgs field_args(bt_ctx* c, bool diffuse, const bt_obs* ob, const dou
                      vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr, c->live_flag, env_out};
 }
 
-static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, const double* vm = nullptr,
-                         const double* env_in = nullptr, double* env_out = nullptr) {
-    const FieldArgs f = field_args(c, diffuse, ob, vm, env_in, env_out);
+// k_env_field over ne env tiles (f.env_tiles starts at the first; ne = 0: all)
+static void launch_field_args(bt_ctx* c, bool diffuse, const FieldArgs& f, hipStream_t st, int ne) {
     // diffusion: one block per (tile, env_ions_per_block ions): few ions per
     // block while tiles alone would leave CUs idle, else one block per tile
     // looping over the ions; field only: one block per tile
     const int ipb = c->tune.env_ions_per_block;
-    dim3 grid(c->n_env_tiles, diffuse ? (c->prm.n_ions + ipb - 1) / ipb : 1);
+    dim3 grid(ne > 0 ? ne : c->n_env_tiles, diffuse ? (c->prm.n_ions + ipb - 1) / ipb : 1);
     if (diffuse && f.rates)
-        hipLaunchKernelGGL((k_env_field<true, true>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<true, true>), grid, dim3(TY * TX), 0, st, f, c->kp);
     else if (diffuse)
-        hipLaunchKernelGGL((k_env_field<true, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<true, false>), grid, dim3(TY * TX), 0, st, f, c->kp);
     else
-        hipLaunchKernelGGL((k_env_field<false, false>), grid, dim3(TY * TX), 0, c->stream, f, c->kp);
+        hipLaunchKernelGGL((k_env_field<false, false>), grid, dim3(TY * TX), 0, st, f, c->kp);
+}
+
+static void launch_field(bt_ctx* c, bool diffuse, const bt_obs* ob = nullptr, const double* vm = nullptr,
+                         const double* env_in = nullptr, double* env_out = nullptr) {
+    launch_field_args(c, diffuse, field_args(c, diffuse, ob, vm, env_in, env_out), c->stream, 0);
 }
 
 // The kernel arguments of one step. parity 0: cell state caller -> workspace;
@@ -1886,7 +1977,8 @@ struct StepArgs {
     ExArgs ex;
 };
 
-static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_step) {
+// G, g: stream group g of G (G = 1: the whole problem)
+static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_step, int G = 1, int g = 0) {
     const bt_buffers& B = c->buf;
     double* ci = parity ? c->c_cell_b : B.c_cell;
     double* li = parity ? c->c_lag_b : B.c_lag;
@@ -1905,24 +1997,39 @@ static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_ste
                            ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
                            ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr},
                ExArgs{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u, c->flags,
-                      c->step_ctr, count_step, c->sq_live, c->n_live}};
+                      c->step_ctr, count_step, (1u << BT_MAX_STREAMS) - 1u, c->sq_live, c->n_live}};
+    if (G > 1) {
+        // the group's slices of the work lists; its own step counter (group 0
+        // also advances the counters of groups this split does not use, so
+        // that every counter holds the step count when a later split reads it)
+        const Span sp = group_span(c, G, g);
+        a.field.env_tiles += 3 * sp.e0;
+        a.mem.tile_ptr += sp.t0;
+        a.mem.tile_cluster += sp.t0;
+        a.mem.step_ctr += g;
+        a.ex.sq_live += sp.l0;
+        a.ex.n_live = sp.nl;
+        a.ex.ctr_mask = (g == 0) ? (((1u << BT_MAX_STREAMS) - 1u) & ~((1u << G) - 2u)) : (1u << g);
+    }
     return a;
 }
 
 // One step on the stream.
 // which: -1 all, 0 membrane/cell, 1 env field+diffusion, 2 exchange, 3 field only
-static void enqueue_step(bt_ctx* c, int parity, int which = -1) {
+static void enqueue_step(bt_ctx* c, int parity, int which = -1, int G = 1, int g = 0) {
     const bt_obs* ob = c->capture;
-    const StepArgs a = step_args(c, parity, ob, which < 0 ? 1 : 0);
-    if (which < 0 || which == 1) launch_field(c, true, ob, a.field.vm, a.field.c_env, a.field.c_env_out);
-    if (which < 0 || which == 0) launch_mem_cell(c, a.mem);
+    const StepArgs a = step_args(c, parity, ob, which < 0 ? 1 : 0, G, g);
+    const Span sp = group_span(c, G, g);
+    hipStream_t st = (G > 1) ? c->gstream[g] : c->stream;
+    if (which < 0 || which == 1) launch_field_args(c, true, a.field, st, sp.ne);
+    if (which < 0 || which == 0) launch_mem_cell(c, a.mem, st, sp.nt);
     if (which < 0 || which == 2) {
-        const int nl = c->n_live > 0 ? c->n_live : 1;    // block 0 also counts the step
+        const int nl = sp.nl > 0 ? sp.nl : 1;    // block 0 also counts the step
         if (c->tune.exchange_form == 0)
-            hipLaunchKernelGGL(k_env_exchange, dim3((nl + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0,
-                               c->stream, a.ex, c->kp);
+            hipLaunchKernelGGL(k_env_exchange, dim3((nl + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0, st, a.ex,
+                               c->kp);
         else
-            hipLaunchKernelGGL(k_env_exchange_sq, dim3(grid_blocks(nl)), dim3(BLOCK), 0, c->stream, a.ex, c->kp);
+            hipLaunchKernelGGL(k_env_exchange_sq, dim3(grid_blocks(nl)), dim3(BLOCK), 0, st, a.ex, c->kp);
     }
     if (which == 3) launch_field(c, false);
     if ((which < 0 || which == 4) && c->net && c->c_net)
@@ -1994,35 +2101,90 @@ static int launch_persistent(bt_ctx* c, int n_steps) {
 }
 
 static void drop_graphs(bt_ctx* c) {
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
-    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-    if (c->gexec_n) { (void)hipGraphExecDestroy(c->gexec_n); c->gexec_n = nullptr; }
-    if (c->graph_n) { (void)hipGraphDestroy(c->graph_n); c->graph_n = nullptr; }
+    for (int g = 0; g < BT_MAX_STREAMS; ++g) {
+        if (c->gexec[g]) { (void)hipGraphExecDestroy(c->gexec[g]); c->gexec[g] = nullptr; }
+        if (c->graph[g]) { (void)hipGraphDestroy(c->graph[g]); c->graph[g] = nullptr; }
+        if (c->gexec_n[g]) { (void)hipGraphExecDestroy(c->gexec_n[g]); c->gexec_n[g] = nullptr; }
+        if (c->graph_n[g]) { (void)hipGraphDestroy(c->graph_n[g]); c->graph_n[g] = nullptr; }
+    }
 }
 
-// capture and instantiate the step graphs (one pair, and tune.graph_pairs
-// pairs when that is more than one) if they do not exist yet
+// whether the problem can be stepped in G stream groups
+static bool streams_allowed(const bt_ctx* c, int G) {
+    return G >= 1 && G <= BT_MAX_STREAMS && (G == 1 || ((int)c->clusters.size() >= G && !c->cl_tile0.empty() &&
+                                                        c->prm.n_chan == 0 && !(c->net && c->c_net)));
+}
+
+// the stream groups of the current tuning (1: the whole problem on the caller's stream)
+static int n_stream_groups(const bt_ctx* c) { return c->tune.persistent ? 1 : std::max(1, (int)c->tune.streams); }
+
+// the groups' streams and the fork/join events, created on first use
+static int ensure_group_streams(bt_ctx* c, int G) {
+    c->gstream[0] = c->stream;
+    if (G > 1 && !c->ev_fork) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    for (int g = 1; g < G; ++g) {
+        if (!c->gstream[g]) HIP_TRY(c, hipStreamCreateWithFlags(&c->gstream[g], hipStreamNonBlocking));
+        if (!c->ev_join[g]) HIP_TRY(c, hipEventCreateWithFlags(&c->ev_join[g], hipEventDisableTiming));
+    }
+    return BT_OK;
+}
+
+// Capture and instantiate the step graphs (one pair, and tune.graph_pairs
+// pairs when that is more than one) of every stream group, each on its
+// group's stream, if they do not exist yet.
 static int capture_graphs(bt_ctx* c) {
-    auto capture = [&](int npairs, hipGraph_t* g, hipGraphExec_t* x) -> int {
-        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int G = n_stream_groups(c);
+    int rc = ensure_group_streams(c, G);
+    if (rc != BT_OK) return rc;
+    auto capture = [&](int g, int npairs, hipGraph_t* gr, hipGraphExec_t* x) -> int {
+        hipStream_t st = (G > 1) ? c->gstream[g] : c->stream;
+        HIP_TRY(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         for (int q = 0; q < npairs; ++q) {
-            enqueue_step(c, 0);
-            enqueue_step(c, 1);
+            enqueue_step(c, 0, -1, G, g);
+            enqueue_step(c, 1, -1, G, g);
         }
-        hipError_t e = hipStreamEndCapture(c->stream, g);
+        hipError_t e = hipStreamEndCapture(st, gr);
         if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-        HIP_TRY(c, hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
-        HIP_TRY(c, hipGraphUpload(*x, c->stream));
+        HIP_TRY(c, hipGraphInstantiate(x, *gr, nullptr, nullptr, 0));
+        HIP_TRY(c, hipGraphUpload(*x, st));
         return BT_OK;
     };
-    if (!c->gexec) {
-        int rc = capture(1, &c->graph, &c->gexec);
-        if (rc != BT_OK) return rc;
+    for (int g = 0; g < G; ++g) {
+        if (!c->gexec[g]) {
+            rc = capture(g, 1, &c->graph[g], &c->gexec[g]);
+            if (rc != BT_OK) return rc;
+        }
+        if (c->tune.graph_pairs > 1 && !c->gexec_n[g]) {
+            rc = capture(g, c->tune.graph_pairs, &c->graph_n[g], &c->gexec_n[g]);
+            if (rc != BT_OK) return rc;
+        }
     }
-    if (c->tune.graph_pairs > 1 && !c->gexec_n) {
-        int rc = capture(c->tune.graph_pairs, &c->graph_n, &c->gexec_n);
-        if (rc != BT_OK) return rc;
+    return BT_OK;
+}
+
+// Replay the step graphs for `pairs` pairs of steps: tune.graph_pairs pairs
+// per launch of the long graph, single pairs for the rest. With stream groups
+// the other groups' streams first wait for the caller's stream, and the
+// caller's stream then waits for all of them.
+static int launch_graphs(bt_ctx* c, int pairs) {
+    const int G = n_stream_groups(c);
+    if (G > 1) {
+        HIP_TRY(c, hipEventRecord(c->ev_fork, c->stream));
+        for (int g = 1; g < G; ++g) HIP_TRY(c, hipStreamWaitEvent(c->gstream[g], c->ev_fork, 0));
     }
+    const int gp = c->tune.graph_pairs;
+    for (int g = 0; g < G; ++g) {
+        hipStream_t st = (G > 1) ? c->gstream[g] : c->stream;
+        int done = 0;
+        if (c->gexec_n[g])
+            for (; done + gp <= pairs; done += gp) HIP_TRY(c, hipGraphLaunch(c->gexec_n[g], st));
+        for (; done < pairs; ++done) HIP_TRY(c, hipGraphLaunch(c->gexec[g], st));
+    }
+    if (G > 1)
+        for (int g = 1; g < G; ++g) {
+            HIP_TRY(c, hipEventRecord(c->ev_join[g], c->gstream[g]));
+            HIP_TRY(c, hipStreamWaitEvent(c->stream, c->ev_join[g], 0));
+        }
     return BT_OK;
 }
 
@@ -2223,6 +2385,31 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     }
     c->n_env_tiles = (int)(tiles.size() / 3);
     {
+        // per cluster, the start of its cell tiles, env tiles, live squares and
+        // cells, for stream groups; left empty unless each list is in cluster order
+        const int nq = n_clusters, nt = b->n_tiles, ne = c->n_env_tiles, nl = (int)sq_live.size();
+        bool ordered = true;
+        for (int t = 1; t < nt && ordered; ++t) ordered = tcl[t] >= tcl[t - 1];
+        for (int l = 1; l < nl && ordered; ++l) ordered = sqc[sq_live[l]] >= sqc[sq_live[l - 1]];
+        if (ordered) {
+            auto starts = [&](auto key, int n) {
+                std::vector<int> out(nq + 1);
+                int i = 0;
+                for (int q = 0; q <= nq; ++q) {
+                    while (i < n && key(i) < q) ++i;
+                    out[q] = i;
+                }
+                return out;
+            };
+            c->cl_tile0 = starts([&](int t) { return tcl[t]; }, nt);
+            c->cl_env0 = starts([&](int t) { return tiles[3 * t]; }, ne);
+            c->cl_live0 = starts([&](int l) { return sqc[sq_live[l]]; }, nl);
+            c->cl_cell0.assign(nq + 1, 0);
+            for (int q = 0; q < nq; ++q)
+                c->cl_cell0[q + 1] = c->cl_cell0[q] + (tp[c->cl_tile0[q + 1]] - tp[c->cl_tile0[q]]);
+        }
+    }
+    {
         // Dm and gjopen that are one value over a cluster's membranes (read once:
         // they do not change within a phase on this path)
         std::vector<double> dm((size_t)p->n_ions * p->n_mems), gj((size_t)p->n_mems);
@@ -2295,6 +2482,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     c->tune.exchange_form = (c->n_live < EX_SPLIT_SQUARES) ? 0 : 1;
     c->tune.graph_pairs = BT_GRAPH_PAIRS;
     c->tune.persistent = 0;
+    c->tune.streams = 1;
     if (e == hipSuccess && !sq_live.empty())
         e = hipMemcpy(c->sq_live, sq_live.data(), sizeof(int32_t) * sq_live.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->live_flag, live_flag.data(), ns, hipMemcpyHostToDevice);
@@ -2322,6 +2510,11 @@ int bt_destroy(bt_ctx* c) {
     if (!c) return BT_OK;
     (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
+    for (int g = 1; g < BT_MAX_STREAMS; ++g) {
+        if (c->gstream[g]) (void)hipStreamDestroy(c->gstream[g]);
+        if (c->ev_join[g]) (void)hipEventDestroy(c->ev_join[g]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
                     c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles,
                     c->vm_prev, c->net, c->bar};
@@ -2360,11 +2553,8 @@ int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
         // first call that replays graphs, so later calls only launch them
         int rc = capture_graphs(c);
         if (rc != BT_OK) return rc;
-        const int gp = c->tune.graph_pairs;
-        int done = 0;
-        if (c->gexec_n)
-            for (; done + gp <= pairs; done += gp) HIP_TRY(c, hipGraphLaunch(c->gexec_n, c->stream));
-        for (; done < pairs; ++done) HIP_TRY(c, hipGraphLaunch(c->gexec, c->stream));
+        rc = launch_graphs(c, pairs);
+        if (rc != BT_OK) return rc;
     } else {
         for (int s = 0; s < pairs; ++s) {
             enqueue_step(c, 0);
@@ -2401,7 +2591,13 @@ int bt_set_tuning(bt_ctx* c, const bt_tuning* t) {
     if (t->persistent && persistent_grid(c) < 1)
         return set_err(c, BT_EINVAL, "no persistent step kernel for this problem (it needs 5 or 6 ions, no "
                                      "reaction network, and cooperative launches)");
+    if (t->streams < 1 || t->streams > BT_MAX_STREAMS) return set_err(c, BT_EINVAL, "streams must be in 1..4");
+    if (t->streams > 1 && !streams_allowed(c, t->streams))
+        return set_err(c, BT_EINVAL, "stream groups need at least that many clusters, work lists in cluster order, "
+                                     "no channels and no reaction network");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int g = 1; g < BT_MAX_STREAMS; ++g)
+        if (c->gstream[g]) HIP_TRY(c, hipStreamSynchronize(c->gstream[g]));
     c->tune = *t;
     drop_graphs(c);
     return BT_OK;
@@ -2456,7 +2652,9 @@ static hipError_t state_copy(bt_ctx* c, const std::vector<std::pair<void*, size_
 
 int bt_capture_graphs(bt_ctx* c) {
     if (!c) return BT_EINVAL;
-    const bool fresh = !c->gexec || (c->tune.graph_pairs > 1 && !c->gexec_n);
+    bool fresh = false;
+    for (int g = 0; g < n_stream_groups(c); ++g)
+        fresh = fresh || !c->gexec[g] || (c->tune.graph_pairs > 1 && !c->gexec_n[g]);
     int rc = capture_graphs(c);
     if (rc != BT_OK || !fresh) return rc;
     // replay each graph once from a saved copy of the state and put the state
@@ -2469,10 +2667,12 @@ int bt_capture_graphs(bt_ctx* c) {
     char* save = nullptr;
     HIP_TRY(c, hipMalloc((void**)&save, sbytes));
     hipError_t e = state_copy(c, regions, save, false);
-    if (e == hipSuccess && c->gexec_n) e = hipGraphLaunch(c->gexec_n, c->stream);
-    if (e == hipSuccess) e = hipGraphLaunch(c->gexec, c->stream);
-    if (e == hipSuccess) e = state_copy(c, regions, save, true);
+    if (e == hipSuccess) {
+        rc = launch_graphs(c, c->tune.graph_pairs > 1 ? c->tune.graph_pairs + 1 : 1);
+        if (rc == BT_OK) e = state_copy(c, regions, save, true);
+    }
     (void)hipFree(save);
+    if (rc != BT_OK) return rc;
     if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("graph warm-up: ") + hipGetErrorString(e));
     return BT_OK;
 }
@@ -2528,16 +2728,23 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
             restore();
             if (rc == BT_OK && t[6 + f] < bx) { bx = t[6 + f]; best.exchange_form = f; }
         }
-        // whole steps: replayed graphs or the persistent kernel, with the
-        // variants chosen above (the persistent kernel deals out its own work)
-        if (rc == BT_OK && e == hipSuccess && persistent_grid(c) > 0) {
+        // whole steps: replayed graphs on one stream, the persistent kernel,
+        // replayed graphs on 2 and on 4 stream groups, with the variants
+        // chosen above (the persistent kernel deals out its own work)
+        if (rc == BT_OK && e == hipSuccess) {
             const int n_steps = 2 * std::max(n_launches, 4);
             double bs = 1e30;
-            for (int pm = 0; pm <= 1 && rc == BT_OK && e == hipSuccess; ++pm) {
+            struct Whole { int slot, persistent, streams; };
+            const Whole cands[4] = {{8, 0, 1}, {9, 1, 1}, {10, 0, 2}, {11, 0, 4}};
+            for (const Whole& w : cands) {
+                if (rc != BT_OK || e != hipSuccess) break;
+                if (w.persistent && persistent_grid(c) < 1) continue;
+                if (!streams_allowed(c, w.streams)) continue;
                 c->tune = best;
-                c->tune.persistent = pm;
+                c->tune.persistent = w.persistent;
+                c->tune.streams = w.streams;
                 drop_graphs(c);
-                if (!pm) rc = capture_graphs(c);
+                if (!w.persistent) rc = capture_graphs(c);
                 hipEvent_t e0, e1;
                 if (rc == BT_OK) {
                     HIP_TRY(c, hipEventCreate(&e0));
@@ -2555,11 +2762,15 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
                     unsigned fl = 0;
                     HIP_TRY(c, hipMemcpy(&fl, c->flags, sizeof(fl), hipMemcpyDeviceToHost));
                     if (fl & BT_FLAG_BARRIER) rc = set_err(c, BT_EHIP, "autotune: a grid barrier gave up waiting");
-                    t[8 + pm] = (double)ms / n_steps;
+                    t[w.slot] = (double)ms / n_steps;
                     c->steps_done -= n_steps + warm;
                 }
                 restore();
-                if (rc == BT_OK && t[8 + pm] < bs) { bs = t[8 + pm]; best.persistent = pm; }
+                if (rc == BT_OK && t[w.slot] < bs) {
+                    bs = t[w.slot];
+                    best.persistent = w.persistent;
+                    best.streams = w.streams;
+                }
             }
         }
         c->tune = prev;
@@ -2665,11 +2876,15 @@ int bt_set_network(bt_ctx* c, const bt_network* net, double* c_net) {
     } else {
         c->c_net = nullptr;
     }
-    // captured steps do not match the new step body any more, and the
-    // persistent kernel has no network phase
+    // captured steps do not match the new step body any more, and neither
+    // the persistent kernel nor stream groups have a network phase
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int g = 1; g < BT_MAX_STREAMS; ++g)
+        if (c->gstream[g]) HIP_TRY(c, hipStreamSynchronize(c->gstream[g]));
     drop_graphs(c);
     c->pers_grid = -1;
     if (c->tune.persistent && persistent_grid(c) < 1) c->tune.persistent = 0;
+    if (!streams_allowed(c, c->tune.streams)) c->tune.streams = 1;
     return BT_OK;
 }
 

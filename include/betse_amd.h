@@ -224,7 +224,14 @@ typedef struct {
     int32_t persistent;         /* 1: bt_step runs its pairs of steps in cooperative launches of one persistent
                                    kernel (grid barriers between a step's phases; 5 or 6 ions, no reaction
                                    network); 0: replayed step graphs */
+    int32_t streams;            /* 1..BT_MAX_STREAMS: with replayed graphs, the clusters of a batch are split
+                                   into this many groups of about equal cells, each stepped by its own graphs
+                                   on its own stream, so that one group's kernels run in the ramps and tails of
+                                   the others' (clusters do not interact within a step); more than 1 needs at
+                                   least that many clusters, no channels and no reaction network */
 } bt_tuning;
+
+#define BT_MAX_STREAMS 4
 
 /* Capture and instantiate the step graphs now (bt_step otherwise does it on
  * its first call that replays graphs), so that no later bt_step pays for it.
@@ -238,11 +245,13 @@ int bt_set_tuning(bt_ctx* ctx, const bt_tuning* tuning);
 
 /* Time each kernel variant over n_launches launches (HIP events on the
  * context stream) and keep the fastest per kernel, then time whole steps
- * (2 * max(n_launches, 4)) replayed from graphs and in the persistent kernel
- * where that exists; the chosen variants go to *chosen (may be NULL) and the
- * average ms of each trial to trial_ms (may be NULL; n_trial_ms entries,
- * unused ones -1) in the order: mem_waves 1, 2; env_ions_per_block 1, 2, 3,
- * n_ions; exchange_form 0, 1 (ms per launch); persistent 0, 1 (ms per step).
+ * (2 * max(n_launches, 4)) replayed from graphs on one stream, in the
+ * persistent kernel where that exists, and replayed from graphs on 2 and on 4
+ * streams where the problem allows it; the chosen variants go to *chosen (may
+ * be NULL) and the average ms of each trial to trial_ms (may be NULL;
+ * n_trial_ms entries, unused ones -1) in the order: mem_waves 1, 2;
+ * env_ions_per_block 1, 2, 3, n_ions; exchange_form 0, 1 (ms per launch);
+ * persistent 0, 1; streams 2, 4 (ms per step).
  * All the state a step modifies in place is saved before and restored after
  * each trial, so a phase continues from where it was; the state must be
  * stable (no NaN flagged). Allocates a temporary copy of that state.
@@ -371,7 +380,9 @@ int bt_gnn_adam(float* p, const float* g, float* m, float* v, int64_t count, flo
 
 /* Wait for the stream; return BT_OK or BT_UNSTABLE (a NaN was produced; the
  * state is frozen at the step that produced it and *bad_step receives its
- * 0-based index counted from context creation, or -1). */
+ * 0-based index counted from context creation, or -1). With streams > 1
+ * (bt_tuning) the clusters of the other stream groups stop where their own
+ * stream had got to within the same bt_step call. */
 int bt_sync(bt_ctx* ctx, int64_t* bad_step);
 
 /* Average device time in ms of one launch of a step kernel (0 membrane/cell,

@@ -2,8 +2,9 @@
 variant of the step kernels.
 
 * The launch variants (``bt_tuning``: one or two wavefronts per cell tile,
-  1..n_ions ions per env block, the per-ion or per-square exchange) divide the
-  same arithmetic differently between lanes, so they must give bit-identical
+  1..n_ions ions per env block, the per-ion or per-square exchange, graph
+  length, the persistent kernel, stream groups) divide the same arithmetic
+  differently between lanes and launches, so they must give bit-identical
   states; the default variant matches the oracle at the FP64 bars.
 * The batched workload (256 clusters of 100-5000 cells, ~500k cells) for 16
   steps, a sample of its clusters (the largest and smallest among them)
@@ -32,6 +33,8 @@ VARIANTS = [{"mem_waves": 1}, {"mem_waves": 2}, {"env_ions_per_block": 1}, {"env
             {"graph_pairs": 1}, {"graph_pairs": 3}]
 PERSISTENT_VARIANTS = [{"persistent": 1}, {"persistent": 1, "env_ions_per_block": 2},
                        {"persistent": 1, "env_ions_per_block": 1}]
+STREAM_VARIANTS = [{"streams": 2}, {"streams": 4}, {"streams": 3, "graph_pairs": 1},
+                   {"streams": 2, "graph_pairs": 3, "exchange_form": 0}]
 
 
 def say(msg):
@@ -105,6 +108,33 @@ def test_persistent_kernel_is_bit_identical(flux_mode):
 
 
 @pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
+def test_stream_groups_are_bit_identical(flux_mode):
+    """The clusters split into groups, each stepped by its own graphs on its
+    own stream: clusters do not interact within a step, so the states are the
+    one-stream states bit for bit (odd step counts included)."""
+    check_variants(flux_mode, STREAM_VARIANTS, "stream groups")
+
+
+@pytest.mark.parametrize("cluster", [0, 4])
+def test_stream_groups_nan_halts_at_its_step(cluster):
+    """A NaN planted in a cluster of the first or the last stream group stops
+    the run with BetseSimUnstableException at that step."""
+    import torch
+    from betse_amd.exceptions import BetseSimUnstableException
+    worlds = batched_worlds(n_clusters=5, cells_range=(30, 400), grid_range=(16, 40), seed=3)
+    say(f"case: NaN in cluster {cluster}, two stream groups")
+    b = batch(worlds, "fp64", tuning={"streams": 2})
+    b.step_n(6)
+    c0 = sum(c.n_cells for _, c in worlds[:cluster])
+    with torch.cuda.stream(b.stream):
+        b._dev["c_cell"][1, c0 + 3] = float("nan")
+    b.stream.synchronize()
+    with pytest.raises(BetseSimUnstableException) as ei:
+        b.step_n(20)
+    assert ei.value.step == 6
+
+
+@pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
 def test_persistent_kernel_with_channels_is_bit_identical(flux_mode):
     """The persistent step kernel with voltage-gated channels (six ions: the
     gating coefficients are a phase of their own) against the replayed graphs."""
@@ -138,7 +168,9 @@ def test_autotune_leaves_the_trajectory_unchanged():
     a.step_n(20)
     rep = a.run_autotune(5)
     say(f"  autotune: {rep}")
-    assert set(rep["chosen"]) == {"mem_waves", "env_ions_per_block", "exchange_form", "graph_pairs", "persistent"}
+    assert set(rep["chosen"]) == {"mem_waves", "env_ions_per_block", "exchange_form", "graph_pairs", "persistent",
+                                  "streams"}
+    assert rep["chosen"]["streams"] == 1           # channels: one stream group
     assert rep["trial_ms"]["persistent=1 (ms per step)"] is not None
     a.step_n(20)
     b = batch(worlds, "fp64", tuning=rep["chosen"])
@@ -187,18 +219,21 @@ def test_batched_workload_sample_matches_oracle(c3_worlds, flux_mode, vm_tol, re
 
 
 @pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
-def test_batched_workload_persistent_kernel(c3_worlds, flux_mode):
-    """The batched workload in the persistent step kernel against the replayed
-    graphs, bit for bit, over 16 steps."""
+def test_batched_workload_persistent_kernel_and_streams(c3_worlds, flux_mode):
+    """The batched workload in the persistent step kernel and in 2 and 4
+    stream groups against the replayed graphs on one stream, bit for bit, over
+    16 steps."""
     runs = []
-    for tuning in (None, {"persistent": 1}):
+    tunings = (None, {"persistent": 1}, {"streams": 2}, {"streams": 4})
+    for tuning in tunings:
         say(f"case: batched workload {flux_mode}, tuning {tuning}")
         b = batch(c3_worlds, flux_mode, tuning=tuning)
         b.step_n(16)
         runs.append(state(b))
         del b
-    for k in runs[0]:
-        assert np.array_equal(runs[0][k], runs[1][k]), k
+    for t, r in zip(tunings[1:], runs[1:]):
+        for k in runs[0]:
+            assert np.array_equal(runs[0][k], r[k]), (t, k)
 
 
 @pytest.fixture(scope="module")
