@@ -113,6 +113,10 @@ struct KCluster {
     // its membranes (NaN otherwise): the membrane kernel then skips the loads
     double dm_u[BT_MAX_IONS];
     double gjo_u;
+    // bit i: ion i has no membrane flux anywhere in the cluster (Dm zero on
+    // every membrane and no pump or channel carrying it), so its membrane
+    // flux rows are exact zeros that need neither writing nor summing
+    unsigned still_mem;
     // likewise D_env per ion over the cluster's squares: k_env_field then
     // stages the constant instead of loading the array
     double denv_u[BT_MAX_IONS];
@@ -411,6 +415,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
     const KCluster* KC = A.kcl + A.tile_cluster[tile];
     const double nak_coef = KC->nak_coef;
     const double gjo_u = KC->gjo_u;
+    const unsigned still_mem = KC->still_mem;
     if (W == 0 && j == 0) *L.bad = 0;
 
     if (m < m1) {
@@ -428,12 +433,16 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         auto gather_ion = [&](int s) {
             const int i = W * NS + s;
             const double du = KC->dm_u[i];
-            // the cluster's uniform Dm when it has one, else the membrane's
+            // the cluster's uniform Dm when it has one, else the membrane's;
+            // nothing for a law with zero permeability (still ion, zero
+            // gap-junction diffusivity), which ghk_reuse does not evaluate
             const double* dmp = (du == du) ? &KC->dm_u[i] : &A.Dm[i * M + m];
-            g_env[s] = A.c_env[i * G2 + k];
-            g_lag[s] = A.c_lag_in[i * C + mc];
-            g_lagn[s] = A.c_lag_in[i * C + cn];
-            g_dm[s] = *dmp;
+            const bool still = (still_mem >> i) & 1u;
+            const bool gstill = (P.dgj_surf[i] == 0.0);
+            g_env[s] = still ? 0.0 : A.c_env[i * G2 + k];
+            g_lag[s] = (still && gstill) ? 0.0 : A.c_lag_in[i * C + mc];
+            g_lagn[s] = gstill ? 0.0 : A.c_lag_in[i * C + cn];
+            g_dm[s] = still ? 0.0 : *dmp;
         };
         double h_m[NQ], h_h[NQ], h_mod[NQ], h_out[NQ], h_in[NQ];
         // the update coefficients a_m, b_m, a_h, b_h of each gate (per cell, from k_env_field)
@@ -518,7 +527,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             }
             const double x = f * sa;
             L.mem[i][j] = x;
-            A.fsa[(size_t)i * M + pos] = x;
+            if (!((still_mem >> i) & 1u)) A.fsa[(size_t)i * M + pos] = x;    // the exchange reads no still row
             L.gj[i][j] = (-fgj) * sa;
         }
         // voltage-gated channels (networks.py:3115-3215): gates from Vmem carrying
@@ -911,17 +920,24 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
                 sC[e / VC][e % VC] = val;
             }
         }
+        // D_env zero over the cluster (block-uniform): the Nernst-Planck flux
+        // is a signed zero everywhere and the update keeps the
+        // (boundary-reset) concentrations, so its stages are skipped
+        const bool still = (s_du[i] == 0.0);
+        if (!still) {
 #pragma unroll
-        for (int it = 0; it < ND; ++it) {
-            const int e = tid + it * TY * TX;
-            if (e < ER * EC) {
-                const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
-                sD[e / EC][e % EC] = (r >= 0 && r < ny && cc >= 0 && cc < nx) ? td[it] : 0.0;
+            for (int it = 0; it < ND; ++it) {
+                const int e = tid + it * TY * TX;
+                if (e < ER * EC) {
+                    const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
+                    sD[e / EC][e % EC] = (r >= 0 && r < ny && cc >= 0 && cc < nx) ? td[it] : 0.0;
+                }
             }
         }
         // the next ion's loads, in flight through this ion's flux stages
         if (i + 1 < i_hi) load_ion(i + 1, s_du[i + 1]);
         __syncthreads();
+        if (!still) {
         // Nernst-Planck fluxes (sim_toolbox.py:409-411, gv = -E, u = 0, mu = 0)
         for (int e = tid; e < TY * (TX + 2); e += TY * TX) {
             const int lr = e / (TX + 2), lc = e % (TX + 2);  // fx at (r0+lr, cc0-1+lc)
@@ -956,28 +972,32 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
             sFy[lr][lc] = fy;
         }
         __syncthreads();
+        }
         // fd.divergence(-fx, -fy) with the sign-flipped one-sided edges (Q4), Euler step
         {
             const int lr = tid / TX, lc = tid % TX;
             const int r = r0 + lr, cc = cc0 + lc;
             if (r < ny && cc < nx) {
-                double gx, gy;
-                if (cc == 0) gx = per_dl(((-sFx[lr][lc + 1]) - (-sFx[lr][lc + 2])));
-                else if (cc == nx - 1) gx = per_dl(((-sFx[lr][lc]) - (-sFx[lr][lc + 1])));
-                else gx = per_d2(-((-sFx[lr][lc]) - (-sFx[lr][lc + 2])));
-                if (r == 0) gy = per_dl(-((-sFy[lr + 2][lc]) - (-sFy[lr + 1][lc])));
-                else if (r == ny - 1) gy = per_dl(-((-sFy[lr + 1][lc]) - (-sFy[lr][lc])));
-                else gy = per_d2(-((-sFy[lr][lc]) - (-sFy[lr + 2][lc])));
-                const double div = gx + gy;
+                double cn = sC[lr + 2][lc + 2];
+                if (!still) {
+                    double gx, gy;
+                    if (cc == 0) gx = per_dl(((-sFx[lr][lc + 1]) - (-sFx[lr][lc + 2])));
+                    else if (cc == nx - 1) gx = per_dl(((-sFx[lr][lc]) - (-sFx[lr][lc + 1])));
+                    else gx = per_d2(-((-sFx[lr][lc]) - (-sFx[lr][lc + 2])));
+                    if (r == 0) gy = per_dl(-((-sFy[lr + 2][lc]) - (-sFy[lr + 1][lc])));
+                    else if (r == ny - 1) gy = per_dl(-((-sFy[lr + 1][lc]) - (-sFy[lr][lc])));
+                    else gy = per_d2(-((-sFy[lr][lc]) - (-sFy[lr + 2][lc])));
+                    const double div = gx + gy;
+                    cn = cn + div * P.dt;
+                }
                 const size_t e = (size_t)i * P.n_sq + so + r * nx + cc;
-                const double cn = sC[lr + 2][lc + 2] + div * P.dt;
                 unsigned lv = live_b;
                 asm volatile("" : "+v"(lv));
                 if (lv != 0) A.c_mid[e] = cn;
                 else A.c_env_out[e] = cn;
                 if (A.obs_fx) {
-                    A.obs_fx[(size_t)i * P.n_sq + so + r * nx + cc] = sFx[lr][lc + 1];
-                    A.obs_fy[(size_t)i * P.n_sq + so + r * nx + cc] = sFy[lr + 1][lc];
+                    A.obs_fx[(size_t)i * P.n_sq + so + r * nx + cc] = still ? 0.0 : sFx[lr][lc + 1];
+                    A.obs_fy[(size_t)i * P.n_sq + so + r * nx + cc] = still ? 0.0 : sFy[lr + 1][lc];
                 }
             }
         }
@@ -1094,8 +1114,12 @@ __device__ __forceinline__ void exchange_square(const ExArgs& A, const KParams& 
             const double Sc = neg_seg_sum(A.fch + (size_t)ch * P.n_mems, a, b);
             cn = cn + (Sc / K.env_vol) * P.dt;
         }
-        const double S = neg_seg_sum(A.fsa + (size_t)i * P.n_mems, a, b);
-        cn = cn + (S / K.env_vol) * P.dt;
+        // a still ion's membrane fluxes are zeros: their sum would add a
+        // signed zero, which leaves cn as it is
+        if (!((K.still_mem >> i) & 1u)) {
+            const double S = neg_seg_sum(A.fsa + (size_t)i * P.n_mems, a, b);
+            cn = cn + (S / K.env_vol) * P.dt;
+        }
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         rho = (i == 0) ? P.zF[0] * cn : rho + P.zF[i] * cn;
     }
@@ -2435,6 +2459,12 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
             bool same = true;
             for (int m = lo[q]; m < hi[q] && same; ++m) same = (gj[m] == gj[lo[q]]);
             c->kcl_host[q].gjo_u = same ? gj[lo[q]] : NAN;
+            unsigned still = 0;
+            for (int i = 0; i < p->n_ions; ++i)
+                if (c->kcl_host[q].dm_u[i] == 0.0 && i != c->kp.i_na && i != c->kp.i_k && i != c->kp.i_ca &&
+                    c->kp.ion_chan[i] < 0)
+                    still |= 1u << i;
+            c->kcl_host[q].still_mem = still;
         }
         // D_env likewise over each cluster's grid (read once, like Dm)
         std::vector<double> de((size_t)p->n_ions * p->n_squares);
@@ -2494,6 +2524,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     if (e == hipSuccess) e = hipMemsetAsync(c->step_ctr, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->bar, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->u, 0, ns * 8, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->fsa, 0, ni * nm * 8, c->stream);   // still rows stay zero
     if (e == hipSuccess)
         e = hipMemsetAsync(c->rates, 0, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * 4 * nc * 8, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

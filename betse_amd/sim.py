@@ -283,6 +283,7 @@ class HostState:
     # ------------------------------------------------------------------
     def _set_state(self, h: dict, full: bool) -> None:
         shp = self.cells.X.shape
+        self.__dict__.pop("vm_ave_obs", None)       # observables of an earlier state
         self.cc_cells = h["c_cell"]
         self.vm_cell = h["vm_cell"]
         self.cc_env = h["c_env"]
@@ -336,7 +337,10 @@ class HostState:
         self.dd_time.append(snap[0])
         vm = self.vm
         self.vm_time.append(vm)
-        self.vm_ave_time.append(np.add.reduceat(vm, self.cells.mem_ptr[:-1]) / self.cells.num_mems)
+        # update_V's vm_ave (sim.py:2038): the sampled step derived it on the device
+        ave = self.__dict__.get("vm_ave_obs")
+        self.vm_ave_time.append(ave if ave is not None else
+                                np.add.reduceat(vm, self.cells.mem_ptr[:-1]) / self.cells.num_mems)
         self.rho_cells_time.append(self.rho_cells)
         self.venv_time.append(self.v_env)
         self.efield_ecm_x_time.append(self.E_env_x)
@@ -348,8 +352,12 @@ class HostState:
     # ------------------------------------------------------------------
     @property
     def vm(self) -> np.ndarray:
-        """Vmem per membrane: ``(1/cm)*rho_surf[mem_to_cells] - Phi_b[map_mem2ecm]``."""
-        return self.vm_cell[self.cells.mem_to_cells] - self.Phi_b[self.cells.map_mem2ecm]
+        """Vmem per membrane: ``(1/cm)*rho_surf[mem_to_cells] - Phi_b[map_mem2ecm]``
+        (x - 0.0 is x, so a zero Phi_b is not gathered)."""
+        vm = self.vm_cell[self.cells.mem_to_cells]
+        if self.Phi_b.any():
+            vm = vm - self.Phi_b[self.cells.map_mem2ecm]
+        return vm
 
     @property
     def cc_at_mem(self) -> np.ndarray:
@@ -365,7 +373,7 @@ class HostState:
         return np.dot(self.zs * self.p.F, self.cc_cells)
 
 
-COPY_THREADS = max(1, min(8, len(os.sched_getaffinity(0))))   # host threads of a sampled step's copy-out
+COPY_THREADS = max(1, min(16, len(os.sched_getaffinity(0))))  # host threads of a sampled step's copy-out
 
 
 class DeviceFetch:
