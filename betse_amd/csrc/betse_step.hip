@@ -363,7 +363,8 @@ struct MemCellArgs {
     const double* c_mid;      // post-diffusion env (Ca pump and channel gathers)
     const double* c_cell_in; const double* c_lag_in; const double* vm_in;
     double* c_cell_out; double* c_lag_out; double* vm_out;
-    double* fsa;              // [n_ions][n_mems] flux_mem * mem_sa, square-sorted
+    void* fsa;                // [n_ions][n_mems] flux_mem * mem_sa, square-sorted: FP64, or FP32 on the
+                              // FP32-flux path (summed in FP64 by the exchange)
     double* gates;            // [n_chan][2][n_mems]
     const double* chan_mod;   // [n_chan][n_cells]
     const double* rates;      // [n_chan][4][n_cells] gating rates of this step (k_env_field)
@@ -527,7 +528,10 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
             }
             const double x = f * sa;
             L.mem[i][j] = x;
-            if (!((still_mem >> i) & 1u)) A.fsa[(size_t)i * M + pos] = x;    // the exchange reads no still row
+            if (!((still_mem >> i) & 1u)) {                  // the exchange reads no still row
+                if constexpr (sizeof(FT) == 4) static_cast<float*>(A.fsa)[(size_t)i * M + pos] = (float)x;
+                else static_cast<double*>(A.fsa)[(size_t)i * M + pos] = x;
+            }
             L.gj[i][j] = (-fgj) * sa;
         }
         // voltage-gated channels (networks.py:3115-3215): gates from Vmem carrying
@@ -1022,7 +1026,8 @@ k_env_field(FieldArgs A, KParams P) {
 // ---------------------------------------------------------------------------
 struct ExArgs {
     const int32_t* sq_ptr; const uint8_t* has_mem; const int32_t* sq_cluster; const KCluster* kcl;
-    const double* c_mid; const double* fsa; const double* fch; double* c_env; double* u;
+    const double* c_mid; const void* fsa; const double* fch; double* c_env; double* u;
+    int fsa32;                // fsa holds FP32 values (the FP32-flux path)
     // step counters: block 0 of a counted step advances step_ctr[g] for each
     // bit g of ctr_mask (one counter per stream group; all of them when the
     // launch covers the whole problem)
@@ -1035,17 +1040,24 @@ struct ExArgs {
 // sum of -x[q] over q in [a, b), added in order from a; the loads are issued
 // eight at a time (clamped to b - 1, past-the-end values unused) so one
 // square's membranes cost one memory latency instead of one each
-__device__ __forceinline__ double neg_seg_sum(const double* x, int a, int b) {
+template <typename T>
+__device__ __forceinline__ double neg_seg_sum(const T* x, int a, int b) {
     double S = 0.0;
     for (int q0 = a; q0 < b; q0 += 8) {
-        double t[8];
+        T t[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) t[u] = x[min(q0 + u, b - 1)];
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (q0 + u < b) S = S + (-t[u]);
+            if (q0 + u < b) S = S + (-(double)t[u]);
     }
     return S;
+}
+
+// row i of the membrane fluxes (FP64 or FP32), summed negated over [a, b) in FP64
+__device__ __forceinline__ double neg_flux_sum(const void* fsa, int fsa32, size_t row, int a, int b) {
+    return fsa32 ? neg_seg_sum(static_cast<const float*>(fsa) + row, a, b)
+                 : neg_seg_sum(static_cast<const double*>(fsa) + row, a, b);
 }
 
 constexpr int EXQ = 64;
@@ -1086,8 +1098,10 @@ k_env_exchange(ExArgs A, KParams P) {
             const double Sc = neg_seg_sum(A.fch + (size_t)ch * P.n_mems, a, b);
             cn = cn + (Sc / env_vol) * P.dt;
         }
-        const double S = neg_seg_sum(A.fsa + (size_t)i * P.n_mems, a, b);
-        cn = cn + (S / env_vol) * P.dt;
+        if (!((K->still_mem >> i) & 1u)) {         // a still ion's fluxes are zeros
+            const double S = neg_flux_sum(A.fsa, A.fsa32, (size_t)i * P.n_mems, a, b);
+            cn = cn + (S / env_vol) * P.dt;
+        }
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         s_cn[i][j] = cn;
     }
@@ -1117,7 +1131,7 @@ __device__ __forceinline__ void exchange_square(const ExArgs& A, const KParams& 
         // a still ion's membrane fluxes are zeros: their sum would add a
         // signed zero, which leaves cn as it is
         if (!((K.still_mem >> i) & 1u)) {
-            const double S = neg_seg_sum(A.fsa + (size_t)i * P.n_mems, a, b);
+            const double S = neg_flux_sum(A.fsa, A.fsa32, (size_t)i * P.n_mems, a, b);
             cn = cn + (S / K.env_vol) * P.dt;
         }
         A.c_env[(size_t)i * P.n_sq + k] = cn;
@@ -1768,7 +1782,7 @@ struct bt_ctx {
     int n_tiles = 0;
     // workspace
     double* c_cell_b = nullptr; double* c_lag_b = nullptr; double* vm_b = nullptr;
-    double* c_mid = nullptr; double* fsa = nullptr; double* fch = nullptr; double* u = nullptr;
+    double* c_mid = nullptr; void* fsa = nullptr; double* fch = nullptr; double* u = nullptr;
     double* rates = nullptr;  // [n_chan][4][n_cells] gating rates of the current step
     double* c_env_b = nullptr;           // env state of odd steps
     int32_t* sq_live = nullptr;          // squares the exchange updates, ascending
@@ -2020,7 +2034,8 @@ static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_ste
                            c->flags, c->bad_step, c->step_ctr,
                            ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
                            ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr},
-               ExArgs{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u, c->flags,
+               ExArgs{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u,
+                      c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0, c->flags,
                       c->step_ctr, count_step, (1u << BT_MAX_STREAMS) - 1u, c->sq_live, c->n_live}};
     if (G > 1) {
         // the group's slices of the work lists; its own step counter (group 0
