@@ -15,6 +15,7 @@ everything else may differ.
 from __future__ import annotations
 
 import ctypes
+from collections import deque
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -260,6 +261,13 @@ class BatchSimulator(DeviceFetch):
             out[k] = a[..., lo:hi]
         return out
 
+    def _sample_worker(self):
+        w = self.__dict__.get("_worker")
+        if w is None:
+            from concurrent.futures import ThreadPoolExecutor
+            w = self._worker = ThreadPoolExecutor(max_workers=1)
+        return w
+
     def _sample_keys(self):
         d = self._dev
         state_keys = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"] + (["c_net"] if "c_net" in d else [])
@@ -274,8 +282,10 @@ class BatchSimulator(DeviceFetch):
         self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
         state_keys, keep = self._sample_keys()
         d = self._dev
-        self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}})
-        self._fetch_finish_async(state_keys + keep)
+        for slot in (0, 1):
+            self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}}, slot)
+            self._fetch_finish_async(state_keys + keep, slot)
+        self._sample_worker()
         self.sync()
 
     def run_loop(self, n_steps: int, t_resample: Optional[float] = None, callbacks=None) -> int:
@@ -294,7 +304,6 @@ class BatchSimulator(DeviceFetch):
         obs, t_obs = self._obs_buffers() if sampled_idx else (None, None)
         state_keys, keep = self._sample_keys()
         use_graph = 1 if self.use_graph else 0
-        pending = None
         cleared = False
 
         def clear():
@@ -306,34 +315,51 @@ class BatchSimulator(DeviceFetch):
                     st.clear_storage()
                 cleared = True
 
-        def finish(t_sample):
-            h = self._fetch_finish_async(state_keys + keep)
+        def finish(t_sample, slot):
+            h = self._fetch_finish_async(state_keys + keep, slot)
             for i, st in enumerate(self.states):
                 hi = self._split(h, i)
                 st._set_state(hi, False)
                 st._set_observables(hi)
                 st.write2storage(t_sample)
-            if callbacks is not None:
-                callbacks.progressed_next()
+
+        # A sample's host side (copy-out and every cluster's write2storage) runs
+        # on a worker thread, in sample order, while this thread keeps the
+        # device stepping; two snapshot slots alternate, so a sample waits for
+        # the one two before it. Progress callbacks run here, in order, once a
+        # sample is stored.
+        pending = deque()
+
+        def drain(block_until: int):
+            while pending and (len(pending) > block_until or pending[0].done()):
+                pending.popleft().result()
+                if callbacks is not None:
+                    callbacks.progressed_next()
 
         pos = 0
-        for s in sampled_idx:
-            self._check(self._lib.bt_step_observed(self._ctx, int(s + 1 - pos), use_graph, ctypes.byref(obs)))
-            pos = s + 1
+        slot = 0
+        try:
+            for s in sampled_idx:
+                self._check(self._lib.bt_step_observed(self._ctx, int(s + 1 - pos), use_graph, ctypes.byref(obs)))
+                pos = s + 1
+                clear()
+                self.sync()
+                self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
+                drain(1)                          # this slot's previous sample is stored
+                self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}}, slot)
+                pending.append(self._sample_worker().submit(finish, time_steps[s], slot))
+                slot ^= 1
+            if pos < len(time_steps):
+                self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, use_graph))
             clear()
-            if pending is not None:
-                finish(pending)
-                pending = None
+            drain(0)
             self.sync()
-            self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
-            self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}})
-            pending = time_steps[s]
-        if pos < len(time_steps):
-            self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, use_graph))
-        clear()
-        if pending is not None:
-            finish(pending)
-        self.sync()
+        except BaseException:
+            # the samples taken before the failure are stored, as the
+            # reference's loop has stored them when it raises
+            for f in pending:
+                f.result()
+            raise
         return len(sampled_idx)
 
     def snapshot(self):

@@ -393,48 +393,56 @@ class DeviceFetch:
                     pinned[k] = h
                 h.copy_(t, non_blocking=True)
 
-    def _fetch_start_async(self, tensors: dict) -> None:
-        """Sampled steps: snapshot the (FP64) tensors into one flat device buffer
+    def _fetch_start_async(self, tensors: dict, slot: int = 0) -> None:
+        """Sampled steps: snapshot the (FP64) tensors into a flat device buffer
         on the simulation stream (a single multi-tensor copy), then copy that
         buffer to pinned host memory on a side stream (default priority), so the
-        host transfer overlaps the next steps. ``_fetch_finish_async`` waits for
-        it. The buffers are reused: a caller finishes one fetch before it starts
-        the next."""
+        host transfer overlaps the next steps. ``_fetch_finish_async(keys,
+        slot)`` waits for it. Each slot has its own pair of buffers, reused: the
+        snapshot waits on the device for the slot's previous transfer, and a
+        caller finishes a slot's fetch before it starts the slot's next one."""
         torch = self.torch
         layout = [(k, tuple(t.shape)) for k, t in tensors.items()]
         if any(t.dtype != torch.float64 for t in tensors.values()):
             raise BetseSimException("sampled-step fetch expects FP64 tensors")
         if self.__dict__.get("_flat_layout") != layout:
-            total = sum(int(np.prod(shape)) for _, shape in layout)
-            self._flat_dev = torch.empty(total, dtype=torch.float64, device=self.device)
-            self._flat_host = torch.empty(total, dtype=torch.float64, pin_memory=True)
+            self._flat_dev, self._flat_host, self._copied = {}, {}, {}
             self._flat_at, o = {}, 0
             for k, shape in layout:
                 n = int(np.prod(shape))
                 self._flat_at[k] = (o, n, shape)
                 o += n
+            self._flat_total = o
             self._flat_layout = layout
+        if slot not in self._flat_dev:
+            self._flat_dev[slot] = torch.empty(self._flat_total, dtype=torch.float64, device=self.device)
+            self._flat_host[slot] = torch.empty(self._flat_total, dtype=torch.float64, pin_memory=True)
         if self.__dict__.get("_copy_stream") is None:
             self._copy_stream = torch.cuda.Stream(device=self.device)     # default priority
+        fdev = self._flat_dev[slot]
         with torch.cuda.device(self.device):
             with torch.cuda.stream(self.stream):
-                dst = [self._flat_dev[o:o + n].view(shape) for o, n, shape in self._flat_at.values()]
+                prev = self._copied.get(slot)
+                if prev is not None:
+                    self.stream.wait_event(prev)
+                dst = [fdev[o:o + n].view(shape) for o, n, shape in self._flat_at.values()]
                 torch._foreach_copy_(dst, [tensors[k] for k in self._flat_at])
                 taken = torch.cuda.Event()
                 taken.record(self.stream)
             with torch.cuda.stream(self._copy_stream):
                 self._copy_stream.wait_event(taken)
-                self._flat_host.copy_(self._flat_dev, non_blocking=True)
-                self._copied = torch.cuda.Event()
-                self._copied.record(self._copy_stream)
+                self._flat_host[slot].copy_(fdev, non_blocking=True)
+                done = torch.cuda.Event()
+                done.record(self._copy_stream)
+                self._copied[slot] = done
 
-    def _fetch_finish_async(self, keys) -> dict:
-        """Host arrays of the last ``_fetch_start_async``: views into one fresh
-        NumPy copy of the pinned buffer (which is reused), so each sample's
-        arrays are its own. The copy runs in chunks on a few host threads
-        (NumPy releases the GIL while it copies)."""
-        self.__dict__.pop("_copied").synchronize()
-        src = self._flat_host.numpy()
+    def _fetch_finish_async(self, keys, slot: int = 0) -> dict:
+        """Host arrays of the slot's last ``_fetch_start_async``: views into one
+        fresh NumPy copy of the pinned buffer (which is reused), so each
+        sample's arrays are its own. The copy runs in chunks on a few host
+        threads (NumPy releases the GIL while it copies)."""
+        self._copied[slot].synchronize()
+        src = self._flat_host[slot].numpy()
         flat = np.empty_like(src)
         n = len(src)
         if n < (1 << 22):
