@@ -140,6 +140,16 @@ __device__ __forceinline__ bool halted(const unsigned* flags) {
     return __atomic_load_n(flags, __ATOMIC_RELAXED) != 0u;
 }
 
+// Barrier for a block whose threads exchange data through LDS only: the
+// fences are restricted to LDS, so the barrier does not wait for the block's
+// outstanding global stores (__syncthreads fences global memory as well, and
+// so waits for every store the block has issued before it).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------------------
 // flux laws
 // ---------------------------------------------------------------------------
@@ -384,8 +394,8 @@ struct MemLds {
 };
 
 // Synchronisation of the lanes that share a tile's LDS: the block (WSYNC
-// false: the tile's one or two wavefronts are the block) or the wavefront
-// alone (WSYNC true: in the persistent step kernel every wavefront of a block
+// false: the tile's one or two wavefronts are the block; an LDS-only barrier,
+// so the tile's flux stores need not have landed) or the wavefront alone (WSYNC true: in the persistent step kernel every wavefront of a block
 // works on a tile of its own; the lanes of a wavefront run in step and its LDS
 // accesses complete in order, so a compiler fence is all it needs).
 template <bool WSYNC>
@@ -395,7 +405,7 @@ __device__ __forceinline__ void tile_sync() {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     } else {
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -854,7 +864,7 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
     // axis-0 pass (scipy correlate1d: centre tap, then pairs outermost first)
     for (int e = tid; e < TR * UC; e += TY * TX) {
         const int lr = e / UC + 4, lc = e % UC;          // row in sU
@@ -863,7 +873,7 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
         for (int q = 4; q >= 1; --q) o = o + (sU[lr - q][lc] + sU[lr + q][lc]) * P.gw[q];
         sT[e / UC][lc] = o;
     }
-    __syncthreads();
+    lds_barrier();
     // axis-1 pass -> v_env (+ Phi_b == 0)
     for (int e = tid; e < VR * VC; e += TY * TX) {
         const int lr = e / VC, lc = e % VC + 4;          // column in sT
@@ -872,7 +882,7 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
         for (int q = 4; q >= 1; --q) o = o + (sT[lr][lc - q] + sT[lr][lc + q]) * P.gw[q];
         sV[lr][e % VC] = o;
     }
-    __syncthreads();
+    lds_barrier();
     // E = -grad(screen * v_env), one-sided at the world edges (finitediff.py:1236-1266)
     for (int e = tid; e < ER * EC; e += TY * TX) {
         const int er = e / EC, ec = e % EC;
@@ -893,7 +903,7 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
         sEx[er][ec] = ex;
         sEy[er][ec] = ey;
     }
-    __syncthreads();
+    lds_barrier();
     // a halted run writes nothing (every thread read the same flag)
     if (stop) return;
     if (!DIFFUSE) {
@@ -910,7 +920,7 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
         const double cb = P.c_bound[i];
         const double zi = P.zq[i];
         const double zq_kt = (zi * P.q) / P.kbT;        // alpha = D * zq_kt
-        __syncthreads();
+        lds_barrier();
         // concentrations of ion i (loaded one ion ahead) with the world
         // boundary reset to c_env_bound (Q3)
 #pragma unroll
@@ -940,7 +950,7 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
         }
         // the next ion's loads, in flight through this ion's flux stages
         if (i + 1 < i_hi) load_ion(i + 1, s_du[i + 1]);
-        __syncthreads();
+        lds_barrier();
         if (!still) {
         // Nernst-Planck fluxes (sim_toolbox.py:409-411, gv = -E, u = 0, mu = 0)
         for (int e = tid; e < TY * (TX + 2); e += TY * TX) {
@@ -975,7 +985,7 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
             }
             sFy[lr][lc] = fy;
         }
-        __syncthreads();
+        lds_barrier();
         }
         // fd.divergence(-fx, -fy) with the sign-flipped one-sided edges (Q4), Euler step
         {
@@ -1105,7 +1115,7 @@ k_env_exchange(ExArgs A, KParams P) {
         A.c_env[(size_t)i * P.n_sq + k] = cn;
         s_cn[i][j] = cn;
     }
-    __syncthreads();
+    lds_barrier();
     if (i == 0 && live) {
         double rho = P.zF[0] * s_cn[0][j];
         for (int ii = 1; ii < P.n_ions; ++ii) rho = rho + P.zF[ii] * s_cn[ii][j];
@@ -1257,7 +1267,7 @@ k_network(const KNet* __restrict__ net, double* __restrict__ cn, int n_cells, do
     const int S = net->n_species;
     // cells past the end evaluate on ones and store nothing
     for (int s = w; s < S; s += NET_WAVES) s_c[s][t] = live ? cn[(size_t)s * n_cells + cell] : 1.0;
-    __syncthreads();
+    lds_barrier();
     const int R = net->n_rates;
     for (int j = w; j < R; j += NET_WAVES) {
         const KNetRate& r = net->rates[j];
@@ -1306,7 +1316,7 @@ k_network(const KNet* __restrict__ net, double* __restrict__ cn, int n_cells, do
         }
         s_r[j][t] = rate;
     }
-    __syncthreads();
+    lds_barrier();
     bool bad = false;
     if (live) {
         for (int s = w; s < S; s += NET_WAVES) {
@@ -1319,7 +1329,7 @@ k_network(const KNet* __restrict__ net, double* __restrict__ cn, int n_cells, do
         }
     }
     if (bad) s_bad = 1;
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0 && s_bad) {
         // the step's index: step_ctr counts the steps whose exchange has run
         atomicOr(flags, 1u);
@@ -1382,7 +1392,7 @@ k_obs_mem(ObsMemArgs A, KParams P) {
         s_sa[j] = A.mem_sa[m];
         s_lc[j] = A.mem_cell[m] - c0;
     }
-    __syncthreads();
+    lds_barrier();
     if (c < c1) {
         double s = 0.0;
         for (int q = ca; q < cb; ++q) s = (q == ca) ? s_j[q] * s_sa[q] : s + s_j[q] * s_sa[q];
@@ -1391,7 +1401,7 @@ k_obs_mem(ObsMemArgs A, KParams P) {
         s_wm[j] = (nm - 1) / nm;
         s_wo[j] = 1 / nm;
     }
-    __syncthreads();
+    lds_barrier();
     if (m < m1) {
         const int lc = s_lc[j];
         const double jn = s_wm[lc] * s_j[j] + s_ave[lc] * s_wo[lc];
@@ -1404,7 +1414,7 @@ k_obs_mem(ObsMemArgs A, KParams P) {
         A.E_gj_x[m] = egj * A.mem_nx[m];
         A.E_gj_y[m] = egj * A.mem_ny[m];
     }
-    __syncthreads();
+    lds_barrier();
     if (c < c1) {
         const double v = A.vm_cell[c];
         double sx = s_jx[ca], sy = s_jy[ca], svm = v;
@@ -1576,7 +1586,7 @@ __device__ __forceinline__ void gemm_tile(const double* A, const double* B, doub
 #pragma unroll
         for (int r = 0; r < 4; ++r) part[wave - 1][lane][r] = acc[r];
     }
-    __syncthreads();
+    lds_barrier();
     if (wave == 0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -1719,13 +1729,13 @@ k_step_persistent(PersistArgs PA, KParams P) {
         // phase A: env field and diffusion (and the gating coefficients). The
         // flags change only in the membrane phase, so one read decides for the block.
         if (threadIdx.x == 0) s_stop = halted(FA.flags) ? 1 : 0;
-        __syncthreads();
+        lds_barrier();
         if (!s_stop) {
             for (int it = blockIdx.x; it < n_env_items; it += nb) {
                 const int item = (nb % N_XCD == 0) ? xcd_chunk(it, n_env_items) : it;
                 env_field_tile<true, false>(FA, P, item % PA.n_env_tiles, item / PA.n_env_tiles, PA.n_groups, lds,
                                             -1, 0, false);
-                __syncthreads();
+                lds_barrier();
             }
             if (RATES) {
                 const int items = P.n_chan * P.n_cells;
