@@ -147,6 +147,7 @@ SIGNATURES = {
     "bt_validate": (ctypes.c_int, [ctypes.POINTER(BtParams), ctypes.POINTER(BtCluster), ctypes.c_int32,
                                     ctypes.POINTER(BtBuffers)]),
     "bt_capture_graphs": (ctypes.c_int, [_vp]),
+    "bt_refresh_params": (ctypes.c_int, [_vp]),
     "bt_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(BtTuning)]),
     "bt_set_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(BtTuning)]),
     "bt_autotune": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(BtTuning), ctypes.POINTER(ctypes.c_double),

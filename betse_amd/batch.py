@@ -148,6 +148,33 @@ class BatchSimulator(DeviceFetch):
             self._check(self._lib.bt_capture_graphs(self._ctx))
 
     # ------------------------------------------------------------------
+    def set_membrane_params(self, cluster: int, Dm: Optional[np.ndarray] = None, gjopen: Optional[np.ndarray] = None,
+                            D_env: Optional[np.ndarray] = None) -> None:
+        """An event on one cluster of the batch (``Simulator.set_membrane_params``):
+        its new ``Dm_cells`` [n_ions][M_k], ``gjopen`` [M_k] or ``D_env``
+        [n_ions][G_k²] written into its slices of the batch's buffers, then
+        ``bt_refresh_params``."""
+        torch = self.torch
+        coff, moff, soff = self.offsets
+        m0, m1 = int(moff[cluster]), int(moff[cluster + 1])
+        s0, s1 = int(soff[cluster]), int(soff[cluster + 1])
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            for key, a, lo, hi in (("Dm", Dm, m0, m1), ("gjopen", gjopen, m0, m1), ("D_env", D_env, s0, s1)):
+                if a is None:
+                    continue
+                dst = self._dev[key][..., lo:hi]
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                if a.shape != tuple(dst.shape):
+                    raise BetseSimException(f"{key}: shape {a.shape}, cluster {cluster} needs {tuple(dst.shape)}")
+                dst.copy_(torch.from_numpy(a).to(self.device))
+        self.stream.synchronize()
+        self._check(self._lib.bt_refresh_params(self._ctx))
+        st = self.states[cluster]
+        if Dm is not None:
+            st.Dm_cells = np.array(Dm, dtype=np.float64)
+        if gjopen is not None:
+            st.gjopen = np.array(gjopen, dtype=np.float64)
+
     def get_tuning(self) -> dict:
         t = _native.BtTuning()
         self._check(self._lib.bt_get_tuning(self._ctx, ctypes.byref(t)))

@@ -1807,6 +1807,7 @@ struct bt_ctx {
     // entries each; empty where the clusters' work lists are not in cluster
     // order); the groups' streams ([0] is the caller's) and fork/join events
     std::vector<int> cl_tile0, cl_env0, cl_live0;
+    std::vector<int> cl_mem_lo, cl_mem_hi;   // each cluster's membrane range
     std::vector<long long> cl_cell0;
     hipStream_t gstream[BT_MAX_STREAMS] = {};
     hipEvent_t ev_fork = nullptr;
@@ -2358,6 +2359,62 @@ const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : g
 
 int64_t bt_steps_done(const bt_ctx* ctx) { return ctx ? ctx->steps_done : -1; }
 
+// The per-cluster constants read from the caller's Dm, gjopen and D_env: a
+// value that is the same over all of a cluster's membranes (squares) becomes
+// a constant of the cluster table, so the kernels skip its loads; a still ion
+// (no membrane flux anywhere in the cluster: Dm zero, no pump or channel
+// carrying it) is marked. Into kcl_host; the caller uploads the table.
+static int derive_uniform(bt_ctx* c) {
+    const bt_params* p = &c->prm;
+    const int nq = (int)c->clusters.size();
+    std::vector<double> dm((size_t)p->n_ions * p->n_mems), gj((size_t)p->n_mems), de((size_t)p->n_ions * p->n_squares);
+    hipError_t e = hipMemcpy(dm.data(), c->buf.Dm, dm.size() * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(gj.data(), c->buf.gjopen, gj.size() * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(de.data(), c->buf.D_env, de.size() * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("reading Dm, gjopen, D_env: ") + hipGetErrorString(e));
+    for (int q = 0; q < nq; ++q) {
+        KCluster& K = c->kcl_host[q];
+        const int lo = c->cl_mem_lo[q], hi = c->cl_mem_hi[q];
+        for (int i = 0; i < BT_MAX_IONS; ++i) K.dm_u[i] = K.denv_u[i] = NAN;
+        K.gjo_u = NAN;
+        K.still_mem = 0;
+        if (hi > lo) {
+            for (int i = 0; i < p->n_ions; ++i) {
+                const double* d = dm.data() + (size_t)i * p->n_mems;
+                bool same = true;
+                for (int m = lo; m < hi && same; ++m) same = (d[m] == d[lo]);
+                K.dm_u[i] = same ? d[lo] : NAN;
+            }
+            bool same = true;
+            for (int m = lo; m < hi && same; ++m) same = (gj[m] == gj[lo]);
+            K.gjo_u = same ? gj[lo] : NAN;
+            for (int i = 0; i < p->n_ions; ++i)
+                if (K.dm_u[i] == 0.0 && i != c->kp.i_na && i != c->kp.i_k && i != c->kp.i_ca && c->kp.ion_chan[i] < 0)
+                    K.still_mem |= 1u << i;
+        }
+        const size_t k0 = (size_t)c->clusters[q].sq_off;
+        const size_t k1 = k0 + (size_t)c->clusters[q].grid_ny * c->clusters[q].grid_nx;
+        for (int i = 0; i < p->n_ions; ++i) {
+            const double* d = de.data() + (size_t)i * p->n_squares;
+            bool same = true;
+            for (size_t k = k0; k < k1 && same; ++k) same = (d[k] == d[k0]);
+            K.denv_u[i] = same ? d[k0] : NAN;
+        }
+    }
+    return BT_OK;
+}
+
+int bt_refresh_params(bt_ctx* c) {
+    if (!c) return BT_EINVAL;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int g = 1; g < BT_MAX_STREAMS; ++g)
+        if (c->gstream[g]) HIP_TRY(c, hipStreamSynchronize(c->gstream[g]));
+    int rc = derive_uniform(c);
+    if (rc != BT_OK) return rc;
+    HIP_TRY(c, hipMemcpy(c->kcl, c->kcl_host.data(), sizeof(KCluster) * c->kcl_host.size(), hipMemcpyHostToDevice));
+    return BT_OK;
+}
+
 int bt_validate(const bt_params* p, const bt_cluster* cl, int32_t n_clusters, const bt_buffers* h) {
     if (!p || !cl || !h) return set_err(nullptr, BT_EINVAL, "null argument");
     int rc = check_params(p, cl, n_clusters, h);
@@ -2459,53 +2516,18 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
         }
     }
     {
-        // Dm and gjopen that are one value over a cluster's membranes (read once:
-        // they do not change within a phase on this path)
-        std::vector<double> dm((size_t)p->n_ions * p->n_mems), gj((size_t)p->n_mems);
-        hipError_t e = hipMemcpy(dm.data(), b->Dm, dm.size() * 8, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(gj.data(), b->gjopen, gj.size() * 8, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            delete c;
-            return set_err(nullptr, BT_EHIP, std::string("reading Dm: ") + hipGetErrorString(e));
-        }
-        std::vector<int> lo(n_clusters, p->n_mems), hi(n_clusters, 0);
+        // each cluster's membrane range, for the per-cluster constants
+        c->cl_mem_lo.assign(n_clusters, p->n_mems);
+        c->cl_mem_hi.assign(n_clusters, 0);
         for (int t = 0; t < b->n_tiles; ++t) {
-            lo[tcl[t]] = std::min(lo[tcl[t]], mp[tp[t]]);
-            hi[tcl[t]] = std::max(hi[tcl[t]], mp[tp[t + 1]]);
+            c->cl_mem_lo[tcl[t]] = std::min(c->cl_mem_lo[tcl[t]], mp[tp[t]]);
+            c->cl_mem_hi[tcl[t]] = std::max(c->cl_mem_hi[tcl[t]], mp[tp[t + 1]]);
         }
-        for (int q = 0; q < n_clusters; ++q) {
-            if (hi[q] <= lo[q]) continue;
-            for (int i = 0; i < p->n_ions; ++i) {
-                const double* d = dm.data() + (size_t)i * p->n_mems;
-                bool same = true;
-                for (int m = lo[q]; m < hi[q] && same; ++m) same = (d[m] == d[lo[q]]);
-                c->kcl_host[q].dm_u[i] = same ? d[lo[q]] : NAN;
-            }
-            bool same = true;
-            for (int m = lo[q]; m < hi[q] && same; ++m) same = (gj[m] == gj[lo[q]]);
-            c->kcl_host[q].gjo_u = same ? gj[lo[q]] : NAN;
-            unsigned still = 0;
-            for (int i = 0; i < p->n_ions; ++i)
-                if (c->kcl_host[q].dm_u[i] == 0.0 && i != c->kp.i_na && i != c->kp.i_k && i != c->kp.i_ca &&
-                    c->kp.ion_chan[i] < 0)
-                    still |= 1u << i;
-            c->kcl_host[q].still_mem = still;
-        }
-        // D_env likewise over each cluster's grid (read once, like Dm)
-        std::vector<double> de((size_t)p->n_ions * p->n_squares);
-        e = hipMemcpy(de.data(), b->D_env, de.size() * 8, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
+        const int rc = derive_uniform(c);
+        if (rc != BT_OK) {
+            const std::string msg = c->err;
             delete c;
-            return set_err(nullptr, BT_EHIP, std::string("reading D_env: ") + hipGetErrorString(e));
-        }
-        for (int q = 0; q < n_clusters; ++q) {
-            const size_t k0 = (size_t)cl[q].sq_off, k1 = k0 + (size_t)cl[q].grid_ny * cl[q].grid_nx;
-            for (int i = 0; i < p->n_ions; ++i) {
-                const double* d = de.data() + (size_t)i * p->n_squares;
-                bool same = true;
-                for (size_t k = k0; k < k1 && same; ++k) same = (d[k] == d[k0]);
-                c->kcl_host[q].denv_u[i] = same ? d[k0] : NAN;
-            }
+            return set_err(nullptr, rc, msg);
         }
     }
     (void)hipGetDevice(&c->device);

@@ -556,6 +556,33 @@ class Simulator(HostState, DeviceFetch):
             self._check(self._lib.bt_capture_graphs(self._ctx))
 
     # ------------------------------------------------------------------
+    # events
+    # ------------------------------------------------------------------
+    def set_membrane_params(self, Dm: Optional[np.ndarray] = None, gjopen: Optional[np.ndarray] = None,
+                            D_env: Optional[np.ndarray] = None) -> None:
+        """An event's new membrane diffusivities (``Dm_cells``, [n_ions][M]),
+        gap-junction openings (``gjopen``, [M]) or env diffusivities (``D_env``,
+        [n_ions][G²]) for the next steps, as ``TissueHandler.fire_events`` ->
+        ``makeAllChanges`` sets them (``tishandler.py:709-724,1321-1332``):
+        written into the device buffers, then ``bt_refresh_params``."""
+        torch = self.torch
+        with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
+            for key, a in (("Dm", Dm), ("gjopen", gjopen), ("D_env", D_env)):
+                if a is None:
+                    continue
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                dst = self._dev[key]
+                if a.size != dst.numel():
+                    raise BetseSimException(f"{key}: {a.size} values for a buffer of {dst.numel()}")
+                dst.copy_(torch.from_numpy(a.reshape(dst.shape)).to(self.device))
+        self.stream.synchronize()
+        self._check(self._lib.bt_refresh_params(self._ctx))
+        if Dm is not None:
+            self.Dm_cells = np.array(Dm, dtype=np.float64)
+        if gjopen is not None:
+            self.gjopen = np.array(gjopen, dtype=np.float64)
+
+    # ------------------------------------------------------------------
     # launch variants of the step kernels
     # ------------------------------------------------------------------
     def get_tuning(self) -> dict:

@@ -233,6 +233,17 @@ typedef struct {
 
 #define BT_MAX_STREAMS 4
 
+/* Events that change the membranes' or the environment's diffusivities
+ * (TissueHandler.fire_events -> makeAllChanges, tishandler.py:709-724 and
+ * 1321-1332: Dm_cells = Dm_scheduled + Dm_base; gap-junction blocks through
+ * gjopen; D_env): after the caller has written new values into the Dm, gjopen
+ * or D_env buffers bound at bt_create, this re-derives the per-cluster
+ * constants bt_create took from them (values that are one over a cluster, ions
+ * without membrane flux), so the next steps use the new values; captured
+ * graphs stay valid. An applied boundary voltage (bound_V, Phi_b) is not
+ * supported on this path. Synchronous. */
+int bt_refresh_params(bt_ctx* ctx);
+
 /* Capture and instantiate the step graphs now (bt_step otherwise does it on
  * its first call that replays graphs), so that no later bt_step pays for it.
  * Newly captured graphs are also replayed once from a saved copy of the state,

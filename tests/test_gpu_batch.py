@@ -134,3 +134,34 @@ def test_batch_sampled_observables_match_oracle():
         rel = jenv_rel(o.storage[-1], p)
         np.testing.assert_allclose(st.B_field.ravel(), o.B_field.ravel(), rtol=0, atol=rel * np.abs(o.B_field).max())
         say(f"cluster {k}: {n_samp} sampled steps match")
+
+
+def test_batch_event_on_one_cluster():
+    """An event on one cluster of a batch (set_membrane_params): that cluster
+    follows the oracle stepped through the change, the others are untouched."""
+    from test_gpu_parity import event_params, oracle_stepper
+    worlds = batched_worlds(n_clusters=3, cells_range=(30, 300), grid_range=(16, 32), seed=11)
+    say(f"case: event on cluster 1 of a batch of 3: cells {[c.n_cells for _, c in worlds]}")
+    from betse_amd.batch import BatchSimulator
+    b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode="fp64")
+    b.init("init")
+    b.step_n(10)
+    orcs = [oracle_stepper(p, c) for p, c in worlds]
+    for o in orcs:
+        for _ in range(10):
+            o.step("init", observables=False)
+    Dm, gjopen = event_params(orcs[1], worlds[1][1])
+    orcs[1].Dm_cells = Dm.copy()
+    orcs[1].gj_block = 0.5
+    b.set_membrane_params(1, Dm=Dm, gjopen=gjopen)
+    for o in orcs:
+        for _ in range(15):
+            o.step("init", observables=False)
+    b.step_n(15)
+    b.pull()
+    for k, (o, st) in enumerate(zip(orcs, b.states)):
+        dv = np.abs(st.vm - o.vm).max()
+        dc = (np.abs(st.cc_cells - o.cc_cells) / np.abs(o.cc_cells)).max()
+        de = (np.abs(st.cc_env - o.cc_env) / np.abs(o.cc_env)).max()
+        say(f"cluster {k}: |dVm|={dv:.3e} rel dc={dc:.3e} rel denv={de:.3e}")
+        assert dv <= 1e-9 and dc <= 1e-10 and de <= 1e-10

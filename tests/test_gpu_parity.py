@@ -629,3 +629,55 @@ def test_static_noise_kept_in_init_dropped_in_sim():
     sim.step_n(30)
     sim.pull()
     check(sim, o, "static noise init+sim")
+
+
+def oracle_stepper(p, cells, kind="init"):
+    """The oracle set up as run_sim_core sets it up, stepped one step at a time."""
+    o = OracleSim(cells, p.copy(), mode="sparse")
+    o.init_core()
+    o.init_dynamics(kind)
+    o.p.set_phase_time(kind)
+    o.update_V(observables=False)
+    return o
+
+
+def event_params(o, cells):
+    """An event's new parameters: Na Dm tripled on every other membrane (no
+    longer one value over the cluster), K Dm halved everywhere, gap junctions
+    half blocked (gj_block = 0.5)."""
+    iNa, iK = o.names.index("Na"), o.names.index("K")
+    Dm = o.Dm_cells.copy()
+    Dm[iNa, ::2] *= 3.0
+    Dm[iK] *= 0.5
+    gjopen = 0.5 * np.ones(cells.n_mems) * cells.gj_default_weights
+    return Dm, gjopen
+
+
+def test_event_changes_membrane_params(tiny):
+    """An event in the middle of a phase (TissueHandler.fire_events ->
+    makeAllChanges changing Dm and the gap-junction block): the GPU run
+    (set_membrane_params -> bt_refresh_params, replayed graphs kept) matches
+    the oracle stepped through the same change, at the FP64 bars."""
+    p, cells = tiny
+    from betse_amd.sim import SimPhase, SimPhaseKind, Simulator
+    say("case: event mid-phase (Dm, gap-junction block), 10 + 15 steps")
+    sim = Simulator(p.copy(), device="cuda:0")
+    ph = SimPhase(SimPhaseKind.INIT, sim.p, cells, sim)
+    sim.init_core(ph)
+    sim.prepare(ph)
+    sim.step_n(10)
+    o = oracle_stepper(p, cells)
+    for _ in range(10):
+        o.step("init", observables=False)
+    Dm, gjopen = event_params(o, cells)
+    o.Dm_cells = Dm.copy()
+    o.gj_block = 0.5
+    sim.set_membrane_params(Dm=Dm, gjopen=gjopen)
+    for _ in range(15):
+        o.step("init", observables=False)
+    sim.step_n(15)
+    sim.pull()
+    check(sim, o, "after the event")
+    # the event changed the trajectory: without it the run differs
+    ref = gpu_run(p, cells, 25)
+    assert np.abs(ref.cc_cells - sim.cc_cells).max() > 1e-6 * np.abs(sim.cc_cells).max()
