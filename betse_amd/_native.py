@@ -108,7 +108,7 @@ class BtBuffers(ctypes.Structure):
         "tile_cluster", "sq_cluster",
         "mem_sa", "gjopen", "cell_vol", "diviterm", "Dm", "D_env", "sq_has_mem",
         "c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "mem_nx", "mem_ny", "cell_sa",
-        "gates", "chan_mod")] + [
+        "gates", "chan_mod", "Phi_b")] + [
         ("n_tiles", ctypes.c_int32), ("tile_m", ctypes.c_int32)]
 
 
@@ -148,6 +148,7 @@ SIGNATURES = {
                                     ctypes.POINTER(BtBuffers)]),
     "bt_capture_graphs": (ctypes.c_int, [_vp]),
     "bt_refresh_params": (ctypes.c_int, [_vp]),
+    "bt_stage_boundary": (ctypes.c_int, [_vp, _vp]),
     "bt_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(BtTuning)]),
     "bt_set_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(BtTuning)]),
     "bt_autotune": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(BtTuning), ctypes.POINTER(ctypes.c_double),

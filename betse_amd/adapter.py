@@ -16,10 +16,11 @@ one phase's loop to the GPU and take the results back:
   The reference holds ``cc_at_mem`` and ``vm`` per membrane
   (``sim.py:744-745, 2027-2029``); on this path both are constant over a
   cell's membranes (``cc_at_mem = cc_cells[mem_to_cells]`` after
-  ``update_intra``, ``vm = rho_surf[mem_to_cells] - Phi_b[map_mem2ecm]`` with
-  ``Phi_b == 0``), so the GPU state keeps them per cell (``c_lag``,
-  ``vm_cell``). Going to the GPU, a per-membrane field that is not constant
-  within a cell raises instead of being averaged away.
+  ``update_intra``; ``vm + Phi_b[map_mem2ecm] = rho_surf[mem_to_cells]``), so
+  the GPU state keeps them per cell (``c_lag``, ``vm_cell`` = Vmem plus the
+  applied boundary potential ``Phi_b`` at the membrane's square) beside
+  ``Phi_b`` per square. Going to the GPU, a per-membrane field that is not
+  constant within a cell raises instead of being averaged away.
 
 Every function reads its input by attribute name only (duck typing), so it
 works on the reference's live objects and on test stand-ins alike.
@@ -113,12 +114,14 @@ def gpu_params_from(ref_p, *, channels: Sequence = (), network=None, **overrides
     return p
 
 
-def _per_cell(x: np.ndarray, mem_ptr: np.ndarray, name: str) -> np.ndarray:
-    """Per-cell values of a per-membrane field ([..., M]) that is constant over
-    each cell's membranes."""
+def _per_cell(x: np.ndarray, mem_ptr: np.ndarray, name: str, atol: float = 0.0) -> np.ndarray:
+    """Per-cell values (each cell's first membrane's) of a per-membrane field
+    ([..., M]) that is constant over each cell's membranes, exactly or, where
+    it was rebuilt from a difference, within ``atol``."""
     first = x[..., mem_ptr[:-1]]
     counts = np.diff(mem_ptr)
-    if not np.array_equal(np.repeat(first, counts, axis=-1), x):
+    dev = np.abs(np.repeat(first, counts, axis=-1) - x)
+    if not (np.array_equal(np.repeat(first, counts, axis=-1), x) if atol == 0.0 else bool(np.all(dev <= atol))):
         raise BetseSimException(f"{name} differs between the membranes of a cell: the GPU state keeps it per cell")
     return first.copy()
 
@@ -145,11 +148,21 @@ def _to_gpu(ref, st: HostState, cells) -> None:
     st.cc_cells = np.array(ref.cc_cells, dtype=np.float64)
     st.cc_env = np.array(ref.cc_env, dtype=np.float64)
     st.c_lag = _per_cell(np.asarray(ref.cc_at_mem, dtype=np.float64), mem_ptr, "cc_at_mem")
-    Phi_b = np.asarray(getattr(ref, "Phi_b", np.zeros(1)))
-    if np.any(Phi_b != 0):
-        raise BetseSimException("an applied boundary voltage (Phi_b != 0) is not supported by the GPU step")
-    st.vm_cell = _per_cell(np.asarray(ref.vm, dtype=np.float64), mem_ptr, "vm")
     shp = np.asarray(cells.X).shape
+    Phi_b = np.array(getattr(ref, "Phi_b", np.zeros(int(np.prod(shp)))), dtype=np.float64).ravel()
+    vm = np.asarray(ref.vm, dtype=np.float64)
+    if np.any(Phi_b != 0):
+        # rho_surf/cm = vm + Phi_b per membrane, up to the rounding of the
+        # reference's subtraction (a few ulps of |vm| + |Phi_b|)
+        pb = Phi_b[np.asarray(cells.map_mem2ecm)]
+        tol = 8 * np.finfo(np.float64).eps * float(np.abs(vm).max() + np.abs(pb).max())
+        st.vm_cell = _per_cell(vm + pb, mem_ptr, "vm + Phi_b[map_mem2ecm]", atol=tol)
+    else:
+        st.vm_cell = _per_cell(vm, mem_ptr, "vm")
+    st.Phi_b = Phi_b
+    bound_V = getattr(ref, "bound_V", None)
+    if bound_V is not None:
+        st.bound_V = {k: float(bound_V[k]) for k in ("T", "B", "L", "R")}
     st.E_env_x = np.array(ref.E_env_x, dtype=np.float64).reshape(shp)
     st.E_env_y = np.array(ref.E_env_y, dtype=np.float64).reshape(shp)
     st.v_env = np.array(ref.v_env, dtype=np.float64).ravel()
@@ -166,13 +179,15 @@ def _from_gpu(st: HostState, ref, cells) -> None:
     ref.cc_cells = st.cc_cells.copy()
     ref.cc_env = st.cc_env.copy()
     ref.cc_at_mem = st.c_lag[:, m2c].copy()
-    ref.vm = st.vm_cell[m2c].copy()
+    Phi_b = np.asarray(st.Phi_b, dtype=np.float64)
+    ref.vm = st.vm_cell[m2c] - Phi_b[np.asarray(cells.map_mem2ecm)] if np.any(Phi_b != 0) else st.vm_cell[m2c].copy()
     ref.vm_ave = np.add.reduceat(ref.vm, np.concatenate([[0], np.cumsum(cells.num_mems)[:-1]])) / cells.num_mems
     ref.rho_cells = np.dot(st.zs * st.p.F, st.cc_cells)
     ref.E_env_x = st.E_env_x.copy()
     ref.E_env_y = st.E_env_y.copy()
     ref.v_env = st.v_env.copy()
-    ref.Phi_b = np.zeros(st.cc_env.shape[1])
+    ref.Phi_b = Phi_b.copy()
+    ref.bound_V = dict(st.bound_V)
     for k in STATE_OPTIONAL:
         setattr(ref, k, getattr(st, k).copy())
     if len(getattr(st, "channel_specs", ())):

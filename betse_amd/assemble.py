@@ -118,6 +118,7 @@ def batch_state(states: Sequence, cells_list: Sequence, sim_phase: bool) -> Dict
         "v_env": np.concatenate([st.v_env for st in states]),
         "gates": np.concatenate([st.chan_gates for st in states], axis=2) if nch else np.zeros((1, 2, 1)),
         "chan_mod": np.concatenate([st.chan_mod for st in states], axis=1) if nch else np.ones((1, 1)),
+        "Phi_b": np.concatenate([st.Phi_b for st in states]),
     }
     if states[0].network is not None:
         h["c_net"] = np.concatenate([st.c_net for st in states], axis=1)

@@ -21,6 +21,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _native
+from .boundary import SIDES, boundary_potential
 from .cells import Cells
 from .channels import draw_channel_state
 from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableException
@@ -146,8 +147,26 @@ class BatchSimulator(DeviceFetch):
             self.run_autotune()
         if self.use_graph:
             self._check(self._lib.bt_capture_graphs(self._ctx))
+        self._phib = host["Phi_b"].copy()      # the latest Phi_b staged or in effect
+        self._phib_on = bool(np.any(self._phib != 0))
 
     # ------------------------------------------------------------------
+    def set_boundary_voltage(self, cluster: int, bound_V: dict) -> None:
+        """The applied-voltage event on one cluster of the batch
+        (``Simulator.set_boundary_voltage``): its ``bound_V`` set at the start of
+        the next step, its ``Phi_b`` from that step's end on; the other
+        clusters keep theirs (``bt_stage_boundary`` takes the whole batch's)."""
+        st = self.states[cluster]
+        soff = self.offsets[2]
+        bv = {k: float(bound_V.get(k, st.bound_V[k])) for k in SIDES}
+        phi = boundary_potential(bv, self.worlds[cluster][1].X.shape)
+        nxt = self._phib.copy()
+        nxt[int(soff[cluster]):int(soff[cluster + 1])] = phi
+        self._check(self._lib.bt_stage_boundary(self._ctx, nxt.ctypes.data))
+        self._phib = nxt
+        st.bound_V = bv
+        self._phib_on = self._phib_on or bool(np.any(phi != 0))
+
     def set_membrane_params(self, cluster: int, Dm: Optional[np.ndarray] = None, gjopen: Optional[np.ndarray] = None,
                             D_env: Optional[np.ndarray] = None) -> None:
         """An event on one cluster of the batch (``Simulator.set_membrane_params``):
@@ -235,7 +254,8 @@ class BatchSimulator(DeviceFetch):
         self.stream.synchronize()
         d = self._dev
         coff, moff, soff = self.offsets
-        keys = ("c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "gates") + (("c_net",) if "c_net" in d else ())
+        keys = ("c_cell", "c_lag", "vm_cell", "c_env", "E_x", "E_y", "v_env", "gates", "Phi_b") + (
+            ("c_net",) if "c_net" in d else ())
         h = {k: d[k].cpu().numpy() for k in keys}
         for i, (st, (p, c)) in enumerate(zip(self.states, self.worlds)):
             a, b = coff[i], coff[i + 1]
@@ -247,6 +267,7 @@ class BatchSimulator(DeviceFetch):
             st.E_env_x = h["E_x"][sa:sb].reshape(c.X.shape).copy()
             st.E_env_y = h["E_y"][sa:sb].reshape(c.X.shape).copy()
             st.v_env = h["v_env"][sa:sb].copy()
+            st.Phi_b = h["Phi_b"][sa:sb].copy()
             if len(st.channel_specs):
                 st.chan_gates = h["gates"][:, :, moff[i]:moff[i + 1]].copy()
             if "c_net" in h:
@@ -298,6 +319,8 @@ class BatchSimulator(DeviceFetch):
     def _sample_keys(self):
         d = self._dev
         state_keys = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"] + (["c_net"] if "c_net" in d else [])
+        if self.__dict__.get("_phib_on"):
+            state_keys.append("Phi_b")
         return state_keys, list(HostState._OBS_KEEP)
 
     def prepare_sampling(self) -> None:

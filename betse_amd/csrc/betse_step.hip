@@ -117,6 +117,9 @@ struct KCluster {
     // every membrane and no pump or channel carrying it), so its membrane
     // flux rows are exact zeros that need neither writing nor summing
     unsigned still_mem;
+    // nonzero where an applied boundary voltage (Phi_b) reaches the cluster's
+    // squares: membranes then take their Vmem less Phi_b, v_env adds it
+    int phib;
     // likewise D_env per ion over the cluster's squares: k_env_field then
     // stages the constant instead of loading the array
     double denv_u[BT_MAX_IONS];
@@ -382,6 +385,7 @@ struct MemCellArgs {
     unsigned* flags; unsigned long long* bad_step; const long long* step_ctr;
     // observed step only (NULL otherwise): raw fluxes in membrane order
     double* obs_fm; double* obs_fgj; double* obs_fch; double* obs_rate;
+    const double* Phi_b = nullptr;   // applied boundary voltage per square (clusters with phib)
 };
 
 // LDS of one k_mem_cell block, shared by its wavefronts
@@ -437,8 +441,14 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         const int pos = A.mem_pos[m];
         const double sa = A.mem_sa[m];
         const double gjo = (gjo_u == gjo_u) ? gjo_u : A.gjopen[m];
-        const double v = A.vm_in[mc];
-        const double vn = A.vm_in[cn];
+        // Vmem of the membrane and of its partner: (1/cm) rho_surf of the cell
+        // less Phi_b at the membrane's square (update_V, sim.py:2029)
+        double v = A.vm_in[mc];
+        double vn = A.vm_in[cn];
+        if (KC->phib) {
+            v = v - A.Phi_b[k];
+            vn = vn - A.Phi_b[A.mem_sq[A.nn_mem[m]]];
+        }
         // the gathers of one ion slot / channel slot, issued together
         double g_env[NS], g_lag[NS], g_lagn[NS], g_dm[NS];
         auto gather_ion = [&](int s) {
@@ -702,6 +712,7 @@ struct FieldArgs {
     // c_env_out; squares with membranes go to c_mid for k_env_exchange
     const uint8_t* live;      // per square: 1 if k_env_exchange updates it
     double* c_env_out;        // the step's env state buffer (never the one read here)
+    const double* Phi_b = nullptr;   // applied boundary voltage per square (clusters with phib)
 };
 
 // RATES: also the gating rates (diffuse mode with channels). Divisions by the
@@ -874,12 +885,16 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
         sT[e / UC][lc] = o;
     }
     lds_barrier();
-    // axis-1 pass -> v_env (+ Phi_b == 0)
+    // axis-1 pass -> v_env (+ Phi_b, ion_current.py:104)
     for (int e = tid; e < VR * VC; e += TY * TX) {
         const int lr = e / VC, lc = e % VC + 4;          // column in sT
         double o = sT[lr][lc] * P.gw[0];
 #pragma unroll
         for (int q = 4; q >= 1; --q) o = o + (sT[lr][lc - q] + sT[lr][lc + q]) * P.gw[q];
+        if (K.phib) {
+            const int r = r0 - 2 + lr, cc = cc0 - 2 + e % VC;
+            if (r >= 0 && r < ny && cc >= 0 && cc < nx) o = o + A.Phi_b[so + r * nx + cc];
+        }
         sV[lr][e % VC] = o;
     }
     lds_barrier();
@@ -1348,6 +1363,9 @@ struct ObsMemArgs {
     double* Jn; double* I_mem; double* E_gj_x; double* E_gj_y;
     double* J_cell_x; double* J_cell_y; double* vm_ave;
     double smooth_cells;
+    // applied boundary voltage per square (NULL: none) and map_mem2ecm: the
+    // membranes' Vmem is their cell's less Phi_b at their square
+    const double* Phi_b; const double* Phi_b_gj; const int32_t* mem_sq;
 };
 
 // raw normal current of membrane m: -sum(zF*f_mem) + extra_J_mem (channels) +
@@ -1376,6 +1394,7 @@ __global__ void __launch_bounds__(TMB)
 k_obs_mem(ObsMemArgs A, KParams P) {
     __shared__ double s_j[TMB], s_sa[TMB], s_jx[TMB], s_jy[TMB];
     __shared__ double s_ave[TMB], s_wm[TMB], s_wo[TMB];      // per cell of the tile
+    __shared__ double s_vm[TMB];                              // Vmem of each membrane
     __shared__ int s_lc[TMB];                                 // cell of each membrane, in the tile
     const int tile = blockIdx.x;
     const int c0 = A.tile_ptr[tile], c1 = A.tile_ptr[tile + 1];
@@ -1409,19 +1428,27 @@ k_obs_mem(ObsMemArgs A, KParams P) {
         A.I_mem[m] = -jn * s_sa[j];
         s_jx[j] = (jn * A.mem_nx[m]) * s_sa[j];
         s_jy[j] = (jn * A.mem_ny[m]) * s_sa[j];
-        const double vgj = A.vm_gj[A.mem_cell[A.nn_mem[m]]] - A.vm_gj[c0 + lc];
+        double vgj;
+        if (A.Phi_b) {
+            const int nn = A.nn_mem[m];
+            vgj = (A.vm_gj[A.mem_cell[nn]] - A.Phi_b_gj[A.mem_sq[nn]]) -
+                  (A.vm_gj[c0 + lc] - A.Phi_b_gj[A.mem_sq[m]]);
+            s_vm[j] = A.vm_cell[c0 + lc] - A.Phi_b[A.mem_sq[m]];
+        } else {
+            vgj = A.vm_gj[A.mem_cell[A.nn_mem[m]]] - A.vm_gj[c0 + lc];
+            s_vm[j] = A.vm_cell[c0 + lc];
+        }
         const double egj = -vgj / P.gj_len;
         A.E_gj_x[m] = egj * A.mem_nx[m];
         A.E_gj_y[m] = egj * A.mem_ny[m];
     }
     lds_barrier();
     if (c < c1) {
-        const double v = A.vm_cell[c];
-        double sx = s_jx[ca], sy = s_jy[ca], svm = v;
+        double sx = s_jx[ca], sy = s_jy[ca], svm = s_vm[ca];
         for (int q = ca + 1; q < cb; ++q) {
             sx = sx + s_jx[q];
             sy = sy + s_jy[q];
-            svm = svm + v;
+            svm = svm + s_vm[q];
         }
         A.J_cell_x[c] = sx / A.cell_sa[c];
         A.J_cell_y[c] = sy / A.cell_sa[c];
@@ -1517,7 +1544,7 @@ k_obs_rhs(const double* jxo, const double* jyo, double* Bw, const ObsQ* Q, int n
 // right-hand sides are zero without an applied voltage).
 __global__ void __launch_bounds__(BLOCK)
 k_obs_hh_out(const double* Xw, const ObsQ* Q, const int32_t* sq_cluster, const KCluster* kcl, double* Jx,
-             double* Jy, double* Jtx, double* Jty, double* Bf, double mu, int n_sq) {
+             double* Jy, double* Jtx, double* Jty, double* Bf, double mu, int n_sq, const double* Phi_b) {
     const int k = blockIdx.x * BLOCK + threadIdx.x;
     if (k >= n_sq) return;
     const int q = sq_cluster[k];
@@ -1529,17 +1556,22 @@ k_obs_hh_out(const double* Xw, const ObsQ* Q, const int32_t* sq_cluster, const K
     auto F = [&](const double* X, int rr, int cc) {
         return (rr == 0 || cc == 0 || rr == ny - 1 || cc == nx - 1) ? 0.0 : X[(rr - 1) * g.n2p + (cc - 1)];
     };
-    auto grad = [&](const double* X, double& gx, double& gy) {
-        if (c == 0) gx = (F(X, r, 1) - F(X, r, 0)) / K.delta;
-        else if (c == nx - 1) gx = (F(X, r, nx - 1) - F(X, r, nx - 2)) / K.delta;
-        else gx = -(F(X, r, c - 1) - F(X, r, c + 1)) / K.two_delta;
-        if (r == 0) gy = (F(X, 1, c) - F(X, 0, c)) / K.delta;
-        else if (r == ny - 1) gy = (F(X, ny - 1, c) - F(X, ny - 2, c)) / K.delta;
-        else gy = -(F(X, r - 1, c) - F(X, r + 1, c)) / K.two_delta;
+    // the potential with an applied boundary voltage: the boundary rows of its
+    // solve hold -bound_V, so it is the zero-boundary solution less Phi_b (the
+    // harmonic extension of bound_V with the same corner rule)
+    const double* Pb = (Phi_b && K.phib) ? Phi_b + K.sq_off : nullptr;
+    auto FB = [&](int rr, int cc) { return Pb ? F(BB, rr, cc) - Pb[rr * nx + cc] : F(BB, rr, cc); };
+    auto grad = [&](auto G, double& gx, double& gy) {
+        if (c == 0) gx = (G(r, 1) - G(r, 0)) / K.delta;
+        else if (c == nx - 1) gx = (G(r, nx - 1) - G(r, nx - 2)) / K.delta;
+        else gx = -(G(r, c - 1) - G(r, c + 1)) / K.two_delta;
+        if (r == 0) gy = (G(1, c) - G(0, c)) / K.delta;
+        else if (r == ny - 1) gy = (G(ny - 1, c) - G(ny - 2, c)) / K.delta;
+        else gy = -(G(r - 1, c) - G(r + 1, c)) / K.two_delta;
     };
     double gAx, gAy, Gx, Gy;
-    grad(AA, gAx, gAy);
-    grad(BB, Gx, Gy);
+    grad([&](int rr, int cc) { return F(AA, rr, cc); }, gAx, gAy);
+    grad(FB, Gx, Gy);
     const double Fx = -gAy, Fy = gAx;
     Jx[k] = Fx;
     Jy[k] = Fy;
@@ -1808,6 +1840,7 @@ struct bt_ctx {
     // order); the groups' streams ([0] is the caller's) and fork/join events
     std::vector<int> cl_tile0, cl_env0, cl_live0;
     std::vector<int> cl_mem_lo, cl_mem_hi;   // each cluster's membrane range
+    bool any_phib = false;                   // some cluster has an applied boundary voltage
     std::vector<long long> cl_cell0;
     hipStream_t gstream[BT_MAX_STREAMS] = {};
     hipEvent_t ev_fork = nullptr;
@@ -1830,6 +1863,11 @@ struct bt_ctx {
     GemmProb* gemm_probs = nullptr; int32_t* block_prob = nullptr; int n_prob = 0; int n_gemm_blocks = 0;
     // Vmem per cell before the last observed step: update_gj's E_gj uses it (sim.py:2166-2172)
     double* vm_prev = nullptr; bool vm_prev_ok = false;
+    // applied boundary voltage: the Phi_b staged by bt_stage_boundary for the
+    // end of the next step, and the Phi_b in effect during the last observed
+    // step (its E_gj takes Vmem from before the step)
+    double* phib_stage = nullptr; bool phib_staged = false;
+    double* phib_prev = nullptr;
     // reaction network (device copy of the description) and its state buffer
     KNet* net = nullptr; double* c_net = nullptr; int net_species = 0;
     std::string err;
@@ -1995,7 +2033,7 @@ static FieldArgs field_args(bt_ctx* c, bool diffuse, const bt_obs* ob, const dou
     const bt_buffers& B = c->buf;
     return FieldArgs{c->env_tiles, c->kcl, c->u, env_in ? env_in : B.c_env, B.D_env, c->c_mid, B.v_env, B.E_x, B.E_y,
                      c->flags, ob ? ob->fluxes_env_x : nullptr, ob ? ob->fluxes_env_y : nullptr,
-                     vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr, c->live_flag, env_out};
+                     vm, (diffuse && c->prm.n_chan > 0) ? c->rates : nullptr, c->live_flag, env_out, B.Phi_b};
 }
 
 // k_env_field over ne env tiles (f.env_tiles starts at the first; ne = 0: all)
@@ -2044,7 +2082,7 @@ static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_ste
                            ci, li, vi, co, lo, vo, c->fsa, B.gates, B.chan_mod, c->rates, c->fch,
                            c->flags, c->bad_step, c->step_ctr,
                            ob ? ob->fluxes_mem : nullptr, ob ? ob->fluxes_gj : nullptr,
-                           ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr},
+                           ob ? ob->chan_flux : nullptr, ob ? ob->rate_NaKATP : nullptr, B.Phi_b},
                ExArgs{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u,
                       c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0, c->flags,
                       c->step_ctr, count_step, (1u << BT_MAX_STREAMS) - 1u, c->sq_live, c->n_live}};
@@ -2353,7 +2391,7 @@ __global__ void k_fp32_math(const float* x, const float* y, float* out, int n, i
 
 extern "C" {
 
-int bt_version(void) { return 200; }
+int bt_version(void) { return 300; }
 
 const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
@@ -2368,10 +2406,14 @@ static int derive_uniform(bt_ctx* c) {
     const bt_params* p = &c->prm;
     const int nq = (int)c->clusters.size();
     std::vector<double> dm((size_t)p->n_ions * p->n_mems), gj((size_t)p->n_mems), de((size_t)p->n_ions * p->n_squares);
+    std::vector<double> pb(c->buf.Phi_b ? (size_t)p->n_squares : 0);
     hipError_t e = hipMemcpy(dm.data(), c->buf.Dm, dm.size() * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(gj.data(), c->buf.gjopen, gj.size() * 8, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(de.data(), c->buf.D_env, de.size() * 8, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return set_err(c, BT_EHIP, std::string("reading Dm, gjopen, D_env: ") + hipGetErrorString(e));
+    if (e == hipSuccess && !pb.empty()) e = hipMemcpy(pb.data(), c->buf.Phi_b, pb.size() * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess)
+        return set_err(c, BT_EHIP, std::string("reading Dm, gjopen, D_env, Phi_b: ") + hipGetErrorString(e));
+    c->any_phib = false;
     for (int q = 0; q < nq; ++q) {
         KCluster& K = c->kcl_host[q];
         const int lo = c->cl_mem_lo[q], hi = c->cl_mem_hi[q];
@@ -2400,6 +2442,12 @@ static int derive_uniform(bt_ctx* c) {
             for (size_t k = k0; k < k1 && same; ++k) same = (d[k] == d[k0]);
             K.denv_u[i] = same ? d[k0] : NAN;
         }
+        K.phib = 0;
+        for (size_t k = k0; k < k1 && !pb.empty() && !K.phib; ++k) K.phib = (pb[k] != 0.0);
+        if (K.phib && p->n_chan > 0)
+            return set_err(c, BT_EINVAL, "an applied boundary voltage (Phi_b) with voltage-gated channels is not "
+                                         "supported (their gating rates are per cell)");
+        c->any_phib = c->any_phib || K.phib;
     }
     return BT_OK;
 }
@@ -2595,7 +2643,7 @@ int bt_destroy(bt_ctx* c) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
                     c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles,
-                    c->vm_prev, c->net, c->bar};
+                    c->vm_prev, c->net, c->bar, c->phib_stage, c->phib_prev};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (void* q : c->obs_alloc) (void)hipFree(q);
@@ -2616,10 +2664,51 @@ int bt_update_v(bt_ctx* c) {
     return BT_OK;
 }
 
+// A staged Phi_b becomes the one in effect (update_V's Phi_b at the end of the
+// step the change fires in), and the per-cluster flags follow it.
+static int apply_staged_boundary(bt_ctx* c) {
+    HIP_TRY(c, hipMemcpyAsync(c->buf.Phi_b, c->phib_stage, (size_t)c->prm.n_squares * 8, hipMemcpyDeviceToDevice,
+                              c->stream));
+    c->phib_staged = false;
+    return bt_refresh_params(c);
+}
+
+int bt_stage_boundary(bt_ctx* c, const double* Phi_b_next) {
+    if (!c || !Phi_b_next) return BT_EINVAL;
+    if (!c->buf.Phi_b) return set_err(c, BT_EINVAL, "the context was created without a Phi_b buffer");
+    const size_t n = (size_t)c->prm.n_squares;
+    for (size_t k = 0; k < n; ++k) {
+        if (!std::isfinite(Phi_b_next[k])) return set_err(c, BT_EINVAL, "Phi_b is not finite");
+        if (c->prm.n_chan > 0 && Phi_b_next[k] != 0.0)
+            return set_err(c, BT_EINVAL, "an applied boundary voltage (Phi_b) with voltage-gated channels is not "
+                                         "supported (their gating rates are per cell)");
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!c->phib_stage) HIP_TRY(c, hipMalloc((void**)&c->phib_stage, n * 8));
+    HIP_TRY(c, hipMemcpy(c->phib_stage, Phi_b_next, n * 8, hipMemcpyHostToDevice));
+    c->phib_staged = true;
+    return BT_OK;
+}
+
 int bt_step(bt_ctx* c, int32_t n_steps, int32_t use_graph) {
     if (!c) return BT_EINVAL;
     if (n_steps < 0) return set_err(c, BT_EINVAL, "n_steps < 0");
     if (n_steps == 0) return BT_OK;
+    if (c->phib_staged) {
+        // the step the change fires in runs with the Phi_b in effect; the
+        // staged one holds from its end on
+        enqueue_step(c, 0);
+        int rc = copy_back(c);
+        if (rc == BT_OK) rc = apply_staged_boundary(c);
+        if (rc != BT_OK) return rc;
+        c->steps_done += 1;
+        if (n_steps == 1) {
+            enqueue_step(c, 0, 3);
+            HIP_TRY(c, hipGetLastError());
+            return BT_OK;
+        }
+        return bt_step(c, n_steps - 1, use_graph);
+    }
     int pairs = n_steps / 2;
     if (c->tune.persistent && pairs > 0) {
         // every pair of steps in cooperative launches of the persistent kernel
@@ -2761,6 +2850,8 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
     unsigned flags = 0;
     HIP_TRY(c, hipMemcpy(&flags, c->flags, sizeof(flags), hipMemcpyDeviceToHost));
     if (flags) return set_err(c, BT_EINVAL, "autotune needs a stable state (a NaN was flagged)");
+    if (c->phib_staged)
+        return set_err(c, BT_EINVAL, "autotune with a staged boundary change: tune before staging it");
     // every timed launch starts from the state as it is now: keep a copy of all
     // that a step modifies in place and put it back after each trial
     const auto regions = state_regions(c);
@@ -2976,6 +3067,13 @@ int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs
         HIP_TRY(c, hipMemsetAsync(obs->rate_NaKATP, 0, (size_t)c->prm.n_mems * 8, c->stream));
     const size_t vbytes = (size_t)c->prm.n_cells * 8;
     if (!c->vm_prev) HIP_TRY(c, hipMalloc((void**)&c->vm_prev, vbytes));
+    const size_t pbytes = (size_t)c->prm.n_squares * 8;
+    if (c->buf.Phi_b && !c->phib_prev) HIP_TRY(c, hipMalloc((void**)&c->phib_prev, pbytes));
+    if (c->phib_staged && n_steps > 1) {
+        int rc = bt_step(c, 1, use_graph);       // the step the change fires in
+        if (rc != BT_OK) return rc;
+        --n_steps;
+    }
     const int ahead = n_steps - 1;
     if (ahead > 0) {
         // keep the graph on the caller buffers: an even number of graph steps, then eager ones
@@ -2986,15 +3084,20 @@ int bt_step_observed(bt_ctx* c, int32_t n_steps, int32_t use_graph, const bt_obs
     if (rest == 2) {
         enqueue_step(c, 0);
         HIP_TRY(c, hipMemcpyAsync(c->vm_prev, c->vm_b, vbytes, hipMemcpyDeviceToDevice, c->stream));
+        if (c->phib_prev)
+            HIP_TRY(c, hipMemcpyAsync(c->phib_prev, c->buf.Phi_b, pbytes, hipMemcpyDeviceToDevice, c->stream));
         c->capture = obs;
         enqueue_step(c, 1);
         c->capture = nullptr;
     } else {
         HIP_TRY(c, hipMemcpyAsync(c->vm_prev, c->buf.vm_cell, vbytes, hipMemcpyDeviceToDevice, c->stream));
+        if (c->phib_prev)
+            HIP_TRY(c, hipMemcpyAsync(c->phib_prev, c->buf.Phi_b, pbytes, hipMemcpyDeviceToDevice, c->stream));
         c->capture = obs;
         enqueue_step(c, 0);
         c->capture = nullptr;
         int rc = copy_back(c);
+        if (rc == BT_OK && c->phib_staged) rc = apply_staged_boundary(c);
         if (rc != BT_OK) return rc;
     }
     enqueue_step(c, 0, 3);
@@ -3138,7 +3241,8 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
     ObsMemArgs a{B.mem_ptr, B.nn_mem, B.mem_sa, B.mem_nx, B.mem_ny, B.cell_sa, B.vm_cell,
                  c->vm_prev_ok ? c->vm_prev : B.vm_cell, B.mem_cell, B.tile_ptr,
                  o->fluxes_mem, o->fluxes_gj, o->chan_flux, o->Jn, o->I_mem, o->E_gj_x, o->E_gj_y,
-                 o->J_cell_x, o->J_cell_y, o->vm_ave, c->prm.smooth_cells};
+                 o->J_cell_x, o->J_cell_y, o->vm_ave, c->prm.smooth_cells,
+                 B.Phi_b, (c->vm_prev_ok && c->phib_prev) ? c->phib_prev : B.Phi_b, B.mem_sq};
     hipLaunchKernelGGL(k_obs_mem, dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
     hipLaunchKernelGGL(k_obs_env_rhs, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
                        (const double*)o->fluxes_env_x, (const double*)o->fluxes_env_y, c->jxo, c->jyo, c->kp);
@@ -3154,7 +3258,8 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
                            (const GemmProb*)(c->gemm_probs + (size_t)st * c->n_prob), (const int32_t*)c->block_prob);
     hipLaunchKernelGGL(k_obs_hh_out, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream, (const double*)c->T2w,
                        (const ObsQ*)c->obs_q, (const int32_t*)B.sq_cluster, (const KCluster*)c->kcl, o->J_env_x,
-                       o->J_env_y, o->Jtx, o->Jty, o->B_field, c->prm.mu, c->kp.n_sq);
+                       o->J_env_y, o->Jtx, o->Jty, o->B_field, c->prm.mu, c->kp.n_sq,
+                       c->any_phib ? B.Phi_b : nullptr);
     HIP_TRY(c, hipGetLastError());
     return BT_OK;
 }

@@ -28,6 +28,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import _native
+from .boundary import SIDES, boundary_potential
 from .cells import Cells
 from .channels import CHANNEL_KINDS, draw_channel_state, init_gates
 from .exceptions import BetseNativeError, BetseSimException, BetseSimUnstableException
@@ -296,6 +297,8 @@ class HostState:
                 self.chan_gates = h["gates"]
         if self.network is not None:
             self.c_net = h["c_net"]
+        if "Phi_b" in h:
+            self.Phi_b = h["Phi_b"]
 
     # the observables write2storage keeps; the raw fluxes stay on the device (pull_raw_fluxes)
     _OBS_KEEP = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "J_env_x",
@@ -554,10 +557,24 @@ class Simulator(HostState, DeviceFetch):
             self.set_tuning(self.tuning)
         if self.use_graph:
             self._check(self._lib.bt_capture_graphs(self._ctx))
+        self._phib_on = bool(np.any(self.Phi_b != 0))
 
     # ------------------------------------------------------------------
     # events
     # ------------------------------------------------------------------
+    def set_boundary_voltage(self, bound_V: dict) -> None:
+        """The applied-voltage event (``TissueEventVoltage.fire``,
+        ``tisevevolt.py:75-89``): ``bound_V`` (keys ``T``, ``B``, ``L``, ``R``;
+        missing ones keep their value) set at the start of the next step. That
+        step still runs with the ``Phi_b`` in effect; ``get_current`` derives
+        the new ``Phi_b`` at its end (``boundary_potential``), which then holds
+        (``bt_stage_boundary``)."""
+        bv = {k: float(bound_V.get(k, self.bound_V[k])) for k in SIDES}
+        phi = boundary_potential(bv, self.cells.X.shape)
+        self._check(self._lib.bt_stage_boundary(self._ctx, phi.ctypes.data))
+        self.bound_V = bv
+        self._phib_on = self._phib_on or bool(np.any(phi != 0))
+
     def set_membrane_params(self, Dm: Optional[np.ndarray] = None, gjopen: Optional[np.ndarray] = None,
                             D_env: Optional[np.ndarray] = None) -> None:
         """An event's new membrane diffusivities (``Dm_cells``, [n_ions][M]),
@@ -625,6 +642,8 @@ class Simulator(HostState, DeviceFetch):
 
     def _state_names(self, full: bool):
         names = ["c_cell", "vm_cell", "c_env", "E_x", "E_y", "v_env"]
+        if self.__dict__.get("_phib_on"):
+            names.append("Phi_b")
         if full:
             names.append("c_lag")
             if len(self.channel_specs):
@@ -803,7 +822,8 @@ class Simulator(HostState, DeviceFetch):
         """Phase-end state (stand-in for ``_pickle_phase``, ``sim.py:1888-1950``)."""
         np.savez_compressed(path, cc_cells=self.cc_cells, c_lag=self.c_lag, vm_cell=self.vm_cell,
                             cc_env=self.cc_env, E_env_x=self.E_env_x, E_env_y=self.E_env_y,
-                            v_env=self.v_env, names=np.asarray(self.names))
+                            v_env=self.v_env, names=np.asarray(self.names), Phi_b=self.Phi_b,
+                            bound_V=np.asarray([self.bound_V[k] for k in SIDES], dtype=np.float64))
 
     def load(self, path: str) -> None:
         with np.load(path, allow_pickle=False) as z:
@@ -814,3 +834,6 @@ class Simulator(HostState, DeviceFetch):
             self.E_env_x = z["E_env_x"]
             self.E_env_y = z["E_env_y"]
             self.v_env = z["v_env"]
+            if "Phi_b" in z:
+                self.Phi_b = z["Phi_b"]
+                self.bound_V = dict(zip(SIDES, (float(v) for v in z["bound_V"])))

@@ -116,7 +116,8 @@ typedef struct {
     /* state (read/write, in place between calls) */
     double* c_cell;             /* [n_ions][n_cells] cc_cells                     */
     double* c_lag;              /* [n_ions][n_cells] cc_at_mem per cell (Q1)      */
-    double* vm_cell;            /* [n_cells] Vmem of each cell (Phi_b == 0)       */
+    double* vm_cell;            /* [n_cells] (1/cm) rho_surf of each cell: the Vmem
+                                   of its membranes before - Phi_b[map_mem2ecm]  */
     double* c_env;              /* [n_ions][ny*nx] cc_env                         */
     double* E_x;                /* [ny*nx] E_env_x                                */
     double* E_y;                /* [ny*nx] E_env_y                                */
@@ -128,6 +129,11 @@ typedef struct {
     /* voltage-gated channels (may be NULL when n_chan == 0) */
     double* gates;              /* [n_chan][2][n_mems] m, h gates (in place)      */
     const double* chan_mod;     /* [n_chan][n_cells] channel modulator per cell   */
+    /* applied boundary voltage (bound_V events; ion_current.py:79-90): Phi_b
+       = lapENVinv . (-div_Jb) per square, subtracted from each membrane's Vmem
+       and added to v_env; NULL or all zeros: none. Not with channels. State:
+       bt_stage_boundary writes a staged change into it. */
+    double* Phi_b;              /* [ny*nx]                                        */
     int32_t n_tiles;
     int32_t tile_m;             /* must equal BT_TILE_MEMBRANES                   */
 } bt_buffers;
@@ -236,13 +242,28 @@ typedef struct {
 /* Events that change the membranes' or the environment's diffusivities
  * (TissueHandler.fire_events -> makeAllChanges, tishandler.py:709-724 and
  * 1321-1332: Dm_cells = Dm_scheduled + Dm_base; gap-junction blocks through
- * gjopen; D_env): after the caller has written new values into the Dm, gjopen
- * or D_env buffers bound at bt_create, this re-derives the per-cluster
+ * gjopen; D_env): after the caller has written new values into the Dm, gjopen,
+ * D_env (or Phi_b) buffers bound at bt_create, this re-derives the per-cluster
  * constants bt_create took from them (values that are one over a cluster, ions
- * without membrane flux), so the next steps use the new values; captured
- * graphs stay valid. An applied boundary voltage (bound_V, Phi_b) is not
- * supported on this path. Synchronous. */
+ * without membrane flux, clusters with an applied voltage), so the next steps
+ * use the new values; captured graphs stay valid. BT_EINVAL for a nonzero
+ * Phi_b with voltage-gated channels. Synchronous. */
 int bt_refresh_params(bt_ctx* ctx);
+
+/* The applied boundary voltage event (TissueEventVoltage.fire,
+ * tisevevolt.py:75-89, sets sim.bound_V at the start of a step; get_current,
+ * ion_current.py:84-107, turns it into Phi_b at that step's end): Phi_b_next
+ * is the new potential per square (host memory, n_squares values: the harmonic
+ * extension of bound_V that get_current solves for, zero where a cluster has
+ * none). The next step still runs with the Phi_b in effect; from its end on,
+ * Phi_b_next is copied into the Phi_b buffer bound at bt_create and the
+ * per-cluster constants follow (as bt_refresh_params): Vmem per membrane is
+ * its cell's less Phi_b at its square, v_env adds Phi_b, and the observables of
+ * that step take E_gj from the Vmem before it and the rest from after it, as
+ * the reference does. BT_EINVAL without a Phi_b buffer, for a non-finite value,
+ * or for a nonzero value with voltage-gated channels (their gating rates are
+ * per cell). Synchronous; bt_autotune refuses while a change is staged. */
+int bt_stage_boundary(bt_ctx* ctx, const double* Phi_b_next);
 
 /* Capture and instantiate the step graphs now (bt_step otherwise does it on
  * its first call that replays graphs), so that no later bt_step pays for it.

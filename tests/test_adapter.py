@@ -94,7 +94,22 @@ def test_state_not_constant_over_a_cell_raises(world):
                           vm=vm, E_env_x=np.zeros(G), E_env_y=np.zeros(G), v_env=np.zeros(G), Phi_b=np.zeros(G))
     with pytest.raises(BetseSimException, match="vm differs"):
         copy_state(ref, st, cells)
-    ref.vm = np.zeros(cells.n_mems)
-    ref.Phi_b = np.full(G, 1e-3)
-    with pytest.raises(BetseSimException, match="Phi_b"):
+    # with an applied boundary potential Vmem differs within a cell by Phi_b at
+    # each membrane's square, and that is what the per-cell state gives back
+    rng = np.random.RandomState(3)
+    ref.Phi_b = rng.uniform(-1e-3, 1e-3, G)
+    ref.bound_V = {"T": 1e-3, "B": -1e-3, "L": 0.0, "R": 0.0}
+    rho = rng.randn(cells.n_cells) * 0.01
+    ref.vm = rho[cells.mem_to_cells] - ref.Phi_b[cells.map_mem2ecm]
+    copy_state(ref, st, cells)
+    np.testing.assert_allclose(st.vm_cell, rho, rtol=0, atol=1e-17)
+    np.testing.assert_array_equal(st.Phi_b, ref.Phi_b)
+    back = SimpleNamespace()
+    copy_state(st, back, cells)
+    np.testing.assert_allclose(back.vm, ref.vm, rtol=0, atol=1e-17)
+    np.testing.assert_array_equal(back.Phi_b, ref.Phi_b)
+    assert back.bound_V == ref.bound_V
+    np.testing.assert_allclose(st.vm, ref.vm, rtol=0, atol=1e-17)
+    ref.vm[0] += 1e-3                                   # not rho - Phi_b: no per-cell value
+    with pytest.raises(BetseSimException, match="differs"):
         copy_state(ref, st, cells)
