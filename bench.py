@@ -81,7 +81,15 @@ def workload_b_step(worlds) -> dict:
                   n_species=net.n_species if net is not None else 0)
 
 
-def measured_traffic(workload: str, flux_mode: str):
+def step_kernel_names(tuning: dict, network: bool):
+    """The kernels of one step under a tuning: the env kernel does the exchange
+    with exchange_form 2 (k_mem_cell runs first), else a kernel of its own."""
+    names = ("mem_cell", "env_field_exchange") if tuning.get("exchange_form") == 2 else \
+        ("mem_cell", "env_field", "env_exchange")
+    return names + (("network",) if network else ())
+
+
+def measured_traffic(workload: str, flux_mode: str, kernels=("mem_cell", "env_field", "env_exchange")):
     """HBM bytes per step of the step kernels from the committed rocprofv3 PMC
     summary (profiles/hbm_traffic_<workload>.txt: FETCH_SIZE and WRITE_SIZE per
     dispatch, scaled by the plain-copy calibration), or None when a kernel of
@@ -95,9 +103,10 @@ def measured_traffic(workload: str, flux_mode: str):
             t = line.split()
             if len(t) >= 6 and t[0] == "kernel" and t[3] == flux_mode:
                 per[t[1]] = float(t[5])
-    need = {"mem_cell", "env_field", "env_exchange"}
+    need = set(kernels) - {"network"}
     if not need <= set(per):
         return None
+    per = {k: v for k, v in per.items() if k in set(kernels)}
     return {"bytes": sum(per.values()), "kernels": per}
 
 
@@ -301,7 +310,7 @@ def time_loop(worlds, mode, steps, warmup):
     p1.record(b.stream)
     b.sync()
     pure_us = p0.elapsed_time(p1) / n_pure * 1e3
-    names = ("mem_cell", "env_field", "env_exchange") + (("network",) if "c_net" in b._dev else ())
+    names = step_kernel_names(b.get_tuning(), "c_net" in b._dev)
     kernels = {k: b.time_kernel(k, 20) * 1e3 for k in names}
     res = {"wall_s": wall, "events_ms": ev_ms, "sampled_steps": n_samp, "kernels_us": kernels,
            "step_us": pure_us, "tuning": b.get_tuning(), "autotune": b.autotune_report}
@@ -559,7 +568,7 @@ def main(argv=None):
     bs = workload_b_step(worlds)
     ev_step_us = r["step_us"]
     achieved = bs["bytes"] / (ev_step_us * 1e-6) / 1e9
-    traffic = measured_traffic(a.config, head)
+    traffic = measured_traffic(a.config, head, tuple(r["kernels_us"]))
     kern = r["kernels_us"]
     dom = max(kern, key=kern.get)
     line = {

@@ -28,7 +28,9 @@ BT_FLUX_FP32 = 1
 BT_TILE_MEMBRANES = 64
 BT_TILE_CELLS = 32
 
-KERNEL_IDS = {"mem_cell": 0, "env_field": 1, "env_exchange": 2, "env_field_only": 3, "network": 4}
+# bt_time_kernel ids (env_field_exchange: the env kernel of exchange_form 2, which does the exchange)
+KERNEL_IDS = {"mem_cell": 0, "env_field": 1, "env_exchange": 2, "env_field_only": 3, "network": 4,
+              "env_field_exchange": 1}
 
 BT_NET_MAX_SPECIES = 16
 BT_NET_MAX_RATES = 32
@@ -134,9 +136,10 @@ class BtTuning(ctypes.Structure):
 AUTOTUNE_TRIALS = ("mem_waves=1", "mem_waves=2", "env_ions_per_block=1", "env_ions_per_block=2",
                    "env_ions_per_block=3", "env_ions_per_block=n_ions", "exchange_form=0", "exchange_form=1",
                    "persistent=0 (ms per step)", "persistent=1 (ms per step)", "streams=2 (ms per step)",
-                   "streams=4 (ms per step)")
+                   "streams=4 (ms per step)", "exchange_form=2 (ms per step)")
 
 BT_MAX_STREAMS = 4
+BT_EXCHANGE_FUSED = 2
 
 
 # symbol -> (restype, argtypes); every function include/betse_amd.h declares

@@ -115,7 +115,8 @@ struct KCluster {
     double gjo_u;
     // bit i: ion i has no membrane flux anywhere in the cluster (Dm zero on
     // every membrane and no pump or channel carrying it), so its membrane
-    // flux rows are exact zeros that need neither writing nor summing
+    // flux rows are exact zeros that need neither writing nor summing;
+    // bit PHIB_BIT: phib below is nonzero (read with the ion bits, one load)
     unsigned still_mem;
     // nonzero where an applied boundary voltage (Phi_b) reaches the cluster's
     // squares: membranes then take their Vmem less Phi_b, v_env adds it
@@ -124,6 +125,9 @@ struct KCluster {
     // stages the constant instead of loading the array
     double denv_u[BT_MAX_IONS];
 };
+
+// KCluster::still_mem bit of clusters with an applied boundary voltage (above every ion's bit)
+constexpr unsigned PHIB_BIT = 1u << 31;
 
 // x clamped into [0, n - 1]
 __device__ __forceinline__ int clampi(int x, int n) { return min(max(x, 0), n - 1); }
@@ -445,7 +449,7 @@ __device__ __forceinline__ void mem_cell_wave(const MemCellArgs& A, const KParam
         // less Phi_b at the membrane's square (update_V, sim.py:2029)
         double v = A.vm_in[mc];
         double vn = A.vm_in[cn];
-        if (KC->phib) {
+        if (still_mem & PHIB_BIT) {
             v = v - A.Phi_b[k];
             vn = vn - A.Phi_b[A.mem_sq[A.nn_mem[m]]];
         }
@@ -713,6 +717,16 @@ struct FieldArgs {
     const uint8_t* live;      // per square: 1 if k_env_exchange updates it
     double* c_env_out;        // the step's env state buffer (never the one read here)
     const double* Phi_b = nullptr;   // applied boundary voltage per square (clusters with phib)
+    // the exchange done by this kernel (k_mem_cell ran first: no ion's
+    // membrane law reads post-diffusion env): each square with membranes adds
+    // its membranes' fluxes (square-sorted rows of fsa over [sq_ptr[k],
+    // sq_ptr[k+1])) to its diffused value, and the surface charge of the
+    // step's env state goes to u_out (not u, which neighbouring tiles still
+    // read); the block of tile 0 advances the step counters of ctr_mask
+    const int32_t* sq_ptr = nullptr; const uint8_t* has_mem = nullptr; const void* fsa = nullptr;
+    int fsa32 = 0;
+    double* u_out = nullptr;
+    long long* step_ctr = nullptr; unsigned ctr_mask = 0; int count_step = 0;
 };
 
 // RATES: also the gating rates (diffuse mode with channels). Divisions by the
@@ -738,6 +752,37 @@ __device__ __forceinline__ void gate_coef_item(const FieldArgs& A, const KParams
 }
 
 
+// sum of -x[q] over q in [a, b), added in order from a; the loads are issued
+// eight at a time (clamped to b - 1, past-the-end values unused) so one
+// square's membranes cost one memory latency instead of one each
+template <typename T, int W = 8>
+__device__ __forceinline__ double neg_seg_sum(const T* x, int a, int b, double S = 0.0) {
+    for (int q0 = a; q0 < b; q0 += W) {
+        T t[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) t[u] = x[min(q0 + u, b - 1)];
+#pragma unroll
+        for (int u = 0; u < W; ++u)
+            if (q0 + u < b) S = S + (-(double)t[u]);
+    }
+    return S;
+}
+
+// row i of the membrane fluxes (FP64 or FP32), summed negated over [a, b) in
+// FP64, continuing the sum S
+template <int W = 8>
+__device__ __forceinline__ double neg_flux_sum(const void* fsa, int fsa32, size_t row, int a, int b,
+                                               double S = 0.0) {
+    return fsa32 ? neg_seg_sum<float, W>(static_cast<const float*>(fsa) + row, a, b, S)
+                 : neg_seg_sum<double, W>(static_cast<const double*>(fsa) + row, a, b, S);
+}
+
+// FUSE: the first FUSE_PF membrane fluxes of a square's row, loaded at the
+// start of the ion's iteration so that their latency passes in its flux
+// stages; the rest FUSE_W at a time
+constexpr int FUSE_PF = 2;
+constexpr int FUSE_W = 4;
+
 // LDS of one k_env_field tile, in doubles
 constexpr int ENV_UH = 6;                                        // u on tile +- 6
 constexpr int ENV_A1 = (TY + 2 * ENV_UH) * (TX + 2 * ENV_UH) + (TY + 4) * (TX + 2 * ENV_UH);
@@ -749,10 +794,11 @@ constexpr int ENV_LDS = ENV_POOL + (TY + 4) * (TX + 4) + 2 * (TY + 2) * (TX + 2)
 // [ion_id * ipb, ion_id * ipb + ipb), ipb = ceil(n_ions / n_groups)), with
 // the block's LDS at lds (ENV_LDS doubles). With RATES the block's threads also
 // evaluate the gating-rate items rate_first, rate_first + rate_stride, ...
-template <bool DIFFUSE, bool RATES>
+template <bool DIFFUSE, bool RATES, bool FUSE = false>
 __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams& P, int tile_id, int ion_id,
                                                int n_groups, double* lds, int rate_first, int rate_stride,
                                                bool check_halt = true) {
+    static_assert(!FUSE || (DIFFUSE && !RATES), "the fused exchange is a diffusion step without channels");
     // the halt flag is read first and tested before the first store, so its
     // latency overlaps the block's first loads (a caller that has already
     // decided for the whole block passes check_halt = false)
@@ -809,35 +855,75 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
     // (an empty asm at the store keeps the compiler from testing it, and so
     // waiting for every load issued before it, right after the load)
     unsigned live_b = 0;
+    int fa = 0, fb = 0;             // FUSE: the square's membranes in fsa's rows
+    unsigned hm = 0;
     if (DIFFUSE) {
         const int r = r0 + tid / TX, cc = cc0 + tid % TX;
-        if (r < ny && cc < nx) live_b = A.live[so + r * nx + cc];
+        if (r < ny && cc < nx) {
+            live_b = A.live[so + r * nx + cc];
+            if (FUSE) {
+                fa = A.sq_ptr[so + r * nx + cc];
+                fb = A.sq_ptr[so + r * nx + cc + 1];
+                hm = A.has_mem[so + r * nx + cc];
+            }
+        }
     }
-    // one ion's concentrations on tile +- 2 and D_env on tile +- 1, from
-    // clamped in-grid addresses (du: the cluster's D_env of the ion when it is
-    // one value, no loads; NaN when it varies)
-    constexpr int NC = (VR * VC + TY * TX - 1) / (TY * TX);
-    constexpr int ND = (ER * EC + TY * TX - 1) / (TY * TX);
-    double tc[NC], td[ND];
+    // The squares of ion data a thread stages: its own square of the tile
+    // (row tid / TX, column tid % TX) and one square of the ring around the
+    // tile, for the concentrations on tile +- 2 (the RC squares of the
+    // (TY + 4) x (TX + 4) block outside the tile: thread t < RC takes ring
+    // square t) and for D_env on tile +- 1 (RD squares, t < RD). Their
+    // clamped in-grid offsets, LDS slots and edge flags are set once per tile,
+    // so an ion's loads and staging take no index arithmetic.
+    static_assert(TY == 16 && TX == 16, "the halo flux mapping assumes 16 x 16 tiles");
+    constexpr int RC = VR * VC - TY * TX;
+    constexpr int RD = ER * EC - TY * TX;
+    // ring square t of the block of width w around the tile: the w top rows,
+    // the w bottom rows, then the w left and w right squares of each tile row
+    auto ring = [](int t, int w, int& sr, int& sc) {
+        const int bw = TX + 2 * w;
+        if (t < w * bw) {
+            sr = t / bw;
+            sc = t % bw;
+        } else if (t < 2 * w * bw) {
+            t -= w * bw;
+            sr = TY + w + t / bw;
+            sc = t % bw;
+        } else {
+            t -= 2 * w * bw;
+            sr = w + t / (2 * w);
+            const int q = t % (2 * w);
+            sc = (q < w) ? q : TX + q;
+        }
+    };
+    auto on_edge = [&](int r, int cc) { return r == 0 || r == ny - 1 || cc == 0 || cc == nx - 1; };
+    const int lr = tid / TX, lc = tid % TX;                // own square in the tile
+    const int r_own = r0 + lr, c_own = cc0 + lc;
+    const bool own_in = r_own < ny && c_own < nx;
+    const bool own_edge = on_edge(r_own, c_own);
+    const int off_own = clampi(r_own, ny) * nx + clampi(c_own, nx);
+    int crs, ccs, drs, dcs;                                // ring squares in sC and sD
+    ring(min(tid, RC - 1), 2, crs, ccs);
+    ring(min(tid, RD - 1), 1, drs, dcs);
+    const int rc_r = r0 - 2 + crs, rc_c = cc0 - 2 + ccs;
+    const int rd_r = r0 - 1 + drs, rd_c = cc0 - 1 + dcs;
+    const bool cring_in = rc_r >= 0 && rc_r < ny && rc_c >= 0 && rc_c < nx;
+    const bool cring_edge = on_edge(rc_r, rc_c);
+    const bool dring_in = rd_r >= 0 && rd_r < ny && rd_c >= 0 && rd_c < nx;
+    const int off_cring = clampi(rc_r, ny) * nx + clampi(rc_c, nx);
+    const int off_dring = clampi(rd_r, ny) * nx + clampi(rd_c, nx);
+    // one ion's concentrations and D_env at those squares, from clamped
+    // in-grid addresses (du: the cluster's D_env of the ion when it is one
+    // value, no loads; NaN when it varies)
+    double tc_own = 0.0, tc_ring = 0.0, td_own = 0.0, td_ring = 0.0;
     auto load_ion = [&](int i, double du) {
         const double* cenv = A.c_env + (size_t)i * P.n_sq + so;
         const double* D = A.D_env + (size_t)i * P.n_sq + so;
-#pragma unroll
-        for (int it = 0; it < NC; ++it) {
-            const int e = min(tid + it * TY * TX, VR * VC - 1);
-            const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
-            tc[it] = cenv[clampi(r, ny) * nx + clampi(cc, nx)];
-        }
-        if (du == du) {
-#pragma unroll
-            for (int it = 0; it < ND; ++it) td[it] = du;
-        } else {
-#pragma unroll
-            for (int it = 0; it < ND; ++it) {
-                const int e = min(tid + it * TY * TX, ER * EC - 1);
-                const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
-                td[it] = D[clampi(r, ny) * nx + clampi(cc, nx)];
-            }
+        tc_own = cenv[off_own];
+        tc_ring = cenv[off_cring];
+        if (!(du == du)) {
+            td_own = D[off_own];
+            td_ring = D[off_dring];
         }
     };
     if (DIFFUSE && i_lo < i_hi) load_ion(i_lo, A.kcl[et[0]].denv_u[i_lo]);
@@ -921,6 +1007,9 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
     lds_barrier();
     // a halted run writes nothing (every thread read the same flag)
     if (stop) return;
+    if (FUSE && A.count_step && tile_id == 0 && ion_id == 0 && tid == 0)
+        for (int g = 0; g < BT_MAX_STREAMS; ++g)
+            if (A.ctr_mask & (1u << g)) A.step_ctr[g] = A.step_ctr[g] + 1;
     if (!DIFFUSE) {
         const int r = r0 + tid / TX, cc = cc0 + tid % TX;
         if (r < ny && cc < nx) {
@@ -931,74 +1020,91 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
         return;
     }
 
+    double rho = 0.0;               // FUSE: the square's charge over the ions, in ion order
     for (int i = i_lo; i < i_hi; ++i) {
         const double cb = P.c_bound[i];
         const double zi = P.zq[i];
         const double zq_kt = (zi * P.q) / P.kbT;        // alpha = D * zq_kt
         lds_barrier();
         // concentrations of ion i (loaded one ion ahead) with the world
-        // boundary reset to c_env_bound (Q3)
-#pragma unroll
-        for (int it = 0; it < NC; ++it) {
-            const int e = tid + it * TY * TX;
-            if (e < VR * VC) {
-                const int r = r0 - 2 + e / VC, cc = cc0 - 2 + e % VC;
-                double val = 0.0;
-                if (r >= 0 && r < ny && cc >= 0 && cc < nx)
-                    val = (r == 0 || r == ny - 1 || cc == 0 || cc == nx - 1) ? cb : tc[it];
-                sC[e / VC][e % VC] = val;
-            }
-        }
+        // boundary reset to c_env_bound (Q3), zero outside the grid
+        sC[lr + 2][lc + 2] = own_in ? (own_edge ? cb : tc_own) : 0.0;
+        if (tid < RC) sC[crs][ccs] = cring_in ? (cring_edge ? cb : tc_ring) : 0.0;
         // D_env zero over the cluster (block-uniform): the Nernst-Planck flux
         // is a signed zero everywhere and the update keeps the
-        // (boundary-reset) concentrations, so its stages are skipped
-        const bool still = (s_du[i] == 0.0);
-        if (!still) {
+        // (boundary-reset) concentrations, so its stages are skipped. D_env
+        // one value over the cluster (block-uniform) is that value, with no
+        // staging; else it is staged on tile +- 1, zero outside the grid.
+        const double du_i = s_du[i];
+        const bool still = (du_i == 0.0);
+        const bool duni = (du_i == du_i);
+        // FUSE: this square's first membrane fluxes of ion i (none for a still
+        // ion, whose rows are zeros, or a square without membranes)
+        double pf[(FUSE && FUSE_PF > 0) ? FUSE_PF : 1];
+        const bool fsum = FUSE && FUSE_PF > 0 && live_b != 0 && fa < fb && !((K.still_mem >> i) & 1u);
+        if (FUSE && fsum) {
+            const size_t row = (size_t)i * P.n_mems;
 #pragma unroll
-            for (int it = 0; it < ND; ++it) {
-                const int e = tid + it * TY * TX;
-                if (e < ER * EC) {
-                    const int r = r0 - 1 + e / EC, cc = cc0 - 1 + e % EC;
-                    sD[e / EC][e % EC] = (r >= 0 && r < ny && cc >= 0 && cc < nx) ? td[it] : 0.0;
-                }
+            for (int u = 0; u < FUSE_PF; ++u) {
+                const int q = min(fa + u, fb - 1);
+                pf[u] = A.fsa32 ? (double)static_cast<const float*>(A.fsa)[row + q]
+                                : static_cast<const double*>(A.fsa)[row + q];
             }
+        }
+        if (!still && !duni) {
+            sD[lr + 1][lc + 1] = own_in ? td_own : 0.0;
+            if (tid < RD) sD[drs][dcs] = dring_in ? td_ring : 0.0;
         }
         // the next ion's loads, in flight through this ion's flux stages
         if (i + 1 < i_hi) load_ion(i + 1, s_du[i + 1]);
         lds_barrier();
         if (!still) {
         // Nernst-Planck fluxes (sim_toolbox.py:409-411, gv = -E, u = 0, mu = 0)
-        for (int e = tid; e < TY * (TX + 2); e += TY * TX) {
-            const int lr = e / (TX + 2), lc = e % (TX + 2);  // fx at (r0+lr, cc0-1+lc)
-            const int r = r0 + lr, cc = cc0 - 1 + lc;
-            double fx = 0.0;
-            if (r < ny && cc >= 0 && cc < nx) {
-                const int sr = lr + 2, sc = lc + 1;            // in sC
+        // at the square (r, cc), which is (sr, sc) in sC and (er, ec) in sD, sEx and sEy
+        auto np_flux = [&](int r, int cc, int sr, int sc, int er, int ec, double& fx, double& fy, bool want_x,
+                           bool want_y) {
+            const double Dv = duni ? du_i : sD[er][ec];
+            const double alpha = Dv * zq_kt;
+            if (want_x) {
                 double g;
                 if (cc == 0) g = per_dl((sC[sr][sc + 1] - sC[sr][sc]));
                 else if (cc == nx - 1) g = per_dl((sC[sr][sc] - sC[sr][sc - 1]));
                 else g = per_d2(-(sC[sr][sc - 1] - sC[sr][sc + 1]));
-                const double Dv = sD[lr + 1][lc];
-                const double alpha = Dv * zq_kt;
-                fx = (-Dv) * g - (alpha * (-sEx[lr + 1][lc])) * sC[sr][sc];
+                fx = (-Dv) * g - (alpha * (-sEx[er][ec])) * sC[sr][sc];
             }
-            sFx[lr][lc] = fx;
-        }
-        for (int e = tid; e < (TY + 2) * TX; e += TY * TX) {
-            const int lr = e / TX, lc = e % TX;              // fy at (r0-1+lr, cc0+lc)
-            const int r = r0 - 1 + lr, cc = cc0 + lc;
-            double fy = 0.0;
-            if (r >= 0 && r < ny && cc < nx) {
-                const int sr = lr + 1, sc = lc + 2;
+            if (want_y) {
                 double g;
                 if (r == 0) g = per_dl((sC[sr + 1][sc] - sC[sr][sc]));
                 else if (r == ny - 1) g = per_dl((sC[sr][sc] - sC[sr - 1][sc]));
                 else g = per_d2(-(sC[sr - 1][sc] - sC[sr + 1][sc]));
-                const double Dv = sD[lr][lc + 1];
-                const double alpha = Dv * zq_kt;
-                fy = (-Dv) * g - (alpha * (-sEy[lr][lc + 1])) * sC[sr][sc];
+                fy = (-Dv) * g - (alpha * (-sEy[er][ec])) * sC[sr][sc];
             }
-            sFy[lr][lc] = fy;
+        };
+        {
+            // both fluxes at the thread's own square
+            double fx = 0.0, fy = 0.0;
+            if (own_in) np_flux(r_own, c_own, lr + 2, lc + 2, lr + 1, lc + 1, fx, fy, true, true);
+            sFx[lr][lc + 1] = fx;
+            sFy[lr + 1][lc] = fy;
+        }
+        if (tid < 2 * TY + 2 * TX) {
+            // the halo the divergence also reads, by the first wavefront: fx
+            // in the columns left and right of the tile, fy in the rows above
+            // and below it
+            const bool xh = tid < 2 * TY;
+            const int t = xh ? tid : tid - 2 * TY;
+            const int a = t % TX, side = (t / TX) ? TX + 1 : 0;     // 16 squares per side
+            const int hr = xh ? a : side, hc = xh ? side : a;       // in sFy rows / sFx columns
+            const int r = xh ? r0 + a : r0 - 1 + side, cc = xh ? cc0 - 1 + side : cc0 + a;
+            double f = 0.0;
+            if (r >= 0 && r < ny && cc >= 0 && cc < nx) {
+                double fx = 0.0, fy = 0.0;
+                if (xh) np_flux(r, cc, a + 2, side + 1, a + 1, side, fx, fy, true, false);
+                else np_flux(r, cc, side + 1, a + 2, side, a + 1, fx, fy, false, true);
+                f = xh ? fx : fy;
+            }
+            if (xh) sFx[hr][hc] = f;
+            else sFy[hr][hc] = f;
         }
         lds_barrier();
         }
@@ -1022,8 +1128,30 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
                 const size_t e = (size_t)i * P.n_sq + so + r * nx + cc;
                 unsigned lv = live_b;
                 asm volatile("" : "+v"(lv));
-                if (lv != 0) A.c_mid[e] = cn;
-                else A.c_env_out[e] = cn;
+                if (FUSE) {
+                    // the exchange of k_env_exchange for this square and ion:
+                    // a still ion's membrane fluxes are zeros, whose sum
+                    // would add a signed zero
+                    if (lv != 0 && !((K.still_mem >> i) & 1u)) {
+                        double S = 0.0;
+                        if (fsum) {
+#pragma unroll
+                            for (int u = 0; u < FUSE_PF; ++u)
+                                if (fa + u < fb) S = S + (-pf[u]);
+                            if (fb > fa + FUSE_PF)
+                                S = neg_flux_sum<FUSE_W>(A.fsa, A.fsa32, (size_t)i * P.n_mems, fa + FUSE_PF, fb, S);
+                        } else {
+                            S = neg_flux_sum<FUSE_W>(A.fsa, A.fsa32, (size_t)i * P.n_mems, fa, fb);
+                        }
+                        cn = cn + (S / K.env_vol) * P.dt;
+                    }
+                    A.c_env_out[e] = cn;
+                    rho = (i == 0) ? P.zF[0] * cn : rho + P.zF[i] * cn;
+                } else if (lv != 0) {
+                    A.c_mid[e] = cn;
+                } else {
+                    A.c_env_out[e] = cn;
+                }
                 if (A.obs_fx) {
                     A.obs_fx[(size_t)i * P.n_sq + so + r * nx + cc] = still ? 0.0 : sFx[lr][lc + 1];
                     A.obs_fy[(size_t)i * P.n_sq + so + r * nx + cc] = still ? 0.0 : sFy[lr + 1][lc];
@@ -1031,17 +1159,22 @@ __device__ __forceinline__ void env_field_tile(const FieldArgs& A, const KParams
             }
         }
     }
+    if (FUSE) {
+        const int r = r0 + tid / TX, cc = cc0 + tid % TX;
+        if (r < ny && cc < nx && live_b != 0)
+            A.u_out[so + r * nx + cc] = hm ? ((rho * K.surf_scale) / K.mean_sa) / K.kee : 0.0;
+    }
     if (RATES) gating_rates();
 }
 
-template <bool DIFFUSE, bool RATES>
+template <bool DIFFUSE, bool RATES, bool FUSE = false>
 __global__ void __launch_bounds__(TY * TX)
 k_env_field(FieldArgs A, KParams P) {
     __shared__ double lds[ENV_LDS];
     // (tile, ion group) from the XCD-chunked linear block index, tiles fastest
     const int lin = xcd_chunk(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const int tile_id = lin % gridDim.x, ion_id = lin / gridDim.x;
-    env_field_tile<DIFFUSE, RATES>(A, P, tile_id, ion_id, gridDim.y, lds,
+    env_field_tile<DIFFUSE, RATES, FUSE>(A, P, tile_id, ion_id, gridDim.y, lds,
                                    (blockIdx.y * gridDim.x + blockIdx.x) * (TY * TX) + threadIdx.x,
                                    gridDim.x * gridDim.y * (TY * TX));
 }
@@ -1061,29 +1194,6 @@ struct ExArgs {
     // the others keep u = 0 and take their env state from k_env_field
     const int32_t* sq_live; int n_live;
 };
-
-// sum of -x[q] over q in [a, b), added in order from a; the loads are issued
-// eight at a time (clamped to b - 1, past-the-end values unused) so one
-// square's membranes cost one memory latency instead of one each
-template <typename T>
-__device__ __forceinline__ double neg_seg_sum(const T* x, int a, int b) {
-    double S = 0.0;
-    for (int q0 = a; q0 < b; q0 += 8) {
-        T t[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) t[u] = x[min(q0 + u, b - 1)];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (q0 + u < b) S = S + (-(double)t[u]);
-    }
-    return S;
-}
-
-// row i of the membrane fluxes (FP64 or FP32), summed negated over [a, b) in FP64
-__device__ __forceinline__ double neg_flux_sum(const void* fsa, int fsa32, size_t row, int a, int b) {
-    return fsa32 ? neg_seg_sum(static_cast<const float*>(fsa) + row, a, b)
-                 : neg_seg_sum(static_cast<const double*>(fsa) + row, a, b);
-}
 
 constexpr int EXQ = 64;
 
@@ -1825,6 +1935,7 @@ struct bt_ctx {
     // workspace
     double* c_cell_b = nullptr; double* c_lag_b = nullptr; double* vm_b = nullptr;
     double* c_mid = nullptr; void* fsa = nullptr; double* fch = nullptr; double* u = nullptr;
+    double* u_b = nullptr;    // surface charge of odd steps with the fused exchange
     double* rates = nullptr;  // [n_chan][4][n_cells] gating rates of the current step
     double* c_env_b = nullptr;           // env state of odd steps
     int32_t* sq_live = nullptr;          // squares the exchange updates, ascending
@@ -1964,7 +2075,7 @@ constexpr int BT_MEM_WG_TILES = 4096;
 constexpr int BT_GRAPH_PAIRS = 8;
 constexpr int BT_MAX_GRAPH_PAIRS = 64;
 constexpr int BT_PERSIST_STEPS = 512;   // steps per cooperative launch of the persistent kernel (even)
-constexpr int BT_AUTOTUNE_TRIALS = 12;
+constexpr int BT_AUTOTUNE_TRIALS = 13;
 
 // A stream group's share of the step's work lists: its cell tiles [t0, t0 +
 // nt), env tiles [e0, e0 + ne) and live squares [l0, l0 + nl).
@@ -2043,7 +2154,9 @@ static void launch_field_args(bt_ctx* c, bool diffuse, const FieldArgs& f, hipSt
     // looping over the ions; field only: one block per tile
     const int ipb = c->tune.env_ions_per_block;
     dim3 grid(ne > 0 ? ne : c->n_env_tiles, diffuse ? (c->prm.n_ions + ipb - 1) / ipb : 1);
-    if (diffuse && f.rates)
+    if (diffuse && f.u_out)
+        hipLaunchKernelGGL((k_env_field<true, false, true>), grid, dim3(TY * TX), 0, st, f, c->kp);
+    else if (diffuse && f.rates)
         hipLaunchKernelGGL((k_env_field<true, true>), grid, dim3(TY * TX), 0, st, f, c->kp);
     else if (diffuse)
         hipLaunchKernelGGL((k_env_field<true, false>), grid, dim3(TY * TX), 0, st, f, c->kp);
@@ -2064,8 +2177,11 @@ struct StepArgs {
     ExArgs ex;
 };
 
-// G, g: stream group g of G (G = 1: the whole problem)
-static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_step, int G = 1, int g = 0) {
+// G, g: stream group g of G (G = 1: the whole problem); fused: the env kernel
+// does the exchange (launched by enqueue_step with that tuning; the persistent
+// kernel keeps its own exchange phase)
+static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_step, int G = 1, int g = 0,
+                          bool fused = false) {
     const bt_buffers& B = c->buf;
     double* ci = parity ? c->c_cell_b : B.c_cell;
     double* li = parity ? c->c_lag_b : B.c_lag;
@@ -2086,6 +2202,20 @@ static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_ste
                ExArgs{B.sq_ptr, B.sq_has_mem, B.sq_cluster, c->kcl, c->c_mid, c->fsa, c->fch, eo, c->u,
                       c->prm.flux_mode == BT_FLUX_FP32 ? 1 : 0, c->flags,
                       c->step_ctr, count_step, (1u << BT_MAX_STREAMS) - 1u, c->sq_live, c->n_live}};
+    if (fused) {
+        // the env kernel does the exchange: the surface charge ping-pongs
+        // with the env state (the kernel reads u on a halo while it writes its
+        // own squares' next value)
+        a.field.u = parity ? c->u_b : c->u;
+        a.field.u_out = parity ? c->u : c->u_b;
+        a.field.sq_ptr = B.sq_ptr;
+        a.field.has_mem = B.sq_has_mem;
+        a.field.fsa = c->fsa;
+        a.field.fsa32 = a.ex.fsa32;
+        a.field.step_ctr = c->step_ctr;
+        a.field.count_step = count_step;
+        a.field.ctr_mask = a.ex.ctr_mask;
+    }
     if (G > 1) {
         // the group's slices of the work lists; its own step counter (group 0
         // also advances the counters of groups this split does not use, so
@@ -2098,6 +2228,7 @@ static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_ste
         a.ex.sq_live += sp.l0;
         a.ex.n_live = sp.nl;
         a.ex.ctr_mask = (g == 0) ? (((1u << BT_MAX_STREAMS) - 1u) & ~((1u << G) - 2u)) : (1u << g);
+        a.field.ctr_mask = a.ex.ctr_mask;
     }
     return a;
 }
@@ -2106,12 +2237,19 @@ static StepArgs step_args(bt_ctx* c, int parity, const bt_obs* ob, int count_ste
 // which: -1 all, 0 membrane/cell, 1 env field+diffusion, 2 exchange, 3 field only
 static void enqueue_step(bt_ctx* c, int parity, int which = -1, int G = 1, int g = 0) {
     const bt_obs* ob = c->capture;
-    const StepArgs a = step_args(c, parity, ob, which < 0 ? 1 : 0, G, g);
+    const bool fused = c->tune.exchange_form == BT_EXCHANGE_FUSED;
+    const StepArgs a = step_args(c, parity, ob, which < 0 ? 1 : 0, G, g, fused);
     const Span sp = group_span(c, G, g);
     hipStream_t st = (G > 1) ? c->gstream[g] : c->stream;
-    if (which < 0 || which == 1) launch_field_args(c, true, a.field, st, sp.ne);
-    if (which < 0 || which == 0) launch_mem_cell(c, a.mem, st, sp.nt);
-    if (which < 0 || which == 2) {
+    if (fused) {
+        // membranes first (they read the pre-diffusion env only), then the
+        // env kernel with the exchange
+        if (which < 0 || which == 0) launch_mem_cell(c, a.mem, st, sp.nt);
+        if (which < 0 || which == 1) launch_field_args(c, true, a.field, st, sp.ne);
+    }
+    if (!fused && (which < 0 || which == 1)) launch_field_args(c, true, a.field, st, sp.ne);
+    if (!fused && (which < 0 || which == 0)) launch_mem_cell(c, a.mem, st, sp.nt);
+    if (!fused && (which < 0 || which == 2)) {
         const int nl = sp.nl > 0 ? sp.nl : 1;    // block 0 also counts the step
         if (c->tune.exchange_form == 0)
             hipLaunchKernelGGL(k_env_exchange, dim3((nl + EXQ - 1) / EXQ), dim3(EXQ * c->prm.n_ions), 0, st, a.ex,
@@ -2203,6 +2341,13 @@ static bool streams_allowed(const bt_ctx* c, int G) {
                                                         c->prm.n_chan == 0 && !(c->net && c->c_net)));
 }
 
+// the exchange fused into the env kernel: no membrane law may read the
+// post-diffusion env (Ca pump, channels), and one block must hold every ion
+// of a square (its charge)
+static bool fused_allowed(const bt_ctx* c, const bt_tuning* t) {
+    return c->prm.n_chan == 0 && c->prm.i_ca < 0 && t->env_ions_per_block == c->prm.n_ions;
+}
+
 // the stream groups of the current tuning (1: the whole problem on the caller's stream)
 static int n_stream_groups(const bt_ctx* c) { return c->tune.persistent ? 1 : std::max(1, (int)c->tune.streams); }
 
@@ -2284,6 +2429,8 @@ static int copy_back(bt_ctx* c) {
     HIP_TRY(c, hipMemcpyAsync(c->buf.vm_cell, c->vm_b, nc * 8, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->buf.c_env, c->c_env_b, ni * (size_t)c->kp.n_sq * 8, hipMemcpyDeviceToDevice,
                               c->stream));
+    if (c->tune.exchange_form == BT_EXCHANGE_FUSED)
+        HIP_TRY(c, hipMemcpyAsync(c->u, c->u_b, (size_t)c->kp.n_sq * 8, hipMemcpyDeviceToDevice, c->stream));
     return BT_OK;
 }
 
@@ -2444,6 +2591,7 @@ static int derive_uniform(bt_ctx* c) {
         }
         K.phib = 0;
         for (size_t k = k0; k < k1 && !pb.empty() && !K.phib; ++k) K.phib = (pb[k] != 0.0);
+        if (K.phib) K.still_mem |= PHIB_BIT;
         if (K.phib && p->n_chan > 0)
             return set_err(c, BT_EINVAL, "an applied boundary voltage (Phi_b) with voltage-gated channels is not "
                                          "supported (their gating rates are per cell)");
@@ -2592,6 +2740,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     alloc((void**)&c->fch, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * nm * 8);
     alloc((void**)&c->rates, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * 4 * nc * 8);
     alloc((void**)&c->u, ns * 8);
+    alloc((void**)&c->u_b, ns * 8);
     alloc((void**)&c->flags, 256);
     alloc((void**)&c->bad_step, 256);
     alloc((void**)&c->step_ctr, 256);
@@ -2608,6 +2757,9 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     c->tune.graph_pairs = BT_GRAPH_PAIRS;
     c->tune.persistent = 0;
     c->tune.streams = 1;
+    // large problems whose membrane laws read the pre-diffusion env only:
+    // the exchange in the env kernel (measured faster on the 500k-cell batch)
+    if (c->tune.exchange_form == 1 && fused_allowed(c, &c->tune)) c->tune.exchange_form = BT_EXCHANGE_FUSED;
     if (e == hipSuccess && !sq_live.empty())
         e = hipMemcpy(c->sq_live, sq_live.data(), sizeof(int32_t) * sq_live.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->live_flag, live_flag.data(), ns, hipMemcpyHostToDevice);
@@ -2619,6 +2771,7 @@ int bt_create(bt_ctx** out, const bt_params* p, const bt_cluster* cl, int32_t n_
     if (e == hipSuccess) e = hipMemsetAsync(c->step_ctr, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->bar, 0, 256, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->u, 0, ns * 8, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->u_b, 0, ns * 8, c->stream);    // u of squares without membranes
     if (e == hipSuccess) e = hipMemsetAsync(c->fsa, 0, ni * nm * 8, c->stream);   // still rows stay zero
     if (e == hipSuccess)
         e = hipMemsetAsync(c->rates, 0, (p->n_chan > 0 ? (size_t)p->n_chan : 1) * 4 * nc * 8, c->stream);
@@ -2641,7 +2794,7 @@ int bt_destroy(bt_ctx* c) {
         if (c->ev_join[g]) (void)hipEventDestroy(c->ev_join[g]);
     }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->rates, c->c_env_b,
+    void* ptrs[] = {c->c_cell_b, c->c_lag_b, c->vm_b, c->c_mid, c->fsa, c->fch, c->u, c->u_b, c->rates, c->c_env_b,
                     c->sq_live, c->live_flag, c->flags, c->bad_step, c->step_ctr, c->kcl, c->env_tiles,
                     c->vm_prev, c->net, c->bar, c->phib_stage, c->phib_prev};
     for (void* q : ptrs)
@@ -2751,7 +2904,11 @@ int bt_set_tuning(bt_ctx* c, const bt_tuning* t) {
     if (t->mem_waves != 1 && t->mem_waves != 2) return set_err(c, BT_EINVAL, "mem_waves must be 1 or 2");
     if (t->env_ions_per_block < 1 || t->env_ions_per_block > c->prm.n_ions)
         return set_err(c, BT_EINVAL, "env_ions_per_block must be in 1..n_ions");
-    if (t->exchange_form != 0 && t->exchange_form != 1) return set_err(c, BT_EINVAL, "exchange_form must be 0 or 1");
+    if (t->exchange_form < 0 || t->exchange_form > BT_EXCHANGE_FUSED)
+        return set_err(c, BT_EINVAL, "exchange_form must be 0, 1 or 2");
+    if (t->exchange_form == BT_EXCHANGE_FUSED && !fused_allowed(c, t))
+        return set_err(c, BT_EINVAL, "the exchange fused into the env kernel needs no channels, no Ca pump and "
+                                     "env_ions_per_block = n_ions");
     if (t->graph_pairs < 1 || t->graph_pairs > BT_MAX_GRAPH_PAIRS)
         return set_err(c, BT_EINVAL, "graph_pairs must be in 1..64");
     if (t->persistent != 0 && t->persistent != 1) return set_err(c, BT_EINVAL, "persistent must be 0 or 1");
@@ -2903,13 +3060,18 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
         if (rc == BT_OK && e == hipSuccess) {
             const int n_steps = 2 * std::max(n_launches, 4);
             double bs = 1e30;
-            struct Whole { int slot, persistent, streams; };
-            const Whole cands[4] = {{8, 0, 1}, {9, 1, 1}, {10, 0, 2}, {11, 0, 4}};
+            struct Whole { int slot, persistent, streams; bool fused; };
+            const Whole cands[5] = {{8, 0, 1, false}, {9, 1, 1, false}, {10, 0, 2, false}, {11, 0, 4, false},
+                                    {12, 0, 1, true}};
+            bt_tuning fused_t = best;
+            fused_t.exchange_form = BT_EXCHANGE_FUSED;
+            fused_t.env_ions_per_block = c->prm.n_ions;
             for (const Whole& w : cands) {
                 if (rc != BT_OK || e != hipSuccess) break;
                 if (w.persistent && persistent_grid(c) < 1) continue;
                 if (!streams_allowed(c, w.streams)) continue;
-                c->tune = best;
+                if (w.fused && !fused_allowed(c, &fused_t)) continue;
+                c->tune = w.fused ? fused_t : best;
                 c->tune.persistent = w.persistent;
                 c->tune.streams = w.streams;
                 drop_graphs(c);
@@ -2937,6 +3099,10 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
                 restore();
                 if (rc == BT_OK && t[w.slot] < bs) {
                     bs = t[w.slot];
+                    if (w.fused) {
+                        best.exchange_form = fused_t.exchange_form;
+                        best.env_ions_per_block = fused_t.env_ions_per_block;
+                    }
                     best.persistent = w.persistent;
                     best.streams = w.streams;
                 }

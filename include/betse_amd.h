@@ -225,7 +225,10 @@ int bt_destroy(bt_ctx* ctx);
 typedef struct {
     int32_t mem_waves;          /* k_mem_cell wavefronts per cell tile: 1, or 2 (each a contiguous block of ions) */
     int32_t env_ions_per_block; /* ions per k_env_field block, 1..n_ions (n_ions: one block per 16x16 tile) */
-    int32_t exchange_form;      /* k_env_exchange: 0 one wavefront per ion over 64 squares, 1 one thread per square */
+    int32_t exchange_form;      /* k_env_exchange: 0 one wavefront per ion over 64 squares, 1 one thread per square;
+                                   BT_EXCHANGE_FUSED: no exchange kernel, k_mem_cell runs first and k_env_field
+                                   adds each square's membrane fluxes to its diffused value (needs no channels, no
+                                   Ca pump and env_ions_per_block = n_ions) */
     int32_t graph_pairs;        /* pairs of steps per replayed step graph, 1..64 */
     int32_t persistent;         /* 1: bt_step runs its pairs of steps in cooperative launches of one persistent
                                    kernel (grid barriers between a step's phases; 5 or 6 ions, no reaction
@@ -238,6 +241,7 @@ typedef struct {
 } bt_tuning;
 
 #define BT_MAX_STREAMS 4
+#define BT_EXCHANGE_FUSED 2
 
 /* Events that change the membranes' or the environment's diffusivities
  * (TissueHandler.fire_events -> makeAllChanges, tishandler.py:709-724 and
@@ -283,7 +287,8 @@ int bt_set_tuning(bt_ctx* ctx, const bt_tuning* tuning);
  * be NULL) and the average ms of each trial to trial_ms (may be NULL;
  * n_trial_ms entries, unused ones -1) in the order: mem_waves 1, 2;
  * env_ions_per_block 1, 2, 3, n_ions; exchange_form 0, 1 (ms per launch);
- * persistent 0, 1; streams 2, 4 (ms per step).
+ * persistent 0, 1; streams 2, 4; exchange_form BT_EXCHANGE_FUSED with all ions
+ * per env block (ms per step).
  * All the state a step modifies in place is saved before and restored after
  * each trial, so a phase continues from where it was; the state must be
  * stable (no NaN flagged). Allocates a temporary copy of that state.
@@ -418,8 +423,9 @@ int bt_gnn_adam(float* p, const float* g, float* m, float* v, int64_t count, flo
 int bt_sync(bt_ctx* ctx, int64_t* bad_step);
 
 /* Average device time in ms of one launch of a step kernel (0 membrane/cell,
- * 1 env field + diffusion, 2 env exchange, 3 env field only, 4 reaction
- * network; 4 needs an attached network),
+ * 1 env field + diffusion (+ the exchange with exchange_form
+ * BT_EXCHANGE_FUSED), 2 env exchange (nothing with BT_EXCHANGE_FUSED), 3 env
+ * field only, 4 reaction network; 4 needs an attached network),
  * timed with HIP events over n_launches back-to-back launches on the context
  * stream. Leaves the state undefined: call after the timed work. */
 int bt_time_kernel(bt_ctx* ctx, int32_t kernel, int32_t n_launches, double* ms_out);

@@ -2,10 +2,11 @@
 variant of the step kernels.
 
 * The launch variants (``bt_tuning``: one or two wavefronts per cell tile,
-  1..n_ions ions per env block, the per-ion or per-square exchange, graph
-  length, the persistent kernel, stream groups) divide the same arithmetic
-  differently between lanes and launches, so they must give bit-identical
-  states; the default variant matches the oracle at the FP64 bars.
+  1..n_ions ions per env block, the per-ion or per-square exchange or the
+  exchange fused into the env kernel, graph length, the persistent kernel,
+  stream groups) divide the same arithmetic differently between lanes and
+  launches, so they must give bit-identical states; the default variant
+  matches the oracle at the FP64 bars.
 * The batched workload (256 clusters of 100-5000 cells, ~500k cells) for 16
   steps, a sample of its clusters (the largest and smallest among them)
   against per-cluster oracle runs.
@@ -30,11 +31,15 @@ REL_TOL = 1e-10
 
 VARIANTS = [{"mem_waves": 1}, {"mem_waves": 2}, {"env_ions_per_block": 1}, {"env_ions_per_block": 2},
             {"env_ions_per_block": 3}, {"env_ions_per_block": 5}, {"exchange_form": 0}, {"exchange_form": 1},
-            {"graph_pairs": 1}, {"graph_pairs": 3}]
+            {"graph_pairs": 1}, {"graph_pairs": 3}, {"exchange_form": 2, "env_ions_per_block": 5},
+            {"exchange_form": 2, "env_ions_per_block": 5, "graph_pairs": 3, "mem_waves": 2}]
 PERSISTENT_VARIANTS = [{"persistent": 1}, {"persistent": 1, "env_ions_per_block": 2},
-                       {"persistent": 1, "env_ions_per_block": 1}]
+                       {"persistent": 1, "env_ions_per_block": 1},
+                       # pairs in the persistent kernel, the odd step with the fused exchange
+                       {"persistent": 1, "exchange_form": 2, "env_ions_per_block": 5}]
 STREAM_VARIANTS = [{"streams": 2}, {"streams": 4}, {"streams": 3, "graph_pairs": 1},
-                   {"streams": 2, "graph_pairs": 3, "exchange_form": 0}]
+                   {"streams": 2, "graph_pairs": 3, "exchange_form": 0},
+                   {"streams": 3, "exchange_form": 2, "env_ions_per_block": 5}]
 
 
 def say(msg):
@@ -115,15 +120,18 @@ def test_stream_groups_are_bit_identical(flux_mode):
     check_variants(flux_mode, STREAM_VARIANTS, "stream groups")
 
 
-@pytest.mark.parametrize("cluster", [0, 4])
-def test_stream_groups_nan_halts_at_its_step(cluster):
-    """A NaN planted in a cluster of the first or the last stream group stops
-    the run with BetseSimUnstableException at that step."""
+@pytest.mark.parametrize("cluster,tuning", [(0, {"streams": 2}), (4, {"streams": 2}),
+                                            (2, {"exchange_form": 2, "env_ions_per_block": 5}),
+                                            (4, {"streams": 2, "exchange_form": 2, "env_ions_per_block": 5})])
+def test_stream_groups_nan_halts_at_its_step(cluster, tuning):
+    """A NaN planted in a cluster of the first or the last stream group (or
+    with the exchange fused into the env kernel, whose step counter the env
+    kernel advances) stops the run with BetseSimUnstableException at that step."""
     import torch
     from betse_amd.exceptions import BetseSimUnstableException
     worlds = batched_worlds(n_clusters=5, cells_range=(30, 400), grid_range=(16, 40), seed=3)
-    say(f"case: NaN in cluster {cluster}, two stream groups")
-    b = batch(worlds, "fp64", tuning={"streams": 2})
+    say(f"case: NaN in cluster {cluster}, tuning {tuning}")
+    b = batch(worlds, "fp64", tuning=tuning)
     b.step_n(6)
     c0 = sum(c.n_cells for _, c in worlds[:cluster])
     with torch.cuda.stream(b.stream):
@@ -150,6 +158,12 @@ def test_persistent_kernel_with_channels_is_bit_identical(flux_mode):
         b.step_n(31)
         runs.append((state(b), b._dev["gates"].cpu().numpy()))
         del b
+    # channels read the post-diffusion env in the membrane kernel: no fused exchange
+    from betse_amd.exceptions import BetseNativeError
+    b = batch(worlds, flux_mode)
+    with pytest.raises(BetseNativeError, match="fused"):
+        b.set_tuning({"exchange_form": 2, "env_ions_per_block": 6})
+    del b
     for k in runs[0][0]:
         assert np.array_equal(runs[0][0][k], runs[1][0][k]), k
     assert np.array_equal(runs[0][1], runs[1][1])
@@ -191,8 +205,10 @@ def c3_worlds():
 @pytest.mark.parametrize("flux_mode,vm_tol,rel_tol", [("fp64", VM_TOL, REL_TOL), ("fp32", 1e-6, 1e-6)])
 def test_batched_workload_sample_matches_oracle(c3_worlds, flux_mode, vm_tol, rel_tol):
     """The real batched workload, with the launch variants its size selects
-    (one wavefront per tile, one exchange thread per square) and the other
-    ones forced, against per-cluster oracle runs of a sample of clusters."""
+    (one wavefront per tile, the exchange in the env kernel) and the other
+    ones forced (two wavefronts per tile, the exchange kernel with one thread
+    per square, two ions per env block), against per-cluster oracle runs of a
+    sample of clusters."""
     worlds = c3_worlds
     n = 16
     sizes = np.array([c.n_cells for _, c in worlds])
@@ -201,7 +217,7 @@ def test_batched_workload_sample_matches_oracle(c3_worlds, flux_mode, vm_tol, re
     tune = b.get_tuning()
     say(f"case: batched workload {flux_mode}, {len(worlds)} clusters, {sizes.sum()} cells, {n} steps, "
         f"default variants {tune}; sample {pick} ({sizes[pick].tolist()} cells)")
-    assert tune["mem_waves"] == 1 and tune["exchange_form"] == 1
+    assert tune["mem_waves"] == 1 and tune["exchange_form"] == 2       # the exchange in the env kernel
     b.step_n(n)
     s_def = state(b)
     b.pull()
@@ -209,7 +225,7 @@ def test_batched_workload_sample_matches_oracle(c3_worlds, flux_mode, vm_tol, re
         p, cells = worlds[k]
         compare(b.states[k], oracle_steps(p, cells, n), f"  cluster {k} ({cells.n_cells} cells)", vm_tol, rel_tol)
     del b
-    other = {"mem_waves": 2, "exchange_form": 0, "env_ions_per_block": 2}
+    other = {"mem_waves": 2, "exchange_form": 1, "env_ions_per_block": 2}
     say(f"  forced variants {other}")
     b2 = batch(worlds, flux_mode, tuning=other)
     b2.step_n(n)
@@ -220,11 +236,14 @@ def test_batched_workload_sample_matches_oracle(c3_worlds, flux_mode, vm_tol, re
 
 @pytest.mark.parametrize("flux_mode", ["fp64", "fp32"])
 def test_batched_workload_persistent_kernel_and_streams(c3_worlds, flux_mode):
-    """The batched workload in the persistent step kernel and in 2 and 4
-    stream groups against the replayed graphs on one stream, bit for bit, over
-    16 steps."""
+    """The batched workload in the persistent step kernel, in 2 and 4 stream
+    groups and with the exchange fused into the env kernel (one and two
+    stream groups) against the replayed graphs on one stream, bit for bit,
+    over 16 steps."""
     runs = []
-    tunings = (None, {"persistent": 1}, {"streams": 2}, {"streams": 4})
+    tunings = (None, {"persistent": 1}, {"streams": 2}, {"streams": 4},
+               {"exchange_form": 2, "env_ions_per_block": 5}, {"exchange_form": 2, "env_ions_per_block": 5,
+                                                               "streams": 2})
     for tuning in tunings:
         say(f"case: batched workload {flux_mode}, tuning {tuning}")
         b = batch(c3_worlds, flux_mode, tuning=tuning)
