@@ -3018,7 +3018,11 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
     HIP_TRY(c, hipMalloc((void**)&save, sbytes));
     hipError_t e = state_copy(c, regions, save, false);
     const bt_tuning prev = c->tune;
-    bt_tuning best = prev;
+    // the per-kernel trials run the three-kernel step (the fused exchange is
+    // timed as whole steps below)
+    bt_tuning split = prev;
+    if (split.exchange_form == BT_EXCHANGE_FUSED) split.exchange_form = 1;
+    bt_tuning best = split;
     double t[BT_AUTOTUNE_TRIALS];
     for (int q = 0; q < BT_AUTOTUNE_TRIALS; ++q) t[q] = -1.0;
     int rc = BT_OK;
@@ -3028,13 +3032,14 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
     if (e == hipSuccess) {
         // membrane kernel: one or two wavefronts per tile
         double bm = 1e30;
+        c->tune = split;
         for (int w = 1; w <= 2 && rc == BT_OK; ++w) {
             c->tune.mem_waves = w;
             rc = time_launches(c, 0, n_launches, &t[w - 1]);
             restore();
             if (rc == BT_OK && t[w - 1] < bm) { bm = t[w - 1]; best.mem_waves = w; }
         }
-        c->tune = prev;
+        c->tune = split;
         // env kernel: 1, 2, 3 or all ions per block
         const int cand[4] = {1, 2, 3, c->prm.n_ions};
         double be = 1e30;
@@ -3045,7 +3050,7 @@ int bt_autotune(bt_ctx* c, int32_t n_launches, bt_tuning* chosen, double* trial_
             restore();
             if (rc == BT_OK && t[2 + q] < be) { be = t[2 + q]; best.env_ions_per_block = cand[q]; }
         }
-        c->tune = prev;
+        c->tune = split;
         // exchange: per ion or per square
         double bx = 1e30;
         for (int f = 0; f <= 1 && rc == BT_OK; ++f) {
