@@ -199,11 +199,57 @@ __device__ __forceinline__ float expm1_32(float x) {
     p = fmaf(p, x, 0.5f);                     // 1/2!
     return fmaf(p * x, x, x);
 }
+// a * b + c with c in an SGPR pair. A polynomial's 64-bit coefficients are
+// then built by the scalar unit; left to itself the compiler builds each in
+// VGPRs with two v_mov per use (wave64: four cycles each), which doubled the
+// cost of the library's exp.
+__device__ __forceinline__ double fma_sc(double a, double b, double c) {
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+}
+
+// 1/k! rounded once (k! is exact in FP64 up to 18!)
+constexpr double inv_fact(int k) {
+    double f = 1.0;
+    for (int i = 2; i <= k; ++i) f *= (double)i;
+    return 1.0 / f;
+}
+
+// exp(x) on the FP64 path: x = k ln2 + r with k = rint(x log2 e) and r by two
+// FMAs against ln 2 split in a 32-bit head and its remainder (Cody and Waite;
+// |r| <= ln2/2 and k r exact for the laws' |x| < 700), exp(r) by its Taylor
+// series to r^13 in Horner form (truncation below 5e-18 relative), then 2^k
+// by ldexp. Overflow gives +inf, underflow 0, NaN NaN.
+__device__ __forceinline__ double exp64(double x) {
+    constexpr double LOG2E = 1.4426950408889634;
+    constexpr double LN2_HI = 6.93147180369123816490e-01;   // ln 2 to 32 bits
+    constexpr double LN2_LO = 1.90821492927058770002e-10;   // ln 2 - LN2_HI
+    const double k = __builtin_rint(x * LOG2E);
+    const double r = fma(-k, LN2_LO, fma(-k, LN2_HI, x));
+    double p = fma_sc(r, inv_fact(13), inv_fact(12));
+#pragma unroll
+    for (int j = 11; j >= 3; --j) p = fma_sc(p, r, inv_fact(j));
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)k);
+}
+// expm1(x) on the FP64 path for |x| <= 1/2: x + x^2 (1/2! + x (1/3! + ... +
+// x / 17!)) (truncation below 2e-21 relative).
+__device__ __forceinline__ double expm1_64(double x) {
+    double q = fma_sc(x, inv_fact(17), inv_fact(16));
+#pragma unroll
+    for (int j = 15; j >= 3; --j) q = fma_sc(q, x, inv_fact(j));
+    q = fma(q, x, 0.5);
+    return fma(q * x, x, x);
+}
+
 __device__ __forceinline__ float exp_path(float x) { return exp32(x); }
-__device__ __forceinline__ double exp_path(double x) { return exp(x); }
+__device__ __forceinline__ double exp_path(double x) { return exp64(x); }
 
 __device__ __forceinline__ float expm1_path(float x) { return expm1_32(x); }
-__device__ __forceinline__ double expm1_path(double x) { return expm1(x); }
+__device__ __forceinline__ double expm1_path(double x) { return expm1_64(x); }
 
 // electroflux (sim_toolbox.py:18-69), flux = -(Dc alpha / d) (cB - cA e^-alpha)
 // / (-expm1(-alpha)) with alpha = zc vBA F / (R T); zc and vBA already carry
@@ -2536,6 +2582,12 @@ __global__ void k_fp32_math(const float* x, const float* y, float* out, int n, i
     else out[t] = dv(x[t], y[t]);
 }
 
+__global__ void k_fp64_math(const double* x, double* out, int n, int fn) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    out[t] = (fn == 0) ? exp64(x[t]) : expm1_64(x[t]);
+}
+
 extern "C" {
 
 int bt_version(void) { return 300; }
@@ -3295,6 +3347,13 @@ int bt_fp32_math(const float* x, const float* y, float* out, int32_t n, int32_t 
     if (!x || !out || n < 0 || fn < 0 || fn > 2 || (fn == 2 && !y)) return BT_EINVAL;
     if (n == 0) return BT_OK;
     hipLaunchKernelGGL(k_fp32_math, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, y, out, n, fn);
+    return hipGetLastError() == hipSuccess ? BT_OK : BT_EHIP;
+}
+
+int bt_fp64_math(const double* x, double* out, int32_t n, int32_t fn, void* stream) {
+    if (!x || !out || n < 0 || fn < 0 || fn > 1) return BT_EINVAL;
+    if (n == 0) return BT_OK;
+    hipLaunchKernelGGL(k_fp64_math, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, x, out, n, fn);
     return hipGetLastError() == hipSuccess ? BT_OK : BT_EHIP;
 }
 
