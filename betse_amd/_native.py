@@ -163,7 +163,7 @@ SIGNATURES = {
     "bt_observables": (ctypes.c_int, [_vp, ctypes.POINTER(BtObs)]),
     "bt_gemm_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "bt_fp32_math": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
-    "bt_fp64_math": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "bt_fp64_math": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
     "bt_time_kernel": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "bt_set_network": (ctypes.c_int, [_vp, _vp, _vp]),
     "bt_residual": (ctypes.c_int, [_vp, _vp, _vp, _vp]),

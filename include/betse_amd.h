@@ -366,10 +366,11 @@ int bt_gemm_f64(bt_ctx* ctx, const double* A, const double* B, double* C,
  * are FP64): fn 0 out = exp(x), 1 out = expm1(x) for |x| <= 1/2, 2 out = x / y. */
 int bt_fp32_math(const float* x, const float* y, float* out, int32_t n, int32_t fn, void* stream);
 
-/* The FP64 path's own exponentials on n device doubles, asynchronous on
- * `stream` (exposed for tests; the reference calls NumPy's exp and expm1,
- * sim_toolbox.py:18-69): fn 0 out = exp(x), 1 out = expm1(x) for |x| <= 1/2. */
-int bt_fp64_math(const double* x, double* out, int32_t n, int32_t fn, void* stream);
+/* The FP64 path's own elementary functions on n device doubles, asynchronous
+ * on `stream` (exposed for tests; the reference calls NumPy's exp, expm1 and
+ * division, sim_toolbox.py:18-122): fn 0 out = exp(x), 1 out = expm1(x) for
+ * |x| <= 1/2, 2 out = x / y. */
+int bt_fp64_math(const double* x, const double* y, double* out, int32_t n, int32_t fn, void* stream);
 
 /* ------------------------------------------------------------------------
  * Warm-start graph network (no counterpart in the reference: it predicts the
