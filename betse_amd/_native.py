@@ -116,7 +116,7 @@ class BtBuffers(ctypes.Structure):
 
 OBS_FIELDS = ("fluxes_mem", "fluxes_gj", "fluxes_env_x", "fluxes_env_y", "chan_flux", "rate_NaKATP",
               "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave",
-              "J_env_x", "J_env_y", "Jtx", "Jty", "B_field")
+              "J_env_x", "J_env_y", "Jtx", "Jty", "B_field", "vm")
 
 
 class BtObs(ctypes.Structure):

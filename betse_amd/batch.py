@@ -286,7 +286,7 @@ class BatchSimulator(DeviceFetch):
         shapes = {"fluxes_mem": (ni, M), "fluxes_gj": (ni, M), "fluxes_env_x": (ni, G2), "fluxes_env_y": (ni, G2),
                   "chan_flux": (nch, M), "rate_NaKATP": (M,), "Jn": (M,), "I_mem": (M,), "E_gj_x": (M,),
                   "E_gj_y": (M,), "J_cell_x": (C,), "J_cell_y": (C,), "vm_ave": (C,), "J_env_x": (G2,),
-                  "J_env_y": (G2,), "Jtx": (G2,), "Jty": (G2,), "B_field": (G2,)}
+                  "J_env_y": (G2,), "Jtx": (G2,), "Jty": (G2,), "B_field": (G2,), "vm": (M,)}
         with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
             t = {k: torch.zeros(v, dtype=torch.float64, device=self.device) for k, v in shapes.items()}
         obs = _native.BtObs()
@@ -298,7 +298,7 @@ class BatchSimulator(DeviceFetch):
 
     # per-cluster slicing of the batch's fields: by cell, membrane or square
     _CELL_FIELDS = ("c_cell", "vm_cell", "c_net", "J_cell_x", "J_cell_y", "vm_ave")
-    _MEM_FIELDS = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y")
+    _MEM_FIELDS = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "vm")
 
     def _split(self, h: dict, i: int) -> dict:
         coff, moff, soff = self.offsets

@@ -1537,6 +1537,7 @@ struct ObsMemArgs {
     // applied boundary voltage per square (NULL: none) and map_mem2ecm: the
     // membranes' Vmem is their cell's less Phi_b at their square
     const double* Phi_b; const double* Phi_b_gj; const int32_t* mem_sq;
+    double* vm_mem;           // Vmem per membrane (write2storage's vm), or NULL
 };
 
 // raw normal current of membrane m: -sum(zF*f_mem) + extra_J_mem (channels) +
@@ -1610,6 +1611,7 @@ k_obs_mem(ObsMemArgs A, KParams P) {
             s_vm[j] = A.vm_cell[c0 + lc];
         }
         const double egj = -vgj / P.gj_len;
+        if (A.vm_mem) A.vm_mem[m] = s_vm[j];
         A.E_gj_x[m] = egj * A.mem_nx[m];
         A.E_gj_y[m] = egj * A.mem_ny[m];
     }
@@ -2607,7 +2609,7 @@ __global__ void k_fp64_math(const double* x, const double* y, double* out, int n
 
 extern "C" {
 
-int bt_version(void) { return 300; }
+int bt_version(void) { return 301; }
 
 const char* bt_last_error(const bt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
 
@@ -3489,7 +3491,7 @@ int bt_observables(bt_ctx* c, const bt_obs* o) {
                  c->vm_prev_ok ? c->vm_prev : B.vm_cell, B.mem_cell, B.tile_ptr,
                  o->fluxes_mem, o->fluxes_gj, o->chan_flux, o->Jn, o->I_mem, o->E_gj_x, o->E_gj_y,
                  o->J_cell_x, o->J_cell_y, o->vm_ave, c->prm.smooth_cells,
-                 B.Phi_b, (c->vm_prev_ok && c->phib_prev) ? c->phib_prev : B.Phi_b, B.mem_sq};
+                 B.Phi_b, (c->vm_prev_ok && c->phib_prev) ? c->phib_prev : B.Phi_b, B.mem_sq, o->vm};
     hipLaunchKernelGGL(k_obs_mem, dim3(c->n_tiles), dim3(TMB), 0, c->stream, a, c->kp);
     hipLaunchKernelGGL(k_obs_env_rhs, dim3(grid_blocks(c->kp.n_sq)), dim3(BLOCK), 0, c->stream,
                        (const double*)o->fluxes_env_x, (const double*)o->fluxes_env_y, c->jxo, c->jyo, c->kp);

@@ -285,6 +285,7 @@ class HostState:
     def _set_state(self, h: dict, full: bool) -> None:
         shp = self.cells.X.shape
         self.__dict__.pop("vm_ave_obs", None)       # observables of an earlier state
+        self.__dict__.pop("vm_obs", None)
         self.cc_cells = h["c_cell"]
         self.vm_cell = h["vm_cell"]
         self.cc_env = h["c_env"]
@@ -301,7 +302,7 @@ class HostState:
             self.Phi_b = h["Phi_b"]
 
     # the observables write2storage keeps; the raw fluxes stay on the device (pull_raw_fluxes)
-    _OBS_KEEP = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "J_env_x",
+    _OBS_KEEP = ("rate_NaKATP", "Jn", "I_mem", "E_gj_x", "E_gj_y", "J_cell_x", "J_cell_y", "vm_ave", "vm", "J_env_x",
                  "J_env_y", "Jtx", "Jty", "B_field")
 
     def _set_observables(self, h: dict) -> None:
@@ -311,6 +312,7 @@ class HostState:
         self.E_gj_x, self.E_gj_y = h["E_gj_x"], h["E_gj_y"]
         self.J_cell_x, self.J_cell_y = h["J_cell_x"], h["J_cell_y"]
         self.vm_ave_obs = h["vm_ave"]
+        self.vm_obs = h["vm"]
         self.J_env_x, self.J_env_y = h["J_env_x"].reshape(shp), h["J_env_y"].reshape(shp)
         self.Jtx, self.Jty = h["Jtx"].reshape(shp), h["Jty"].reshape(shp)
         self.B_field = h["B_field"].reshape(shp)
@@ -338,7 +340,10 @@ class HostState:
             snap = (np.copy(self.Dm_cells), self.gjopen * 1)
             self._phase_snapshot = snap
         self.dd_time.append(snap[0])
-        vm = self.vm
+        # update_V's vm per membrane: the sampled step derived it on the device
+        vm = self.__dict__.get("vm_obs")
+        if vm is None:
+            vm = self.vm
         self.vm_time.append(vm)
         # update_V's vm_ave (sim.py:2038): the sampled step derived it on the device
         ave = self.__dict__.get("vm_ave_obs")
@@ -683,7 +688,7 @@ class Simulator(HostState, DeviceFetch):
         shapes = {"fluxes_mem": (ni, M), "fluxes_gj": (ni, M), "fluxes_env_x": (ni, G2), "fluxes_env_y": (ni, G2),
                   "chan_flux": (nch, M), "rate_NaKATP": (M,), "Jn": (M,), "I_mem": (M,), "E_gj_x": (M,),
                   "E_gj_y": (M,), "J_cell_x": (C,), "J_cell_y": (C,), "vm_ave": (C,), "J_env_x": (G2,),
-                  "J_env_y": (G2,), "Jtx": (G2,), "Jty": (G2,), "B_field": (G2,)}
+                  "J_env_y": (G2,), "Jtx": (G2,), "Jty": (G2,), "B_field": (G2,), "vm": (M,)}
         with torch.cuda.device(self.device), torch.cuda.stream(self.stream):
             t = {k: torch.zeros(v, dtype=f64, device=self.device) for k, v in shapes.items()}
         obs = _native.BtObs()

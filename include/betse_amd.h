@@ -344,6 +344,8 @@ typedef struct {
     double* Jtx;           /* [ny*nx] J_env + curl-free part                     */
     double* Jty;           /* [ny*nx]                                            */
     double* B_field;       /* [ny*nx] mu * stream function                       */
+    double* vm;            /* [n_mems] Vmem per membrane (update_V's vm, what
+                              write2storage stores), or NULL                     */
 } bt_obs;
 
 /* bt_step whose last step also writes the raw fluxes into obs. */
