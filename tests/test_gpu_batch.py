@@ -165,3 +165,35 @@ def test_batch_event_on_one_cluster():
         de = (np.abs(st.cc_env - o.cc_env) / np.abs(o.cc_env)).max()
         say(f"cluster {k}: |dVm|={dv:.3e} rel dc={dc:.3e} rel denv={de:.3e}")
         assert dv <= 1e-9 and dc <= 1e-10 and de <= 1e-10
+
+
+def test_sampled_steps_with_the_fused_exchange_are_bit_identical():
+    """The loop with sampled steps (observed steps, observables, write2storage)
+    with the exchange in the env kernel against the exchange kernel: every
+    stored series of every cluster bit for bit (odd and even step counts
+    between samples, the observed step's env fluxes taken from the env
+    kernel that also does the exchange)."""
+    from betse_amd.batch import BatchSimulator
+    worlds = batched_worlds(n_clusters=4, cells_range=(30, 300), grid_range=(16, 32), seed=9)
+    n = 47
+    for p, c in worlds:
+        p.init_time_sampling = 9 * p.init_time_step
+        p.set_phase_time("init")
+    runs = []
+    for tuning in ({"exchange_form": 1}, {"exchange_form": 2, "env_ions_per_block": 5}):
+        say(f"case: sampled loop, 4 clusters, tuning {tuning}")
+        b = BatchSimulator([(p.copy(), c) for p, c in worlds], device="cuda:0", flux_mode="fp64", tuning=tuning)
+        b.init("init")
+        assert b.get_tuning()["exchange_form"] == tuning["exchange_form"]
+        n_samp = b.run_loop(n)
+        assert n_samp >= 5
+        runs.append(b.states)
+    keys = [k for k in vars(runs[0][0]) if k.endswith("_time") and isinstance(getattr(runs[0][0], k), list)]
+    assert {"vm_time", "cc_env_time", "I_tot_x_time", "venv_time"} <= set(keys)
+    for k_cl, (sa, sb) in enumerate(zip(*runs)):
+        for key in keys:
+            la, lb = getattr(sa, key), getattr(sb, key)
+            assert len(la) == len(lb) > 0, key
+            for x, y in zip(la, lb):
+                assert np.array_equal(np.asarray(x), np.asarray(y)), (k_cl, key)
+    say(f"sampled loop: {len(keys)} series of {len(runs[0])} clusters bit-identical")
