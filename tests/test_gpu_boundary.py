@@ -127,13 +127,15 @@ def test_observables_of_the_step_a_change_fires_in(tiny):
     check_phib(sim, o, "two steps after a change")
 
 
-VARIANTS = [dict(), dict(persistent=1), dict(streams=2), dict(mem_waves=2, exchange_form=1, env_ions_per_block=1)]
+VARIANTS = [dict(), dict(persistent=1), dict(streams=2), dict(mem_waves=2, exchange_form=1, env_ions_per_block=1),
+            dict(exchange_form=2, env_ions_per_block=5)]
 
 
 def test_batch_voltage_on_one_cluster_every_variant():
     """A voltage on cluster 1 of a batch of 3: that cluster follows the oracle
     stepped through it, the others are untouched, and every launch variant
-    (persistent kernel, stream groups, wavefront splits) gives the same bits."""
+    (persistent kernel, stream groups, wavefront splits, the exchange in the
+    env kernel) gives the same bits."""
     from betse_amd.batch import BatchSimulator
     worlds = batched_worlds(n_clusters=3, cells_range=(30, 300), grid_range=(16, 32), seed=11)
     say(f"case: bound_V on cluster 1 of a batch of 3: cells {[c.n_cells for _, c in worlds]}")
