@@ -386,17 +386,40 @@ class BatchSimulator(DeviceFetch):
                 if callbacks is not None:
                     callbacks.progressed_next()
 
+        # Per sample: wait for the device to reach it (the instability check),
+        # queue its observables and its snapshot on the device, queue the
+        # steps up to the next sample (or the rest), and only then wait for
+        # the host side of the sample two before (its slot's pinned buffer)
+        # and queue the snapshot's host copy: the device is not left idle
+        # while this thread waits for the worker.
         pos = 0
         slot = 0
+
+        def queue_steps_after(n):
+            # the steps after sampled step n: up to the next sampled step, or the rest
+            nonlocal pos
+            nxt = sampled_idx[n + 1] if n + 1 < len(sampled_idx) else None
+            if nxt is not None:
+                self._check(self._lib.bt_step_observed(self._ctx, int(nxt + 1 - pos), use_graph, ctypes.byref(obs)))
+                pos = nxt + 1
+            elif pos < len(time_steps):
+                self._check(self._lib.bt_step(self._ctx, len(time_steps) - pos, use_graph))
+                pos = len(time_steps)
+
         try:
-            for s in sampled_idx:
-                self._check(self._lib.bt_step_observed(self._ctx, int(s + 1 - pos), use_graph, ctypes.byref(obs)))
-                pos = s + 1
+            if sampled_idx:
+                s0 = sampled_idx[0]
+                self._check(self._lib.bt_step_observed(self._ctx, int(s0 + 1), use_graph, ctypes.byref(obs)))
+                pos = s0 + 1
+            for n, s in enumerate(sampled_idx):
                 clear()
                 self.sync()
                 self._check(self._lib.bt_observables(self._ctx, ctypes.byref(obs)))
+                self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}}, slot,
+                                        host=False)
+                queue_steps_after(n)
                 drain(1)                          # this slot's previous sample is stored
-                self._fetch_start_async({**{k: d[k] for k in state_keys}, **{k: t_obs[k] for k in keep}}, slot)
+                self._fetch_host_async(slot)
                 pending.append(self._sample_worker().submit(finish, time_steps[s], slot))
                 slot ^= 1
             if pos < len(time_steps):

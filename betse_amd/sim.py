@@ -401,14 +401,16 @@ class DeviceFetch:
                     pinned[k] = h
                 h.copy_(t, non_blocking=True)
 
-    def _fetch_start_async(self, tensors: dict, slot: int = 0) -> None:
+    def _fetch_start_async(self, tensors: dict, slot: int = 0, host: bool = True) -> None:
         """Sampled steps: snapshot the (FP64) tensors into a flat device buffer
         on the simulation stream (a single multi-tensor copy), then copy that
         buffer to pinned host memory on a side stream (default priority), so the
         host transfer overlaps the next steps. ``_fetch_finish_async(keys,
         slot)`` waits for it. Each slot has its own pair of buffers, reused: the
         snapshot waits on the device for the slot's previous transfer, and a
-        caller finishes a slot's fetch before it starts the slot's next one."""
+        caller finishes a slot's fetch before it starts the slot's next one.
+        ``host=False`` takes the device snapshot only; ``_fetch_host_async(slot)``
+        then queues its host copy (once the slot's previous sample is read)."""
         torch = self.torch
         layout = [(k, tuple(t.shape)) for k, t in tensors.items()]
         if any(t.dtype != torch.float64 for t in tensors.values()):
@@ -437,6 +439,16 @@ class DeviceFetch:
                 torch._foreach_copy_(dst, [tensors[k] for k in self._flat_at])
                 taken = torch.cuda.Event()
                 taken.record(self.stream)
+        self.__dict__.setdefault("_taken", {})[slot] = taken
+        if host:
+            self._fetch_host_async(slot)
+
+    def _fetch_host_async(self, slot: int = 0) -> None:
+        """The host copy of the slot's device snapshot, on the side stream."""
+        torch = self.torch
+        fdev = self._flat_dev[slot]
+        taken = self._taken.pop(slot)
+        with torch.cuda.device(self.device):
             with torch.cuda.stream(self._copy_stream):
                 self._copy_stream.wait_event(taken)
                 self._flat_host[slot].copy_(fdev, non_blocking=True)
