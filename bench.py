@@ -503,8 +503,9 @@ def main(argv=None):
                     help="only run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(each GPU box has one GPU; the sharding design is in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU only; --gpus {a.gpus} is not supported (the step runs on the one GPU; "
+              "several GPUs would each take a strip of the grid with a halo exchange per step, not built)",
+              file=sys.stderr)
         sys.exit(2)
     if a.copy_calib_mib:
         import torch
