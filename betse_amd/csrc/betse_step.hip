@@ -29,8 +29,8 @@
 //                   sparse form of M_divmap_mem2ecm) -> env update; then
 //                   rho_env -> u (sim_toolbox.py:1191-1229, ion_current.py:75-97)
 //                   over the squares with membranes only      c_mid -> next c_env
-//   k_network       (when a reaction network is attached) a lane per cell in
-//                   each of four wavefronts that split the rates:
+//   k_network       (when a reaction network is attached) a lane per two cells
+//                   in each of four wavefronts that split the rates:
 //                   the substances' explicit Euler update (networks.py:2805-2982)
 //
 // The env state ping-pongs between the caller's c_env and a workspace copy, like
@@ -1426,92 +1426,155 @@ __device__ __forceinline__ double hill_inh(double c, double rkm, double n) {
 }
 
 // A block holds NET_CELLS cells and NET_WAVES wavefronts: lane l of every
-// wavefront works on cell l, and wavefront w evaluates the rates w, w +
-// NET_WAVES, ... from the pre-step concentrations (rate-uniform per wavefront,
-// so the network description is read through scalar loads and no lane
-// diverges from its wavefront). After a barrier, wavefront w updates the
-// substances w, w + NET_WAVES, ...: delta[s] sums coeff * rate over the
-// substance's terms in ascending rate order (np.dot(reaction_matrix, rates)),
-// in the reference's order (growth/decay terms of every substance, then
-// reactions), then c + delta*dt and no_negs.
-constexpr int NET_CELLS = 64;
+// wavefront works on the NET_CPL cells l, l + 64, ... side by side, and
+// wavefront w evaluates the rates w, w + NET_WAVES, ... from the pre-step
+// concentrations (rate-uniform per wavefront, so the network description is
+// read through scalar loads, once for the lane's cells, and no lane diverges
+// from its wavefront). After a barrier, wavefront w updates the substances w,
+// w + NET_WAVES, ...: delta[s] sums coeff * rate over the substance's terms in
+// ascending rate order (np.dot(reaction_matrix, rates)), in the reference's
+// order (growth/decay terms of every substance, then reactions), then
+// c + delta*dt and no_negs. The concentrations and rates sit in LDS sized to
+// the network at launch ((S + R) rows of NET_CELLS doubles), so a 100k-cell
+// problem's blocks all fit the GPU at once.
+constexpr int NET_LANES = 64;
+constexpr int NET_CPL = 2;
+constexpr int NET_CELLS = NET_LANES * NET_CPL;
 constexpr int NET_WAVES = 4;
-constexpr int NET_BLOCK = NET_CELLS * NET_WAVES;
+constexpr int NET_BLOCK = NET_LANES * NET_WAVES;
+
+static size_t net_lds_bytes(int n_species, int n_rates) {
+    return (size_t)(n_species + n_rates) * NET_CELLS * sizeof(double);
+}
 
 __global__ void __launch_bounds__(NET_BLOCK)
 k_network(const KNet* __restrict__ net, double* __restrict__ cn, int n_cells, double dt,
           unsigned* flags, unsigned long long* bad_step, const long long* step_ctr, int step_ahead) {
     if (halted(flags)) return;
-    __shared__ double s_c[BT_NET_MAX_SPECIES][NET_CELLS];
-    __shared__ double s_r[BT_NET_MAX_RATES][NET_CELLS];
+    extern __shared__ double net_lds[];
     __shared__ int s_bad;
     if (threadIdx.x == 0) s_bad = 0;
-    const int t = threadIdx.x % NET_CELLS;
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / NET_CELLS));
-    const int cell = blockIdx.x * NET_CELLS + t;
-    const bool live = cell < n_cells;
+    const int t = threadIdx.x % NET_LANES;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / NET_LANES));
+    const int cell0 = blockIdx.x * NET_CELLS + t;
     const int S = net->n_species;
-    // cells past the end evaluate on ones and store nothing
-    for (int s = w; s < S; s += NET_WAVES) s_c[s][t] = live ? cn[(size_t)s * n_cells + cell] : 1.0;
-    lds_barrier();
     const int R = net->n_rates;
+    double* s_c = net_lds;                              // [S][NET_CELLS]
+    double* s_r = net_lds + (size_t)S * NET_CELLS;      // [R][NET_CELLS]
+    // cells past the end evaluate on ones and store nothing
+    for (int s = w; s < S; s += NET_WAVES) {
+#pragma unroll
+        for (int k = 0; k < NET_CPL; ++k) {
+            const int cell = cell0 + k * NET_LANES;
+            s_c[s * NET_CELLS + t + k * NET_LANES] = (cell < n_cells) ? cn[(size_t)s * n_cells + cell] : 1.0;
+        }
+    }
+    lds_barrier();
+    // the concentration of substance sp in the lane's k-th cell
+    auto conc = [&](int sp, int k) -> double { return s_c[sp * NET_CELLS + t + k * NET_LANES]; };
     for (int j = w; j < R; j += NET_WAVES) {
         const KNetRate& r = net->rates[j];
         // alpha = (product of activator terms) * (product of inhibitor terms), each
         // product taken left to right; an empty product is np.ones
-        double act = 1.0, inh = 1.0;
+        double act[NET_CPL], inh[NET_CPL], rate[NET_CPL];
+#pragma unroll
+        for (int k = 0; k < NET_CPL; ++k) act[k] = inh[k] = 1.0;
         for (int q = 0; q < r.n_act; ++q) {
-            const double h = hill_act(s_c[r.act[q]][t], r.act_rkm[q], r.act_n[q]);
-            act = (q == 0) ? h : act * h;
+            const int sp = r.act[q];
+            const double rkm = r.act_rkm[q], n = r.act_n[q];
+#pragma unroll
+            for (int k = 0; k < NET_CPL; ++k) {
+                const double h = hill_act(conc(sp, k), rkm, n);
+                act[k] = (q == 0) ? h : act[k] * h;
+            }
         }
         for (int q = 0; q < r.n_inh; ++q) {
-            const double h = hill_inh(s_c[r.inh[q]][t], r.inh_rkm[q], r.inh_n[q]);
-            inh = (q == 0) ? h : inh * h;
+            const int sp = r.inh[q];
+            const double rkm = r.inh_rkm[q], n = r.inh_n[q];
+#pragma unroll
+            for (int k = 0; k < NET_CPL; ++k) {
+                const double h = hill_inh(conc(sp, k), rkm, n);
+                inh[k] = (q == 0) ? h : inh[k] * h;
+            }
         }
-        double rate;
         if (r.kind == BT_RATE_GROWTH) {
             // mod*r_prod*alpha - r_decay*cc - decay_max*1*cc, cc = c/gad_scale (networks.py:1233-1249)
-            const double cc = s_c[r.species][t] * r.inv_gad;
-            rate = ((1.0 * r.r_prod) * act) * inh;
-            rate = rate - r.r_decay * cc;
-            rate = rate - ((r.decay_max * 1.0) * 1.0) * cc;
+#pragma unroll
+            for (int k = 0; k < NET_CPL; ++k) {
+                const double cc = conc(r.species, k) * r.inv_gad;
+                double v = ((1.0 * r.r_prod) * act[k]) * inh[k];
+                v = v - r.r_decay * cc;
+                rate[k] = v - ((r.decay_max * 1.0) * 1.0) * cc;
+            }
         } else {
-            double x;
+            double x[NET_CPL];
             if (r.reversible) {
                 // Q = prod(1e-3*c**k) / prod(1e-3*c**k) (mol/L), rate ~ (1.0 - Q/Keq) (networks.py:1346-1391, 1520-1524)
-                double num = 1.0, den = 1.0;
+                double num[NET_CPL], den[NET_CPL];
                 for (int q = 0; q < r.n_prod; ++q) {
-                    const double f = 1.0e-3 * hill_pow(s_c[r.prod[q]][t], r.prod_coeff[q]);
-                    num = (q == 0) ? f : num * f;
+                    const int sp = r.prod[q];
+                    const double e = r.prod_coeff[q];
+#pragma unroll
+                    for (int k = 0; k < NET_CPL; ++k) {
+                        const double f = 1.0e-3 * hill_pow(conc(sp, k), e);
+                        num[k] = (q == 0) ? f : num[k] * f;
+                    }
                 }
                 for (int q = 0; q < r.n_reac; ++q) {
-                    const double f = 1.0e-3 * hill_pow(s_c[r.reac[q]][t], r.reac_coeff[q]);
-                    den = (q == 0) ? f : den * f;
+                    const int sp = r.reac[q];
+                    const double e = r.reac_coeff[q];
+#pragma unroll
+                    for (int k = 0; k < NET_CPL; ++k) {
+                        const double f = 1.0e-3 * hill_pow(conc(sp, k), e);
+                        den[k] = (q == 0) ? f : den[k] * f;
+                    }
                 }
-                x = 1.0 - ((num / den) * r.inv_keq);
+#pragma unroll
+                for (int k = 0; k < NET_CPL; ++k) x[k] = 1.0 - ((num[k] / den[k]) * r.inv_keq);
             } else {
                 // forward coefficient: product over reactants of ((c/Km)**k)/(1 + (c/Km)**k)
-                double fwd = 1.0;
+                double fwd[NET_CPL];
+#pragma unroll
+                for (int k = 0; k < NET_CPL; ++k) fwd[k] = 1.0;
                 for (int q = 0; q < r.n_reac; ++q) {
-                    const double f = hill_act(s_c[r.reac[q]][t], r.reac_rkm[q], r.reac_coeff[q]);
-                    fwd = (q == 0) ? f : fwd * f;
+                    const int sp = r.reac[q];
+                    const double rkm = r.reac_rkm[q], e = r.reac_coeff[q];
+#pragma unroll
+                    for (int k = 0; k < NET_CPL; ++k) {
+                        const double f = hill_act(conc(sp, k), rkm, e);
+                        fwd[k] = (q == 0) ? f : fwd[k] * f;
+                    }
                 }
-                x = fwd - (0.0 * 0.0);
+#pragma unroll
+                for (int k = 0; k < NET_CPL; ++k) x[k] = fwd[k] - (0.0 * 0.0);
             }
-            rate = ((r.vmax * act) * inh) * x;
+#pragma unroll
+            for (int k = 0; k < NET_CPL; ++k) rate[k] = ((r.vmax * act[k]) * inh[k]) * x[k];
         }
-        s_r[j][t] = rate;
+#pragma unroll
+        for (int k = 0; k < NET_CPL; ++k) s_r[j * NET_CELLS + t + k * NET_LANES] = rate[k];
     }
     lds_barrier();
     bool bad = false;
-    if (live) {
-        for (int s = w; s < S; s += NET_WAVES) {
-            double d = 0.0;
-            for (int q = net->sub_ptr[s]; q < net->sub_ptr[s + 1]; ++q) d = d + net->term_coeff[q] * s_r[net->term_rate[q]][t];
-            double v = s_c[s][t] + d * dt;
+    for (int s = w; s < S; s += NET_WAVES) {
+        double d[NET_CPL];
+#pragma unroll
+        for (int k = 0; k < NET_CPL; ++k) d[k] = 0.0;
+        for (int q = net->sub_ptr[s]; q < net->sub_ptr[s + 1]; ++q) {
+            const double co = net->term_coeff[q];
+            const int jr = net->term_rate[q];
+#pragma unroll
+            for (int k = 0; k < NET_CPL; ++k) d[k] = d[k] + co * s_r[jr * NET_CELLS + t + k * NET_LANES];
+        }
+#pragma unroll
+        for (int k = 0; k < NET_CPL; ++k) {
+            const int cell = cell0 + k * NET_LANES;
+            double v = conc(s, k) + d[k] * dt;
             if (v < 0.0) v = 0.0;                    // stb.no_negs clamps negatives ...
-            if (isnan(v)) bad = true;                // ... and raises on NaN
-            cn[(size_t)s * n_cells + cell] = v;
+            if (cell < n_cells) {
+                if (isnan(v)) bad = true;            // ... and raises on NaN
+                cn[(size_t)s * n_cells + cell] = v;
+            }
         }
     }
     if (bad) s_bad = 1;
@@ -2043,7 +2106,7 @@ struct bt_ctx {
     double* phib_stage = nullptr; bool phib_staged = false;
     double* phib_prev = nullptr;
     // reaction network (device copy of the description) and its state buffer
-    KNet* net = nullptr; double* c_net = nullptr; int net_species = 0;
+    KNet* net = nullptr; double* c_net = nullptr; int net_species = 0, net_rates = 0;
     std::string err;
 };
 
@@ -2322,8 +2385,8 @@ static void enqueue_step(bt_ctx* c, int parity, int which = -1, int G = 1, int g
     }
     if (which == 3) launch_field(c, false);
     if ((which < 0 || which == 4) && c->net && c->c_net)
-        hipLaunchKernelGGL(k_network, dim3((c->prm.n_cells + NET_CELLS - 1) / NET_CELLS), dim3(NET_BLOCK), 0,
-                           c->stream, (const KNet*)c->net, c->c_net, c->prm.n_cells, c->prm.dt, c->flags,
+        hipLaunchKernelGGL(k_network, dim3((c->prm.n_cells + NET_CELLS - 1) / NET_CELLS), dim3(NET_BLOCK),
+                           net_lds_bytes(c->net_species, c->net_rates), c->stream, (const KNet*)c->net, c->c_net, c->prm.n_cells, c->prm.dt, c->flags,
                            c->bad_step, (const long long*)c->step_ctr, which < 0 ? 1 : 0);
 }
 
@@ -3284,6 +3347,7 @@ int bt_set_network(bt_ctx* c, const bt_network* net, double* c_net) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         c->c_net = c_net;
         c->net_species = net->n_species;
+        c->net_rates = net->n_rates;
     } else {
         c->c_net = nullptr;
     }
