@@ -15,8 +15,8 @@ horizon with its 19 sampled steps. The headline path is FP64 throughout; the FP3
 (FP32 flux laws, FP64 state, charge and voltage) is reported beside it.
 
 Printed: one JSON line with cell-steps/s, the roofline of the step (the
-compulsory bytes of one ideal fused pass per step, per unit as the
-hot-path scope table of the survey defines them, over the measured step time),
+compulsory bytes of one ideal fused pass per step, per membrane, cell and
+env square as ``b_step`` states them, over the measured step time),
 the measured HBM traffic per step from the committed rocprofv3 summaries, the
 launch variants the autotuner chose, and the CPU baseline: the FP64 NumPy
 oracle on the host's cores (clusters in parallel processes) and on one
