@@ -2425,10 +2425,27 @@ static int persistent_grid(bt_ctx* c) {
     return c->pers_grid;
 }
 
+// The blocks a persistent launch uses: as many as the largest phase has work
+// items for (env (tile, ion group) items, cell tiles at PW per block, squares
+// with membranes and (channel, cell) gating items at PB per block), rounded up
+// to a multiple of the XCD count, and at most what stays resident. Every block
+// arrives at each grid barrier, so a small problem waits on few arrivals.
+static int persistent_launch_grid(bt_ctx* c) {
+    const int cap = persistent_grid(c);
+    if (cap < 1) return cap;
+    const long long groups = (c->prm.n_ions + c->tune.env_ions_per_block - 1) / c->tune.env_ions_per_block;
+    long long need = (long long)c->n_env_tiles * groups;
+    need = std::max(need, (long long)(c->n_tiles + PW - 1) / PW);
+    need = std::max(need, (long long)(c->n_live + PB - 1) / PB);
+    need = std::max(need, ((long long)c->prm.n_chan * c->prm.n_cells + PB - 1) / PB);
+    need = ((std::max(need, 1LL) + N_XCD - 1) / N_XCD) * N_XCD;
+    return (int)std::min<long long>(need, cap);
+}
+
 // n_steps (even) steps in cooperative launches of the persistent kernel
 static int launch_persistent(bt_ctx* c, int n_steps) {
     const void* k = persistent_kernel(c);
-    const int grid = persistent_grid(c);
+    const int grid = persistent_launch_grid(c);
     if (!k || grid < 1) return set_err(c, BT_EINVAL, "no persistent step kernel for this problem");
     PersistArgs pa;
     for (int par = 0; par < 2; ++par) {
