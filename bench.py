@@ -286,10 +286,25 @@ def gpu_batch(worlds, mode, autotune=True):
 def time_loop(worlds, mode, steps, warmup):
     """K steps of the batch's run_loop after W warm-up steps (the graphs are
     captured at set-up), bracketed by device synchronisations; HIP events on
-    the simulation stream give the device time."""
+    the simulation stream give the device time. The step alone (480 replayed
+    steps on HIP events, about 0.1 s: the roofline's denominator) is measured
+    before the warm-up steps: the set-up's host work (pinned snapshot buffers)
+    leaves the GPU idle long enough for its clocks to drop, and its first
+    milliseconds of steps then run up to 10% slower, which a 20-step timed
+    region would otherwise take in. Each step kernel repeated alone
+    (``bt_time_kernel``) is timed after the timed steps."""
     import torch
     b = gpu_batch(worlds, mode)
     b.prepare_sampling()              # set-up of the sampled steps, outside the timed region
+    # the step alone (replayed graphs, no sampled step), for the roofline
+    n_pure = 480
+    p0 = torch.cuda.Event(enable_timing=True)
+    p1 = torch.cuda.Event(enable_timing=True)
+    p0.record(b.stream)
+    b.step_n(n_pure, sync=False)
+    p1.record(b.stream)
+    b.sync()
+    pure_us = p0.elapsed_time(p1) / n_pure * 1e3
     b.step_n(warmup)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -301,15 +316,6 @@ def time_loop(worlds, mode, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ev_ms = e0.elapsed_time(e1)
-    # the step alone (replayed graphs, no sampled step), for the roofline
-    n_pure = 96
-    p0 = torch.cuda.Event(enable_timing=True)
-    p1 = torch.cuda.Event(enable_timing=True)
-    p0.record(b.stream)
-    b.step_n(n_pure, sync=False)
-    p1.record(b.stream)
-    b.sync()
-    pure_us = p0.elapsed_time(p1) / n_pure * 1e3
     names = step_kernel_names(b.get_tuning(), "c_net" in b._dev)
     kernels = {k: b.time_kernel(k, 20) * 1e3 for k in names}
     res = {"wall_s": wall, "events_ms": ev_ms, "sampled_steps": n_samp, "kernels_us": kernels,
