@@ -26,6 +26,7 @@ thread, plus the reference's own dense-operator form on the small cluster.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import multiprocessing
 import os
@@ -320,7 +321,9 @@ def time_loop(worlds, mode, steps, warmup):
     kernels = {k: b.time_kernel(k, 20) * 1e3 for k in names}
     res = {"wall_s": wall, "events_ms": ev_ms, "sampled_steps": n_samp, "kernels_us": kernels,
            "step_us": pure_us, "tuning": b.get_tuning(), "autotune": b.autotune_report}
+    b.close()
     del b
+    gc.collect()
     return res
 
 
@@ -337,7 +340,9 @@ def time_sampled(worlds, mode, steps):
     n_samp = b.run_loop(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    b.close()
     del b
+    gc.collect()
     return {"steps": steps, "sampled_steps": n_samp, "wall_s": wall}
 
 
@@ -351,7 +356,9 @@ def gpu_parity(worlds, pick, states, n_steps, mode):
         dv = max(dv, float(np.abs(st.vm - o["vm"]).max()))
         dc = max(dc, float((np.abs(st.cc_cells - o["cc_cells"]) / np.abs(o["cc_cells"]).clip(1e-300)).max()))
         de = max(de, float((np.abs(st.cc_env - o["cc_env"]) / np.abs(o["cc_env"]).clip(1e-300)).max()))
+    b.close()
     del b
+    gc.collect()
     return {"steps": n_steps, "clusters": len(pick), "max_abs_dVm_V": dv, "max_rel_dc_cell": dc,
             "max_rel_dc_env": de}
 

@@ -226,6 +226,19 @@ class BatchSimulator(DeviceFetch):
         if self._ctx:
             self._lib.bt_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
+        w = self.__dict__.pop("_worker", None)
+        if w is not None:
+            w.shutdown(wait=True)
+
+    def close(self) -> None:
+        """Release the context, the sample worker thread, the device state and
+        the pinned snapshot buffers now rather than at garbage collection
+        (an instability the device recorded is not raised here)."""
+        if self._ctx:
+            self.stream.synchronize()
+        self._destroy()
+        for k in ("_dev", "_flat_dev", "_flat_host", "_pinned", "_obs"):
+            self.__dict__.pop(k, None)
 
     def __del__(self):
         try:

@@ -188,6 +188,9 @@ def test_sampled_steps_with_the_fused_exchange_are_bit_identical():
         n_samp = b.run_loop(n)
         assert n_samp >= 5
         runs.append(b.states)
+        b.close()          # the stored series outlive the context; a second close does nothing
+        b.close()
+        assert "_worker" not in vars(b) and "_dev" not in vars(b)
     keys = [k for k in vars(runs[0][0]) if k.endswith("_time") and isinstance(getattr(runs[0][0], k), list)]
     assert {"vm_time", "cc_env_time", "I_tot_x_time", "venv_time"} <= set(keys)
     for k_cl, (sa, sb) in enumerate(zip(*runs)):
